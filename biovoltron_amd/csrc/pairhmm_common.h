@@ -1,0 +1,88 @@
+// Shared host/device definitions for the MI355X PairHMM.
+#pragma once
+
+#include <cstdint>
+
+namespace bvp {
+
+// One unit of kernel work: a read against a contiguous run of haplotypes
+// (all haplotypes of its region in the FP32 pass, one haplotype in the FP64
+// pass).  Results go to out[out0 + h] for h in [0, nhap).
+struct alignas(8) Task {
+  int32_t read;
+  int32_t hap0;
+  int32_t nhap;
+  int32_t band;  // index into the band table, or -1 for the full matrix
+  int64_t out0;
+};
+
+// Per-pair result form, written next to every log10 likelihood.
+enum Form : uint8_t {
+  kFormFP32 = 0,        // FP32 forward, S_f >= 1e-28f
+  kFormFP64 = 1,        // FP32 underflowed, FP64 re-run
+  kFormBandFP32 = 2,    // banded FP32 forward accepted
+  kFormBandFallback = 3,// band rejected (boundary mass), full FP32 used
+  kFormPending = 255,   // internal: queued for the FP64 pass
+};
+
+// Read base classes.  Bits 0-2: match class (slot in the haplotype match
+// window); bit 7: wildcard (the read base matches everything).
+constexpr uint8_t kWildcard = 0x80;
+
+constexpr int kLanes = 64;        // wavefront width
+constexpr int kRowsPerStripe = 63;// lane 0 carries the previous stripe's last row
+constexpr int kBlock = 64;        // anti-diagonal steps per staging block
+constexpr int kSlots = 8;         // match classes per haplotype column
+constexpr int kCarryPad = 192;    // extra carry entries past the haplotype end
+
+// Rescue threshold of the reference driver (pairhmm_common.h:16).
+constexpr float kMinAccepted = 1e-28f;
+
+// Transition mode.
+enum Mode : int {
+  kModeAvx = 0,     // per-base transitions from the quality bytes (AvxPairHMM)
+  kModeScalar = 1,  // constant transition matrix (scalar PairHMM)
+};
+
+struct Tables {
+  const float* ph2pr_f;   // [128]
+  const double* ph2pr_d;  // [128]
+  const float* mm_f;      // [8256]: match-to-match for ins,del quals < 128
+  const double* mm_d;     // [8256]
+  const double* err_scalar;  // [256]: scalar-mode error prob by raw byte
+  double trans[7];        // scalar-mode {MM, MI, MD, IM, II, DM, DD}
+};
+
+// Kernel arguments (one struct, passed by value).
+struct KArgs {
+  // reads: per-position class byte and raw quality bytes
+  const uint8_t* rclass;
+  const uint8_t* q;
+  const uint8_t* ins;
+  const uint8_t* del;
+  const uint8_t* gcp;
+  const int64_t* read_off;
+  // haplotypes: per-position match bitmask over read classes
+  const uint8_t* hmask;
+  const int64_t* hap_off;
+  // work
+  const Task* tasks;
+  int n_tasks;
+  const int* n_tasks_dev;  // when set, the task count is read from device memory
+  int* counter;
+  // results
+  double* out;
+  uint8_t* form;
+  Task* rescue;         // FP32 pass: pairs to re-run in FP64
+  int* rescue_count;
+  // per-wavefront carry scratch
+  void* carry;
+  int64_t carry_stride;  // Pair2<T> entries per wavefront
+  Tables tab;
+  int mode;
+  int renorm;   // FP64: rescale the carry row after every stripe
+  int out_raw;  // write the scaled sum S instead of log10 L (testcase ABI)
+  unsigned long long* rescue_cells;  // FP32 pass: sum of R*H over rescued pairs
+};
+
+}  // namespace bvp

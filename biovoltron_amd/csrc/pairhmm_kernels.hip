@@ -1,0 +1,367 @@
+// MI355X (gfx950) PairHMM forward kernels.
+//
+// Arithmetic contract (reference: lib/intel_native/pairhmm/avx-pairhmm-template.h
+// :83-128 per-row transitions, :136-177 boundary, :183-198 cell recurrence,
+// :299-345 final sum; pairhmm.hpp:55-80 for the constant-transition mode):
+//
+//   M[r][j] = prior(r,j) * (M[r-1][j-1]*mm_r + I[r-1][j-1]*im_r + D[r-1][j-1]*dm_r)
+//   I[r][j] = M[r-1][j]*mx_r + I[r-1][j]*xx_r
+//   D[r][j] = M[r][j-1]*my_r + D[r][j-1]*yy_r
+//   row 0: M = I = 0, D = K/H;  column 0: M = I = D = 0
+//   S = sum_j (M[R][j] + I[R][j])
+//
+// Wavefront layout.  A wavefront sweeps anti-diagonals over a stripe of 63
+// read rows: lane l (1..63) holds row r0-1+l and at step t works on column
+// j = t - l.  Lane 0 is a feeder that holds the previous stripe's last row.
+// Each lane publishes, per cell, the two terms its lower neighbour needs:
+//   Z = M*mm' + I*im' + D*dm'   (' = next row's transitions)
+//   J = M*mx' + I*xx'
+// so the neighbour's M is one DPP-shifted multiply, dpp(Z two steps back) *
+// prior, and its I is dpp(J one step back) -- I is never materialised.  Per
+// cell this is 9 VALU instructions (1 prior, 1 M, 2 D, 3 Z, 2 J) with every
+// cross-lane move folded into a v_*_dpp wave_shr:1 operand.
+//
+// Carry between stripes: the stripe's last row writes (Z, J) per column into a
+// per-wavefront global scratch row, staged through LDS in 64-step blocks so
+// that global traffic is coalesced and the per-step work stays in LDS.  The
+// haplotype enters as a per-column match bitmask over up to 8 read-base
+// classes, expanded into a sliding LDS window of f16 selectors; the prior is
+// one fused multiply-add: prior = sel * (pm - pmis) + pmis.
+#include <hip/hip_runtime.h>
+
+#include "pairhmm_common.h"
+
+namespace bvp {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair2 {
+  T z, j;
+};
+
+template <typename T>
+struct Lds {
+  Pair2<T> cin[kBlock];       // carry-in, columns [64b, 64b+64)
+  Pair2<T> cout[2 * kBlock];  // carry-out window, columns [64b-64, 64b+64)
+  _Float16 win[2 * kBlock][kSlots];  // match selectors, columns [64b-64, 64b+64)
+};
+
+__device__ __forceinline__ float dpp_shr(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138 /*wave_shr:1*/, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ double dpp_shr(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x138, 0xf, 0xf, true);
+  return __longlong_as_double(static_cast<long long>(static_cast<unsigned>(lo)) |
+                              (static_cast<long long>(hi) << 32));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <typename T>
+struct Num;
+template <>
+struct Num<float> {
+  static constexpr float K = 0x1p120f;  // Context.h:109 (FP32)
+  static constexpr double log10K = 36.123599479677743;  // 120*log10(2)
+  __device__ static float ph2pr(const Tables& t, int q) { return t.ph2pr_f[q]; }
+  __device__ static float mm(const Tables& t, int i) { return t.mm_f[i]; }
+  __device__ static float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+};
+template <>
+struct Num<double> {
+  static constexpr double K = 0x1p1020;  // Context.h:149 (FP64)
+  static constexpr double log10K = 307.05178458212172;  // 1020*log10(2)
+  __device__ static double ph2pr(const Tables& t, int q) { return t.ph2pr_d[q]; }
+  __device__ static double mm(const Tables& t, int i) { return t.mm_d[i]; }
+  __device__ static double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+};
+
+
+constexpr int kRenormTarget = 1000;  // carry max is rescaled to 2^1000
+
+template <typename T>
+__device__ __forceinline__ void row_params(const KArgs& a, int64_t ro, int R, int r, int lane,
+                                           T& pmis, T& dp, T& my, T& yy, int& cls, T& mm_n,
+                                           T& im_n, T& dm_n, T& mx_n, T& xx_n) {
+  using N = Num<T>;
+  T pm = T(0);
+  pmis = T(0); my = T(0); yy = T(0); cls = 0;
+  mm_n = T(0); im_n = T(0); dm_n = T(0); mx_n = T(0); xx_n = T(0);
+  if (lane >= 1 && r >= 1 && r <= R) {
+    const int64_t p = ro + r - 1;
+    const uint8_t rc = a.rclass[p];
+    T e;
+    if (a.mode == kModeAvx) {
+      e = N::ph2pr(a.tab, a.q[p] & 127);
+      my = N::ph2pr(a.tab, a.del[p] & 127);
+      yy = N::ph2pr(a.tab, a.gcp[p] & 127);
+    } else {
+      e = static_cast<T>(a.tab.err_scalar[a.q[p]]);
+      my = static_cast<T>(a.tab.trans[2]);
+      yy = static_cast<T>(a.tab.trans[6]);
+    }
+    pm = T(1) - e;
+    pmis = e / T(3);
+    if (rc & kWildcard) pmis = pm;
+    cls = rc & 7;
+  }
+  // prior = sel * dp + pmis gives pm for a match selector of 1.
+  dp = pm - pmis;
+  if (lane >= 1 && r == R) {
+    // Last read row: J = M + I, the per-column summand of S.
+    mx_n = T(1);
+    xx_n = T(1);
+  } else if (r + 1 >= 1 && r + 1 <= R) {
+    if (a.mode == kModeAvx) {
+      const int64_t p1 = ro + r;  // row r+1
+      const int iq = a.ins[p1] & 127, dq = a.del[p1] & 127, cq = a.gcp[p1] & 127;
+      const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
+      mm_n = N::mm(a.tab, ((hi * (hi + 1)) >> 1) + lo);
+      im_n = T(1) - N::ph2pr(a.tab, cq);
+      dm_n = im_n;
+      mx_n = N::ph2pr(a.tab, iq);
+      xx_n = N::ph2pr(a.tab, cq);
+    } else {
+      mm_n = static_cast<T>(a.tab.trans[0]);
+      mx_n = static_cast<T>(a.tab.trans[1]);
+      im_n = static_cast<T>(a.tab.trans[3]);
+      xx_n = static_cast<T>(a.tab.trans[4]);
+      dm_n = static_cast<T>(a.tab.trans[5]);
+    }
+  }
+}
+
+// One anti-diagonal step for lane-local state.  ZS is the slot holding Z from
+// two steps back; it receives this step's Z.
+#define BVP_STEP(U, ZS)                                                       \
+  {                                                                           \
+    const T sel = static_cast<T>(selp[(U) * kSlots]);                          \
+    const T prior = N::fma(sel, dp, pmis);                                     \
+    const T M = dpp_shr(ZS) * prior;                                           \
+    const T D = N::fma(D1, yy, M1 * my);                                       \
+    const T jin = dpp_shr(J1);                                                 \
+    T zn = N::fma(M, mm_n, N::fma(D, dm_n, jin * im_n));                       \
+    T jn = N::fma(M, mx_n, jin * xx_n);                                        \
+    if (lane == 0) {                                                           \
+      const Pair2<T> c = s.cin[(U)];                                           \
+      zn = c.z;                                                                \
+      jn = c.j;                                                                \
+    }                                                                          \
+    if (lane == L) outp[(U)] = Pair2<T>{zn, jn};                               \
+    ZS = zn;                                                                   \
+    J1 = jn;                                                                   \
+    M1 = M;                                                                    \
+    D1 = D;                                                                    \
+  }
+
+#define BVP_STEP2(U) BVP_STEP((U), Za) BVP_STEP((U) + 1, Zb)
+#define BVP_STEP16(U) \
+  BVP_STEP2((U)) BVP_STEP2((U) + 2) BVP_STEP2((U) + 4) BVP_STEP2((U) + 6) \
+  BVP_STEP2((U) + 8) BVP_STEP2((U) + 10) BVP_STEP2((U) + 12) BVP_STEP2((U) + 14)
+
+// Forward probability of one (read, haplotype) pair.  Returns the scaled sum
+// S (as a double) and the power-of-two exponent removed by renormalisation.
+template <typename T>
+__device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int lane,
+                               int read, int hap, int& shift_total) {
+  using N = Num<T>;
+  const int64_t ro = a.read_off[read];
+  const int R = static_cast<int>(a.read_off[read + 1] - ro);
+  const int64_t ho = a.hap_off[hap];
+  const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+  const int nstripes = (R + kRowsPerStripe - 1) / kRowsPerStripe;
+  shift_total = 0;
+  T scale = T(1);
+  double acc = 0.0;
+  for (int st = 0; st < nstripes; ++st) {
+    const int r0 = kRowsPerStripe * st + 1;
+    const bool last = (st == nstripes - 1);
+    const int L = last ? (R - r0 + 1) : kRowsPerStripe;
+    const int r = r0 - 1 + lane;
+    T pmis, dp, my, yy, mm_n, im_n, dm_n, mx_n, xx_n;
+    int cls;
+    row_params<T>(a, ro, R, r, lane, pmis, dp, my, yy, cls, mm_n, im_n, dm_n, mx_n, xx_n);
+    // Row 0 of the matrix as seen by row 1: Z = (K/H) * dm_1, J = 0.
+    const T z_row0 = __shfl((T(N::K) / static_cast<T>(H)) * dm_n, 0);
+    const int steps = H + L + 1;
+    const int nb = (steps + kBlock - 1) / kBlock;
+    // Prefetch block 0.
+    T pre_z, pre_j;
+    uint8_t pre_m;
+    {
+      const int c = lane;
+      if (st == 0) {
+        pre_z = z_row0;
+        pre_j = T(0);
+      } else {
+        const Pair2<T> v = carry[c];
+        pre_z = v.z * scale;
+        pre_j = v.j * scale;
+      }
+      pre_m = (c >= 1 && c <= H) ? a.hmask[ho + c - 1] : uint8_t(0);
+    }
+    T Za = T(0), Zb = T(0), J1 = T(0), M1 = T(0), D1 = T(0);
+    T maxabs = T(0);
+    const _Float16* selp = &s.win[kBlock - lane][cls];
+    Pair2<T>* outp = &s.cout[kBlock - L];
+    for (int b = 0; b < nb; ++b) {
+      __syncthreads();
+      s.cin[lane] = Pair2<T>{pre_z, pre_j};
+      {
+        _Float16 lo[kSlots];
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) lo[k] = (b == 0) ? _Float16(0) : s.win[kBlock + lane][k];
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) s.win[lane][k] = lo[k];
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k)
+          s.win[kBlock + lane][k] = ((pre_m >> k) & 1) ? _Float16(1) : _Float16(0);
+      }
+      {
+        const int c = kBlock * (b + 1) + lane;
+        if (st == 0) {
+          pre_z = z_row0;
+          pre_j = T(0);
+        } else {
+          const Pair2<T> v = carry[c];
+          pre_z = v.z * scale;
+          pre_j = v.j * scale;
+        }
+        pre_m = (c >= 1 && c <= H) ? a.hmask[ho + c - 1] : uint8_t(0);
+      }
+      __syncthreads();
+      const int left = steps - kBlock * b;
+      BVP_STEP16(0)
+      if (left > 16) {
+        BVP_STEP16(16)
+        if (left > 32) {
+          BVP_STEP16(32)
+          if (left > 48) {
+            BVP_STEP16(48)
+          }
+        }
+      }
+      __syncthreads();
+      // Flush the 64 columns the last row produced in this block.
+      const int c = kBlock * b - L + lane;
+      const Pair2<T> v = s.cout[kBlock - L + lane];
+      if (c >= 0 && c <= H) {
+        if (!last) {
+          carry[c] = v;
+          maxabs = fmax(maxabs, fmax(fabs(v.z), fabs(v.j)));
+        } else if (c >= 1) {
+          acc += static_cast<double>(v.j);
+        }
+      }
+    }
+    if (!last) {
+      // Make this stripe's carry row visible to the next stripe's loads.
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (a.renorm) {
+        const double m = wave_max(static_cast<double>(maxabs));
+        if (m > 0.0) {
+          const int e = ilogb(m);
+          const int k = kRenormTarget - e;
+          scale = static_cast<T>(ldexp(1.0, k));
+          shift_total += k;
+        } else {
+          scale = T(1);
+        }
+      }
+    }
+  }
+  return wave_sum(acc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
+  __shared__ Lds<T> s;
+  const int lane = threadIdx.x;
+  Pair2<T>* carry = static_cast<Pair2<T>*>(a.carry) + static_cast<int64_t>(blockIdx.x) * a.carry_stride;
+  const int n_tasks = a.n_tasks_dev ? *a.n_tasks_dev : a.n_tasks;
+  for (;;) {
+    int ti = 0;
+    if (lane == 0) ti = atomicAdd(a.counter, 1);
+    ti = __builtin_amdgcn_readfirstlane(__shfl(ti, 0));
+    if (ti >= n_tasks) break;
+    const Task tk = a.tasks[ti];
+    for (int h = 0; h < tk.nhap; ++h) {
+      int shift = 0;
+      const double S = pair_forward<T>(a, s, carry, lane, tk.read, tk.hap0 + h, shift);
+      const int64_t o = tk.out0 + h;
+      if (lane == 0) {
+        if (a.out_raw) {
+          a.out[o] = ldexp(S, -shift);
+          a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
+        } else if constexpr (sizeof(T) == 4) {
+          if (S < static_cast<double>(kMinAccepted) && a.rescue != nullptr) {
+            const int k = atomicAdd(a.rescue_count, 1);
+            a.rescue[k] = Task{tk.read, tk.hap0 + h, 1, -1, o};
+            a.form[o] = kFormPending;
+            const unsigned long long cells =
+                static_cast<unsigned long long>(a.read_off[tk.read + 1] - a.read_off[tk.read]) *
+                static_cast<unsigned long long>(a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h]);
+            atomicAdd(a.rescue_cells, cells);
+          } else {
+            a.out[o] = log10(S) - Num<T>::log10K;
+            a.form[o] = kFormFP32;
+          }
+        } else {
+          a.out[o] = log10(S) - static_cast<double>(shift) * 0.30102999566398120 - Num<T>::log10K;
+          a.form[o] = kFormFP64;
+        }
+      }
+    }
+  }
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace bvp
+
+// ---- launchers (host) ----------------------------------------------------
+
+namespace bvp {
+
+hipError_t launch_zero_ints(int* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(pairhmm_forward_kernel<T>, dim3(grid), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_forward<float>(const KArgs&, int, hipStream_t);
+template hipError_t launch_forward<double>(const KArgs&, int, hipStream_t);
+
+template <typename T>
+int max_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pairhmm_forward_kernel<T>, 64, 0) != hipSuccess)
+    return 8;
+  return n > 0 ? n : 1;
+}
+template int max_blocks_per_cu<float>();
+template int max_blocks_per_cu<double>();
+
+}  // namespace bvp

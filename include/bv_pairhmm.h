@@ -1,0 +1,153 @@
+/*
+ * bv_pairhmm.h -- C ABI of the MI355X PairHMM read-likelihood stage.
+ *
+ * Drop-in boundary for joyeli/Biovoltron's PairHMM path.  Plain C types only
+ * (no HIP or torch types); every function returns 0 on success or a negative
+ * status (BV_E*), with a per-thread message from bv_pairhmm_last_error().
+ * Buffers passed in are host memory unless a name says otherwise; the library
+ * copies its inputs and keeps no caller pointer after a call returns.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   bv_compute_full_prob        compute_full_prob_avxs<float> / _avxd<double>
+ *                               (lib/intel_native/pairhmm/avx-pairhmm-template.h:210,
+ *                               instantiated in avx-pairhmm.h:7-15, called through
+ *                               the pointers of pairhmm_avx.hpp:33-34,113-114)
+ *   bv_pairhmm_batch /          AvxPairHMM::computeLikelihoodsNative
+ *   bv_plan_*                   (include/biovoltron/algo/align/inexact_match/
+ *                               pairhmm_avx.hpp:121-157): FP32 first, FP64 re-run
+ *                               when S_f < 1e-28f, log10 minus log10(K); for many
+ *                               regions in one launch sequence
+ *   BV_MODE_SCALAR              PairHMM::compute_likelihoods with a TrasMatrx
+ *                               (pairhmm.hpp:116-141, recurrence :55-80)
+ *   bv_pairhmm_init             AvxPairHMM::initNative (pairhmm_avx.hpp:83-119)
+ *   (C++ linkage, in the same   shacc_pairhmm::calculate
+ *    library)                   (lib/intel_native/pairhmm/shacc_pairhmm.h:10-36)
+ *
+ * Normalisation and filtering (pairhmm.hpp:83-114) are host-side and live in
+ * the C++ adapter include/biovoltron_amd/gpu_pairhmm.hpp.
+ */
+#ifndef BV_PAIRHMM_H
+#define BV_PAIRHMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BV_OK 0
+#define BV_EINVAL (-22)
+#define BV_ENOMEM (-12)
+#define BV_ENODEV (-19)
+#define BV_EHIP (-5)
+#define BV_ESTATE (-1)
+
+/* Transition/emission model. */
+#define BV_MODE_AVX 0    /* per-base GOP/GCP transitions, FP32 + FP64 rescue */
+#define BV_MODE_SCALAR 1 /* constant transition matrix, FP64 (scalar PairHMM) */
+
+/* Per-pair result forms (bv_plan_results form[] values). */
+#define BV_FORM_FP32 0
+#define BV_FORM_FP64 1
+#define BV_FORM_BAND_FP32 2
+#define BV_FORM_BAND_FALLBACK 3
+
+/* Initialise the device context and upload the tables (idempotent).  device
+ * is a HIP ordinal; -1 keeps the current device. */
+int bv_pairhmm_init(int device);
+void bv_pairhmm_shutdown(void);
+const char* bv_pairhmm_last_error(void);
+
+typedef struct bv_device_info {
+  int32_t device;
+  int32_t compute_units;
+  int32_t blocks_per_cu_f32;
+  int32_t blocks_per_cu_f64;
+  char arch[64];
+} bv_device_info;
+int bv_pairhmm_device_info(bv_device_info* info);
+
+/* The reference's per-pair descriptor (pairhmm_common.h:20-24). */
+typedef struct bv_testcase {
+  int rslen, haplen;
+  const char *q, *i, *d, *c;
+  const char *hap, *rs;
+} bv_testcase;
+
+/* Per-pair kernel ABI: scaled sum S for each testcase, FP32 (precision 32,
+ * K = 2^120) or FP64 (precision 64, K = 2^1020), exactly the quantity the
+ * reference kernels return (not a log). */
+int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* out_scaled);
+
+/* Regions of reads x haplotypes in packed structure-of-arrays form.  Region g
+ * owns reads [region_read_off[g], region_read_off[g+1]) and haplotypes
+ * [region_hap_off[g], region_hap_off[g+1]); read r's bytes are
+ * [read_off[r], read_off[r+1]) of the five read arrays (quality bytes are raw
+ * Phred values as the reference's testcase carries them); haplotype h's bytes
+ * are [hap_off[h], hap_off[h+1]) of hap_bases.  Results are numbered region
+ * by region, read-major: pair = sum of earlier regions' reads*haps
+ * + r_local * n_haps(g) + h_local. */
+typedef struct bv_region_batch {
+  int64_t n_regions;
+  const int64_t* region_read_off;
+  const int64_t* region_hap_off;
+  int64_t n_reads;
+  const int64_t* read_off;
+  const uint8_t* read_bases;
+  const uint8_t* read_qual;
+  const uint8_t* read_ins;
+  const uint8_t* read_del;
+  const uint8_t* read_gcp;
+  int64_t n_haps;
+  const int64_t* hap_off;
+  const uint8_t* hap_bases;
+} bv_region_batch;
+
+typedef struct bv_options {
+  int32_t mode;        /* BV_MODE_* */
+  int32_t fp64_renorm; /* FP64: rescale per stripe so it never underflows (1),
+                          or reproduce the reference's 2^1020-only scaling (0) */
+  int32_t force_fp64;  /* skip the FP32 pass */
+  int32_t grid_f32;    /* persistent grid override, 0 = occupancy-derived */
+  int32_t grid_f64;
+  int32_t reserved[3];
+  double trans[7];     /* BV_MODE_SCALAR: {MM, MI, MD, IM, II, DM, DD}
+                          (pairhmm.hpp:15-33 order); all zero = ORIGINAL_DEFAULT */
+} bv_options;
+
+void bv_options_default(bv_options* opt);
+
+typedef struct bv_plan bv_plan;
+
+typedef struct bv_plan_stats {
+  int64_t n_pairs;
+  int64_t cells;        /* sum over pairs of R*H */
+  int64_t fp64_pairs;   /* pairs computed by the FP64 pass in the last run */
+  int64_t fp64_cells;
+  int64_t tasks;
+  int32_t grid_f32, grid_f64;
+  float last_ms_f32, last_ms_f64; /* kernel times of the last timed run */
+} bv_plan_stats;
+
+/* Uploads a batch to device memory and builds the length-ordered work list. */
+int bv_plan_create(const bv_region_batch* batch, const bv_options* opt, bv_plan** plan);
+/* Enqueues one full evaluation (FP32 pass, FP64 re-run of underflowed pairs)
+ * on the plan's stream; asynchronous. */
+int bv_plan_run(bv_plan* plan);
+/* Runs the plan iters times back to back and returns the mean wall time per
+ * run measured with HIP events on the plan's stream (inputs resident). */
+int bv_plan_time(bv_plan* plan, int iters, float* ms_per_run);
+/* Waits for the plan's stream and copies results to host buffers sized
+ * n_pairs (either pointer may be NULL). */
+int bv_plan_results(bv_plan* plan, double* log10_out, uint8_t* form_out);
+int bv_plan_stats_get(bv_plan* plan, bv_plan_stats* stats);
+void bv_plan_destroy(bv_plan* plan);
+
+/* One-shot: create, run, fetch, destroy. */
+int bv_pairhmm_batch(const bv_region_batch* batch, const bv_options* opt,
+                     double* log10_out, uint8_t* form_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BV_PAIRHMM_H */

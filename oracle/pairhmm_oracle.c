@@ -1,0 +1,400 @@
+/*
+ * PairHMM oracle -- TEST INFRASTRUCTURE ONLY (see pairhmm_oracle.h).
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
+ */
+#include "pairhmm_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+#include <xmmintrin.h>
+
+/* Context.h:7-11 */
+#define JAC_STEP 0.0001
+#define JAC_SIZE 80001 /* (int)(MAX_JACOBIAN_TOLERANCE / STEP) + 1 */
+#define MAXQ 254
+#define MM_SIZE (((MAXQ + 1) * (MAXQ + 2)) >> 1)
+
+static float jac_f[JAC_SIZE];
+static double jac_d[JAC_SIZE];
+static float mm_f[MM_SIZE];
+static double mm_d[MM_SIZE];
+static float ph2pr_f[128];
+static double ph2pr_d[128];
+static uint8_t base_code[256];
+static int g_ready = 0;
+
+/* Context.h:63-90: log10(10^a + 10^b) with a quantised Jacobian correction.
+ * The float context evaluates it in float (NUMBER = float). */
+static float log10sum_f(float a, float b) {
+  float lo = a, hi = b;
+  if (lo > hi) { lo = b; hi = a; }
+  float diff = hi - lo;
+  if (diff >= 8.0f) return hi;
+  float x = diff * (float)(1.0 / JAC_STEP);
+  int k = (x > 0.0f) ? (int)(x + 0.5f) : (int)(x - 0.5f);
+  return hi + jac_f[k];
+}
+
+static double log10sum_d(double a, double b) {
+  double lo = a, hi = b;
+  if (lo > hi) { lo = b; hi = a; }
+  double diff = hi - lo;
+  if (diff >= 8.0) return hi;
+  double x = diff * (1.0 / JAC_STEP);
+  int k = (x > 0.0) ? (int)(x + 0.5) : (int)(x - 0.5);
+  return hi + jac_d[k];
+}
+
+static inline int mm_index(int iq, int dq) {
+  int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
+  return ((hi * (hi + 1)) >> 1) + lo; /* Context.h:123-134 */
+}
+
+void bvo_init(void) {
+  if (g_ready) return;
+  /* Context.h:42-47 */
+  for (int k = 0; k < JAC_SIZE; ++k) {
+    double v = log10(1.0 + pow(10.0, -((double)k) * JAC_STEP));
+    jac_d[k] = v;
+    jac_f[k] = (float)v;
+  }
+  /* Context.h:50-61, once per precision. */
+  const double inv_ln10 = 1.0 / log(10.0);
+  for (int i = 0, off = 0; i <= MAXQ; off += ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double sf = (double)log10sum_f((float)(-0.1 * i), (float)(-0.1 * j));
+      double lf = log1p(-fmin(1.0, pow(10.0, sf))) * inv_ln10;
+      mm_f[off + j] = (float)pow(10.0, lf);
+      double sd = log10sum_d(-0.1 * i, -0.1 * j);
+      double ld = log1p(-fmin(1.0, pow(10.0, sd))) * inv_ln10;
+      mm_d[off + j] = pow(10.0, ld);
+    }
+  }
+  /* Context.h:105-107,145-147 */
+  for (int x = 0; x < 128; ++x) {
+    ph2pr_d[x] = pow(10.0, -((double)x) / 10.0);
+    ph2pr_f[x] = powf(10.f, -((float)x) / 10.f);
+  }
+  /* pairhmm_common.h:26-44: A0 C1 T2 G3 N4, every other byte 0. */
+  memset(base_code, 0, sizeof(base_code));
+  base_code['A'] = 0;
+  base_code['C'] = 1;
+  base_code['T'] = 2;
+  base_code['G'] = 3;
+  base_code['N'] = 4;
+  g_ready = 1;
+}
+
+float bvo_ph2pr_f32(int q) { bvo_init(); return ph2pr_f[q & 127]; }
+double bvo_ph2pr_f64(int q) { bvo_init(); return ph2pr_d[q & 127]; }
+float bvo_mm_f32(int iq, int dq) { bvo_init(); return mm_f[mm_index(iq & 127, dq & 127)]; }
+double bvo_mm_f64(int iq, int dq) { bvo_init(); return mm_d[mm_index(iq & 127, dq & 127)]; }
+
+/* ---- stripe restatement, FP32 and FP64 -------------------------------- */
+#define T float
+#define FN prob_stripe_f32
+#define INIT_K ldexpf(1.f, 120)
+#define MM_TAB mm_f
+#define PH2PR ph2pr_f
+#include "dp_stripe.inc"
+#undef T
+#undef FN
+#undef INIT_K
+#undef MM_TAB
+#undef PH2PR
+
+#define T double
+#define FN prob_stripe_f64
+#define INIT_K ldexp(1.0, 1020)
+#define MM_TAB mm_d
+#define PH2PR ph2pr_d
+#include "dp_stripe.inc"
+#undef T
+#undef FN
+#undef INIT_K
+#undef MM_TAB
+#undef PH2PR
+
+static unsigned set_ftz(int on) {
+  unsigned old = _mm_getcsr();
+  if (on) _mm_setcsr(old | _MM_FLUSH_ZERO_ON);
+  return old;
+}
+
+float bvo_prob_f32(int R, int H, const uint8_t* rs, const uint8_t* q,
+                   const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                   const uint8_t* hap, int ftz) {
+  bvo_init();
+  unsigned old = set_ftz(ftz);
+  float s = prob_stripe_f32(R, H, rs, q, ins, del, gcp, hap);
+  _mm_setcsr(old);
+  return s;
+}
+
+double bvo_prob_f64(int R, int H, const uint8_t* rs, const uint8_t* q,
+                    const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                    const uint8_t* hap, int ftz) {
+  bvo_init();
+  unsigned old = set_ftz(ftz);
+  double s = prob_stripe_f64(R, H, rs, q, ins, del, gcp, hap);
+  _mm_setcsr(old);
+  return s;
+}
+
+/* pairhmm_avx.hpp:140-148 and pairhmm_common.h:16 (MIN_ACCEPTED). */
+double bvo_log10_reference(int R, int H, const uint8_t* rs, const uint8_t* q,
+                           const uint8_t* ins, const uint8_t* del,
+                           const uint8_t* gcp, const uint8_t* hap, int ftz,
+                           int* rescued) {
+  bvo_init();
+  float sf = bvo_prob_f32(R, H, rs, q, ins, del, gcp, hap, ftz);
+  if (sf < 1e-28f) {
+    double sd = bvo_prob_f64(R, H, rs, q, ins, del, gcp, hap, ftz);
+    if (rescued) *rescued = 1;
+    return log10(sd) - log10(ldexp(1.0, 1020));
+  }
+  if (rescued) *rescued = 0;
+  return (double)(log10f(sf) - log10f(ldexpf(1.f, 120)));
+}
+
+/* ---- FP64 textbook DP with per-row renormalisation -------------------- */
+
+typedef struct {
+  double mm, im, dm, mi, ii, md, dd, pm, pmis;
+  uint8_t code; /* mode 0: base code; mode 1: raw byte */
+} row_par;
+
+static void row_params(int mode, int r, const uint8_t* rs, const uint8_t* q,
+                       const uint8_t* ins, const uint8_t* del,
+                       const uint8_t* gcp, row_par* p) {
+  if (mode == 0) {
+    int iq = ins[r] & 127, dq = del[r] & 127, cq = gcp[r] & 127, qq = q[r] & 127;
+    p->mm = mm_d[mm_index(iq, dq)];
+    p->im = 1.0 - ph2pr_d[cq];
+    p->dm = p->im;
+    p->mi = ph2pr_d[iq];
+    p->ii = ph2pr_d[cq];
+    p->md = ph2pr_d[dq];
+    p->dd = ph2pr_d[cq];
+    double e = ph2pr_d[qq];
+    p->pm = 1.0 - e;
+    p->pmis = e / 3.0;
+    p->code = base_code[rs[r]];
+  } else {
+    /* pairhmm.hpp:32-33 constants, quality_utils.hpp:25-28,71-76 priors. */
+    p->mm = 0.9998; p->mi = 0.0001; p->md = 0.0001;
+    p->im = 0.9; p->ii = 0.1; p->dm = 0.9; p->dd = 0.1;
+    int qq = (int)(int8_t)q[r];
+    double e = pow(10.0, qq / -10.0);
+    p->pm = 1.0 - e;
+    p->pmis = e / 3.0;
+    p->code = rs[r];
+  }
+}
+
+static inline int is_match(int mode, uint8_t rc, uint8_t hb) {
+  if (mode == 0) {
+    uint8_t hc = base_code[hb];
+    return rc == hc || rc == 4 || hc == 4;
+  }
+  return rc == hb || rc == 'N' || hb == 'N';
+}
+
+double bvo_log10_exact(int R, int H, const uint8_t* rs, const uint8_t* q,
+                       const uint8_t* ins, const uint8_t* del,
+                       const uint8_t* gcp, const uint8_t* hap, int mode) {
+  bvo_init();
+  double* buf = (double*)calloc((size_t)6 * (size_t)(H + 1), sizeof(double));
+  double *Mp = buf, *Ip = buf + (H + 1), *Dp = buf + 2 * (H + 1);
+  double *Mc = buf + 3 * (H + 1), *Ic = buf + 4 * (H + 1), *Dc = buf + 5 * (H + 1);
+  for (int j = 0; j <= H; ++j) Dp[j] = 1.0 / (double)H;
+  long log2scale = 0; /* values are stored times 2^-log2scale */
+  for (int r = 1; r <= R; ++r) {
+    row_par p;
+    row_params(mode, r - 1, rs, q, ins, del, gcp, &p);
+    Mc[0] = Ic[0] = Dc[0] = 0.0;
+    double mx = 0.0;
+    for (int j = 1; j <= H; ++j) {
+      double prior = is_match(mode, p.code, hap[j - 1]) ? p.pm : p.pmis;
+      Mc[j] = (Mp[j - 1] * p.mm + Ip[j - 1] * p.im + Dp[j - 1] * p.dm) * prior;
+      Ic[j] = Mp[j] * p.mi + Ip[j] * p.ii;
+      Dc[j] = Mc[j - 1] * p.md + Dc[j - 1] * p.dd;
+      double v = Mc[j] > Ic[j] ? Mc[j] : Ic[j];
+      v = v > Dc[j] ? v : Dc[j];
+      mx = mx > v ? mx : v;
+    }
+    if (mx > 0.0) {
+      int e;
+      frexp(mx, &e);
+      for (int j = 0; j <= H; ++j) {
+        Mc[j] = ldexp(Mc[j], -e);
+        Ic[j] = ldexp(Ic[j], -e);
+        Dc[j] = ldexp(Dc[j], -e);
+      }
+      log2scale += e;
+    }
+    double* t;
+    t = Mp; Mp = Mc; Mc = t;
+    t = Ip; Ip = Ic; Ic = t;
+    t = Dp; Dp = Dc; Dc = t;
+  }
+  double s = 0.0;
+  for (int j = 1; j <= H; ++j) s += Mp[j] + Ip[j];
+  free(buf);
+  return log10(s) + (double)log2scale * log10(2.0);
+}
+
+/* pairhmm.hpp:55-80: no renormalisation, K = 2^1020. */
+double bvo_scalar_log10(int R, int H, const uint8_t* rs, const uint8_t* q,
+                        const uint8_t* hap, const double* tr) {
+  bvo_init();
+  double* buf = (double*)calloc((size_t)6 * (size_t)(H + 1), sizeof(double));
+  double *Mp = buf, *Ip = buf + (H + 1), *Dp = buf + 2 * (H + 1);
+  double *Mc = buf + 3 * (H + 1), *Ic = buf + 4 * (H + 1), *Dc = buf + 5 * (H + 1);
+  const double K = pow(2.0, 1020);
+  for (int j = 0; j <= H; ++j) Dp[j] = K / (double)H;
+  for (int r = 1; r <= R; ++r) {
+    int qq = (int)(int8_t)q[r - 1];
+    double e = pow(10.0, qq / -10.0);
+    uint8_t x = rs[r - 1];
+    Mc[0] = Ic[0] = Dc[0] = 0.0;
+    for (int j = 1; j <= H; ++j) {
+      uint8_t y = hap[j - 1];
+      double prior = (x == y || x == 'N' || y == 'N') ? 1.0 - e : e / 3.0;
+      Mc[j] = prior * (Mp[j - 1] * tr[0] + Ip[j - 1] * tr[3] + Dp[j - 1] * tr[5]);
+      Ic[j] = Mp[j] * tr[1] + Ip[j] * tr[4];
+      Dc[j] = Mc[j - 1] * tr[2] + Dc[j - 1] * tr[6];
+    }
+    double* t;
+    t = Mp; Mp = Mc; Mc = t;
+    t = Ip; Ip = Ic; Ic = t;
+    t = Dp; Dp = Dc; Dc = t;
+  }
+  double s = 0.0;
+  for (int j = 1; j <= H; ++j) s += Mp[j] + Ip[j];
+  free(buf);
+  return log10(s) - log10(K);
+}
+
+/* ---- forward-backward posteriors (FP64, per-row scaling) -------------- */
+
+double bvo_band_posterior(int R, int H, const uint8_t* rs, const uint8_t* q,
+                          const uint8_t* ins, const uint8_t* del,
+                          const uint8_t* gcp, const uint8_t* hap,
+                          double* mass, double* row_sum) {
+  bvo_init();
+  const size_t W = (size_t)H + 2;
+  const size_t N = ((size_t)R + 2) * W;
+  double* FM = (double*)calloc(N, sizeof(double));
+  double* FI = (double*)calloc(N, sizeof(double));
+  double* FD = (double*)calloc(N, sizeof(double));
+  double* BM = (double*)calloc(N, sizeof(double));
+  double* BI = (double*)calloc(N, sizeof(double));
+  double* BD = (double*)calloc(N, sizeof(double));
+  row_par* P = (row_par*)malloc(sizeof(row_par) * (size_t)(R + 2));
+  double* fs = (double*)calloc((size_t)R + 2, sizeof(double)); /* log2 scale of forward row r */
+  double* bs = (double*)calloc((size_t)R + 2, sizeof(double)); /* log2 scale of backward row r */
+#define AT(a, r, j) a[(size_t)(r) * W + (size_t)(j)]
+  for (int r = 1; r <= R; ++r) row_params(0, r - 1, rs, q, ins, del, gcp, &P[r]);
+  /* forward */
+  for (int j = 0; j <= H; ++j) AT(FD, 0, j) = 1.0 / (double)H;
+  for (int r = 1; r <= R; ++r) {
+    double mx = 0.0;
+    for (int j = 1; j <= H; ++j) {
+      double prior = is_match(0, P[r].code, hap[j - 1]) ? P[r].pm : P[r].pmis;
+      AT(FM, r, j) = (AT(FM, r - 1, j - 1) * P[r].mm + AT(FI, r - 1, j - 1) * P[r].im +
+                      AT(FD, r - 1, j - 1) * P[r].dm) * prior;
+      AT(FI, r, j) = AT(FM, r - 1, j) * P[r].mi + AT(FI, r - 1, j) * P[r].ii;
+      AT(FD, r, j) = AT(FM, r, j - 1) * P[r].md + AT(FD, r, j - 1) * P[r].dd;
+      double v = fmax(AT(FM, r, j), fmax(AT(FI, r, j), AT(FD, r, j)));
+      mx = fmax(mx, v);
+    }
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);
+    for (int j = 0; j <= H; ++j) {
+      AT(FM, r, j) = ldexp(AT(FM, r, j), -e);
+      AT(FI, r, j) = ldexp(AT(FI, r, j), -e);
+      AT(FD, r, j) = ldexp(AT(FD, r, j), -e);
+    }
+    fs[r] = fs[r - 1] + e;
+  }
+  double total = 0.0;
+  for (int j = 1; j <= H; ++j) total += AT(FM, R, j) + AT(FI, R, j);
+  double log2L = log2(total) + fs[R];
+  /* backward: B(r,j) = P(rows r+1..R emitted | state at (r,j)); the end
+   * condition is B_M(R,j) = B_I(R,j) = 1, B_D(R,j) = 0. */
+  for (int j = 1; j <= H; ++j) { AT(BM, R, j) = 1.0; AT(BI, R, j) = 1.0; }
+  bs[R] = 0.0;
+  for (int r = R - 1; r >= 1; --r) {
+    double mx = 0.0;
+    for (int j = H; j >= 1; --j) {
+      double pr_next = (j + 1 <= H)
+          ? (is_match(0, P[r + 1].code, hap[j]) ? P[r + 1].pm : P[r + 1].pmis) : 0.0;
+      double mnext = (j + 1 <= H) ? AT(BM, r + 1, j + 1) * pr_next : 0.0;
+      double dnext = (j + 1 <= H) ? AT(BD, r, j + 1) : 0.0;
+      AT(BM, r, j) = P[r + 1].mm * mnext + P[r + 1].mi * AT(BI, r + 1, j) + P[r].md * dnext;
+      AT(BI, r, j) = P[r + 1].im * mnext + P[r + 1].ii * AT(BI, r + 1, j);
+      AT(BD, r, j) = P[r + 1].dm * mnext + P[r].dd * dnext;
+      mx = fmax(mx, fmax(AT(BM, r, j), fmax(AT(BI, r, j), AT(BD, r, j))));
+    }
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);
+    for (int j = 1; j <= H; ++j) {
+      AT(BM, r, j) = ldexp(AT(BM, r, j), -e);
+      AT(BI, r, j) = ldexp(AT(BI, r, j), -e);
+      AT(BD, r, j) = ldexp(AT(BD, r, j), -e);
+    }
+    bs[r] = bs[r + 1] + e;
+  }
+  for (int o = 0; o < R + H - 1; ++o) mass[o] = 0.0;
+  for (int r = 1; r <= R; ++r) {
+    double rsum = 0.0;
+    double sc = exp2(fs[r] + bs[r] - log2L);
+    for (int j = 1; j <= H; ++j) {
+      double pmi = (AT(FM, r, j) * AT(BM, r, j) + AT(FI, r, j) * AT(BI, r, j)) * sc;
+      double pd = AT(FD, r, j) * AT(BD, r, j) * sc;
+      rsum += pmi;
+      mass[(j - r) + R - 1] += pmi + pd;
+    }
+    if (row_sum) row_sum[r - 1] = rsum;
+  }
+  /* D cells carry no read base, so the mass above double counts nothing but
+   * is not normalised to 1 per row; normalise the offset histogram. */
+  double tot = 0.0;
+  for (int o = 0; o < R + H - 1; ++o) tot += mass[o];
+  if (tot > 0.0)
+    for (int o = 0; o < R + H - 1; ++o) mass[o] /= tot;
+#undef AT
+  free(FM); free(FI); free(FD); free(BM); free(BI); free(BD);
+  free(P); free(fs); free(bs);
+  return log2L * log10(2.0);
+}
+
+/* ---- region driver (pairhmm_avx.hpp:121-157) --------------------------- */
+
+long bvo_region_log10(int n_reads, const int64_t* read_off, const uint8_t* rs,
+                      const uint8_t* q, const uint8_t* ins, const uint8_t* del,
+                      const uint8_t* gcp, int n_haps, const int64_t* hap_off,
+                      const uint8_t* hap, int n_threads, double* out) {
+  bvo_init();
+  long rescued = 0;
+  if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads) reduction(+ : rescued)
+  for (int i = 0; i < n_reads; ++i) {
+    /* FTZ on every worker, as the reference's calling thread has it. */
+    _mm_setcsr(_mm_getcsr() | _MM_FLUSH_ZERO_ON);
+    const int64_t ro = read_off[i];
+    const int R = (int)(read_off[i + 1] - ro);
+    for (int h = 0; h < n_haps; ++h) {
+      const int64_t ho = hap_off[h];
+      const int H = (int)(hap_off[h + 1] - ho);
+      int resc = 0;
+      out[(size_t)i * (size_t)n_haps + (size_t)h] = bvo_log10_reference(
+          R, H, rs + ro, q + ro, ins + ro, del + ro, gcp + ro, hap + ho, 1, &resc);
+      rescued += resc;
+    }
+  }
+  return rescued;
+}

@@ -1,0 +1,100 @@
+/*
+ * PairHMM oracle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference CPU PairHMM, used as the parity
+ * checker for the HIP path.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product path
+ * (biovoltron_amd/) links or calls it.
+ *
+ * Restated reference (joyeli/Biovoltron):
+ *   tables         lib/intel_native/pairhmm/Context.h:42-61,63-90,100-175
+ *   base codes     lib/intel_native/pairhmm/pairhmm_common.h:16,26-44
+ *   DP + striping  lib/intel_native/pairhmm/avx-pairhmm-template.h:83-346
+ *   carry shifts   lib/intel_native/pairhmm/avx-vector-shift.h:3-56
+ *   driver rule    include/biovoltron/algo/align/inexact_match/pairhmm_avx.hpp:121-157
+ *   scalar mode    include/biovoltron/algo/align/inexact_match/pairhmm.hpp:39-80
+ *   post-process   include/biovoltron/algo/align/inexact_match/pairhmm.hpp:83-114
+ *
+ * How each function is pinned (what checks it besides its own formula):
+ *   bvo_prob_f32 / bvo_prob_f64   bit-exact vs the reference kernels compiled
+ *                                 from their own sources (oracle/_ref), KATs,
+ *                                 and the closed forms for R=1 and R=H=1.
+ *   bvo_log10_exact               brute-force path enumeration (R,H<=4),
+ *                                 invariance to the scale K, 1e-9 agreement
+ *                                 with bvo_prob_f64 where that is finite.
+ *   bvo_scalar_log10              brute-force enumeration with the constant
+ *                                 transition matrix.
+ *   bvo_band_posterior            per-row posterior mass sums to 1 (pins the
+ *                                 backward pass), band covering everything.
+ */
+#ifndef BV_PAIRHMM_ORACLE_H
+#define BV_PAIRHMM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Build the tables (idempotent). */
+void bvo_init(void);
+
+/* Table access for cross-checking the product's own tables. */
+float bvo_ph2pr_f32(int q);
+double bvo_ph2pr_f64(int q);
+float bvo_mm_f32(int iq, int dq);
+double bvo_mm_f64(int iq, int dq);
+
+/* Reference stripe semantics.  Returns the scaled sum S (not a log), exactly
+ * as compute_full_prob_avx{s,d} does: FP32 with K=2^120 and 8-row stripes,
+ * FP64 with K=2^1020 and 4-row stripes.  ftz!=0 sets the x86 flush-to-zero
+ * mode during the call, as the reference driver does on its calling thread. */
+float bvo_prob_f32(int R, int H, const uint8_t* rs, const uint8_t* q,
+                   const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                   const uint8_t* hap, int ftz);
+double bvo_prob_f64(int R, int H, const uint8_t* rs, const uint8_t* q,
+                    const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                    const uint8_t* hap, int ftz);
+
+/* Driver rule: FP32 first, FP64 re-run when S_f < 1e-28f, then log10 minus
+ * log10(K).  *rescued (optional) receives 1 when the FP64 run was used. */
+double bvo_log10_reference(int R, int H, const uint8_t* rs, const uint8_t* q,
+                           const uint8_t* ins, const uint8_t* del,
+                           const uint8_t* gcp, const uint8_t* hap, int ftz,
+                           int* rescued);
+
+/* FP64 textbook forward DP with per-row renormalisation (never underflows).
+ * mode 0: per-base transitions as in the AVX path (matchToMatch table,
+ *         ph2pr); mode 1: the scalar PairHMM constant transition matrix with
+ *         priors from 10^(-q/10) and wildcard 'N' only.  Returns log10 L. */
+double bvo_log10_exact(int R, int H, const uint8_t* rs, const uint8_t* q,
+                       const uint8_t* ins, const uint8_t* del,
+                       const uint8_t* gcp, const uint8_t* hap, int mode);
+
+/* Scalar PairHMM (pairhmm.hpp:55-80): FP64, constant transitions
+ * trans[7] = {MM, MI, MD, IM, II, DM, DD}, K = 2^1020, no rescue. */
+double bvo_scalar_log10(int R, int H, const uint8_t* rs, const uint8_t* q,
+                        const uint8_t* hap, const double* trans);
+
+/* Forward-backward posterior mass per diagonal offset o = j - r, for
+ * o in [-(R-1), H-1] (length R+H-1), written to mass[o + R - 1]; the values
+ * sum to 1.  row_sum (optional, length R) receives the per-row total
+ * posterior over M+I (must be 1 for every row).  Returns log10 L. */
+double bvo_band_posterior(int R, int H, const uint8_t* rs, const uint8_t* q,
+                          const uint8_t* ins, const uint8_t* del,
+                          const uint8_t* gcp, const uint8_t* hap,
+                          double* mass, double* row_sum);
+
+/* Region driver as the reference runs it (pairhmm_avx.hpp:121-157): OpenMP
+ * schedule(dynamic,1) over reads, haplotypes serial inside, FP32 then FP64.
+ * Reads and haplotypes are packed byte arrays with offsets (length n+1).
+ * out is read-major [n_reads][n_haps].  Returns the number of rescued pairs. */
+long bvo_region_log10(int n_reads, const int64_t* read_off, const uint8_t* rs,
+                      const uint8_t* q, const uint8_t* ins, const uint8_t* del,
+                      const uint8_t* gcp, int n_haps, const int64_t* hap_off,
+                      const uint8_t* hap, int n_threads, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
