@@ -1,0 +1,169 @@
+"""PairHMM throughput on one MI355X.
+
+A step is one full evaluation of the workload: the FP32 forward pass over all
+read x haplotype pairs of 10,000 synthetic regions, the FP64 re-run of pairs
+whose FP32 sum falls below 1e-28, and the final log10, with inputs resident in
+device memory.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--regions N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "GCUPS (read×hap cells/s) learned-band vs full PairHMM at |Δlog10L|≤1e-3"
+FLOPS_PER_CELL = 12  # avx-pairhmm-template.h:188-197: 8 mul + 4 add
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) datasheet peak
+PEAK_FP32_NOPACK_TFLOPS = 78.6
+
+
+def cpu_baseline(batch, out_gpu, budget_s: float):
+    """The reference kernels (oracle/_ref, built from the reference sources)
+    run as the reference driver runs them -- OpenMP schedule(dynamic,1) over
+    reads, min(64, cores) threads, FP32 then FP64 -- region by region over a
+    bounded sample of the same workload.  Falls back to the oracle's C
+    restatement ("port") where oracle/_ref is absent.  Also reports the GPU's
+    max |dlog10 L| against these CPU results on the sampled pairs."""
+    ref_so = ROOT / "oracle" / "_ref" / "libref_pairhmm.so"
+    port_so = ROOT / "oracle" / "liboracle_pairhmm.so"
+    if ref_so.exists():
+        lib, fn, kind = C.CDLL(str(ref_so)), "ref_region_log10", "reference"
+    else:
+        lib, fn, kind = C.CDLL(str(port_so)), "bvo_region_log10", "port"
+    f = getattr(lib, fn)
+    f.restype = C.c_long
+    f.argtypes = [C.c_int] + [C.c_void_p] * 6 + [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    cores = len(os.sched_getaffinity(0))
+    threads = min(64, cores)
+    b = batch
+    cells = 0
+    elapsed = 0.0
+    regions = 0
+    worst = 0.0
+    rng = np.random.default_rng(1)
+    order = rng.permutation(b.n_regions)
+    for g in order:
+        r0, r1 = int(b.region_read_off[g]), int(b.region_read_off[g + 1])
+        h0, h1 = int(b.region_hap_off[g]), int(b.region_hap_off[g + 1])
+        nr, nh = r1 - r0, h1 - h0
+        ro = np.ascontiguousarray(b.read_off[r0:r1 + 1])
+        ho = np.ascontiguousarray(b.hap_off[h0:h1 + 1])
+        out = np.zeros(nr * nh)
+        t0 = time.perf_counter()
+        f(nr, ro.ctypes.data, b.read_bases.ctypes.data, b.read_qual.ctypes.data,
+          b.read_ins.ctypes.data, b.read_del.ctypes.data, b.read_gcp.ctypes.data,
+          nh, ho.ctypes.data, b.hap_bases.ctypes.data, threads, out.ctypes.data)
+        elapsed += time.perf_counter() - t0
+        rl = np.diff(ro).sum()
+        cells += int(rl) * int(np.diff(ho).sum())
+        p0 = int(b.region_pair_off[g])
+        fin = np.isfinite(out)
+        if fin.any():
+            worst = max(worst, float(np.abs(out_gpu[p0:p0 + nr * nh][fin] - out[fin]).max()))
+        regions += 1
+        if elapsed >= budget_s:
+            break
+    gcups = cells / elapsed / 1e9
+    return dict(value=round(gcups, 4), unit="GCUPS", cores=threads, kind=kind,
+                sample=f"{regions} of {b.n_regions} regions ({cells:.3e} cells, "
+                       f"{100.0 * cells / b.cells:.2f}% of the workload), {elapsed:.1f} s, "
+                       f"{os.cpu_count()} host CPUs visible, {cores} in affinity"), worst
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--regions", type=int, default=10000)
+    ap.add_argument("--workload", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the PairHMM path has no cross-GPU step; "
+              "regions shard independently (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from biovoltron_amd import pairhmm, synth
+
+    t0 = time.perf_counter()
+    b = synth.generate(args.workload, count=args.regions)
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] generated workload {args.workload}: {b.n_regions} regions, {b.n_pairs} pairs, "
+          f"{b.cells:.4e} cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    plan = pairhmm.Plan(pairhmm.PackedBatch.from_synth(b))
+    print(f"[bench] plan ready ({time.perf_counter() - t0 - t_gen:.1f} s)", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        plan.run()
+        plan.results()  # waits for the stream
+    # timed region: K full evaluations, synchronised on both sides
+    plan.results()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run()
+    out, form = plan.results()
+    t2 = time.perf_counter()
+    ms_step = (t2 - t1) * 1e3 / args.steps
+    # split of one evaluation between the passes, from HIP events on the stream
+    ev_ms = plan.time(1)
+    st = plan.stats()
+    info = pairhmm.device_info()
+    print(f"[bench] {ms_step:.2f} ms/step wall, events: total {ev_ms:.2f} ms, fp32 {st.last_ms_f32:.2f} ms, "
+          f"fp64 {st.last_ms_f64:.2f} ms ({st.fp64_pairs} pairs, {st.fp64_cells:.3e} cells)",
+          file=sys.stderr, flush=True)
+    gcups = b.cells / (ms_step / 1e3) / 1e9
+    achieved = FLOPS_PER_CELL * b.cells / (st.last_ms_f32 / 1e3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64-rescue",
+        "data": "synthetic (seeded generator, biovoltron_amd/csrc/synth.cpp)",
+        "config": {
+            "workload": f"batched full-matrix PairHMM, workload {args.workload} generator, "
+                        f"{b.n_regions} regions",
+            "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,
+            "seed": synth.workload_seed(args.workload),
+            "fp64_rescued_pairs": st.fp64_pairs, "fp64_rescued_cells": st.fp64_cells,
+            "device": info.arch.decode(), "compute_units": info.compute_units,
+            "grid_fp32": st.grid_f32, "grid_fp64": st.grid_f64,
+        },
+        "roofline": {
+            "bound": "valu", "kernel": "pairhmm_forward_kernel<float>",
+            "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "frac_of_unpacked_peak": round(achieved / PEAK_FP32_NOPACK_TFLOPS, 4),
+            "flops_per_cell": FLOPS_PER_CELL, "kernel_ms": round(st.last_ms_f32, 3),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        cb, worst = cpu_baseline(b, out, args.cpu_seconds)
+        line["cpu_baseline"] = cb
+        line["config"]["max_abs_dlog10_vs_cpu_sample"] = worst
+    plan.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,767 @@
+// Host side of the MI355X PairHMM: tables, device context, plans, C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "bv_pairhmm.h"
+#include "pairhmm_common.h"
+
+namespace bvp {
+template <typename T>
+hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st);
+template <typename T>
+int max_blocks_per_cu();
+hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
+}  // namespace bvp
+
+namespace {
+
+using namespace bvp;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return fail(BV_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+// ---- tables -------------------------------------------------------------
+// Same quantities as the reference's Context<T> (lib/intel_native/pairhmm/
+// Context.h:42-61 Jacobian and match-to-match tables, :63-90 quantised
+// log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
+// uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
+// avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
+constexpr int kJac = 80001;
+constexpr int kMmEntries = (128 * 129) / 2;
+
+struct HostTables {
+  float ph2pr_f[128];
+  double ph2pr_d[128];
+  float mm_f[kMmEntries];
+  double mm_d[kMmEntries];
+  double err_scalar[256];
+};
+
+template <typename T>
+T log10_sum_quantised(const std::vector<T>& jac, T a, T b) {
+  T lo = a, hi = b;
+  if (lo > hi) std::swap(lo, hi);
+  const T diff = hi - lo;
+  if (diff >= T(8)) return hi;
+  const T x = diff * static_cast<T>(1.0 / 0.0001);
+  const int k = (x > T(0)) ? static_cast<int>(x + T(0.5)) : static_cast<int>(x - T(0.5));
+  return hi + jac[k];
+}
+
+void build_tables(HostTables& t) {
+  std::vector<float> jf(kJac);
+  std::vector<double> jd(kJac);
+  for (int k = 0; k < kJac; ++k) {
+    const double v = std::log10(1.0 + std::pow(10.0, -static_cast<double>(k) * 0.0001));
+    jd[k] = v;
+    jf[k] = static_cast<float>(v);
+  }
+  const double inv_ln10 = 1.0 / std::log(10.0);
+  for (int i = 0, off = 0; i < 128; off += ++i) {
+    for (int j = 0; j <= i; ++j) {
+      const double sf = static_cast<double>(log10_sum_quantised<float>(
+          jf, static_cast<float>(-0.1 * i), static_cast<float>(-0.1 * j)));
+      t.mm_f[off + j] = static_cast<float>(
+          std::pow(10.0, std::log1p(-std::min(1.0, std::pow(10.0, sf))) * inv_ln10));
+      const double sd = log10_sum_quantised<double>(jd, -0.1 * i, -0.1 * j);
+      t.mm_d[off + j] = std::pow(10.0, std::log1p(-std::min(1.0, std::pow(10.0, sd))) * inv_ln10);
+    }
+  }
+  for (int x = 0; x < 128; ++x) {
+    t.ph2pr_d[x] = std::pow(10.0, -static_cast<double>(x) / 10.0);
+    t.ph2pr_f[x] = powf(10.f, -static_cast<float>(x) / 10.f);
+  }
+  // Scalar mode: QualityUtils::qual_to_error_prob (quality_utils.hpp:25-28,
+  // 71-76) indexes a 128-entry table with the raw char; bytes >= 128 are
+  // rejected at plan creation.
+  for (int x = 0; x < 256; ++x) t.err_scalar[x] = x < 128 ? std::pow(10.0, x / -10.0) : 1.0;
+}
+
+// ---- device context -----------------------------------------------------
+struct DeviceCtx {
+  bool ready = false;
+  int device = 0;
+  int cus = 0;
+  int bpc_f32 = 1, bpc_f64 = 1;
+  std::string arch;
+  HostTables host{};
+  float* d_ph2pr_f = nullptr;
+  double* d_ph2pr_d = nullptr;
+  float* d_mm_f = nullptr;
+  double* d_mm_d = nullptr;
+  double* d_err_scalar = nullptr;
+};
+
+std::mutex g_mu;
+DeviceCtx g_ctx;
+
+int ensure_init_locked(int device) {
+  if (g_ctx.ready) return BV_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(BV_ENODEV, "no HIP device");
+  if (device < 0) HIP_TRY(hipGetDevice(&device));
+  if (device >= n) return fail(BV_ENODEV, "device ordinal out of range");
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  g_ctx.device = device;
+  g_ctx.cus = prop.multiProcessorCount;
+  g_ctx.arch = prop.gcnArchName;
+  if (g_ctx.arch.rfind("gfx950", 0) != 0)
+    return fail(BV_ENODEV, "this library is built for gfx950, device is " + g_ctx.arch);
+  build_tables(g_ctx.host);
+  auto& h = g_ctx.host;
+  HIP_TRY(hipMalloc(&g_ctx.d_ph2pr_f, sizeof(h.ph2pr_f)));
+  HIP_TRY(hipMalloc(&g_ctx.d_ph2pr_d, sizeof(h.ph2pr_d)));
+  HIP_TRY(hipMalloc(&g_ctx.d_mm_f, sizeof(h.mm_f)));
+  HIP_TRY(hipMalloc(&g_ctx.d_mm_d, sizeof(h.mm_d)));
+  HIP_TRY(hipMalloc(&g_ctx.d_err_scalar, sizeof(h.err_scalar)));
+  HIP_TRY(hipMemcpy(g_ctx.d_ph2pr_f, h.ph2pr_f, sizeof(h.ph2pr_f), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g_ctx.d_ph2pr_d, h.ph2pr_d, sizeof(h.ph2pr_d), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g_ctx.d_mm_f, h.mm_f, sizeof(h.mm_f), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g_ctx.d_mm_d, h.mm_d, sizeof(h.mm_d), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g_ctx.d_err_scalar, h.err_scalar, sizeof(h.err_scalar), hipMemcpyHostToDevice));
+  g_ctx.bpc_f32 = max_blocks_per_cu<float>();
+  g_ctx.bpc_f64 = max_blocks_per_cu<double>();
+  g_ctx.ready = true;
+  return BV_OK;
+}
+
+// ---- read/haplotype encoding ---------------------------------------------
+// AVX mode (pairhmm_common.h:26-44, avx-pairhmm-template.h:3-35): codes
+// A0 C1 T2 G3, 'N' is a wildcard on either side, every other byte acts as
+// 'A'.  Scalar mode (pairhmm.hpp:48): raw byte equality, uppercase 'N' is the
+// only wildcard; every distinct non-'N' read byte gets a match class.
+struct Alphabet {
+  uint8_t read_class[256];
+  uint8_t hap_mask[256];
+};
+
+void alphabet_avx(Alphabet& a) {
+  for (int b = 0; b < 256; ++b) {
+    a.read_class[b] = 0;
+    a.hap_mask[b] = 1;  // acts as 'A'
+  }
+  const char codes[4] = {'A', 'C', 'T', 'G'};
+  for (int k = 0; k < 4; ++k) {
+    a.read_class[static_cast<uint8_t>(codes[k])] = static_cast<uint8_t>(k);
+    a.hap_mask[static_cast<uint8_t>(codes[k])] = static_cast<uint8_t>(1u << k);
+  }
+  a.read_class[static_cast<uint8_t>('N')] = kWildcard;
+  a.hap_mask[static_cast<uint8_t>('N')] = 0xFF;
+}
+
+int alphabet_scalar(Alphabet& a, const uint8_t* bases, int64_t n) {
+  bool seen[256] = {false};
+  for (int64_t i = 0; i < n; ++i) seen[bases[i]] = true;
+  int next = 0;
+  for (int b = 0; b < 256; ++b) {
+    a.read_class[b] = 0;
+    a.hap_mask[b] = 0;
+  }
+  for (int b = 0; b < 256; ++b) {
+    if (!seen[b] || b == 'N') continue;
+    if (next >= kSlots)
+      return fail(BV_EINVAL, "scalar mode supports at most 8 distinct read base bytes");
+    a.read_class[b] = static_cast<uint8_t>(next);
+    a.hap_mask[b] = static_cast<uint8_t>(1u << next);
+    ++next;
+  }
+  a.read_class[static_cast<uint8_t>('N')] = kWildcard;
+  a.hap_mask[static_cast<uint8_t>('N')] = 0xFF;
+  return BV_OK;
+}
+
+template <typename F>
+void parallel_for(int64_t n, F&& f) {
+  const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < (1 << 16) || nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const int64_t a = t * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([&, a, b] { f(a, b); });
+  }
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+// ---- plan ---------------------------------------------------------------
+struct bv_plan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bv_options opt{};
+  int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
+  int hmax = 0;
+  int n_tasks = 0;
+  int grid32 = 0, grid64 = 0;
+  int64_t stride32 = 0, stride64 = 0;
+  uint8_t* d_rclass = nullptr;
+  uint8_t* d_q = nullptr;
+  uint8_t* d_ins = nullptr;
+  uint8_t* d_del = nullptr;
+  uint8_t* d_gcp = nullptr;
+  int64_t* d_read_off = nullptr;
+  uint8_t* d_hmask = nullptr;
+  int64_t* d_hap_off = nullptr;
+  Task* d_tasks = nullptr;
+  Task* d_rescue = nullptr;
+  int* d_counters = nullptr;  // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] pad
+  unsigned long long* d_rescue_cells = nullptr;
+  double* d_out = nullptr;
+  uint8_t* d_form = nullptr;
+  void* d_carry32 = nullptr;
+  void* d_carry64 = nullptr;
+  float last_ms32 = 0.f, last_ms64 = 0.f;
+  bool timed = false;
+};
+
+namespace {
+
+void free_plan(bv_plan* p) {
+  if (!p) return;
+  // d_rescue_cells points into d_counters and is not freed on its own.
+  void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
+                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_rescue, p->d_counters,
+                  p->d_out, p->d_form, p->d_carry32, p->d_carry64};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto& e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+template <typename T>
+int dev_upload(T** dst, const T* src, int64_t n, hipStream_t st) {
+  const size_t bytes = sizeof(T) * static_cast<size_t>(std::max<int64_t>(n, 1));
+  HIP_TRY(hipMalloc(dst, bytes));
+  if (n > 0) HIP_TRY(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
+  return BV_OK;
+}
+
+int validate(const bv_region_batch* b) {
+  if (!b) return fail(BV_EINVAL, "batch is NULL");
+  if (b->n_regions < 0 || b->n_reads < 0 || b->n_haps < 0) return fail(BV_EINVAL, "negative count");
+  if (b->n_regions > 0 && (!b->region_read_off || !b->region_hap_off))
+    return fail(BV_EINVAL, "region offsets are NULL");
+  if (b->n_reads > 0 && (!b->read_off || !b->read_bases || !b->read_qual || !b->read_ins ||
+                         !b->read_del || !b->read_gcp))
+    return fail(BV_EINVAL, "read arrays are NULL");
+  if (b->n_haps > 0 && (!b->hap_off || !b->hap_bases)) return fail(BV_EINVAL, "hap arrays are NULL");
+  if (b->n_reads >= (1ll << 31) || b->n_haps >= (1ll << 31))
+    return fail(BV_EINVAL, "more than 2^31 reads or haplotypes");
+  for (int64_t g = 0; g < b->n_regions; ++g) {
+    if (b->region_read_off[g] > b->region_read_off[g + 1] ||
+        b->region_hap_off[g] > b->region_hap_off[g + 1])
+      return fail(BV_EINVAL, "region offsets decrease at region " + std::to_string(g));
+    const bool has_reads = b->region_read_off[g + 1] > b->region_read_off[g];
+    const bool has_haps = b->region_hap_off[g + 1] > b->region_hap_off[g];
+    if (has_reads && !has_haps)
+      return fail(BV_EINVAL, "region " + std::to_string(g) + " has reads but no haplotypes");
+  }
+  if (b->n_regions > 0 &&
+      (b->region_read_off[0] < 0 || b->region_read_off[b->n_regions] > b->n_reads ||
+       b->region_hap_off[0] < 0 || b->region_hap_off[b->n_regions] > b->n_haps))
+    return fail(BV_EINVAL, "region offsets out of range");
+  for (int64_t r = 0; r < b->n_reads; ++r) {
+    const int64_t len = b->read_off[r + 1] - b->read_off[r];
+    if (len <= 0) return fail(BV_EINVAL, "read " + std::to_string(r) + " is empty");
+    if (len > (1 << 24)) return fail(BV_EINVAL, "read " + std::to_string(r) + " is longer than 2^24");
+  }
+  for (int64_t h = 0; h < b->n_haps; ++h) {
+    const int64_t len = b->hap_off[h + 1] - b->hap_off[h];
+    if (len <= 0) return fail(BV_EINVAL, "haplotype " + std::to_string(h) + " is empty");
+    if (len > (1 << 24)) return fail(BV_EINVAL, "haplotype " + std::to_string(h) + " is longer than 2^24");
+  }
+  return BV_OK;
+}
+
+KArgs base_args(const bv_plan* p) {
+  KArgs a{};
+  a.rclass = p->d_rclass;
+  a.q = p->d_q;
+  a.ins = p->d_ins;
+  a.del = p->d_del;
+  a.gcp = p->d_gcp;
+  a.read_off = p->d_read_off;
+  a.hmask = p->d_hmask;
+  a.hap_off = p->d_hap_off;
+  a.out = p->d_out;
+  a.form = p->d_form;
+  a.tab.ph2pr_f = g_ctx.d_ph2pr_f;
+  a.tab.ph2pr_d = g_ctx.d_ph2pr_d;
+  a.tab.mm_f = g_ctx.d_mm_f;
+  a.tab.mm_d = g_ctx.d_mm_d;
+  a.tab.err_scalar = g_ctx.d_err_scalar;
+  for (int k = 0; k < 7; ++k) a.tab.trans[k] = p->opt.trans[k];
+  a.mode = p->opt.mode;
+  a.renorm = p->opt.fp64_renorm;
+  a.rescue_cells = p->d_rescue_cells;
+  return a;
+}
+
+int enqueue_run(bv_plan* p, bool timed) {
+  HIP_TRY(launch_zero_ints(p->d_counters, 6, p->stream));  // counters + 64-bit cell sum
+  if (p->n_tasks == 0) return BV_OK;
+  const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
+  if (timed) HIP_TRY(hipEventRecord(p->ev[0], p->stream));
+  if (fp32) {
+    KArgs a = base_args(p);
+    a.tasks = p->d_tasks;
+    a.n_tasks = p->n_tasks;
+    a.counter = p->d_counters + 0;
+    a.rescue = p->d_rescue;
+    a.rescue_count = p->d_counters + 2;
+    a.carry = p->d_carry32;
+    a.carry_stride = p->stride32;
+    HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+  }
+  if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
+  {
+    KArgs a = base_args(p);
+    if (fp32) {
+      a.tasks = p->d_rescue;
+      a.n_tasks = 0;
+      a.n_tasks_dev = p->d_counters + 2;
+    } else {
+      a.tasks = p->d_tasks;
+      a.n_tasks = p->n_tasks;
+    }
+    a.counter = p->d_counters + 1;
+    a.carry = p->d_carry64;
+    a.carry_stride = p->stride64;
+    HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
+  }
+  if (timed) HIP_TRY(hipEventRecord(p->ev[2], p->stream));
+  return BV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bv_pairhmm_last_error(void) { return g_err.c_str(); }
+
+int bv_pairhmm_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return ensure_init_locked(device);
+}
+
+void bv_pairhmm_shutdown(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx.ready) return;
+  (void)hipFree(g_ctx.d_ph2pr_f);
+  (void)hipFree(g_ctx.d_ph2pr_d);
+  (void)hipFree(g_ctx.d_mm_f);
+  (void)hipFree(g_ctx.d_mm_d);
+  (void)hipFree(g_ctx.d_err_scalar);
+  g_ctx = DeviceCtx{};
+}
+
+int bv_pairhmm_device_info(bv_device_info* info) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = ensure_init_locked(-1);
+  if (rc) return rc;
+  if (!info) return fail(BV_EINVAL, "info is NULL");
+  info->device = g_ctx.device;
+  info->compute_units = g_ctx.cus;
+  info->blocks_per_cu_f32 = g_ctx.bpc_f32;
+  info->blocks_per_cu_f64 = g_ctx.bpc_f64;
+  std::snprintf(info->arch, sizeof(info->arch), "%s", g_ctx.arch.c_str());
+  return BV_OK;
+}
+
+// Host copies of the tables, for cross-checking against the reference's.
+int bv_pairhmm_tables(float* ph2pr_f, double* ph2pr_d, float* mm_f, double* mm_d) {
+  static HostTables t;
+  static std::once_flag once;
+  std::call_once(once, [] { build_tables(t); });
+  if (ph2pr_f) std::memcpy(ph2pr_f, t.ph2pr_f, sizeof(t.ph2pr_f));
+  if (ph2pr_d) std::memcpy(ph2pr_d, t.ph2pr_d, sizeof(t.ph2pr_d));
+  if (mm_f) std::memcpy(mm_f, t.mm_f, sizeof(t.mm_f));
+  if (mm_d) std::memcpy(mm_d, t.mm_d, sizeof(t.mm_d));
+  return BV_OK;
+}
+
+void bv_options_default(bv_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->mode = BV_MODE_AVX;
+  opt->fp64_renorm = 1;
+}
+
+int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** out) {
+  if (!out) return fail(BV_EINVAL, "plan output pointer is NULL");
+  *out = nullptr;
+  bv_options opt;
+  if (optp) opt = *optp; else bv_options_default(&opt);
+  if (opt.mode != BV_MODE_AVX && opt.mode != BV_MODE_SCALAR) return fail(BV_EINVAL, "unknown mode");
+  if (opt.mode == BV_MODE_SCALAR) {
+    bool all_zero = true;
+    for (double v : opt.trans) all_zero = all_zero && v == 0.0;
+    if (all_zero) {  // PairHMM::ORIGINAL_DEFAULT (pairhmm.hpp:32-33)
+      const double d[7] = {0.9998, 0.0001, 0.0001, 0.9, 0.1, 0.9, 0.1};
+      std::memcpy(opt.trans, d, sizeof(d));
+    }
+  }
+  int rc = validate(b);
+  if (rc) return rc;
+  Alphabet al;
+  if (opt.mode == BV_MODE_AVX) {
+    alphabet_avx(al);
+  } else {
+    const int64_t nb = b->n_reads ? b->read_off[b->n_reads] - b->read_off[0] : 0;
+    rc = alphabet_scalar(al, b->read_bases + (b->n_reads ? b->read_off[0] : 0), nb);
+    if (rc) return rc;
+    for (int64_t i = 0; i < nb; ++i)
+      if (b->read_qual[b->read_off[0] + i] >= 128)
+        return fail(BV_EINVAL, "scalar mode: quality byte >= 128 (negative char index in the reference)");
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc = ensure_init_locked(-1);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipSetDevice(g_ctx.device));
+
+  auto* p = new bv_plan();
+  p->opt = opt;
+  p->n_reads = b->n_reads;
+  p->n_haps = b->n_haps;
+  auto cleanup = [&](int code) {
+    free_plan(p);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(BV_EHIP, "hipStreamCreate failed"));
+  for (auto& e : p->ev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BV_EHIP, "hipEventCreate failed"));
+
+  // Rebase read and hap byte ranges to start at zero.
+  const int64_t rb0 = b->n_reads ? b->read_off[0] : 0;
+  const int64_t rbytes = b->n_reads ? b->read_off[b->n_reads] - rb0 : 0;
+  const int64_t hb0 = b->n_haps ? b->hap_off[0] : 0;
+  const int64_t hbytes = b->n_haps ? b->hap_off[b->n_haps] - hb0 : 0;
+  p->read_bytes = rbytes;
+  p->hap_bytes = hbytes;
+  std::vector<int64_t> roff(b->n_reads + 1), hoff(b->n_haps + 1);
+  for (int64_t r = 0; r <= b->n_reads; ++r) roff[r] = b->n_reads ? b->read_off[r] - rb0 : 0;
+  for (int64_t h = 0; h <= b->n_haps; ++h) hoff[h] = b->n_haps ? b->hap_off[h] - hb0 : 0;
+  std::vector<uint8_t> rclass(static_cast<size_t>(rbytes)), hmask(static_cast<size_t>(hbytes));
+  parallel_for(rbytes, [&](int64_t a0, int64_t a1) {
+    for (int64_t i = a0; i < a1; ++i) rclass[i] = al.read_class[b->read_bases[rb0 + i]];
+  });
+  parallel_for(hbytes, [&](int64_t a0, int64_t a1) {
+    for (int64_t i = a0; i < a1; ++i) hmask[i] = al.hap_mask[b->hap_bases[hb0 + i]];
+  });
+  int hmax = 0;
+  for (int64_t h = 0; h < b->n_haps; ++h) hmax = std::max<int>(hmax, static_cast<int>(hoff[h + 1] - hoff[h]));
+  p->hmax = hmax;
+
+  // Tasks: one per read, over its region's haplotypes, longest first.
+  std::vector<Task> tasks;
+  tasks.reserve(static_cast<size_t>(b->n_reads));
+  std::vector<int64_t> cost;
+  cost.reserve(static_cast<size_t>(b->n_reads));
+  int64_t pair_base = 0, cells = 0;
+  for (int64_t g = 0; g < b->n_regions; ++g) {
+    const int64_t r0 = b->region_read_off[g], r1 = b->region_read_off[g + 1];
+    const int64_t h0 = b->region_hap_off[g], h1 = b->region_hap_off[g + 1];
+    const int64_t nh = h1 - h0;
+    int64_t hsum = 0;
+    for (int64_t h = h0; h < h1; ++h) hsum += hoff[h + 1] - hoff[h];
+    for (int64_t r = r0; r < r1; ++r) {
+      if (nh == 0) continue;
+      const int64_t R = roff[r + 1] - roff[r];
+      tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
+                           static_cast<int32_t>(nh), -1, pair_base + (r - r0) * nh});
+      cost.push_back(R * hsum);
+      cells += R * hsum;
+    }
+    pair_base += (r1 - r0) * nh;
+  }
+  p->n_pairs = pair_base;
+  p->cells = cells;
+  if (tasks.size() >= (1u << 31)) return cleanup(fail(BV_EINVAL, "too many tasks"));
+  {
+    std::vector<int64_t> idx(tasks.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
+    std::vector<Task> sorted(tasks.size());
+    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = tasks[idx[i]];
+    tasks.swap(sorted);
+  }
+  p->n_tasks = static_cast<int>(tasks.size());
+
+  const int bpc32 = g_ctx.bpc_f32, bpc64 = g_ctx.bpc_f64;
+  p->grid32 = opt.grid_f32 > 0 ? opt.grid_f32 : g_ctx.cus * bpc32;
+  p->grid64 = opt.grid_f64 > 0 ? opt.grid_f64 : g_ctx.cus * bpc64;
+  p->grid32 = std::max(1, std::min<int>(p->grid32, std::max(1, p->n_tasks)));
+  const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
+  p->grid64 = std::max(1, static_cast<int>(std::min<int64_t>(p->grid64, std::max<int64_t>(1, max_fp64_tasks))));
+  p->stride32 = hmax + kCarryPad;
+  p->stride64 = hmax + kCarryPad;
+
+  hipStream_t st = p->stream;
+  if ((rc = dev_upload(&p->d_rclass, rclass.data(), rbytes, st)) ||
+      (rc = dev_upload(&p->d_q, b->read_qual + rb0, rbytes, st)) ||
+      (rc = dev_upload(&p->d_ins, b->read_ins + rb0, rbytes, st)) ||
+      (rc = dev_upload(&p->d_del, b->read_del + rb0, rbytes, st)) ||
+      (rc = dev_upload(&p->d_gcp, b->read_gcp + rb0, rbytes, st)) ||
+      (rc = dev_upload(&p->d_read_off, roff.data(), b->n_reads + 1, st)) ||
+      (rc = dev_upload(&p->d_hmask, hmask.data(), hbytes, st)) ||
+      (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st)) ||
+      (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)))
+    return cleanup(rc);
+  if (hipMalloc(&p->d_rescue, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
+      hipMalloc(&p->d_counters, sizeof(int) * 8) != hipSuccess ||
+      hipMalloc(&p->d_out, sizeof(double) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
+      hipMalloc(&p->d_form, std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
+      hipMalloc(&p->d_carry32, sizeof(float) * 2 * p->stride32 * p->grid32) != hipSuccess ||
+      hipMalloc(&p->d_carry64, sizeof(double) * 2 * p->stride64 * p->grid64) != hipSuccess)
+    return cleanup(fail(BV_ENOMEM, "device allocation failed"));
+  // The 64-bit rescue cell counter shares the zeroed counter block.
+  p->d_rescue_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 4);
+  if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(BV_EHIP, "upload failed"));
+  *out = p;
+  return BV_OK;
+}
+
+int bv_plan_run(bv_plan* p) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  p->timed = false;
+  return enqueue_run(p, false);
+}
+
+int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
+  if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  float sum32 = 0.f, sum64 = 0.f;
+  HIP_TRY(hipEventRecord(p->ev[3], p->stream));
+  hipEvent_t start = p->ev[3];
+  // Per-run split between the FP32 and FP64 passes from events between them.
+  for (int i = 0; i < iters; ++i) {
+    int rc = enqueue_run(p, true);
+    if (rc) return rc;
+    HIP_TRY(hipEventSynchronize(p->ev[2]));
+    float a = 0.f, c = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&c, p->ev[1], p->ev[2]));
+    sum32 += a;
+    sum64 += c;
+  }
+  float total = 0.f;
+  HIP_TRY(hipEventElapsedTime(&total, start, p->ev[2]));
+  p->last_ms32 = sum32 / iters;
+  p->last_ms64 = sum64 / iters;
+  if (ms_per_run) *ms_per_run = total / iters;
+  return BV_OK;
+}
+
+int bv_plan_results(bv_plan* p, double* log10_out, uint8_t* form_out) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (log10_out && p->n_pairs)
+    HIP_TRY(hipMemcpyAsync(log10_out, p->d_out, sizeof(double) * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
+  if (form_out && p->n_pairs)
+    HIP_TRY(hipMemcpyAsync(form_out, p->d_form, p->n_pairs, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return BV_OK;
+}
+
+int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
+  if (!p || !s) return fail(BV_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  int cnt[8] = {0};
+  HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  unsigned long long rcells = 0;
+  std::memcpy(&rcells, cnt + 4, sizeof(rcells));
+  const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
+  s->n_pairs = p->n_pairs;
+  s->cells = p->cells;
+  s->fp64_pairs = fp32 ? cnt[2] : p->n_pairs;
+  s->fp64_cells = fp32 ? static_cast<int64_t>(rcells) : p->cells;
+  s->tasks = p->n_tasks;
+  s->grid_f32 = p->grid32;
+  s->grid_f64 = p->grid64;
+  s->last_ms_f32 = p->last_ms32;
+  s->last_ms_f64 = p->last_ms64;
+  return BV_OK;
+}
+
+void bv_plan_destroy(bv_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(g_ctx.device);
+  (void)hipStreamSynchronize(p->stream);
+  free_plan(p);
+}
+
+int bv_pairhmm_batch(const bv_region_batch* b, const bv_options* opt, double* log10_out,
+                     uint8_t* form_out) {
+  bv_plan* p = nullptr;
+  int rc = bv_plan_create(b, opt, &p);
+  if (rc) return rc;
+  rc = bv_plan_run(p);
+  if (!rc) rc = bv_plan_results(p, log10_out, form_out);
+  bv_plan_destroy(p);
+  return rc;
+}
+
+int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* out_scaled) {
+  if (n < 0 || (n > 0 && (!tcs || !out_scaled))) return fail(BV_EINVAL, "bad arguments");
+  if (precision != 32 && precision != 64) return fail(BV_EINVAL, "precision must be 32 or 64");
+  if (n == 0) return BV_OK;
+  // Each testcase becomes a one-read, one-haplotype region.
+  std::vector<int64_t> rro(n + 1), rho(n + 1), ro(n + 1), ho(n + 1);
+  std::vector<uint8_t> rs, q, ins, del, gcp, hap;
+  for (int k = 0; k < n; ++k) {
+    const bv_testcase& t = tcs[k];
+    if (t.rslen <= 0 || t.haplen <= 0) return fail(BV_EINVAL, "testcase " + std::to_string(k) + " has an empty read or haplotype");
+    rro[k] = k;
+    rho[k] = k;
+    ro[k] = static_cast<int64_t>(rs.size());
+    ho[k] = static_cast<int64_t>(hap.size());
+    rs.insert(rs.end(), t.rs, t.rs + t.rslen);
+    q.insert(q.end(), t.q, t.q + t.rslen);
+    ins.insert(ins.end(), t.i, t.i + t.rslen);
+    del.insert(del.end(), t.d, t.d + t.rslen);
+    gcp.insert(gcp.end(), t.c, t.c + t.rslen);
+    hap.insert(hap.end(), t.hap, t.hap + t.haplen);
+  }
+  rro[n] = n;
+  rho[n] = n;
+  ro[n] = static_cast<int64_t>(rs.size());
+  ho[n] = static_cast<int64_t>(hap.size());
+  bv_region_batch b{n, rro.data(), rho.data(), n, ro.data(), rs.data(), q.data(), ins.data(),
+                    del.data(), gcp.data(), n, ho.data(), hap.data()};
+  bv_options opt;
+  bv_options_default(&opt);
+  opt.fp64_renorm = 0;
+  opt.force_fp64 = precision == 64;
+  bv_plan* p = nullptr;
+  int rc = bv_plan_create(&b, &opt, &p);
+  if (rc) return rc;
+  auto run = [&]() -> int {
+    HIP_TRY(launch_zero_ints(p->d_counters, 6, p->stream));
+    KArgs a = base_args(p);
+    a.tasks = p->d_tasks;
+    a.n_tasks = p->n_tasks;
+    a.counter = p->d_counters;
+    a.out_raw = 1;
+    if (precision == 32) {
+      a.carry = p->d_carry32;
+      a.carry_stride = p->stride32;
+      HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+    } else {
+      a.carry = p->d_carry64;
+      a.carry_stride = p->stride64;
+      HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
+    }
+    return BV_OK;
+  };
+  rc = run();
+  if (!rc) rc = bv_plan_results(p, out_scaled, nullptr);
+  bv_plan_destroy(p);
+  return rc;
+}
+
+}  // extern "C"
+
+// ---- batch accelerator hook (lib/intel_native/pairhmm/shacc_pairhmm.h:10-36) --
+// The reference declares shacc_pairhmm::calculate as a weak C++ symbol and
+// never defines it; this is the strong definition.  results receives
+// log10 likelihoods (FP32 with FP64 rescue), read-major [num_reads][num_haps],
+// as float.  Returns false on any error (message in bv_pairhmm_last_error).
+namespace shacc_pairhmm {
+struct Read {
+  int length;
+  const char* bases;
+  const char* q;
+  const char* i;
+  const char* d;
+  const char* c;
+};
+struct Haplotype {
+  int length;
+  const char* bases;
+};
+struct Batch {
+  int num_reads;
+  int num_haps;
+  long num_cells;
+  Read* reads;
+  Haplotype* haps;
+  float* results;
+};
+
+bool calculate(Batch& batch) {
+  const int nr = batch.num_reads, nh = batch.num_haps;
+  if (nr < 0 || nh < 0 || (nr > 0 && nh > 0 && (!batch.reads || !batch.haps || !batch.results))) {
+    g_err = "shacc_pairhmm::calculate: bad batch";
+    return false;
+  }
+  if (nr == 0 || nh == 0) return true;
+  std::vector<int64_t> rro{0, nr}, rho{0, nh}, ro(nr + 1), ho(nh + 1);
+  std::vector<uint8_t> rs, q, ins, del, gcp, hap;
+  for (int r = 0; r < nr; ++r) {
+    const Read& x = batch.reads[r];
+    ro[r] = static_cast<int64_t>(rs.size());
+    if (x.length <= 0) {
+      g_err = "shacc_pairhmm::calculate: empty read";
+      return false;
+    }
+    rs.insert(rs.end(), x.bases, x.bases + x.length);
+    q.insert(q.end(), x.q, x.q + x.length);
+    ins.insert(ins.end(), x.i, x.i + x.length);
+    del.insert(del.end(), x.d, x.d + x.length);
+    gcp.insert(gcp.end(), x.c, x.c + x.length);
+  }
+  ro[nr] = static_cast<int64_t>(rs.size());
+  for (int h = 0; h < nh; ++h) {
+    ho[h] = static_cast<int64_t>(hap.size());
+    const Haplotype& y = batch.haps[h];
+    if (y.length <= 0) {
+      g_err = "shacc_pairhmm::calculate: empty haplotype";
+      return false;
+    }
+    hap.insert(hap.end(), y.bases, y.bases + y.length);
+  }
+  ho[nh] = static_cast<int64_t>(hap.size());
+  bv_region_batch b{1, rro.data(), rho.data(), nr, ro.data(), rs.data(), q.data(), ins.data(),
+                    del.data(), gcp.data(), nh, ho.data(), hap.data()};
+  std::vector<double> out(static_cast<size_t>(nr) * nh);
+  if (bv_pairhmm_batch(&b, nullptr, out.data(), nullptr) != BV_OK) return false;
+  for (size_t k = 0; k < out.size(); ++k) batch.results[k] = static_cast<float>(out[k]);
+  return true;
+}
+}  // namespace shacc_pairhmm

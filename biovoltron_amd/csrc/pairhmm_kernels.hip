@@ -33,6 +33,21 @@
 
 namespace bvp {
 
+#ifdef BVP_DEBUG_DUMP
+__device__ float bvp_debug[16][8][6];  // [step][lane][M, zn, jn, prior, D, sel]
+#define BVP_DUMP(U)                                                            \
+  if (blockIdx.x == 0 && st == 0 && b == 0 && (U) < 16 && lane < 8) {         \
+    bvp_debug[(U)][lane][0] = static_cast<float>(M);                           \
+    bvp_debug[(U)][lane][1] = static_cast<float>(zn);                          \
+    bvp_debug[(U)][lane][2] = static_cast<float>(jn);                          \
+    bvp_debug[(U)][lane][3] = static_cast<float>(prior);                       \
+    bvp_debug[(U)][lane][4] = static_cast<float>(D);                           \
+    bvp_debug[(U)][lane][5] = static_cast<float>(sel);                         \
+  }
+#else
+#define BVP_DUMP(U)
+#endif
+
 template <typename T>
 struct alignas(2 * sizeof(T)) Pair2 {
   T z, j;
@@ -75,7 +90,7 @@ struct Num;
 template <>
 struct Num<float> {
   static constexpr float K = 0x1p120f;  // Context.h:109 (FP32)
-  static constexpr double log10K = 36.123599479677743;  // 120*log10(2)
+  static constexpr double log10K = 120.0 * 0.30102999566398119521;  // log10(2^120)
   __device__ static float ph2pr(const Tables& t, int q) { return t.ph2pr_f[q]; }
   __device__ static float mm(const Tables& t, int i) { return t.mm_f[i]; }
   __device__ static float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -83,7 +98,7 @@ struct Num<float> {
 template <>
 struct Num<double> {
   static constexpr double K = 0x1p1020;  // Context.h:149 (FP64)
-  static constexpr double log10K = 307.05178458212172;  // 1020*log10(2)
+  static constexpr double log10K = 1020.0 * 0.30102999566398119521;  // log10(2^1020)
   __device__ static double ph2pr(const Tables& t, int q) { return t.ph2pr_d[q]; }
   __device__ static double mm(const Tables& t, int i) { return t.mm_d[i]; }
   __device__ static double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -161,6 +176,7 @@ __device__ __forceinline__ void row_params(const KArgs& a, int64_t ro, int R, in
       jn = c.j;                                                                \
     }                                                                          \
     if (lane == L) outp[(U)] = Pair2<T>{zn, jn};                               \
+    BVP_DUMP(U)                                                                \
     ZS = zn;                                                                   \
     J1 = jn;                                                                   \
     M1 = M;                                                                    \
@@ -271,7 +287,9 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
       // Make this stripe's carry row visible to the next stripe's loads.
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      if (a.renorm) {
+      // Only the FP64 form renormalises; the FP32 form keeps the reference's
+      // fixed 2^120 scaling so its underflow decisions match.
+      if (sizeof(T) == 8 && a.renorm) {
         const double m = wave_max(static_cast<double>(maxabs));
         if (m > 0.0) {
           const int e = ilogb(m);
@@ -362,6 +380,12 @@ int max_blocks_per_cu() {
   return n > 0 ? n : 1;
 }
 template int max_blocks_per_cu<float>();
+
+#ifdef BVP_DEBUG_DUMP
+extern "C" int bv_debug_read(float* out) {
+  return hipMemcpyFromSymbol(out, HIP_SYMBOL(bvp_debug), sizeof(bvp_debug)) == hipSuccess ? 0 : -1;
+}
+#endif
 template int max_blocks_per_cu<double>();
 
 }  // namespace bvp

@@ -67,6 +67,13 @@ typedef struct bv_device_info {
 } bv_device_info;
 int bv_pairhmm_device_info(bv_device_info* info);
 
+/* Host copies of the lookup tables the kernels use (ph2pr[128] per
+ * precision; match-to-match for insertion/deletion quals < 128, 8256 entries
+ * indexed max*(max+1)/2 + min), for checking against the reference's
+ * Context<T> tables (lib/intel_native/pairhmm/Context.h:42-61,105-109,145-149).
+ * Any pointer may be NULL. */
+int bv_pairhmm_tables(float* ph2pr_f, double* ph2pr_d, float* mm_f, double* mm_d);
+
 /* The reference's per-pair descriptor (pairhmm_common.h:20-24). */
 typedef struct bv_testcase {
   int rslen, haplen;

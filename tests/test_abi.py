@@ -1,0 +1,75 @@
+"""C ABI checks that need no GPU: exports, tables, argument validation."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from biovoltron_amd import pairhmm
+
+
+def declared_symbols():
+    txt = (ROOT / "include" / "bv_pairhmm.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(bv_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pairhmm.library()
+    names = declared_symbols()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(pairhmm.EXPORTS) <= set(names)
+
+
+def test_shacc_hook_is_defined():
+    # shacc_pairhmm::calculate(shacc_pairhmm::Batch&), C++ linkage
+    lib = pairhmm.library()
+    assert hasattr(lib, "_ZN13shacc_pairhmm9calculateERNS_5BatchE")
+
+
+def test_product_tables_equal_reference_tables():
+    pf, pd, mf, md = pairhmm.tables()
+    t = np.load(GOLDEN / "tables.npz")
+    assert np.array_equal(pf.view(np.uint32), t["ph2pr_f32"].view(np.uint32))
+    assert np.array_equal(pd.view(np.uint64), t["ph2pr_f64"].view(np.uint64))
+    assert np.array_equal(mf.view(np.uint32), t["mm_f32"].view(np.uint32))
+    assert np.array_equal(md.view(np.uint64), t["mm_f64"].view(np.uint64))
+
+
+def _batch(reads, haps):
+    return pairhmm.PackedBatch.from_regions([(reads, haps)])
+
+
+def _read(seq, q=30):
+    n = len(seq)
+    return dict(seq=seq, q=bytes([q]) * n, i=b"\x2d" * n, d=b"\x2d" * n, c=b"\x0a" * n)
+
+
+@pytest.mark.parametrize("reads,haps,what", [
+    ([_read(b"")], [b"ACGT"], "empty"),
+    ([_read(b"ACGT")], [b""], "empty"),
+    ([_read(b"ACGT")], [], "no haplotypes"),
+])
+def test_invalid_batches_are_errors(reads, haps, what):
+    with pytest.raises(pairhmm.PairHMMError) as e:
+        pairhmm.Plan(_batch(reads, haps))
+    assert "-22" in str(e.value) and what in str(e.value)
+
+
+def test_scalar_mode_rejects_high_quality_bytes():
+    rd = _read(b"ACGT", q=200)
+    with pytest.raises(pairhmm.PairHMMError, match="quality byte"):
+        pairhmm.Plan(_batch([rd], [b"ACGT"]), pairhmm.default_options(mode=pairhmm.MODE_SCALAR))
+
+
+def test_options_defaults():
+    o = pairhmm.default_options()
+    assert o.mode == pairhmm.MODE_AVX and o.fp64_renorm == 1 and o.force_fp64 == 0
+
+
+def test_testcase_abi_rejects_bad_arguments():
+    lib = pairhmm.library()
+    assert lib.bv_compute_full_prob(None, 1, 32, None) == -22
+    assert lib.bv_compute_full_prob(None, 0, 16, None) == -22
