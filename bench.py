@@ -28,6 +28,27 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) datasheet peak
 PEAK_FP32_NOPACK_TFLOPS = 78.6
 
 
+def profiled_traffic(workload: int):
+    """HBM bytes per launch of the FP32 kernel from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by a 1 GiB copy),
+    or None when no summary exists for this workload."""
+    import re
+    path = ROOT / "profiles" / f"pairhmm_forward_workload{workload}.md"
+    if not path.exists():
+        return None
+    in_table = False
+    for line in path.read_text().splitlines():
+        if line.startswith("| kernel | FETCH_SIZE"):
+            in_table = True
+            continue
+        if in_table and line.startswith("| `void bvp::pairhmm_forward_kernel<float>"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            return int((float(cells[3]) + float(cells[4])) * 1e9), path.name
+        if in_table and not line.startswith("|"):
+            in_table = False
+    return None
+
+
 def cpu_baseline(batch, out_gpu, budget_s: float):
     """The reference kernels (oracle/_ref, built from the reference sources)
     run as the reference driver runs them -- OpenMP schedule(dynamic,1) over
@@ -90,6 +111,9 @@ def main():
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="also run a plain device-to-device copy of this size, so byte "
+                         "counters can be scaled against a kernel of known traffic")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -157,11 +181,29 @@ def main():
             "traffic": None,
         },
     }
+    tr = profiled_traffic(args.workload)
+    if tr is not None:
+        line["roofline"]["traffic"] = tr[0]
+        line["roofline"]["traffic_source"] = f"profiles/{tr[1]} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
     if not args.no_cpu:
         cb, worst = cpu_baseline(b, out, args.cpu_seconds)
         line["cpu_baseline"] = cb
         line["config"]["max_abs_dlog10_vs_cpu_sample"] = worst
     plan.close()
+    if args.copy_calibration > 0:
+        # A device-to-device copy of known size through the HIP runtime's own
+        # blit kernel; its FETCH_SIZE / WRITE_SIZE scale the byte counters.
+        hip = C.CDLL("libamdhip64.so")
+        n = args.copy_calibration * (1 << 20)
+        src, dst = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0
+        assert hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+        assert hip.hipMemset(src, 1, C.c_size_t(n)) == 0
+        assert hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        line["config"]["copy_calibration_bytes"] = n
     print(json.dumps(line), flush=True)
 
 

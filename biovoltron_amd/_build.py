@@ -63,6 +63,19 @@ def build_hip() -> Path:
     return HIP_LIB
 
 
+CPP_TEST = ROOT / "tests" / "cpp" / "test_adapter"
+
+
+def build_cpp_tests() -> Path:
+    """The C++ adapter test program (links libbvpairhmm.so)."""
+    tmp = CPP_TEST.with_suffix(".tmp")
+    _run(["g++", "-std=c++20", "-O2", f"-I{INCLUDE}", "-o", str(tmp),
+          str(CPP_TEST.with_suffix(".cpp")), f"-L{PKG}", "-lbvpairhmm",
+          f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../biovoltron_amd"])
+    os.replace(tmp, CPP_TEST)
+    return CPP_TEST
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """The CPU checker (test infrastructure); the reference build only where
     the reference tree exists."""
@@ -74,6 +87,7 @@ def build_oracle(with_reference: bool = True) -> None:
 def build_all() -> None:
     build_synth()
     build_hip()
+    build_cpp_tests()
     build_oracle()
 
 

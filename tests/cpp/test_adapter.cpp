@@ -1,0 +1,101 @@
+// Exercises include/biovoltron_amd/gpu_pairhmm.hpp the way HaplotypeCaller
+// uses PairHMM: a const member called per region with SamRecord-like reads.
+//   test_adapter cpu   normalisation/filtering known answers, error paths
+//   test_adapter gpu   prints the likelihood matrix of a small region
+#include <biovoltron_amd/gpu_pairhmm.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+struct Read {  // the SamRecord members PairHMM reads (sam.hpp:186-259, :451-475)
+  std::string seq, qual, gop, gcp;
+  std::string_view insertion_gop() const { return gop; }
+  std::string_view deletion_gop() const { return gop; }
+  std::string_view overall_gcp() const { return gcp; }
+};
+struct Hap {
+  std::string seq;
+};
+
+static int fails = 0;
+#define EXPECT(c)                                                  \
+  do {                                                             \
+    if (!(c)) {                                                    \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c);      \
+      ++fails;                                                     \
+    }                                                              \
+  } while (0)
+
+static Read make_read(const std::string& s, char q = 30, char gop = 45, char gcp = 10) {
+  return Read{s, std::string(s.size(), q), std::string(s.size(), gop), std::string(s.size(), gcp)};
+}
+
+int cpu_tests() {
+  using namespace biovoltron_amd;
+  // normalize: cap at row best - 4.5
+  std::vector<std::vector<double>> ll{{-1.0, -10.0, -5.0}, {-20.0, -30.0, -22.0}};
+  normalize_likelihoods(ll);
+  EXPECT(ll[0][1] == -5.5 && ll[0][2] == -5.0 && ll[0][0] == -1.0);
+  EXPECT(ll[1][1] == -24.5 && ll[1][2] == -22.0);
+  // filter (as the reference is written): keep rows whose best is below
+  // min(2, ceil(0.02*len)) * -4
+  std::vector<Read> reads{make_read(std::string(150, 'A')), make_read(std::string(150, 'C')),
+                          make_read(std::string(20, 'G'))};
+  std::vector<std::vector<double>> l2{{-3.0, -9.0}, {-9.0, -12.0}, {-4.5, -6.0}};
+  filter_poorly_modeled_reads(reads, l2);
+  EXPECT(reads.size() == 2 && l2.size() == 2);
+  EXPECT(reads[0].seq[0] == 'C' && reads[1].seq[0] == 'G');  // 150bp: thr -8; 20bp: thr -4
+  // error paths: short GOP string, empty read -> exceptions, not UB
+  const GpuPairHMM hmm;
+  std::vector<Hap> haps{{"ACGTACGT"}};
+  std::vector<Read> bad{Read{"ACGT", "????", "--", "++++"}};
+  bool threw = false;
+  try {
+    hmm.compute_likelihoods(haps, bad);
+  } catch (const PairHMMError& e) {
+    threw = std::strstr(e.what(), "insertion_gop") != nullptr;
+  }
+  EXPECT(threw);
+  std::vector<Read> empty{make_read("")};
+  threw = false;
+  try {
+    hmm.compute_likelihoods(haps, empty);
+  } catch (const PairHMMError& e) {
+    threw = e.status == BV_EINVAL;
+  }
+  EXPECT(threw);
+  std::printf("cpu tests: %s\n", fails ? "FAILED" : "ok");
+  return fails ? 1 : 0;
+}
+
+int gpu_run() {
+  using namespace biovoltron_amd;
+  const std::string base = "AATCGAAGGTCGTAAGGACACGGTTGAGCGTTCAGCGTTCATGTGAGTCCTCACCACTTATGGCTCCATAGCC";
+  std::vector<Hap> haps{{base}, {base.substr(0, 30) + "T" + base.substr(31)}, {base.substr(5)}};
+  std::vector<Read> reads{make_read(base.substr(10, 40), 35), make_read(base.substr(0, 50), 25),
+                          make_read(base.substr(20, 30) + "GG", 20)};
+  const GpuPairHMM hmm;
+  const auto raw = raw_likelihoods(haps, reads, [] {
+    bv_options o;
+    bv_options_default(&o);
+    return o;
+  }());
+  for (const auto& row : raw) {
+    for (double v : row) std::printf("%.17g ", v);
+    std::printf("\n");
+  }
+  auto copy = reads;
+  const auto ll = hmm.compute_likelihoods(haps, copy);
+  std::printf("kept %zu\n", ll.size());
+  const auto sc = hmm.compute_likelihoods(haps, reads, kOriginalDefault);
+  std::printf("scalar kept %zu\n", sc.size());
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1 && std::strcmp(argv[1], "gpu") == 0) return gpu_run();
+  return cpu_tests();
+}
