@@ -1,0 +1,28 @@
+// Shared definitions for band prediction (features, MLP).
+#pragma once
+
+#include <cstdint>
+
+#include "pairhmm_common.h"
+
+namespace bvp {
+
+constexpr int kFeat = 36;   // features per pair
+constexpr int kBins = 32;   // seed-offset histogram bins
+constexpr int kHid = 64;    // hidden width
+constexpr int kParams = kHid * kFeat + kHid + kHid * kHid + kHid + 2 * kHid + 2;  // 6658
+constexpr int kBandMaxHap = 1024;  // longest haplotype the feature kernel indexes
+
+struct FeatArgs {
+  const uint8_t* rclass;
+  const uint8_t* q;
+  const int64_t* read_off;
+  const uint8_t* hmask;
+  const int64_t* hap_off;
+  const Task* tasks;
+  int n_tasks;
+  float* X;      // [pairs][kFeat]
+  int* anchor;   // [pairs]
+};
+
+}  // namespace bvp

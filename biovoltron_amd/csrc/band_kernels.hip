@@ -1,0 +1,430 @@
+// Band prediction on the device: per-pair seed features and the band MLP.
+//
+// Features (36 per pair): R/256, H/512, mean and min base quality / 40, and a
+// 32-bin histogram of 12-mer seed-hit diagonal offsets (o = q - p for a read
+// seed at p found at haplotype position q), 4 bp per bin, centred on the
+// median hit offset ("anchor"), each bin divided by the number of seeds.
+// The MLP (36 -> 64 -> 64 -> 2, ReLU, FP32) maps features to the band centre
+// relative to the anchor and the log2 band width; the band is
+// [anchor + dc - hw, anchor + dc + hw].
+//
+// Training kernels (forward, backward, Adam) are in this file too; the
+// trainer in biovoltron_amd/band.py drives them through the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+
+#include "band_common.h"
+#include "pairhmm_common.h"
+
+namespace bvp {
+
+constexpr int kK = 12;                    // seed length
+constexpr int kHashSlots = 2048;          // LDS hash slots (haplotypes up to kBandMaxHap)
+constexpr unsigned long long kEmpty = ~0ull;
+
+struct FeatLds {
+  unsigned long long table[kHashSlots];  // key (low 32 bits: 24-bit k-mer) | position << 32
+  uint8_t hcode[kBandMaxHap + 16];
+};
+
+__device__ __forceinline__ int code_of_mask(uint8_t m) {
+  // single-class masks 1,2,4,8 -> 0..3; anything else (N, 0) -> -1
+  switch (m) {
+    case 1: return 0;
+    case 2: return 1;
+    case 4: return 2;
+    case 8: return 3;
+    default: return -1;
+  }
+}
+
+__device__ __forceinline__ unsigned hash24(unsigned k) {
+  return (k * 2654435761u) >> (32 - 11);  // 11 bits -> 2048 slots
+}
+
+// One wavefront per task (a read and its region's haplotypes).
+__global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
+  __shared__ FeatLds s;
+  const int lane = threadIdx.x;
+  for (int ti = blockIdx.x; ti < a.n_tasks; ti += gridDim.x) {
+    const Task tk = a.tasks[ti];
+    const int64_t ro = a.read_off[tk.read];
+    const int R = static_cast<int>(a.read_off[tk.read + 1] - ro);
+    // read quality summary
+    double qsum = 0.0;
+    int qmin = 255;
+    for (int p = lane; p < R; p += 64) {
+      const int q = a.q[ro + p] & 127;
+      qsum += q;
+      qmin = min(qmin, q);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      qsum += __shfl_xor(qsum, o);
+      qmin = min(qmin, __shfl_xor(qmin, o));
+    }
+    const float fq_mean = static_cast<float>(qsum / R) / 40.f;
+    const float fq_min = static_cast<float>(qmin) / 40.f;
+    // this lane's read seed: positions spread over the read, <= 64 seeds
+    const int nseeds = R >= kK ? min(64, R - kK + 1) : 0;
+    const int stride = nseeds > 0 ? max(1, (R - kK + 1 + 63) / 64) : 1;
+    int my_p = -1;
+    unsigned my_key = 0;
+    if (lane < nseeds) {
+      my_p = lane * stride;
+      if (my_p + kK > R) my_p = -1;
+      if (my_p >= 0) {
+        for (int k = 0; k < kK; ++k) {
+          const uint8_t c = a.rclass[ro + my_p + k];
+          if (c & kWildcard) {
+            my_p = -1;
+            break;
+          }
+          my_key = (my_key << 2) | (c & 3);
+        }
+      }
+    }
+    int nvalid = __popcll(__ballot(my_p >= 0));
+    for (int h = 0; h < tk.nhap; ++h) {
+      const int hap = tk.hap0 + h;
+      const int64_t o = tk.out0 + h;
+      const int64_t ho = a.hap_off[hap];
+      const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+      float* X = a.X + o * kFeat;
+      // lengths and quality
+      if (lane == 0) {
+        X[0] = static_cast<float>(R) / 256.f;
+        X[1] = static_cast<float>(H) / 512.f;
+        X[2] = fq_mean;
+        X[3] = fq_min;
+      }
+      int anchor = (H - R) / 2;
+      int found_off = INT_MIN;
+      if (H <= kBandMaxHap && nvalid > 0) {
+        __syncthreads();
+        for (int i = lane; i < kHashSlots; i += 64) s.table[i] = kEmpty;
+        for (int q = lane; q < H; q += 64) s.hcode[q] = static_cast<uint8_t>(code_of_mask(a.hmask[ho + q]) & 0xff);
+        __syncthreads();
+        for (int q = lane; q + kK <= H; q += 64) {
+          unsigned key = 0;
+          bool ok = true;
+          for (int k = 0; k < kK; ++k) {
+            const int c = static_cast<int8_t>(s.hcode[q + k]);
+            if (c < 0) {
+              ok = false;
+              break;
+            }
+            key = (key << 2) | static_cast<unsigned>(c);
+          }
+          if (!ok) continue;
+          unsigned slot = hash24(key);
+          const unsigned long long entry = static_cast<unsigned long long>(key) |
+                                           (static_cast<unsigned long long>(q) << 32);
+          for (int probe = 0; probe < kHashSlots; ++probe) {
+            const unsigned long long prev = atomicCAS(&s.table[slot], kEmpty, entry);
+            if (prev == kEmpty || static_cast<unsigned>(prev) == key) break;
+            slot = (slot + 1) & (kHashSlots - 1);
+          }
+        }
+        __syncthreads();
+        if (my_p >= 0) {
+          unsigned slot = hash24(my_key);
+          for (int probe = 0; probe < kHashSlots; ++probe) {
+            const unsigned long long e = s.table[slot];
+            if (e == kEmpty) break;
+            if (static_cast<unsigned>(e) == my_key) {
+              found_off = static_cast<int>(e >> 32) - my_p;
+              break;
+            }
+            slot = (slot + 1) & (kHashSlots - 1);
+          }
+        }
+      }
+      // median of the hit offsets as the anchor
+      const bool hit = found_off != INT_MIN;
+      const int nhit = __popcll(__ballot(hit));
+      if (nhit > 0) {
+        int below = 0, equal = 0;
+        for (int l = 0; l < 64; ++l) {
+          const int v = __shfl(found_off, l);
+          const bool vh = __shfl(static_cast<int>(hit), l) != 0;
+          if (vh && hit) {
+            below += v < found_off;
+            equal += v == found_off;
+          }
+        }
+        const int half = (nhit - 1) / 2;
+        const bool is_median = hit && below <= half && half < below + equal;
+        const unsigned long long mb = __ballot(is_median);
+        anchor = __shfl(found_off, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
+      }
+      // 32 bins of 4 bp over [anchor - 64, anchor + 64)
+      const int bin = hit ? (found_off - anchor + 64) >> 2 : -1;
+      const float inv = nvalid > 0 ? 1.f / static_cast<float>(nvalid) : 0.f;
+      int cnt = 0;
+      for (int l = 0; l < 64; ++l) cnt += __shfl(bin, l) == lane;
+      if (lane < kBins) X[4 + lane] = static_cast<float>(cnt) * inv;
+      if (lane == 0) a.anchor[o] = anchor;
+    }
+  }
+}
+
+// ---- MLP --------------------------------------------------------------------
+
+struct MlpParams {
+  const float* W1;  // [kHid][kFeat]
+  const float* b1;  // [kHid]
+  const float* W2;  // [kHid][kHid]
+  const float* b2;
+  const float* W3;  // [2][kHid]
+  const float* b3;  // [2]
+};
+
+__device__ __forceinline__ MlpParams split_params(const float* p) {
+  MlpParams m;
+  m.W1 = p;
+  m.b1 = m.W1 + kHid * kFeat;
+  m.W2 = m.b1 + kHid;
+  m.b2 = m.W2 + kHid * kHid;
+  m.W3 = m.b2 + kHid;
+  m.b3 = m.W3 + 2 * kHid;
+  return m;
+}
+
+// Forward for one sample per thread; weights staged in LDS.  Optionally keeps
+// the hidden activations for the backward pass.
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restrict__ params,
+                                                          const float* __restrict__ X, int64_t n,
+                                                          float* __restrict__ Y,
+                                                          float* __restrict__ H1,
+                                                          float* __restrict__ H2) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = params[i];
+  __syncthreads();
+  const MlpParams m = split_params(w);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    float x[kFeat];
+#pragma unroll
+    for (int k = 0; k < kFeat; ++k) x[k] = X[i * kFeat + k];
+    float h1[kHid];
+#pragma unroll
+    for (int u = 0; u < kHid; ++u) {
+      float acc = m.b1[u];
+#pragma unroll
+      for (int k = 0; k < kFeat; ++k) acc = __builtin_fmaf(m.W1[u * kFeat + k], x[k], acc);
+      h1[u] = acc > 0.f ? acc : 0.f;
+    }
+    float h2[kHid];
+#pragma unroll
+    for (int u = 0; u < kHid; ++u) {
+      float acc = m.b2[u];
+#pragma unroll
+      for (int k = 0; k < kHid; ++k) acc = __builtin_fmaf(m.W2[u * kHid + k], h1[k], acc);
+      h2[u] = acc > 0.f ? acc : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      float acc = m.b3[u];
+#pragma unroll
+      for (int k = 0; k < kHid; ++k) acc = __builtin_fmaf(m.W3[u * kHid + k], h2[k], acc);
+      Y[i * 2 + u] = acc;
+    }
+    if (H1 != nullptr) {
+#pragma unroll
+      for (int u = 0; u < kHid; ++u) H1[i * kHid + u] = h1[u];
+#pragma unroll
+      for (int u = 0; u < kHid; ++u) H2[i * kHid + u] = h2[u];
+    }
+  }
+}
+
+// Backward step 1: per-sample deltas.  Loss = mean over samples of
+// 0.5 * sum_u wt_u * (Y_u - T_u)^2.  D3 = dL/dY, D2 = dL/d(pre-activation 2),
+// D1 = dL/d(pre-activation 1).
+__global__ __launch_bounds__(256) void mlp_delta_kernel(const float* __restrict__ params,
+                                                        const float* __restrict__ Y,
+                                                        const float* __restrict__ T, int64_t n,
+                                                        const float* __restrict__ H1,
+                                                        const float* __restrict__ H2,
+                                                        float* __restrict__ D1,
+                                                        float* __restrict__ D2,
+                                                        float* __restrict__ D3, float w0, float w1,
+                                                        float* __restrict__ loss) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = params[i];
+  __syncthreads();
+  const MlpParams m = split_params(w);
+  const float invn = 1.f / static_cast<float>(n);
+  float lsum = 0.f;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float e0 = Y[i * 2] - T[i * 2], e1 = Y[i * 2 + 1] - T[i * 2 + 1];
+    lsum += 0.5f * (w0 * e0 * e0 + w1 * e1 * e1);
+    const float d3[2] = {w0 * e0 * invn, w1 * e1 * invn};
+    D3[i * 2] = d3[0];
+    D3[i * 2 + 1] = d3[1];
+    float d2[kHid];
+#pragma unroll
+    for (int k = 0; k < kHid; ++k) {
+      const float g = m.W3[k] * d3[0] + m.W3[kHid + k] * d3[1];
+      d2[k] = H2[i * kHid + k] > 0.f ? g : 0.f;
+      D2[i * kHid + k] = d2[k];
+    }
+#pragma unroll 4
+    for (int k = 0; k < kHid; ++k) {
+      float g = 0.f;
+#pragma unroll
+      for (int u = 0; u < kHid; ++u) g = __builtin_fmaf(m.W2[u * kHid + k], d2[u], g);
+      D1[i * kHid + k] = H1[i * kHid + k] > 0.f ? g : 0.f;
+    }
+  }
+  if (loss != nullptr) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, lsum * invn);
+  }
+}
+
+// Backward step 2: one thread per parameter, summing over the batch.
+__global__ __launch_bounds__(256) void mlp_grad_kernel(const float* __restrict__ X, int64_t n,
+                                                       const float* __restrict__ H1,
+                                                       const float* __restrict__ H2,
+                                                       const float* __restrict__ D1,
+                                                       const float* __restrict__ D2,
+                                                       const float* __restrict__ D3,
+                                                       float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kParams) return;
+  // locate parameter p
+  const float* src;  // input activations
+  const float* dst;  // deltas
+  int in_dim, out_dim, u, k;
+  bool bias = false;
+  int q = p;
+  if (q < kHid * kFeat) {
+    in_dim = kFeat; out_dim = kHid; u = q / kFeat; k = q % kFeat; src = X; dst = D1;
+  } else if ((q -= kHid * kFeat) < kHid) {
+    in_dim = kFeat; out_dim = kHid; u = q; k = 0; src = X; dst = D1; bias = true;
+  } else if ((q -= kHid) < kHid * kHid) {
+    in_dim = kHid; out_dim = kHid; u = q / kHid; k = q % kHid; src = H1; dst = D2;
+  } else if ((q -= kHid * kHid) < kHid) {
+    in_dim = kHid; out_dim = kHid; u = q; k = 0; src = H1; dst = D2; bias = true;
+  } else if ((q -= kHid) < 2 * kHid) {
+    in_dim = kHid; out_dim = 2; u = q / kHid; k = q % kHid; src = H2; dst = D3;
+  } else {
+    q -= 2 * kHid;
+    in_dim = kHid; out_dim = 2; u = q; k = 0; src = H2; dst = D3; bias = true;
+  }
+  double g = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const float d = dst[i * out_dim + u];
+    g += bias ? d : d * src[i * in_dim + k];
+  }
+  grad[p] = static_cast<float>(g);
+}
+
+__global__ void adam_kernel(float* __restrict__ params, const float* __restrict__ grad,
+                            float* __restrict__ m1, float* __restrict__ m2, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kParams) return;
+  const float g = grad[p];
+  m1[p] = b1 * m1[p] + (1.f - b1) * g;
+  m2[p] = b2 * m2[p] + (1.f - b2) * g * g;
+  const float mh = m1[p] / c1, vh = m2[p] / c2;
+  params[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Bands from predictions: [anchor + dc - hw, anchor + dc + hw], where
+// dc = 16 * Y0, the width is 2^Y1 (clamped) and hw = width / 2 + margin.
+// A band is only used when the seeds support it: at least two seed hits, and
+// every hit within 64 bp of the anchor lies inside the band.  Otherwise, or
+// when the haplotype is longer than the feature table, or the band is wider
+// than max_width, the pair stays on the full matrix.
+__device__ __forceinline__ bool seeds_support(const float* x, int dc, int hw) {
+  const int R = static_cast<int>(x[0] * 256.f + 0.5f);
+  const int nseeds = R >= kK ? min(64, R - kK + 1) : 0;
+  float tot = 0.f, inside = 0.f;
+#pragma unroll
+  for (int b = 0; b < kBins; ++b) {
+    const float v = x[4 + b];
+    tot += v;
+    // bin b holds offsets anchor - 64 + [4b, 4b + 3]
+    const int lo = -64 + 4 * b, hi = lo + 3;
+    if (lo >= dc - hw && hi <= dc + hw) inside += v;
+  }
+  const float hits = tot * static_cast<float>(nseeds);
+  return hits >= 1.5f && inside >= tot - 1e-6f;
+}
+
+__global__ void band_from_prediction_kernel(const float* __restrict__ Y, const int* __restrict__ anchor,
+                                            const float* __restrict__ X, int64_t n, int margin,
+                                            int max_width, int2* __restrict__ band) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const float* x = X + i * kFeat;
+  const int H = static_cast<int>(x[1] * 512.f + 0.5f);
+  const float lw = fminf(fmaxf(Y[i * 2 + 1], 0.f), 12.f);
+  const int width = static_cast<int>(ceilf(exp2f(lw)));
+  const int hw = (width + 1) / 2 + margin;
+  const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
+  if (H > kBandMaxHap || 2 * hw + 1 > max_width || !seeds_support(x, dc, hw)) {
+    band[i] = int2{1, 0};
+    return;
+  }
+  const int c = anchor[i] + dc;
+  band[i] = int2{c - hw, c + hw};
+}
+
+}  // namespace bvp
+
+// ---- launchers ---------------------------------------------------------------
+namespace bvp {
+
+hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(band_features_kernel, dim3(grid), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
+                              float* H2, hipStream_t st) {
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, blocks))),
+                     dim3(256), 0, st, params, X, n, Y, H1, H2);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward(const float* params, const float* X, const float* Y, const float* T,
+                               int64_t n, const float* H1, const float* H2, float* D1, float* D2,
+                               float* D3, float w0, float w1, float* loss, float* grad,
+                               hipStream_t st) {
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(mlp_delta_kernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, blocks))),
+                     dim3(256), 0, st, params, Y, T, n, H1, H2, D1, D2, D3, w0, w1, loss);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3((kParams + 255) / 256), dim3(256), 0, st, X, n, H1, H2,
+                     D1, D2, D3, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, float lr, int t,
+                       hipStream_t st) {
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, static_cast<float>(t));
+  const float c2 = 1.f - powf(b2, static_cast<float>(t));
+  hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, st, params, grad, m1,
+                     m2, lr, b1, b2, 1e-8f, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const float* X, int64_t n,
+                                       int margin, int max_width, int2* band, hipStream_t st) {
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(band_from_prediction_kernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, blocks))),
+                     dim3(256), 0, st, Y, anchor, X, n, margin, max_width, band);
+  return hipGetLastError();
+}
+
+}  // namespace bvp

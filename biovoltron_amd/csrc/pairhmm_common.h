@@ -34,6 +34,7 @@ constexpr int kRowsPerStripe = 63;// lane 0 carries the previous stripe's last r
 constexpr int kBlock = 64;        // anti-diagonal steps per staging block
 constexpr int kSlots = 8;         // match classes per haplotype column
 constexpr int kCarryPad = 192;    // extra carry entries past the haplotype end
+constexpr int kEdgeCols = 4;      // columns per side measured for the band edge mass
 
 // Rescue threshold of the reference driver (pairhmm_common.h:16).
 constexpr float kMinAccepted = 1e-28f;
@@ -83,6 +84,13 @@ struct KArgs {
   int renorm;   // FP64: rescale the carry row after every stripe
   int out_raw;  // write the scaled sum S instead of log10 L (testcase ABI)
   unsigned long long* rescue_cells;  // FP32 pass: sum of R*H over rescued pairs
+  // Banded evaluation (FP32 pass): per output pair the diagonal band
+  // [o_lo, o_hi] of j - r, or o_lo > o_hi for the full matrix.
+  const int2* band;
+  float band_tol;        // reject the band when the edge-mass ratio exceeds this
+  Task* fallback;        // rejected banded pairs, re-run on the full matrix
+  int* fallback_count;
+  unsigned long long* cells_done;  // cells actually swept (rows x columns)
 };
 
 }  // namespace bvp

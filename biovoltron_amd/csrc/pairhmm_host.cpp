@@ -12,10 +12,22 @@
 #include <thread>
 #include <vector>
 
+#include "band_common.h"
 #include "bv_pairhmm.h"
 #include "pairhmm_common.h"
 
 namespace bvp {
+hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st);
+hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
+                              float* H2, hipStream_t st);
+hipError_t launch_mlp_backward(const float* params, const float* X, const float* Y, const float* T,
+                               int64_t n, const float* H1, const float* H2, float* D1, float* D2,
+                               float* D3, float w0, float w1, float* loss, float* grad,
+                               hipStream_t st);
+hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, float lr, int t,
+                       hipStream_t st);
+hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const float* X, int64_t n,
+                                       int margin, int max_width, int2* band, hipStream_t st);
 template <typename T>
 hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st);
 template <typename T>
@@ -216,7 +228,7 @@ void parallel_for(int64_t n, F&& f) {
 struct bv_plan {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bv_options opt{};
   int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
   int hmax = 0;
@@ -233,13 +245,26 @@ struct bv_plan {
   int64_t* d_hap_off = nullptr;
   Task* d_tasks = nullptr;
   Task* d_rescue = nullptr;
-  int* d_counters = nullptr;  // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] pad
+  Task* d_fallback = nullptr;
+  int2* d_band = nullptr;
+  // learned bands: features and predictions computed on the device each run
+  float* d_X = nullptr;
+  int* d_anchor = nullptr;
+  float* d_Y = nullptr;
+  float* d_model = nullptr;  // MLP parameters (copy)
+  bool has_model = false;
+  int band_margin = 8, band_max_width = 256;
+  bool has_band = false;
+  int64_t band_pairs = 0;
+  // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] fallback count,
+  // [4] fallback queue, [6..7] rescued cells (u64), [8..9] swept cells (u64)
+  int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
   uint8_t* d_form = nullptr;
   void* d_carry32 = nullptr;
   void* d_carry64 = nullptr;
-  float last_ms32 = 0.f, last_ms64 = 0.f;
+  float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f;
   bool timed = false;
 };
 
@@ -250,7 +275,8 @@ void free_plan(bv_plan* p) {
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_rescue, p->d_counters,
-                  p->d_out, p->d_form, p->d_carry32, p->d_carry64};
+                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band,
+                  p->d_X, p->d_anchor, p->d_Y, p->d_model};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
@@ -325,15 +351,53 @@ KArgs base_args(const bv_plan* p) {
   a.mode = p->opt.mode;
   a.renorm = p->opt.fp64_renorm;
   a.rescue_cells = p->d_rescue_cells;
+  a.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
+  a.band_tol = p->opt.band_tol;
   return a;
 }
 
+hipError_t launch_features(bv_plan* p) {
+  FeatArgs f{};
+  f.rclass = p->d_rclass;
+  f.q = p->d_q;
+  f.read_off = p->d_read_off;
+  f.hmask = p->d_hmask;
+  f.hap_off = p->d_hap_off;
+  f.tasks = p->d_tasks;
+  f.n_tasks = p->n_tasks;
+  f.X = p->d_X;
+  f.anchor = p->d_anchor;
+  const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 16));
+  return launch_band_features(f, grid, p->stream);
+}
+
+int ensure_feature_buffers(bv_plan* p) {
+  const int64_t n = std::max<int64_t>(1, p->n_pairs);
+  if (!p->d_X && hipMalloc(&p->d_X, sizeof(float) * kFeat * n) != hipSuccess)
+    return fail(BV_ENOMEM, "feature buffer allocation failed");
+  if (!p->d_anchor && hipMalloc(&p->d_anchor, sizeof(int) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "anchor buffer allocation failed");
+  if (!p->d_Y && hipMalloc(&p->d_Y, sizeof(float) * 2 * n) != hipSuccess)
+    return fail(BV_ENOMEM, "prediction buffer allocation failed");
+  if (!p->d_band && hipMalloc(&p->d_band, sizeof(int2) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "band buffer allocation failed");
+  return BV_OK;
+}
+
 int enqueue_run(bv_plan* p, bool timed) {
-  HIP_TRY(launch_zero_ints(p->d_counters, 6, p->stream));  // counters + 64-bit cell sum
+  HIP_TRY(launch_zero_ints(p->d_counters, 12, p->stream));
   if (p->n_tasks == 0) return BV_OK;
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   if (timed) HIP_TRY(hipEventRecord(p->ev[0], p->stream));
+  if (fp32 && p->has_model) {
+    // Learned bands: seed features, MLP, band table -- all on the device.
+    HIP_TRY(launch_features(p));
+    HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
+    HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_X, p->n_pairs, p->band_margin,
+                                        p->band_max_width, p->d_band, p->stream));
+  }
   if (fp32) {
+    // Pass 1: every pair in FP32, banded where a band is set.
     KArgs a = base_args(p);
     a.tasks = p->d_tasks;
     a.n_tasks = p->n_tasks;
@@ -342,10 +406,31 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.rescue_count = p->d_counters + 2;
     a.carry = p->d_carry32;
     a.carry_stride = p->stride32;
+    if (p->has_band) {
+      a.band = p->d_band;
+      a.fallback = p->d_fallback;
+      a.fallback_count = p->d_counters + 3;
+    }
     HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
+  if (fp32 && p->has_band) {
+    // Pass 2: rejected bands on the full matrix (FP32, same rescue rule).
+    KArgs a = base_args(p);
+    a.tasks = p->d_fallback;
+    a.n_tasks = 0;
+    a.n_tasks_dev = p->d_counters + 3;
+    a.counter = p->d_counters + 4;
+    a.rescue = p->d_rescue;
+    a.rescue_count = p->d_counters + 2;
+    a.carry = p->d_carry32;
+    a.carry_stride = p->stride32;
+    HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+  }
+  if (timed) HIP_TRY(hipEventRecord(p->ev[4], p->stream));
   {
+    // Pass 3: FP64 over the pairs whose FP32 sum fell below 1e-28 (or all
+    // pairs in the FP64-only modes).
     KArgs a = base_args(p);
     if (fp32) {
       a.tasks = p->d_rescue;
@@ -363,7 +448,6 @@ int enqueue_run(bv_plan* p, bool timed) {
   if (timed) HIP_TRY(hipEventRecord(p->ev[2], p->stream));
   return BV_OK;
 }
-
 }  // namespace
 
 extern "C" {
@@ -416,6 +500,7 @@ void bv_options_default(bv_options* opt) {
   std::memset(opt, 0, sizeof(*opt));
   opt->mode = BV_MODE_AVX;
   opt->fp64_renorm = 1;
+  opt->band_tol = 1e-6f;
 }
 
 int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** out) {
@@ -502,7 +587,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       if (nh == 0) continue;
       const int64_t R = roff[r + 1] - roff[r];
       tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
-                           static_cast<int32_t>(nh), -1, pair_base + (r - r0) * nh});
+                           static_cast<int32_t>(nh), 0, pair_base + (r - r0) * nh});
       cost.push_back(R * hsum);
       cells += R * hsum;
     }
@@ -542,14 +627,15 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)))
     return cleanup(rc);
   if (hipMalloc(&p->d_rescue, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
-      hipMalloc(&p->d_counters, sizeof(int) * 8) != hipSuccess ||
+      hipMalloc(&p->d_counters, sizeof(int) * 16) != hipSuccess ||
+      hipMalloc(&p->d_fallback, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_out, sizeof(double) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_form, std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_carry32, sizeof(float) * 2 * p->stride32 * p->grid32) != hipSuccess ||
       hipMalloc(&p->d_carry64, sizeof(double) * 2 * p->stride64 * p->grid64) != hipSuccess)
     return cleanup(fail(BV_ENOMEM, "device allocation failed"));
   // The 64-bit rescue cell counter shares the zeroed counter block.
-  p->d_rescue_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 4);
+  p->d_rescue_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 6);
   if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(BV_EHIP, "upload failed"));
   *out = p;
   return BV_OK;
@@ -565,7 +651,7 @@ int bv_plan_run(bv_plan* p) {
 int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
   if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
   HIP_TRY(hipSetDevice(g_ctx.device));
-  float sum32 = 0.f, sum64 = 0.f;
+  float sum32 = 0.f, sum64 = 0.f, sumfb = 0.f;
   HIP_TRY(hipEventRecord(p->ev[3], p->stream));
   hipEvent_t start = p->ev[3];
   // Per-run split between the FP32 and FP64 passes from events between them.
@@ -573,16 +659,19 @@ int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
     int rc = enqueue_run(p, true);
     if (rc) return rc;
     HIP_TRY(hipEventSynchronize(p->ev[2]));
-    float a = 0.f, c = 0.f;
+    float a = 0.f, fb = 0.f, c = 0.f;
     HIP_TRY(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&c, p->ev[1], p->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&fb, p->ev[1], p->ev[4]));
+    HIP_TRY(hipEventElapsedTime(&c, p->ev[4], p->ev[2]));
     sum32 += a;
+    sumfb += fb;
     sum64 += c;
   }
   float total = 0.f;
   HIP_TRY(hipEventElapsedTime(&total, start, p->ev[2]));
   p->last_ms32 = sum32 / iters;
   p->last_ms64 = sum64 / iters;
+  p->last_msfb = sumfb / iters;
   if (ms_per_run) *ms_per_run = total / iters;
   return BV_OK;
 }
@@ -602,10 +691,11 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   if (!p || !s) return fail(BV_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(g_ctx.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  int cnt[8] = {0};
+  int cnt[16] = {0};
   HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  unsigned long long rcells = 0;
-  std::memcpy(&rcells, cnt + 4, sizeof(rcells));
+  unsigned long long rcells = 0, swept = 0;
+  std::memcpy(&rcells, cnt + 6, sizeof(rcells));
+  std::memcpy(&swept, cnt + 8, sizeof(swept));
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   s->n_pairs = p->n_pairs;
   s->cells = p->cells;
@@ -616,6 +706,29 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->grid_f64 = p->grid64;
   s->last_ms_f32 = p->last_ms32;
   s->last_ms_f64 = p->last_ms64;
+  s->band_pairs = p->has_band ? p->band_pairs : 0;
+  s->fallback_pairs = p->has_band ? cnt[3] : 0;
+  s->cells_swept = static_cast<int64_t>(swept);
+  s->last_ms_fallback = p->last_msfb;
+  s->pad = 0.f;
+  return BV_OK;
+}
+
+int bv_plan_set_bands(bv_plan* p, const int32_t* bands) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (!bands) {
+    p->has_band = false;
+    p->band_pairs = 0;
+    return BV_OK;
+  }
+  if (!p->d_band) HIP_TRY(hipMalloc(&p->d_band, sizeof(int2) * std::max<int64_t>(1, p->n_pairs)));
+  int64_t nb = 0;
+  for (int64_t k = 0; k < p->n_pairs; ++k) nb += bands[2 * k] <= bands[2 * k + 1];
+  HIP_TRY(hipMemcpyAsync(p->d_band, bands, sizeof(int2) * p->n_pairs, hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->has_band = true;
+  p->band_pairs = nb;
   return BV_OK;
 }
 
@@ -672,7 +785,7 @@ int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* o
   int rc = bv_plan_create(&b, &opt, &p);
   if (rc) return rc;
   auto run = [&]() -> int {
-    HIP_TRY(launch_zero_ints(p->d_counters, 6, p->stream));
+    HIP_TRY(launch_zero_ints(p->d_counters, 12, p->stream));
     KArgs a = base_args(p);
     a.tasks = p->d_tasks;
     a.n_tasks = p->n_tasks;
@@ -693,6 +806,393 @@ int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* o
   if (!rc) rc = bv_plan_results(p, out_scaled, nullptr);
   bv_plan_destroy(p);
   return rc;
+}
+
+
+// ---- band model (MLP) ---------------------------------------------------------
+
+struct bv_band_model {
+  float* d_params = nullptr;
+  float* d_m1 = nullptr;
+  float* d_m2 = nullptr;
+  float* d_grad = nullptr;
+  float* d_loss = nullptr;
+  int step = 0;
+  hipStream_t stream = nullptr;
+};
+
+static void free_model(bv_band_model* m) {
+  if (!m) return;
+  for (void* b : {static_cast<void*>(m->d_params), static_cast<void*>(m->d_m1),
+                  static_cast<void*>(m->d_m2), static_cast<void*>(m->d_grad),
+                  static_cast<void*>(m->d_loss)})
+    if (b) (void)hipFree(b);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int bv_band_model_create(bv_band_model** out, uint64_t seed) {
+  if (!out) return fail(BV_EINVAL, "model output pointer is NULL");
+  *out = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_init_locked(-1);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  auto* m = new bv_band_model();
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&m->d_params, sizeof(float) * kParams) != hipSuccess ||
+      hipMalloc(&m->d_m1, sizeof(float) * kParams) != hipSuccess ||
+      hipMalloc(&m->d_m2, sizeof(float) * kParams) != hipSuccess ||
+      hipMalloc(&m->d_grad, sizeof(float) * kParams) != hipSuccess ||
+      hipMalloc(&m->d_loss, sizeof(float)) != hipSuccess) {
+    free_model(m);
+    return fail(BV_ENOMEM, "model allocation failed");
+  }
+  // He-uniform weights, zero biases, from a splitmix64 stream.
+  std::vector<float> p(kParams, 0.f);
+  uint64_t st = seed;
+  auto next = [&st]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return static_cast<double>(z >> 11) * 0x1.0p-53;
+  };
+  auto fill = [&](int off, int fan_in, int count) {
+    const double lim = std::sqrt(6.0 / fan_in);
+    for (int i = 0; i < count; ++i) p[off + i] = static_cast<float>((2.0 * next() - 1.0) * lim);
+  };
+  fill(0, kFeat, kHid * kFeat);
+  fill(kHid * kFeat + kHid, kHid, kHid * kHid);
+  fill(kHid * kFeat + kHid + kHid * kHid + kHid, kHid, 2 * kHid);
+  std::vector<float> zero(kParams, 0.f);
+  if (hipMemcpy(m->d_params, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_m1, zero.data(), sizeof(float) * kParams, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_m2, zero.data(), sizeof(float) * kParams, hipMemcpyHostToDevice) != hipSuccess) {
+    free_model(m);
+    return fail(BV_EHIP, "model upload failed");
+  }
+  *out = m;
+  return BV_OK;
+}
+
+void bv_band_model_destroy(bv_band_model* m) {
+  if (!m) return;
+  (void)hipSetDevice(g_ctx.device);
+  (void)hipStreamSynchronize(m->stream);
+  free_model(m);
+}
+
+int bv_band_model_get_params(const bv_band_model* m, float* out) {
+  if (!m || !out) return fail(BV_EINVAL, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(out, m->d_params, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+  return BV_OK;
+}
+
+int bv_band_model_set_params(bv_band_model* m, const float* in) {
+  if (!m || !in) return fail(BV_EINVAL, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(m->d_params, in, sizeof(float) * kParams, hipMemcpyHostToDevice));
+  m->step = 0;
+  std::vector<float> zero(kParams, 0.f);
+  HIP_TRY(hipMemcpy(m->d_m1, zero.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m->d_m2, zero.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
+  return BV_OK;
+}
+
+namespace {
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  float* f() { return static_cast<float*>(p); }
+};
+int dev_alloc(DevBuf& b, size_t bytes) {
+  if (hipMalloc(&b.p, std::max<size_t>(bytes, 4)) != hipSuccess) return fail(BV_ENOMEM, "device allocation failed");
+  return BV_OK;
+}
+}  // namespace
+
+int bv_band_model_predict(bv_band_model* m, const float* X, int64_t n, float* Y) {
+  if (!m || n < 0 || (n > 0 && (!X || !Y))) return fail(BV_EINVAL, "bad arguments");
+  if (n == 0) return BV_OK;
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  DevBuf dx, dy;
+  int rc;
+  if ((rc = dev_alloc(dx, sizeof(float) * kFeat * n)) || (rc = dev_alloc(dy, sizeof(float) * 2 * n))) return rc;
+  HIP_TRY(hipMemcpyAsync(dx.p, X, sizeof(float) * kFeat * n, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(launch_mlp_forward(m->d_params, dx.f(), n, dy.f(), nullptr, nullptr, m->stream));
+  HIP_TRY(hipMemcpyAsync(Y, dy.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return BV_OK;
+}
+
+// Loss 0.5*mean(w0*e0^2 + w1*e1^2) and its gradient for one batch.
+int bv_band_model_loss_grad(bv_band_model* m, const float* X, const float* T, int64_t n, float w0,
+                            float w1, float* loss, float* grad) {
+  if (!m || n <= 0 || !X || !T) return fail(BV_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  DevBuf dx, dt, dy, h1, h2, d1, d2, d3;
+  int rc;
+  if ((rc = dev_alloc(dx, sizeof(float) * kFeat * n)) || (rc = dev_alloc(dt, sizeof(float) * 2 * n)) ||
+      (rc = dev_alloc(dy, sizeof(float) * 2 * n)) || (rc = dev_alloc(h1, sizeof(float) * kHid * n)) ||
+      (rc = dev_alloc(h2, sizeof(float) * kHid * n)) || (rc = dev_alloc(d1, sizeof(float) * kHid * n)) ||
+      (rc = dev_alloc(d2, sizeof(float) * kHid * n)) || (rc = dev_alloc(d3, sizeof(float) * 2 * n)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(dx.p, X, sizeof(float) * kFeat * n, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipMemcpyAsync(dt.p, T, sizeof(float) * 2 * n, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(launch_zero_ints(reinterpret_cast<int*>(m->d_loss), 1, m->stream));
+  HIP_TRY(launch_mlp_forward(m->d_params, dx.f(), n, dy.f(), h1.f(), h2.f(), m->stream));
+  HIP_TRY(launch_mlp_backward(m->d_params, dx.f(), dy.f(), dt.f(), n, h1.f(),
+                              h2.f(), d1.f(), d2.f(), d3.f(), w0, w1,
+                              m->d_loss, m->d_grad, m->stream));
+  if (loss) HIP_TRY(hipMemcpyAsync(loss, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  if (grad) HIP_TRY(hipMemcpyAsync(grad, m->d_grad, sizeof(float) * kParams, hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return BV_OK;
+}
+
+// Adam training on the device.  X [n][36], T [n][2] (host); the data is
+// uploaded once and every minibatch step (forward, deltas, gradient, Adam)
+// runs as device kernels.  order (host, epochs x n indices, optional) gives
+// the sample order; NULL keeps the given order every epoch.
+int bv_band_model_train(bv_band_model* m, const float* X, const float* T, int64_t n, int epochs,
+                        int batch, float lr, float w0, float w1, const int64_t* order,
+                        float* epoch_loss) {
+  if (!m || n <= 0 || !X || !T || epochs <= 0 || batch <= 0) return fail(BV_EINVAL, "bad arguments");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  const int64_t bsz = std::min<int64_t>(batch, n);
+  DevBuf dx, dt, bx, bt, dy, h1, h2, d1, d2, d3;
+  int rc;
+  if ((rc = dev_alloc(dx, sizeof(float) * kFeat * n)) || (rc = dev_alloc(dt, sizeof(float) * 2 * n)) ||
+      (rc = dev_alloc(bx, sizeof(float) * kFeat * bsz)) || (rc = dev_alloc(bt, sizeof(float) * 2 * bsz)) ||
+      (rc = dev_alloc(dy, sizeof(float) * 2 * bsz)) || (rc = dev_alloc(h1, sizeof(float) * kHid * bsz)) ||
+      (rc = dev_alloc(h2, sizeof(float) * kHid * bsz)) || (rc = dev_alloc(d1, sizeof(float) * kHid * bsz)) ||
+      (rc = dev_alloc(d2, sizeof(float) * kHid * bsz)) || (rc = dev_alloc(d3, sizeof(float) * 2 * bsz)))
+    return rc;
+  // Shuffled copies per epoch are made on the host side of the order array;
+  // the device data is permuted once per epoch by host-built gathers.
+  std::vector<float> xs(static_cast<size_t>(kFeat * n)), ts(static_cast<size_t>(2 * n));
+  for (int ep = 0; ep < epochs; ++ep) {
+    const int64_t* ord = order ? order + static_cast<int64_t>(ep) * n : nullptr;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t k = ord ? ord[i] : i;
+      std::memcpy(&xs[i * kFeat], X + k * kFeat, sizeof(float) * kFeat);
+      ts[2 * i] = T[2 * k];
+      ts[2 * i + 1] = T[2 * k + 1];
+    }
+    HIP_TRY(hipMemcpyAsync(dx.p, xs.data(), sizeof(float) * kFeat * n, hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(hipMemcpyAsync(dt.p, ts.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(launch_zero_ints(reinterpret_cast<int*>(m->d_loss), 1, m->stream));
+    int nb = 0;
+    for (int64_t off = 0; off + bsz <= n; off += bsz, ++nb) {
+      const float* xb = dx.f() + off * kFeat;
+      const float* tb = dt.f() + off * 2;
+      HIP_TRY(launch_mlp_forward(m->d_params, xb, bsz, dy.f(), h1.f(), h2.f(), m->stream));
+      HIP_TRY(launch_mlp_backward(m->d_params, xb, dy.f(), tb, bsz, h1.f(), h2.f(),
+                                  d1.f(), d2.f(), d3.f(), w0, w1, m->d_loss,
+                                  m->d_grad, m->stream));
+      ++m->step;
+      HIP_TRY(launch_adam(m->d_params, m->d_grad, m->d_m1, m->d_m2, lr, m->step, m->stream));
+    }
+    if (epoch_loss) {
+      float l = 0.f;
+      HIP_TRY(hipMemcpyAsync(&l, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+      HIP_TRY(hipStreamSynchronize(m->stream));
+      epoch_loss[ep] = nb > 0 ? l / nb : 0.f;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return BV_OK;
+}
+
+int bv_plan_features(bv_plan* p, float* X_out, int32_t* anchor_out) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  int rc = ensure_feature_buffers(p);
+  if (rc) return rc;
+  if (p->n_tasks > 0) HIP_TRY(launch_features(p));
+  if (X_out && p->n_pairs)
+    HIP_TRY(hipMemcpyAsync(X_out, p->d_X, sizeof(float) * kFeat * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
+  if (anchor_out && p->n_pairs)
+    HIP_TRY(hipMemcpyAsync(anchor_out, p->d_anchor, sizeof(int) * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return BV_OK;
+}
+
+int bv_plan_set_band_model(bv_plan* p, const bv_band_model* m, int margin, int max_width) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (!m) {
+    p->has_model = false;
+    p->has_band = false;
+    return BV_OK;
+  }
+  if (margin < 0 || max_width < 1) return fail(BV_EINVAL, "bad band margin or width");
+  int rc = ensure_feature_buffers(p);
+  if (rc) return rc;
+  if (!p->d_model) HIP_TRY(hipMalloc(&p->d_model, sizeof(float) * kParams));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(p->d_model, m->d_params, sizeof(float) * kParams, hipMemcpyDeviceToDevice));
+  p->has_model = true;
+  p->has_band = true;
+  p->band_margin = margin;
+  p->band_max_width = max_width;
+  p->band_pairs = p->n_pairs;
+  return BV_OK;
+}
+
+int bv_plan_bands(bv_plan* p, int32_t* bands_out) {
+  if (!p || !bands_out) return fail(BV_EINVAL, "NULL argument");
+  if (!p->has_band || !p->d_band) return fail(BV_ESTATE, "plan has no bands");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  HIP_TRY(hipMemcpyAsync(bands_out, p->d_band, sizeof(int2) * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return BV_OK;
+}
+
+// Band labels: the narrowest diagonal span [o_lo, o_hi] of j - r holding at
+// least `mass` of the posterior probability (M, I and D states), from an
+// FP64 forward-backward pass with per-row renormalisation on the host.
+int bv_band_labels(const bv_region_batch* b, double mass, int n_threads, int32_t* lo_hi) {
+  int rc = validate(b);
+  if (rc) return rc;
+  if (!lo_hi || !(mass > 0.0 && mass <= 1.0)) return fail(BV_EINVAL, "bad arguments");
+  static HostTables t;
+  static std::once_flag once;
+  std::call_once(once, [] { build_tables(t); });
+  Alphabet al;
+  alphabet_avx(al);
+  struct Job { int64_t read, hap, out; };
+  std::vector<Job> jobs;
+  int64_t base = 0;
+  for (int64_t g = 0; g < b->n_regions; ++g) {
+    const int64_t r0 = b->region_read_off[g], r1 = b->region_read_off[g + 1];
+    const int64_t h0 = b->region_hap_off[g], h1 = b->region_hap_off[g + 1];
+    for (int64_t r = r0; r < r1; ++r)
+      for (int64_t h = h0; h < h1; ++h) jobs.push_back(Job{r, h, base + (r - r0) * (h1 - h0) + (h - h0)});
+    base += (r1 - r0) * (h1 - h0);
+  }
+  std::atomic<int64_t> next{0};
+  auto worker = [&]() {
+    std::vector<double> F, B, diag;
+    std::vector<double> fs, bs;
+    for (;;) {
+      const int64_t k = next.fetch_add(1);
+      if (k >= static_cast<int64_t>(jobs.size())) break;
+      const Job& jb = jobs[k];
+      const int64_t ro = b->read_off[jb.read], ho = b->hap_off[jb.hap];
+      const int R = static_cast<int>(b->read_off[jb.read + 1] - ro);
+      const int H = static_cast<int>(b->hap_off[jb.hap + 1] - ho);
+      const size_t W = static_cast<size_t>(H) + 2;
+      const size_t N = (static_cast<size_t>(R) + 2) * W;
+      F.assign(3 * N, 0.0);
+      B.assign(3 * N, 0.0);
+      fs.assign(R + 2, 0.0);
+      bs.assign(R + 2, 0.0);
+      auto fm = [&](int r, int j) -> double& { return F[(static_cast<size_t>(r) * W + j) * 3]; };
+      auto fi = [&](int r, int j) -> double& { return F[(static_cast<size_t>(r) * W + j) * 3 + 1]; };
+      auto fd = [&](int r, int j) -> double& { return F[(static_cast<size_t>(r) * W + j) * 3 + 2]; };
+      auto bm = [&](int r, int j) -> double& { return B[(static_cast<size_t>(r) * W + j) * 3]; };
+      auto bi = [&](int r, int j) -> double& { return B[(static_cast<size_t>(r) * W + j) * 3 + 1]; };
+      auto bd = [&](int r, int j) -> double& { return B[(static_cast<size_t>(r) * W + j) * 3 + 2]; };
+      struct P { double mm, g, mi, ii, md, dd, pm, pmis; uint8_t cls; };
+      std::vector<P> prm(R + 2);
+      for (int r = 1; r <= R; ++r) {
+        const int64_t p = ro + r - 1;
+        const int iq = b->read_ins[p] & 127, dq = b->read_del[p] & 127, cq = b->read_gcp[p] & 127;
+        const int lo = std::min(iq, dq), hi = std::max(iq, dq);
+        const double e = t.ph2pr_d[b->read_qual[p] & 127];
+        const uint8_t rc8 = al.read_class[b->read_bases[p]];
+        prm[r] = P{t.mm_d[((hi * (hi + 1)) >> 1) + lo], 1.0 - t.ph2pr_d[cq], t.ph2pr_d[iq], t.ph2pr_d[cq],
+                   t.ph2pr_d[dq], t.ph2pr_d[cq], 1.0 - e, (rc8 & kWildcard) ? 1.0 - e : e / 3.0, rc8};
+      }
+      auto prior = [&](int r, int j) {
+        const uint8_t hm = al.hap_mask[b->hap_bases[ho + j - 1]];
+        return ((hm >> (prm[r].cls & 7)) & 1) ? prm[r].pm : prm[r].pmis;
+      };
+      for (int j = 0; j <= H; ++j) fd(0, j) = 1.0 / H;
+      for (int r = 1; r <= R; ++r) {
+        double mx = 0.0;
+        for (int j = 1; j <= H; ++j) {
+          fm(r, j) = (fm(r - 1, j - 1) * prm[r].mm + (fi(r - 1, j - 1) + fd(r - 1, j - 1)) * prm[r].g) * prior(r, j);
+          fi(r, j) = fm(r - 1, j) * prm[r].mi + fi(r - 1, j) * prm[r].ii;
+          fd(r, j) = fm(r, j - 1) * prm[r].md + fd(r, j - 1) * prm[r].dd;
+          mx = std::max(mx, std::max(fm(r, j), std::max(fi(r, j), fd(r, j))));
+        }
+        int e = 0;
+        if (mx > 0.0) std::frexp(mx, &e);
+        for (int j = 0; j <= H; ++j) {
+          fm(r, j) = std::ldexp(fm(r, j), -e);
+          fi(r, j) = std::ldexp(fi(r, j), -e);
+          fd(r, j) = std::ldexp(fd(r, j), -e);
+        }
+        fs[r] = fs[r - 1] + e;
+      }
+      double tot = 0.0;
+      for (int j = 1; j <= H; ++j) tot += fm(R, j) + fi(R, j);
+      const double log2L = std::log2(tot) + fs[R];
+      for (int j = 1; j <= H; ++j) {
+        bm(R, j) = 1.0;
+        bi(R, j) = 1.0;
+      }
+      for (int r = R - 1; r >= 1; --r) {
+        double mx = 0.0;
+        for (int j = H; j >= 1; --j) {
+          const double mn = j < H ? bm(r + 1, j + 1) * prior(r + 1, j + 1) : 0.0;
+          const double dn = j < H ? bd(r, j + 1) : 0.0;
+          bm(r, j) = prm[r + 1].mm * mn + prm[r + 1].mi * bi(r + 1, j) + prm[r].md * dn;
+          bi(r, j) = prm[r + 1].g * mn + prm[r + 1].ii * bi(r + 1, j);
+          bd(r, j) = prm[r + 1].g * mn + prm[r].dd * dn;
+          mx = std::max(mx, std::max(bm(r, j), std::max(bi(r, j), bd(r, j))));
+        }
+        int e = 0;
+        if (mx > 0.0) std::frexp(mx, &e);
+        for (int j = 1; j <= H; ++j) {
+          bm(r, j) = std::ldexp(bm(r, j), -e);
+          bi(r, j) = std::ldexp(bi(r, j), -e);
+          bd(r, j) = std::ldexp(bd(r, j), -e);
+        }
+        bs[r] = bs[r + 1] + e;
+      }
+      diag.assign(static_cast<size_t>(R + H), 0.0);
+      double all = 0.0;
+      for (int r = 1; r <= R; ++r) {
+        const double sc = std::exp2(fs[r] + bs[r] - log2L);
+        for (int j = 1; j <= H; ++j) {
+          const double v = (fm(r, j) * bm(r, j) + fi(r, j) * bi(r, j) + fd(r, j) * bd(r, j)) * sc;
+          diag[static_cast<size_t>(j - r + R - 1)] += v;
+          all += v;
+        }
+      }
+      // narrowest window holding `mass` of the posterior
+      const int nd = R + H - 1;
+      const double need = mass * all;
+      int best_lo = 0, best_hi = nd - 1;
+      double win = 0.0;
+      int lo = 0;
+      for (int hi = 0; hi < nd; ++hi) {
+        win += diag[hi];
+        while (lo < hi && win - diag[lo] >= need) win -= diag[lo++];
+        if (win >= need && hi - lo < best_hi - best_lo) {
+          best_lo = lo;
+          best_hi = hi;
+        }
+      }
+      lo_hi[2 * jb.out] = best_lo - (R - 1);
+      lo_hi[2 * jb.out + 1] = best_hi - (R - 1);
+    }
+  };
+  if (n_threads <= 0) n_threads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int i = 0; i < n_threads; ++i) th.emplace_back(worker);
+  for (auto& x : th) x.join();
+  return BV_OK;
 }
 
 }  // extern "C"

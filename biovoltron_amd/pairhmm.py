@@ -52,8 +52,8 @@ class _RegionBatch(C.Structure):
 class Options(C.Structure):
     _fields_ = [
         ("mode", C.c_int32), ("fp64_renorm", C.c_int32), ("force_fp64", C.c_int32),
-        ("grid_f32", C.c_int32), ("grid_f64", C.c_int32), ("reserved", C.c_int32 * 3),
-        ("trans", C.c_double * 7),
+        ("grid_f32", C.c_int32), ("grid_f64", C.c_int32), ("band_tol", C.c_float),
+        ("reserved", C.c_int32 * 2), ("trans", C.c_double * 7),
     ]
 
 
@@ -62,6 +62,8 @@ class PlanStats(C.Structure):
         ("n_pairs", C.c_int64), ("cells", C.c_int64), ("fp64_pairs", C.c_int64),
         ("fp64_cells", C.c_int64), ("tasks", C.c_int64), ("grid_f32", C.c_int32),
         ("grid_f64", C.c_int32), ("last_ms_f32", C.c_float), ("last_ms_f64", C.c_float),
+        ("band_pairs", C.c_int64), ("fallback_pairs", C.c_int64), ("cells_swept", C.c_int64),
+        ("last_ms_fallback", C.c_float), ("pad", C.c_float),
     ]
 
 
@@ -127,6 +129,8 @@ def default_options(**kw) -> Options:
         if k == "trans":
             for i, x in enumerate(v):
                 o.trans[i] = float(x)
+        elif k == "band_tol":
+            o.band_tol = float(v)
         else:
             setattr(o, k, int(v))
     return o

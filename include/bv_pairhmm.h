@@ -117,7 +117,9 @@ typedef struct bv_options {
   int32_t force_fp64;  /* skip the FP32 pass */
   int32_t grid_f32;    /* persistent grid override, 0 = occupancy-derived */
   int32_t grid_f64;
-  int32_t reserved[3];
+  float band_tol;      /* banded pairs whose edge-mass ratio exceeds this are
+                          re-run on the full matrix (default 1e-6) */
+  int32_t reserved[2];
   double trans[7];     /* BV_MODE_SCALAR: {MM, MI, MD, IM, II, DM, DD}
                           (pairhmm.hpp:15-33 order); all zero = ORIGINAL_DEFAULT */
 } bv_options;
@@ -134,6 +136,11 @@ typedef struct bv_plan_stats {
   int64_t tasks;
   int32_t grid_f32, grid_f64;
   float last_ms_f32, last_ms_f64; /* kernel times of the last timed run */
+  int64_t band_pairs;     /* pairs given a band in the last run */
+  int64_t fallback_pairs; /* banded pairs rejected and re-run on the full matrix */
+  int64_t cells_swept;    /* FP32 cells actually computed in the last run */
+  float last_ms_fallback;
+  float pad;
 } bv_plan_stats;
 
 /* Uploads a batch to device memory and builds the length-ordered work list. */
@@ -148,7 +155,48 @@ int bv_plan_time(bv_plan* plan, int iters, float* ms_per_run);
  * n_pairs (either pointer may be NULL). */
 int bv_plan_results(bv_plan* plan, double* log10_out, uint8_t* form_out);
 int bv_plan_stats_get(bv_plan* plan, bv_plan_stats* stats);
+/* Per-pair diagonal bands for the FP32 pass: bands[2k], bands[2k+1] =
+ * (o_lo, o_hi) of j - r for pair k (host memory, n_pairs x 2 int32); a pair
+ * with o_lo > o_hi is evaluated on the full matrix.  NULL removes all bands.
+ * No reference counterpart: the reference computes every cell. */
+int bv_plan_set_bands(bv_plan* plan, const int32_t* bands);
 void bv_plan_destroy(bv_plan* plan);
+
+/* ---- learned bands ----------------------------------------------------------
+ * None of these has a reference counterpart: the reference computes every
+ * cell.  A band model is a 36 -> 64 -> 64 -> 2 ReLU MLP (FP32) over per-pair
+ * features (read/haplotype lengths, mean/min base quality, a 32-bin 12-mer
+ * seed-hit offset histogram around the median hit offset, the "anchor");
+ * its outputs are (band centre - anchor) / 16 and log2(band width). */
+#define BV_BAND_FEATURES 36
+#define BV_BAND_PARAMS 6658
+typedef struct bv_band_model bv_band_model;
+int bv_band_model_create(bv_band_model** model, uint64_t seed);
+void bv_band_model_destroy(bv_band_model* model);
+int bv_band_model_get_params(const bv_band_model* model, float* params /* BV_BAND_PARAMS */);
+int bv_band_model_set_params(bv_band_model* model, const float* params);
+/* X [n][36] -> Y [n][2] (host buffers), device forward kernel. */
+int bv_band_model_predict(bv_band_model* model, const float* X, int64_t n, float* Y);
+/* Loss 0.5*mean(w0*(Y0-T0)^2 + w1*(Y1-T1)^2) and its gradient (device kernels). */
+int bv_band_model_loss_grad(bv_band_model* model, const float* X, const float* T, int64_t n,
+                            float w0, float w1, float* loss, float* grad /* BV_BAND_PARAMS */);
+/* Adam on the device over `epochs` passes of minibatches; order (optional,
+ * epochs x n sample indices) fixes the sample order of every epoch. */
+int bv_band_model_train(bv_band_model* model, const float* X, const float* T, int64_t n,
+                        int epochs, int batch, float lr, float w0, float w1,
+                        const int64_t* order, float* epoch_loss);
+/* Per-pair features (device kernel) copied to host: X [n_pairs][36], anchors. */
+int bv_plan_features(bv_plan* plan, float* X_out, int32_t* anchor_out);
+/* Every later run predicts bands on the device (features, MLP, band table)
+ * before the FP32 pass; pairs whose predicted width + 2*margin exceeds
+ * max_width, or whose haplotype is longer than 1024, stay on the full
+ * matrix.  NULL model turns learned bands off. */
+int bv_plan_set_band_model(bv_plan* plan, const bv_band_model* model, int margin, int max_width);
+/* The band table of the last run (n_pairs x 2 int32). */
+int bv_plan_bands(bv_plan* plan, int32_t* bands_out);
+/* Training labels: the narrowest diagonal span holding `mass` of each pair's
+ * posterior (FP64 forward-backward on the host), lo_hi [n_pairs][2]. */
+int bv_band_labels(const bv_region_batch* batch, double mass, int n_threads, int32_t* lo_hi);
 
 /* One-shot: create, run, fetch, destroy. */
 int bv_pairhmm_batch(const bv_region_batch* batch, const bv_options* opt,

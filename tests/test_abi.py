@@ -73,3 +73,28 @@ def test_testcase_abi_rejects_bad_arguments():
     lib = pairhmm.library()
     assert lib.bv_compute_full_prob(None, 1, 32, None) == -22
     assert lib.bv_compute_full_prob(None, 0, 16, None) == -22
+
+
+def test_ctypes_layouts_match_header(tmp_path):
+    """The Python ctypes mirrors have the C structs' sizes and offsets."""
+    import subprocess
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "bv_pairhmm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(bv_options), sizeof(bv_plan_stats),
+         sizeof(bv_region_batch), sizeof(bv_device_info), sizeof(bv_testcase),
+         offsetof(bv_options, trans), offsetof(bv_plan_stats, cells_swept),
+         offsetof(bv_plan_stats, last_ms_fallback));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(pairhmm.Options), C.sizeof(pairhmm.PlanStats), C.sizeof(pairhmm._RegionBatch),
+            C.sizeof(pairhmm.DeviceInfo), C.sizeof(pairhmm._Testcase), pairhmm.Options.trans.offset,
+            pairhmm.PlanStats.cells_swept.offset, pairhmm.PlanStats.last_ms_fallback.offset]
+    assert got == want

@@ -1,0 +1,202 @@
+"""Banded PairHMM, band labels and the band MLP.
+
+CPU: labels from the product's forward-backward vs the oracle's posteriors.
+GPU: banded evaluation vs the full matrix, fallback on bad bands, MLP
+gradients vs a torch FP32 reference, training, and the learned-vs-heuristic
+ablation.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_pairs
+from biovoltron_amd import band, pairhmm, synth
+
+
+def say(*a):
+    print("[band]", *a, flush=True)
+
+
+def fixture_batch(name="workload1.npz", step=1):
+    pairs, d = load_pairs(name)
+    sel = pairs[::step]
+    return sel, pairhmm.PackedBatch.from_regions([([r], [h]) for r, h in sel])
+
+
+def oracle_span(oracle, rd, hap, frac=band.LABEL_MASS):
+    R = len(rd["seq"])
+    _, mass, _ = oracle.band_posterior(rd, hap)
+    need = frac * mass.sum()
+    best, lo, win = (0, len(mass) - 1), 0, 0.0
+    for hi in range(len(mass)):
+        win += mass[hi]
+        while lo < hi and win - mass[lo] >= need:
+            win -= mass[lo]
+            lo += 1
+        if win >= need and hi - lo < best[1] - best[0]:
+            best = (lo, hi)
+    return best[0] - (R - 1), best[1] - (R - 1)
+
+
+def test_labels_match_oracle_posterior(oracle):
+    sel, pb = fixture_batch(step=7)
+    lab = band.labels(pb)
+    for k, (rd, hap) in enumerate(sel):
+        assert tuple(lab[k]) == oracle_span(oracle, rd, hap)
+
+
+def test_targets_roundtrip():
+    lab = np.array([[10, 13], [-5, -5]], np.int32)
+    anchor = np.array([8, -4], np.int32)
+    T = band.targets(lab, anchor)
+    assert np.allclose(T[:, 0] * 16 + anchor, lab.mean(1))
+    assert np.allclose(2 ** T[:, 1], lab[:, 1] - lab[:, 0] + 1)
+
+
+# ---- GPU ----------------------------------------------------------------------
+
+def run_plan(pb, bands=None, model=None, **kw):
+    with pairhmm.Plan(pb) as plan:
+        if bands is not None:
+            band.set_plan_bands(plan, bands)
+        if model is not None:
+            band.set_plan_model(plan, model, **kw)
+        plan.run()
+        out, form = plan.results()
+        st = plan.stats()
+        used = band.plan_bands(plan) if (bands is not None or model is not None) else None
+    return out, form, st, used
+
+
+@pytest.mark.gpu
+def test_covering_band_equals_full():
+    sel, pb = fixture_batch(step=3)
+    say("covering band", len(sel))
+    full, _, _, _ = run_plan(pb)
+    bands = np.array([[-len(r["seq"]) - 2, len(h) + 2] for r, h in sel], np.int32)
+    out, form, st, _ = run_plan(pb, bands=bands)
+    assert np.array_equal(out, full)
+    assert np.all((form == 2) | (form == 1))
+    assert st.fallback_pairs == 0
+
+
+@pytest.mark.gpu
+def test_label_bands_match_full(oracle):
+    sel, pb = fixture_batch()
+    lab = band.labels(pb)
+    bands = np.stack([lab[:, 0] - 8, lab[:, 1] + 8], 1).astype(np.int32)
+    say("label bands", len(sel))
+    full, ff, _, _ = run_plan(pb)
+    out, form, st, _ = run_plan(pb, bands=bands)
+    err = np.abs(out - full)
+    say("max |dlog10| band vs full", float(err.max()), "fallback", st.fallback_pairs,
+        "cells", st.cells_swept)
+    assert err.max() <= 1e-3
+    assert st.fallback_pairs <= len(sel) // 10
+    # fewer cells than the full matrix
+    cells_full = sum(len(r["seq"]) * len(h) for r, h in sel)
+    assert st.cells_swept < 0.8 * cells_full
+
+
+@pytest.mark.gpu
+def test_misplaced_bands_fall_back_to_full():
+    """Bands far from the alignment: the final-row edge test sends (nearly)
+    all of them back to the full matrix, and those results are exactly the
+    full matrix's."""
+    sel, pb = fixture_batch(step=4)
+    lab = band.labels(pb)
+    bands = np.stack([lab[:, 0] + 120, lab[:, 1] + 140], 1).astype(np.int32)
+    say("misplaced bands", len(sel))
+    full, ff, _, _ = run_plan(pb)
+    out, form, st, _ = run_plan(pb, bands=bands)
+    say("fallback", st.fallback_pairs, "of", len(sel))
+    assert st.fallback_pairs >= 0.9 * len(sel)
+    fb = form == 3
+    assert np.array_equal(out[fb], full[fb])
+
+
+def torch_loss_grad(params, X, T, w):
+    import torch
+    p = torch.tensor(params, dtype=torch.float32, requires_grad=True)
+    o = 0
+    W1 = p[o:o + 64 * 36].view(64, 36); o += 64 * 36
+    b1 = p[o:o + 64]; o += 64
+    W2 = p[o:o + 64 * 64].view(64, 64); o += 64 * 64
+    b2 = p[o:o + 64]; o += 64
+    W3 = p[o:o + 128].view(2, 64); o += 128
+    b3 = p[o:o + 2]
+    x = torch.tensor(X)
+    t = torch.tensor(T)
+    h1 = torch.relu(x @ W1.T + b1)
+    h2 = torch.relu(h1 @ W2.T + b2)
+    y = h2 @ W3.T + b3
+    e = y - t
+    loss = 0.5 * (w[0] * e[:, 0] ** 2 + w[1] * e[:, 1] ** 2).mean()
+    loss.backward()
+    return float(loss), p.grad.numpy()
+
+
+@pytest.mark.gpu
+def test_mlp_gradients_match_torch_reference():
+    rng = np.random.default_rng(4)
+    X = rng.random((300, 36)).astype(np.float32)
+    T = rng.normal(size=(300, 2)).astype(np.float32)
+    m = band.BandModel(seed=11)
+    p = m.params()
+    loss, grad = m.loss_grad(X, T)
+    rl, rg = torch_loss_grad(p, X, T, band.LOSS_W)
+    say("loss", loss, rl, "max grad diff", float(np.abs(grad - rg).max()), float(np.abs(rg).max()))
+    assert loss == pytest.approx(rl, rel=1e-4)
+    assert np.allclose(grad, rg, rtol=1e-3, atol=1e-5 * np.abs(rg).max())
+    Y = m.predict(X)
+    import torch
+    with torch.no_grad():
+        o = 0
+        pt = torch.tensor(p)
+        W1 = pt[o:o + 2304].view(64, 36); b1 = pt[2304:2368]
+        W2 = pt[2368:6464].view(64, 64); b2 = pt[6464:6528]
+        W3 = pt[6528:6656].view(2, 64); b3 = pt[6656:6658]
+        ref = (torch.relu(torch.relu(torch.tensor(X) @ W1.T + b1) @ W2.T + b2) @ W3.T + b3).numpy()
+    assert np.allclose(Y, ref, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_mlp_training_reduces_loss():
+    rng = np.random.default_rng(5)
+    X = rng.random((4096, 36)).astype(np.float32)
+    T = np.stack([X[:, 4:8].sum(1) - X[:, 0], np.log2(1 + 8 * X[:, 2])], 1).astype(np.float32)
+    m = band.BandModel(seed=3)
+    losses = m.fit(X, T, epochs=30, batch=256, lr=3e-3)
+    say("losses", losses[0], losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
+
+
+@pytest.mark.gpu
+def test_learned_band_beats_heuristic_band(tmp_path):
+    """Ablation: the trained network against the same banded kernel with a
+    non-learned band (a fixed half-width around the seed anchor)."""
+    data = band.build_training_set(n_pairs=30_000)
+    say("training set", data[0].shape, data[4])
+    model, rep = band.train(epochs=25, data=data, out=tmp_path / "m.npz")
+    say("train losses", rep.losses[0], rep.losses[-1])
+    assert rep.losses[-1] < 0.5 * rep.losses[0]
+    ev = synth.generate(1, seed=synth.SEED_BASE + 999, count=30)
+    pb = pairhmm.PackedBatch.from_synth(ev)
+    full, _, stf, _ = run_plan(pb)
+    res = {}
+    with pairhmm.Plan(pb) as plan:
+        X, anchor = band.plan_features(plan)
+    for name, kw in (("learned", dict(model=model, margin=4, max_width=256)),
+                     ("heuristic16", dict(bands=band.heuristic_bands(X, anchor, 16))),
+                     ("heuristic64", dict(bands=band.heuristic_bands(X, anchor, 64)))):
+        out, form, st, used = run_plan(pb, **kw)
+        err = float(np.abs(out - full).max())
+        res[name] = (err, st.fallback_pairs, st.cells_swept)
+        say(name, "max |dlog10|", err, "fallback", st.fallback_pairs, "cells", st.cells_swept,
+            "of", stf.cells_swept)
+    for name, (err, fb, cells) in res.items():
+        assert err <= 1e-3, name
+    # the trained network does the same job with less work than the
+    # non-learned bands (total FP32 cells, banded sweeps plus fallbacks)
+    assert res["learned"][2] < min(res["heuristic16"][2], res["heuristic64"][2])
+    loaded = band.BandModel.load(tmp_path / "m.npz")
+    assert np.array_equal(loaded.params(), model.params())
