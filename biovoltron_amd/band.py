@@ -6,7 +6,11 @@ are computed on the device per pair (csrc/band_kernels.hip:band_features_kernel)
 Training labels are the narrowest diagonal span holding 99.999 % of each
 pair's posterior mass, from an FP64 forward-backward pass (bv_band_labels).
 
-Targets: T0 = (label centre - anchor) / 16, T1 = log2(label width).
+Targets: T0 = (label centre - anchor) / 16, T1 = log2(label width).  At run
+time (csrc/band_kernels.hip:band_from_prediction_kernel) a pair whose
+predicted width plus margins fits gets the banded kernel's 64 diagonals
+centred on the predicted centre; the banded kernel judges every band by its
+edge-zone touch mass and sends rejected pairs to the full matrix.
 A trained model is saved as an .npz with a format version, the generator
 workload/seeds it was trained on, the epoch count and a SHA-256 of the
 parameters; `train()` can resume from such a file.
@@ -166,7 +170,11 @@ def set_plan_bands(plan: pairhmm.Plan, bands: np.ndarray | None) -> None:
     _check(_lib().bv_plan_set_bands(plan._h, b.ctypes.data))
 
 
-def set_plan_model(plan: pairhmm.Plan, model: BandModel | None, margin=8, max_width=256) -> None:
+def set_plan_model(plan: pairhmm.Plan, model: BandModel | None, margin=4, max_width=48) -> None:
+    """Bands from the network, computed on the device at every run: 64
+    diagonals centred on the predicted posterior span, for pairs whose
+    predicted span plus `margin` diagonals per side is at most `max_width`
+    wide; other pairs run on the full matrix."""
     _check(_lib().bv_plan_set_band_model(plan._h, model.handle if model else None, margin, max_width))
 
 
@@ -195,26 +203,17 @@ def targets(lab: np.ndarray, anchor: np.ndarray) -> np.ndarray:
     return np.stack([t0, t1], 1).astype(np.float32)
 
 
-def seeds_support(X: np.ndarray, dc: np.ndarray, hw: np.ndarray) -> np.ndarray:
-    """The device rule (band_kernels.hip:seeds_support): >= 2 seed hits and
-    every hit within 64 bp of the anchor inside [dc - hw, dc + hw]."""
-    R = np.rint(X[:, 0] * 256).astype(np.int64)
-    nseeds = np.where(R >= 12, np.minimum(64, R - 11), 0)
-    hist = X[:, 4:36]
-    lo = -64 + 4 * np.arange(32)
-    inside_bins = (lo[None, :] >= (dc - hw)[:, None]) & ((lo + 3)[None, :] <= (dc + hw)[:, None])
-    tot = hist.sum(1)
-    inside = (hist * inside_bins).sum(1)
-    return (tot * nseeds >= 1.5) & (inside >= tot - 1e-6)
+BAND_LANES = 64   # diagonals the banded kernel evaluates per pair
+HAP_MAX = 1024    # longest haplotype the banded kernel stages
 
 
-def heuristic_bands(X: np.ndarray, anchor: np.ndarray, half_width: int) -> np.ndarray:
-    """The non-learned band: a fixed half-width around the seed anchor, under
-    the same seed-support rule as the learned band."""
+def heuristic_bands(X: np.ndarray, anchor: np.ndarray) -> np.ndarray:
+    """The band without the network: the banded kernel's 64 diagonals centred
+    on the seed anchor, for every pair with seed hits (the same pairs the
+    learned band considers)."""
     H = np.rint(X[:, 1] * 512).astype(np.int64)
-    hw = np.full(len(X), half_width, np.int64)
-    ok = (H <= 1024) & seeds_support(X, np.zeros(len(X), np.int64), hw)
-    b = np.stack([anchor - half_width, anchor + half_width], 1).astype(np.int32)
+    ok = (H <= HAP_MAX) & (X[:, 4:36].sum(1) > 0)
+    b = np.stack([anchor - BAND_LANES // 2, anchor + BAND_LANES // 2 - 1], 1).astype(np.int32)
     b[~ok] = (1, 0)
     return b
 
@@ -262,6 +261,9 @@ def build_training_set(n_pairs=200_000, workload=1, seed=synth.SEED_BASE + 301, 
     t2 = time.perf_counter()
     lab = labels(batch, n_threads=n_threads)
     t3 = time.perf_counter()
+    # pairs without seed hits have no anchor and always run on the full matrix
+    keep = X[:, 4:36].sum(1) > 0
+    X, anchor, lab = X[keep], anchor[keep], lab[keep]
     T = targets(lab, anchor)
     return X, T, anchor, lab, dict(generate=t1 - t0, features=t2 - t1, labels=t3 - t2)
 

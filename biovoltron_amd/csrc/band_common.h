@@ -3,6 +3,7 @@
 
 #include <cstdint>
 
+#include "pairhmm_band.h"
 #include "pairhmm_common.h"
 
 namespace bvp {
@@ -11,7 +12,7 @@ constexpr int kFeat = 36;   // features per pair
 constexpr int kBins = 32;   // seed-offset histogram bins
 constexpr int kHid = 64;    // hidden width
 constexpr int kParams = kHid * kFeat + kHid + kHid * kHid + kHid + 2 * kHid + 2;  // 6658
-constexpr int kBandMaxHap = 1024;  // longest haplotype the feature kernel indexes
+constexpr int kBandMaxHap = kBandHapMax;  // longest haplotype the feature kernel indexes
 
 struct FeatArgs {
   const uint8_t* rclass;

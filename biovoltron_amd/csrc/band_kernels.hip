@@ -16,6 +16,7 @@
 #include <climits>
 
 #include "band_common.h"
+#include "pairhmm_band.h"
 #include "pairhmm_common.h"
 
 namespace bvp {
@@ -338,28 +339,12 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
   params[p] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// Bands from predictions: [anchor + dc - hw, anchor + dc + hw], where
-// dc = 16 * Y0, the width is 2^Y1 (clamped) and hw = width / 2 + margin.
-// A band is only used when the seeds support it: at least two seed hits, and
-// every hit within 64 bp of the anchor lies inside the band.  Otherwise, or
-// when the haplotype is longer than the feature table, or the band is wider
-// than max_width, the pair stays on the full matrix.
-__device__ __forceinline__ bool seeds_support(const float* x, int dc, int hw) {
-  const int R = static_cast<int>(x[0] * 256.f + 0.5f);
-  const int nseeds = R >= kK ? min(64, R - kK + 1) : 0;
-  float tot = 0.f, inside = 0.f;
-#pragma unroll
-  for (int b = 0; b < kBins; ++b) {
-    const float v = x[4 + b];
-    tot += v;
-    // bin b holds offsets anchor - 64 + [4b, 4b + 3]
-    const int lo = -64 + 4 * b, hi = lo + 3;
-    if (lo >= dc - hw && hi <= dc + hw) inside += v;
-  }
-  const float hits = tot * static_cast<float>(nseeds);
-  return hits >= 1.5f && inside >= tot - 1e-6f;
-}
-
+// Bands from predictions.  The network predicts the posterior span's centre,
+// anchor + 16 * Y0, and its log2 width, Y1.  The banded kernel evaluates 64
+// diagonals centred there; a pair whose predicted span plus `margin` diagonals
+// on each side is wider than `max_width` stays on the full matrix, as do pairs
+// without seed hits (no anchor) and haplotypes longer than the banded kernel
+// stages.
 __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const int* __restrict__ anchor,
                                             const float* __restrict__ X, int64_t n, int margin,
                                             int max_width, int2* __restrict__ band) {
@@ -367,16 +352,17 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
   if (i >= n) return;
   const float* x = X + i * kFeat;
   const int H = static_cast<int>(x[1] * 512.f + 0.5f);
+  float hits = 0.f;
+#pragma unroll
+  for (int b = 0; b < kBins; ++b) hits += x[4 + b];
   const float lw = fminf(fmaxf(Y[i * 2 + 1], 0.f), 12.f);
-  const int width = static_cast<int>(ceilf(exp2f(lw)));
-  const int hw = (width + 1) / 2 + margin;
-  const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
-  if (H > kBandMaxHap || 2 * hw + 1 > max_width || !seeds_support(x, dc, hw)) {
+  const int width = static_cast<int>(ceilf(exp2f(lw))) + 2 * margin;
+  if (H > kBandHapMax || hits <= 0.f || width > max_width) {
     band[i] = int2{1, 0};
     return;
   }
-  const int c = anchor[i] + dc;
-  band[i] = int2{c - hw, c + hw};
+  const int c = anchor[i] + static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
+  band[i] = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
 }
 
 }  // namespace bvp

@@ -12,7 +12,7 @@ struct alignas(8) Task {
   int32_t read;
   int32_t hap0;
   int32_t nhap;
-  int32_t band;  // index into the band table, or -1 for the full matrix
+  int32_t band;  // -2: a pair the banded kernel rejected; otherwise unused
   int64_t out0;
 };
 
@@ -34,7 +34,6 @@ constexpr int kRowsPerStripe = 63;// lane 0 carries the previous stripe's last r
 constexpr int kBlock = 64;        // anti-diagonal steps per staging block
 constexpr int kSlots = 8;         // match classes per haplotype column
 constexpr int kCarryPad = 192;    // extra carry entries past the haplotype end
-constexpr int kEdgeCols = 4;      // columns per side measured for the band edge mass
 
 // Rescue threshold of the reference driver (pairhmm_common.h:16).
 constexpr float kMinAccepted = 1e-28f;
@@ -84,13 +83,7 @@ struct KArgs {
   int renorm;   // FP64: rescale the carry row after every stripe
   int out_raw;  // write the scaled sum S instead of log10 L (testcase ABI)
   unsigned long long* rescue_cells;  // FP32 pass: sum of R*H over rescued pairs
-  // Banded evaluation (FP32 pass): per output pair the diagonal band
-  // [o_lo, o_hi] of j - r, or o_lo > o_hi for the full matrix.
-  const int2* band;
-  float band_tol;        // reject the band when the edge-mass ratio exceeds this
-  Task* fallback;        // rejected banded pairs, re-run on the full matrix
-  int* fallback_count;
-  unsigned long long* cells_done;  // cells actually swept (rows x columns)
+  unsigned long long* cells_done;    // cells computed (R*H per pair)
 };
 
 }  // namespace bvp

@@ -14,9 +14,13 @@
 
 #include "band_common.h"
 #include "bv_pairhmm.h"
+#include "pairhmm_band.h"
 #include "pairhmm_common.h"
 
 namespace bvp {
+hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
+hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st);
+int band_blocks_per_cu();
 hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st);
 hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
                               float* H2, hipStream_t st);
@@ -115,7 +119,7 @@ struct DeviceCtx {
   bool ready = false;
   int device = 0;
   int cus = 0;
-  int bpc_f32 = 1, bpc_f64 = 1;
+  int bpc_f32 = 1, bpc_f64 = 1, bpc_band = 1;
   std::string arch;
   HostTables host{};
   float* d_ph2pr_f = nullptr;
@@ -156,6 +160,7 @@ int ensure_init_locked(int device) {
   HIP_TRY(hipMemcpy(g_ctx.d_err_scalar, h.err_scalar, sizeof(h.err_scalar), hipMemcpyHostToDevice));
   g_ctx.bpc_f32 = max_blocks_per_cu<float>();
   g_ctx.bpc_f64 = max_blocks_per_cu<double>();
+  g_ctx.bpc_band = band_blocks_per_cu();
   g_ctx.ready = true;
   return BV_OK;
 }
@@ -233,7 +238,7 @@ struct bv_plan {
   int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
   int hmax = 0;
   int n_tasks = 0;
-  int grid32 = 0, grid64 = 0;
+  int grid32 = 0, grid64 = 0, grid_band = 0;
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -247,6 +252,7 @@ struct bv_plan {
   Task* d_rescue = nullptr;
   Task* d_fallback = nullptr;
   int2* d_band = nullptr;
+  BandItem* d_items = nullptr;  // banded work list, built on the device each run
   // learned bands: features and predictions computed on the device each run
   float* d_X = nullptr;
   int* d_anchor = nullptr;
@@ -255,9 +261,10 @@ struct bv_plan {
   bool has_model = false;
   int band_margin = 8, band_max_width = 256;
   bool has_band = false;
-  int64_t band_pairs = 0;
-  // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] fallback count,
-  // [4] fallback queue, [6..7] rescued cells (u64), [8..9] swept cells (u64)
+  // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
+  // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
+  // cells (u64), [8..9] swept cells (u64), [10] banded item count, [11] its
+  // queue
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -275,7 +282,7 @@ void free_plan(bv_plan* p) {
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_rescue, p->d_counters,
-                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band,
+                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -352,7 +359,6 @@ KArgs base_args(const bv_plan* p) {
   a.renorm = p->opt.fp64_renorm;
   a.rescue_cells = p->d_rescue_cells;
   a.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
-  a.band_tol = p->opt.band_tol;
   return a;
 }
 
@@ -381,7 +387,37 @@ int ensure_feature_buffers(bv_plan* p) {
     return fail(BV_ENOMEM, "prediction buffer allocation failed");
   if (!p->d_band && hipMalloc(&p->d_band, sizeof(int2) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band buffer allocation failed");
+  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
   return BV_OK;
+}
+
+BArgs band_args(const bv_plan* p) {
+  BArgs b{};
+  b.rclass = p->d_rclass;
+  b.q = p->d_q;
+  b.ins = p->d_ins;
+  b.del = p->d_del;
+  b.gcp = p->d_gcp;
+  b.read_off = p->d_read_off;
+  b.hmask = p->d_hmask;
+  b.hap_off = p->d_hap_off;
+  b.tasks = p->d_tasks;
+  b.n_tasks = p->n_tasks;
+  b.band = p->d_band;
+  b.items = p->d_items;
+  b.item_count = p->d_counters + 10;
+  b.counter = p->d_counters + 11;
+  b.full = p->d_fallback;
+  b.full_count = p->d_counters + 3;
+  b.ph2pr = g_ctx.d_ph2pr_f;
+  b.mm = g_ctx.d_mm_f;
+  b.out = p->d_out;
+  b.form = p->d_form;
+  b.tol = p->opt.band_tol;
+  b.accepted = p->d_counters + 5;
+  b.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
+  return b;
 }
 
 int enqueue_run(bv_plan* p, bool timed) {
@@ -396,8 +432,14 @@ int enqueue_run(bv_plan* p, bool timed) {
     HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_X, p->n_pairs, p->band_margin,
                                         p->band_max_width, p->d_band, p->stream));
   }
-  if (fp32) {
-    // Pass 1: every pair in FP32, banded where a band is set.
+  if (fp32 && p->has_band) {
+    // Pass 1: pairs with an eligible band on the banded kernel; the rest,
+    // and every band it rejects, go to the full-matrix list.
+    const BArgs b = band_args(p);
+    HIP_TRY(launch_band_dispatch(b, p->stream));
+    HIP_TRY(launch_band_forward(b, p->grid_band, p->stream));
+  } else if (fp32) {
+    // Pass 1: every pair in FP32 on the full matrix.
     KArgs a = base_args(p);
     a.tasks = p->d_tasks;
     a.n_tasks = p->n_tasks;
@@ -406,16 +448,11 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.rescue_count = p->d_counters + 2;
     a.carry = p->d_carry32;
     a.carry_stride = p->stride32;
-    if (p->has_band) {
-      a.band = p->d_band;
-      a.fallback = p->d_fallback;
-      a.fallback_count = p->d_counters + 3;
-    }
     HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
   if (fp32 && p->has_band) {
-    // Pass 2: rejected bands on the full matrix (FP32, same rescue rule).
+    // Pass 2: the full-matrix list (FP32, same rescue rule).
     KArgs a = base_args(p);
     a.tasks = p->d_fallback;
     a.n_tasks = 0;
@@ -610,6 +647,8 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid32 = opt.grid_f32 > 0 ? opt.grid_f32 : g_ctx.cus * bpc32;
   p->grid64 = opt.grid_f64 > 0 ? opt.grid_f64 : g_ctx.cus * bpc64;
   p->grid32 = std::max(1, std::min<int>(p->grid32, std::max(1, p->n_tasks)));
+  p->grid_band = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_band, p->n_pairs)));
   const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
   p->grid64 = std::max(1, static_cast<int>(std::min<int64_t>(p->grid64, std::max<int64_t>(1, max_fp64_tasks))));
   p->stride32 = hmax + kCarryPad;
@@ -706,7 +745,7 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->grid_f64 = p->grid64;
   s->last_ms_f32 = p->last_ms32;
   s->last_ms_f64 = p->last_ms64;
-  s->band_pairs = p->has_band ? p->band_pairs : 0;
+  s->band_pairs = p->has_band ? cnt[5] : 0;
   s->fallback_pairs = p->has_band ? cnt[3] : 0;
   s->cells_swept = static_cast<int64_t>(swept);
   s->last_ms_fallback = p->last_msfb;
@@ -719,16 +758,15 @@ int bv_plan_set_bands(bv_plan* p, const int32_t* bands) {
   HIP_TRY(hipSetDevice(g_ctx.device));
   if (!bands) {
     p->has_band = false;
-    p->band_pairs = 0;
     return BV_OK;
   }
-  if (!p->d_band) HIP_TRY(hipMalloc(&p->d_band, sizeof(int2) * std::max<int64_t>(1, p->n_pairs)));
-  int64_t nb = 0;
-  for (int64_t k = 0; k < p->n_pairs; ++k) nb += bands[2 * k] <= bands[2 * k + 1];
+  const int64_t n = std::max<int64_t>(1, p->n_pairs);
+  if (!p->d_band) HIP_TRY(hipMalloc(&p->d_band, sizeof(int2) * n));
+  if (!p->d_items) HIP_TRY(hipMalloc(&p->d_items, sizeof(BandItem) * n));
   HIP_TRY(hipMemcpyAsync(p->d_band, bands, sizeof(int2) * p->n_pairs, hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
+  p->has_model = false;
   p->has_band = true;
-  p->band_pairs = nb;
   return BV_OK;
 }
 
@@ -1043,7 +1081,6 @@ int bv_plan_set_band_model(bv_plan* p, const bv_band_model* m, int margin, int m
   p->has_band = true;
   p->band_margin = margin;
   p->band_max_width = max_width;
-  p->band_pairs = p->n_pairs;
   return BV_OK;
 }
 

@@ -192,8 +192,7 @@ __device__ __forceinline__ void row_params(const KArgs& a, int64_t ro, int R, in
 // S (as a double) and the power-of-two exponent removed by renormalisation.
 template <typename T>
 __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int lane,
-                               int read, int hap, int& shift_total, int o_lo, int o_hi,
-                               bool banded, float& edge_ratio, unsigned long long& cells) {
+                               int read, int hap, int& shift_total) {
   using N = Num<T>;
   const int64_t ro = a.read_off[read];
   const int R = static_cast<int>(a.read_off[read + 1] - ro);
@@ -201,41 +200,16 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
   const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
   const int nstripes = (R + kRowsPerStripe - 1) / kRowsPerStripe;
   shift_total = 0;
-  edge_ratio = 0.f;
-  cells = 0;
   T scale = T(1);
   double acc = 0.0;
-  // Columns the previous stripe's last row was computed on; the carry is
-  // zero outside them (cells outside a band are taken as zero).
-  int prev_lo = 0, prev_hi = H;
   for (int st = 0; st < nstripes; ++st) {
     const int r0 = kRowsPerStripe * st + 1;
     const bool last = (st == nstripes - 1);
     const int L = last ? (R - r0 + 1) : kRowsPerStripe;
     const int r = r0 - 1 + lane;
-    // Column range of this stripe: the full row, or the union of the band's
-    // columns over its rows.  Lane l at step t works on column t - l + base.
-    int c_lo = 1, c_hi = H;
-    if (banded) {
-      c_lo = max(1, r0 + o_lo);
-      c_hi = min(H, r0 + L - 1 + o_hi);
-      if (c_lo > c_hi) {  // the band misses the matrix on these rows
-        edge_ratio = 1.f;
-        return 0.0;
-      }
-    }
-    const int base = c_lo - 1;
-    const int r_last = r0 + L - 1;
-    // Edge windows of the read's last row, next to the band's two edges where
-    // those are not the matrix boundary.  On the last row the forward mass is
-    // the exact posterior of the alignment's end column, so mass near an edge
-    // means the alignment runs along it.  (Forward mass on earlier rows is not
-    // a posterior and is not used.)
-    const int el_lo = r_last + o_lo, el_hi = r_last + o_lo + kEdgeCols - 1;
-    const bool left_cut = banded && last && el_lo > 1;
-    const int er_lo = c_hi - kEdgeCols + 1;
-    const bool right_cut = banded && last && c_hi < H;
-    cells += static_cast<unsigned long long>(L) * static_cast<unsigned long long>(c_hi - base);
+    // Lane l at step t works on column t - l.
+    const int c_hi = H;
+    const int base = 0;
     T pmis, dp, my, yy, mm_n, im_n, dm_n, mx_n, xx_n;
     int cls;
     row_params<T>(a, ro, R, r, lane, pmis, dp, my, yy, cls, mm_n, im_n, dm_n, mx_n, xx_n);
@@ -251,7 +225,7 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
       if (st == 0) {
         pre_z = z_row0;
         pre_j = T(0);
-      } else if (c >= prev_lo && c <= prev_hi) {
+      } else if (c <= H) {
         const Pair2<T> v = carry[c];
         pre_z = v.z * scale;
         pre_j = v.j * scale;
@@ -263,7 +237,6 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
     }
     T Za = T(0), Zb = T(0), J1 = T(0), M1 = T(0), D1 = T(0);
     T maxabs = T(0);
-    double row_tot = 0.0, row_edge = 0.0;
     const _Float16* selp = &s.win[kBlock - lane][cls];
     Pair2<T>* outp = &s.cout[kBlock - L];
     for (int b = 0; b < nb; ++b) {
@@ -284,7 +257,7 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
         if (st == 0) {
           pre_z = z_row0;
           pre_j = T(0);
-        } else if (c >= prev_lo && c <= prev_hi) {
+        } else if (c <= H) {
           const Pair2<T> v = carry[c];
           pre_z = v.z * scale;
           pre_j = v.j * scale;
@@ -311,28 +284,14 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
       const int c = kBlock * b - L + lane + base;
       const Pair2<T> v = s.cout[kBlock - L + lane];
       if (c >= base && c <= c_hi) {
-        const double mass = last ? static_cast<double>(v.j)
-                                 : static_cast<double>(v.z) + static_cast<double>(v.j);
         if (!last) {
           carry[c] = v;
           maxabs = fmax(maxabs, fmax(fabs(v.z), fabs(v.j)));
         } else if (c >= 1) {
-          acc += mass;
-        }
-        if (banded && c >= 1) {
-          row_tot += mass;
-          if ((left_cut && c >= el_lo && c <= el_hi) || (right_cut && c >= er_lo)) row_edge += mass;
+          acc += static_cast<double>(v.j);
         }
       }
     }
-    if (banded && last) {
-      const double tot = wave_sum(row_tot);
-      const double edge = wave_sum(row_edge);
-      const float ratio = tot > 0.0 ? static_cast<float>(edge / tot) : 1.f;
-      edge_ratio = fmaxf(edge_ratio, ratio);
-    }
-    prev_lo = base;
-    prev_hi = c_hi;
     if (!last) {
       // Make this stripe's carry row visible to the next stripe's loads.
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -370,24 +329,15 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
     for (int h = 0; h < tk.nhap; ++h) {
       int shift = 0;
       const int64_t o = tk.out0 + h;
-      int o_lo = 0, o_hi = -1;
-      if (sizeof(T) == 4 && a.band != nullptr && tk.band >= 0) {
-        const int2 bd = a.band[o];
-        o_lo = bd.x;
-        o_hi = bd.y;
-      }
-      const bool banded = o_lo <= o_hi;
-      float edge_ratio = 0.f;
-      unsigned long long cells = 0;
-      const double S = pair_forward<T>(a, s, carry, lane, tk.read, tk.hap0 + h, shift, o_lo, o_hi,
-                                       banded, edge_ratio, cells);
+      const double S = pair_forward<T>(a, s, carry, lane, tk.read, tk.hap0 + h, shift);
       if (lane == 0) {
-        if (a.cells_done != nullptr) atomicAdd(a.cells_done, cells);
-        if (banded && !(edge_ratio <= a.band_tol)) {
-          const int k = atomicAdd(a.fallback_count, 1);
-          a.fallback[k] = Task{tk.read, tk.hap0 + h, 1, -2, o};
-          a.form[o] = kFormPending;
-        } else if (a.out_raw) {
+        if (a.cells_done != nullptr) {
+          const unsigned long long cells =
+              static_cast<unsigned long long>(a.read_off[tk.read + 1] - a.read_off[tk.read]) *
+              static_cast<unsigned long long>(a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h]);
+          atomicAdd(a.cells_done, cells);
+        }
+        if (a.out_raw) {
           a.out[o] = ldexp(S, -shift);
           a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
         } else if constexpr (sizeof(T) == 4) {
@@ -401,7 +351,7 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
             atomicAdd(a.rescue_cells, cells);
           } else {
             a.out[o] = log10(S) - Num<T>::log10K;
-            a.form[o] = banded ? kFormBandFP32 : (tk.band == -2 ? kFormBandFallback : kFormFP32);
+            a.form[o] = tk.band == -2 ? kFormBandFallback : kFormFP32;
           }
         } else {
           a.out[o] = log10(S) - static_cast<double>(shift) * 0.30102999566398120 - Num<T>::log10K;
