@@ -115,7 +115,7 @@ def main():
                     help="trained band model; the learned-band leg is skipped when absent")
     ap.add_argument("--no-band", action="store_true")
     ap.add_argument("--band-margin", type=int, default=4)
-    ap.add_argument("--band-max-width", type=int, default=256)
+    ap.add_argument("--band-max-width", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run a plain device-to-device copy of this size, so byte "
                          "counters can be scaled against a kernel of known traffic")

@@ -52,7 +52,7 @@ def _lib():
         lib.bv_band_model_loss_grad.argtypes = [vp, vp, vp, i64, C.c_float, C.c_float, vp, vp]
         lib.bv_band_model_train.argtypes = [vp, vp, vp, i64, C.c_int, C.c_int, C.c_float,
                                             C.c_float, C.c_float, vp, vp]
-        lib.bv_plan_features.argtypes = [vp, vp, vp]
+        lib.bv_plan_features.argtypes = [vp, vp, vp, vp]
         lib.bv_plan_set_band_model.argtypes = [vp, vp, C.c_int, C.c_int]
         lib.bv_plan_bands.argtypes = [vp, vp]
         lib.bv_plan_set_bands.argtypes = [vp, vp]
@@ -152,13 +152,16 @@ class BandModel:
             pass
 
 
-def plan_features(plan: pairhmm.Plan):
-    """(X [n_pairs, 36], anchor [n_pairs]) computed on the device."""
+def plan_features(plan: pairhmm.Plan, with_ends: bool = False):
+    """(X [n_pairs, 36], anchor [n_pairs]) computed on the device, and with
+    with_ends the seed-anchored read span [n_pairs, 2] (first position, last
+    end of seeds hitting within 8 of the anchor; -1, -1 without)."""
     n = plan.batch.n_pairs
     X = np.zeros((n, N_FEAT), np.float32)
     a = np.zeros(n, np.int32)
-    _check(_lib().bv_plan_features(plan._h, X.ctypes.data, a.ctypes.data))
-    return X, a
+    e = np.zeros((n, 2), np.int32)
+    _check(_lib().bv_plan_features(plan._h, X.ctypes.data, a.ctypes.data, e.ctypes.data))
+    return (X, a, e) if with_ends else (X, a)
 
 
 def set_plan_bands(plan: pairhmm.Plan, bands: np.ndarray | None) -> None:
@@ -171,10 +174,11 @@ def set_plan_bands(plan: pairhmm.Plan, bands: np.ndarray | None) -> None:
 
 
 def set_plan_model(plan: pairhmm.Plan, model: BandModel | None, margin=4, max_width=48) -> None:
-    """Bands from the network, computed on the device at every run: 64
-    diagonals centred on the predicted posterior span, for pairs whose
-    predicted span plus `margin` diagonals per side is at most `max_width`
-    wide; other pairs run on the full matrix."""
+    """Bands from the network, computed on the device at every run: 32 or
+    64 diagonals centred on the predicted posterior span, chosen by the
+    predicted span plus `margin` diagonals per side (32 when that fits between
+    the narrow band's edge zones, 64 up to `max_width`); wider predictions run
+    on the full matrix."""
     _check(_lib().bv_plan_set_band_model(plan._h, model.handle if model else None, margin, max_width))
 
 
@@ -207,14 +211,41 @@ BAND_LANES = 64   # diagonals the banded kernel evaluates per pair
 HAP_MAX = 1024    # longest haplotype the banded kernel stages
 
 
-def heuristic_bands(X: np.ndarray, anchor: np.ndarray) -> np.ndarray:
-    """The band without the network: the banded kernel's 64 diagonals centred
-    on the seed anchor, for every pair with seed hits (the same pairs the
-    learned band considers)."""
+def heuristic_bands(X: np.ndarray, anchor: np.ndarray, width: int = BAND_LANES) -> np.ndarray:
+    """The band without the network: `width` (32 or 64) diagonals centred on
+    the seed anchor, for every pair with seed hits (the same pairs the learned
+    band considers)."""
     H = np.rint(X[:, 1] * 512).astype(np.int64)
     ok = (H <= HAP_MAX) & (X[:, 4:36].sum(1) > 0)
-    b = np.stack([anchor - BAND_LANES // 2, anchor + BAND_LANES // 2 - 1], 1).astype(np.int32)
+    b = np.stack([anchor - width // 2, anchor + width // 2 - 1], 1).astype(np.int32)
     b[~ok] = (1, 0)
+    return b
+
+
+NARROW = 32      # diagonals of the narrow band (two pairs per wavefront)
+EDGE = 4         # edge-zone diagonals per side
+END_SLACK = 8    # read bases the end-anchoring seeds may leave uncovered
+
+
+def narrow_ok(X: np.ndarray, ends: np.ndarray) -> np.ndarray:
+    """The seed conditions of the narrow band (band_kernels.hip,
+    band_from_prediction_kernel): every seed hit within 64 of the anchor
+    inside the narrow band's edge zones, and seeds hitting next to the anchor
+    reaching within END_SLACK bases of both read ends."""
+    R = np.rint(X[:, 0] * 256).astype(np.int64)
+    k_in = NARROW // 2 - EDGE
+    blo = -64 + 4 * np.arange(32)
+    outside = (blo < -k_in) | (blo + 3 > k_in - 1)
+    hits_in = ~((X[:, 4:36] > 0) & outside[None, :]).any(1)
+    return hits_in & (ends[:, 0] >= 0) & (ends[:, 0] <= END_SLACK) & (ends[:, 1] >= R - END_SLACK)
+
+
+def rule_bands(X: np.ndarray, anchor: np.ndarray, ends: np.ndarray) -> np.ndarray:
+    """The learned band's rules without the network: narrow where the seed
+    conditions hold, otherwise 64 diagonals, both centred on the anchor."""
+    b = heuristic_bands(X, anchor, BAND_LANES)
+    nar = narrow_ok(X, ends) & (b[:, 0] <= b[:, 1])
+    b[nar] = np.stack([anchor - NARROW // 2, anchor + NARROW // 2 - 1], 1)[nar]
     return b
 
 

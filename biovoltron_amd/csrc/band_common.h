@@ -14,16 +14,23 @@ constexpr int kHid = 64;    // hidden width
 constexpr int kParams = kHid * kFeat + kHid + kHid * kHid + kHid + 2 * kHid + 2;  // 6658
 constexpr int kBandMaxHap = kBandHapMax;  // longest haplotype the feature kernel indexes
 
+constexpr int kSeedK = 12;         // seed length
+constexpr int kEndSlack = 8;       // read bases a seed hit may leave unanchored at either end
+constexpr int kEndAnchorTol = 8;   // |hit offset - anchor| for an end-anchoring seed
+
 struct FeatArgs {
   const uint8_t* rclass;
   const uint8_t* q;
   const int64_t* read_off;
   const uint8_t* hmask;
   const int64_t* hap_off;
+  const uint32_t* hkey;  // per haplotype position: its 12-mer code, or ~0 (N or past the end)
   const Task* tasks;
   int n_tasks;
   float* X;      // [pairs][kFeat]
   int* anchor;   // [pairs]
+  int2* ends;    // [pairs]: first seed start, last seed end (read positions) among
+                 // hits within kEndAnchorTol of the anchor; (-1, -1) without hits
 };
 
 }  // namespace bvp

@@ -21,14 +21,25 @@
 
 namespace bvp {
 
-constexpr int kK = 12;                    // seed length
-constexpr int kHashSlots = 2048;          // LDS hash slots (haplotypes up to kBandMaxHap)
+constexpr int kK = kSeedK;
+constexpr int kSeedSlots = 128;           // LDS hash slots for one read's <= 64 seeds
 constexpr unsigned long long kEmpty = ~0ull;
+constexpr uint32_t kNoKey = ~0u;
 
 struct FeatLds {
-  unsigned long long table[kHashSlots];  // key (low 32 bits: 24-bit k-mer) | position << 32
-  uint8_t hcode[kBandMaxHap + 16];
+  unsigned long long table[kSeedSlots];  // seed key (24-bit k-mer) | seed lane << 32
+  int first_q[64];                       // per seed: first haplotype position of its k-mer
+  int cnt[64];                           // offset counts (median, histogram)
 };
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
 
 __device__ __forceinline__ int code_of_mask(uint8_t m) {
   // single-class masks 1,2,4,8 -> 0..3; anything else (N, 0) -> -1
@@ -42,10 +53,33 @@ __device__ __forceinline__ int code_of_mask(uint8_t m) {
 }
 
 __device__ __forceinline__ unsigned hash24(unsigned k) {
-  return (k * 2654435761u) >> (32 - 11);  // 11 bits -> 2048 slots
+  return (k * 2654435761u) >> (32 - 7);  // 7 bits -> 128 slots
 }
 
-// One wavefront per task (a read and its region's haplotypes).
+// 12-mer code of every haplotype position (one block per haplotype).
+__global__ __launch_bounds__(256) void hap_keys_kernel(const uint8_t* __restrict__ hmask,
+                                                       const int64_t* __restrict__ hap_off,
+                                                       int64_t n_haps, uint32_t* __restrict__ hkey) {
+  for (int64_t h = blockIdx.x; h < n_haps; h += gridDim.x) {
+    const int64_t ho = hap_off[h];
+    const int H = static_cast<int>(hap_off[h + 1] - ho);
+    for (int q = threadIdx.x; q < H; q += blockDim.x) {
+      uint32_t key = 0;
+      bool ok = q + kK <= H;
+      for (int k = 0; ok && k < kK; ++k) {
+        const int c = code_of_mask(hmask[ho + q + k]);
+        ok = c >= 0;
+        key = (key << 2) | static_cast<uint32_t>(c & 3);
+      }
+      hkey[ho + q] = ok ? key : kNoKey;
+    }
+  }
+}
+
+// One wavefront per task (a read and its region's haplotypes).  The read's
+// seeds (<= 64 12-mers spread over the read) go into a small LDS hash table;
+// every haplotype is scanned once, and each seed keeps the first haplotype
+// position of its k-mer.
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
   __shared__ FeatLds s;
   const int lane = threadIdx.x;
@@ -87,7 +121,19 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
         }
       }
     }
-    int nvalid = __popcll(__ballot(my_p >= 0));
+    const int nvalid = __popcll(__ballot(my_p >= 0));
+    __syncthreads();
+    for (int i = lane; i < kSeedSlots; i += 64) s.table[i] = kEmpty;
+    __syncthreads();
+    if (my_p >= 0) {
+      unsigned slot = hash24(my_key);
+      const unsigned long long entry =
+          static_cast<unsigned long long>(my_key) | (static_cast<unsigned long long>(lane) << 32);
+      for (int probe = 0; probe < kSeedSlots; ++probe) {
+        if (atomicCAS(&s.table[slot], kEmpty, entry) == kEmpty) break;
+        slot = (slot + 1) & (kSeedSlots - 1);
+      }
+    }
     for (int h = 0; h < tk.nhap; ++h) {
       const int hap = tk.hap0 + h;
       const int64_t o = tk.out0 + h;
@@ -105,69 +151,79 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       int found_off = INT_MIN;
       if (H <= kBandMaxHap && nvalid > 0) {
         __syncthreads();
-        for (int i = lane; i < kHashSlots; i += 64) s.table[i] = kEmpty;
-        for (int q = lane; q < H; q += 64) s.hcode[q] = static_cast<uint8_t>(code_of_mask(a.hmask[ho + q]) & 0xff);
+        s.first_q[lane] = INT_MAX;
         __syncthreads();
         for (int q = lane; q + kK <= H; q += 64) {
-          unsigned key = 0;
-          bool ok = true;
-          for (int k = 0; k < kK; ++k) {
-            const int c = static_cast<int8_t>(s.hcode[q + k]);
-            if (c < 0) {
-              ok = false;
-              break;
-            }
-            key = (key << 2) | static_cast<unsigned>(c);
-          }
-          if (!ok) continue;
+          const uint32_t key = a.hkey[ho + q];
+          if (key == kNoKey) continue;
           unsigned slot = hash24(key);
-          const unsigned long long entry = static_cast<unsigned long long>(key) |
-                                           (static_cast<unsigned long long>(q) << 32);
-          for (int probe = 0; probe < kHashSlots; ++probe) {
-            const unsigned long long prev = atomicCAS(&s.table[slot], kEmpty, entry);
-            if (prev == kEmpty || static_cast<unsigned>(prev) == key) break;
-            slot = (slot + 1) & (kHashSlots - 1);
+          for (int probe = 0; probe < kSeedSlots; ++probe) {
+            const unsigned long long e = s.table[slot];
+            if (e == kEmpty) break;
+            if (static_cast<uint32_t>(e) == key) atomicMin(&s.first_q[static_cast<int>(e >> 32)], q);
+            slot = (slot + 1) & (kSeedSlots - 1);
           }
         }
         __syncthreads();
-        if (my_p >= 0) {
-          unsigned slot = hash24(my_key);
-          for (int probe = 0; probe < kHashSlots; ++probe) {
-            const unsigned long long e = s.table[slot];
-            if (e == kEmpty) break;
-            if (static_cast<unsigned>(e) == my_key) {
-              found_off = static_cast<int>(e >> 32) - my_p;
-              break;
-            }
-            slot = (slot + 1) & (kHashSlots - 1);
-          }
-        }
+        if (my_p >= 0 && s.first_q[lane] != INT_MAX) found_off = s.first_q[lane] - my_p;
       }
       // median of the hit offsets as the anchor
       const bool hit = found_off != INT_MIN;
       const int nhit = __popcll(__ballot(hit));
       if (nhit > 0) {
-        int below = 0, equal = 0;
-        for (int l = 0; l < 64; ++l) {
-          const int v = __shfl(found_off, l);
-          const bool vh = __shfl(static_cast<int>(hit), l) != 0;
-          if (vh && hit) {
-            below += v < found_off;
-            equal += v == found_off;
-          }
+        int mn = hit ? found_off : INT_MAX, mx = hit ? found_off : INT_MIN;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          mn = min(mn, __shfl_xor(mn, off));
+          mx = max(mx, __shfl_xor(mx, off));
         }
         const int half = (nhit - 1) / 2;
-        const bool is_median = hit && below <= half && half < below + equal;
-        const unsigned long long mb = __ballot(is_median);
-        anchor = __shfl(found_off, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
+        if (mx - mn < 64) {
+          // counting: offsets mn + [0, 64)
+          s.cnt[lane] = 0;
+          __syncthreads();
+          if (hit) atomicAdd(&s.cnt[found_off - mn], 1);
+          __syncthreads();
+          const int c = s.cnt[lane];
+          const int incl = wave_incl_scan(c, lane);
+          const unsigned long long past = __ballot(incl > half);
+          anchor = mn + static_cast<int>(__ffsll(static_cast<long long>(past)) - 1);
+        } else {
+          int below = 0, equal = 0;
+          for (int l = 0; l < 64; ++l) {
+            const int v = __shfl(found_off, l);
+            const bool vh = __shfl(static_cast<int>(hit), l) != 0;
+            if (vh && hit) {
+              below += v < found_off;
+              equal += v == found_off;
+            }
+          }
+          const bool is_median = hit && below <= half && half < below + equal;
+          const unsigned long long mb = __ballot(is_median);
+          anchor = __shfl(found_off, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
+        }
       }
       // 32 bins of 4 bp over [anchor - 64, anchor + 64)
       const int bin = hit ? (found_off - anchor + 64) >> 2 : -1;
       const float inv = nvalid > 0 ? 1.f / static_cast<float>(nvalid) : 0.f;
-      int cnt = 0;
-      for (int l = 0; l < 64; ++l) cnt += __shfl(bin, l) == lane;
-      if (lane < kBins) X[4 + lane] = static_cast<float>(cnt) * inv;
-      if (lane == 0) a.anchor[o] = anchor;
+      __syncthreads();
+      if (lane < kBins) s.cnt[lane] = 0;
+      __syncthreads();
+      if (bin >= 0 && bin < kBins) atomicAdd(&s.cnt[bin], 1);
+      __syncthreads();
+      if (lane < kBins) X[4 + lane] = static_cast<float>(s.cnt[lane]) * inv;
+      // read ends anchored by seeds that hit next to the anchor
+      const bool near = hit && abs(found_off - anchor) <= kEndAnchorTol;
+      int pf = near ? my_p : INT_MAX, pl = near ? my_p + kK : -1;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        pf = min(pf, __shfl_xor(pf, off));
+        pl = max(pl, __shfl_xor(pl, off));
+      }
+      if (lane == 0) {
+        a.anchor[o] = anchor;
+        a.ends[o] = pl >= 0 ? int2{pf, pl} : int2{-1, -1};
+      }
     }
   }
 }
@@ -340,17 +396,36 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
 }
 
 // Bands from predictions.  The network predicts the posterior span's centre,
-// anchor + 16 * Y0, and its log2 width, Y1.  The banded kernel evaluates 64
-// diagonals centred there; a pair whose predicted span plus `margin` diagonals
-// on each side is wider than `max_width` stays on the full matrix, as do pairs
-// without seed hits (no anchor) and haplotypes longer than the banded kernel
-// stages.
+// anchor + 16 * Y0, and its log2 width, Y1; with `margin` diagonals added on
+// each side that is the predicted span.  It gets:
+//  - 32 diagonals centred on the anchor when the span fits between the narrow
+//    band's edge zones, the centre is within 4 of the anchor, every seed hit
+//    within 64 of the anchor lies inside those zones, and seeds that hit next
+//    to the anchor reach within kEndSlack bases of both read ends (a read end
+//    that matches nowhere is emitted as a run of insertions whose diagonals
+//    sweep up to its length away from the alignment, out of sight of the
+//    kernel's edge-zone test);
+//  - otherwise 64 diagonals centred on the predicted centre when the span is
+//    at most `max_width` wide;
+//  - otherwise the full matrix, as for pairs without seed hits (no anchor) and
+//    haplotypes longer than the banded kernel stages.
+__device__ __forceinline__ bool hits_inside(const float* x, int lo, int hi) {
+#pragma unroll
+  for (int b = 0; b < kBins; ++b) {
+    const int blo = -64 + 4 * b;
+    if (x[4 + b] > 0.f && (blo < lo || blo + 3 > hi)) return false;
+  }
+  return true;
+}
+
 __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const int* __restrict__ anchor,
-                                            const float* __restrict__ X, int64_t n, int margin,
-                                            int max_width, int2* __restrict__ band) {
+                                            const int2* __restrict__ ends, const float* __restrict__ X,
+                                            int64_t n, int margin, int max_width,
+                                            int2* __restrict__ band) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   const float* x = X + i * kFeat;
+  const int R = static_cast<int>(x[0] * 256.f + 0.5f);
   const int H = static_cast<int>(x[1] * 512.f + 0.5f);
   float hits = 0.f;
 #pragma unroll
@@ -361,14 +436,33 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
     band[i] = int2{1, 0};
     return;
   }
-  const int c = anchor[i] + static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
-  band[i] = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
+  const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
+  const int2 e = ends[i];
+  constexpr int kIn = kBandNarrow / 2 - kBandEdge;  // narrow interior: [-kIn, kIn - 1]
+  const bool narrow = width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
+                      hits_inside(x, -kIn, kIn - 1) && e.x >= 0 && e.x <= kEndSlack &&
+                      e.y >= R - kEndSlack;
+  if (narrow) {
+    band[i] = int2{anchor[i] - kBandNarrow / 2, anchor[i] + kBandNarrow / 2 - 1};
+  } else {
+    const int c = anchor[i] + dc;
+    band[i] = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
+  }
 }
 
 }  // namespace bvp
 
 // ---- launchers ---------------------------------------------------------------
 namespace bvp {
+
+hipError_t launch_hap_keys(const uint8_t* hmask, const int64_t* hap_off, int64_t n_haps,
+                           uint32_t* hkey, hipStream_t st) {
+  const int64_t blocks = std::min<int64_t>(std::max<int64_t>(n_haps, 1), 65536);
+  if (n_haps > 0)
+    hipLaunchKernelGGL(hap_keys_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, hmask,
+                       hap_off, n_haps, hkey);
+  return hipGetLastError();
+}
 
 hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st) {
   hipLaunchKernelGGL(band_features_kernel, dim3(grid), dim3(64), 0, st, a);
@@ -405,11 +499,13 @@ hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, f
   return hipGetLastError();
 }
 
-hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const float* X, int64_t n,
-                                       int margin, int max_width, int2* band, hipStream_t st) {
+hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
+                                       const float* X, int64_t n, int margin, int max_width,
+                                       int2* band, hipStream_t st) {
   const int64_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(band_from_prediction_kernel, dim3(static_cast<unsigned>(std::max<int64_t>(1, blocks))),
-                     dim3(256), 0, st, Y, anchor, X, n, margin, max_width, band);
+  if (blocks > 0)
+    hipLaunchKernelGGL(band_from_prediction_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                       Y, anchor, ends, X, n, margin, max_width, band);
   return hipGetLastError();
 }
 

@@ -7,11 +7,13 @@
 
 namespace bvp {
 
-constexpr int kBandLanes = 64;    // diagonals evaluated per banded pair (one per lane)
+constexpr int kBandLanes = 64;    // wavefront width; the wide band's diagonals (one per lane)
+constexpr int kBandNarrow = 32;   // the narrow band: two pairs of one read per wavefront
 constexpr int kBandEdge = 4;      // diagonals per side whose touch mass is measured
 constexpr int kBandHapMax = 1024; // longest haplotype the banded kernel stages in LDS
 
-// One banded pair: diagonals j - r in [o_lo, o_lo + 63].
+// One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
+// list items 2u and 2u+1 share a read; hap < 0 marks padding.
 struct alignas(8) BandItem {
   int32_t read;
   int32_t hap;
@@ -27,6 +29,7 @@ struct BArgs {
   const uint8_t* ins;
   const uint8_t* del;
   const uint8_t* gcp;
+  const uint8_t* gcp_const;  // per read: 1 if the gap continuation is constant along it
   const int64_t* read_off;
   // haplotypes
   const uint8_t* hmask;
@@ -35,10 +38,14 @@ struct BArgs {
   const Task* tasks;
   int n_tasks;
   const int2* band;
-  // banded work list (dispatch output, band kernel input)
+  // banded work lists (dispatch output, band kernel input): 64-wide bands,
+  // and 32-wide bands in read-sharing pairs
   BandItem* items;
   int* item_count;
-  int* counter;       // band kernel work queue
+  int* counter;       // work queue of the 64-wide list
+  BandItem* items32;
+  int* item32_count;
+  int* counter32;     // work queue of the 32-wide list (counts item pairs)
   // pairs for the full-matrix FP32 pass (ineligible or rejected)
   Task* full;
   int* full_count;

@@ -65,13 +65,16 @@ struct alignas(16) RowRec {
 
 struct BandLds {
   RowRec tab[kBandLanes];
-  uint8_t hap[kBandHapMax + 8];  // hap[j] = class mask of column j for 1 <= j <= H; 0 at j = 0, H+1
+  // per lane group: hap[g][j] = class mask of column j for 1 <= j <= H; 0 at j = 0, H+1
+  uint8_t hap[2][kBandHapMax + 8];
 };
 
-// Inclusive decayed scan across the wavefront: returns y with
-// y_k = sum_{m>=0} yy^m x_{k-m}.  f15 is yy^(k%16+1) on DPP rows 1 and 3 and
-// 0 on rows 0 and 2; f31 is yy^(k-31) on rows 2 and 3 and 0 below (the two
-// broadcasts then add exactly the lower rows' prefixes).
+// Inclusive decayed scan over each G-lane group: returns y with
+// y_k = sum_{m>=0} yy^m x_{k-m}, k and k-m in the same group.  Four row_shr
+// levels scan each 16-lane DPP row; then row_bcast:15 adds the lower row's
+// prefix times f15 = yy^(k%16+1) (0 on rows 0 and 2), and for G = 64
+// row_bcast:31 adds the lower half's prefix times f31 = yy^(k-31) (0 below).
+template <int G>
 __device__ __forceinline__ float decay_scan(float x, float yy1, float yy2, float yy4, float yy8,
                                             float f15, float f31) {
   x = __builtin_fmaf(dpp<kRowShr + 1>(x), yy1, x);
@@ -79,7 +82,7 @@ __device__ __forceinline__ float decay_scan(float x, float yy1, float yy2, float
   x = __builtin_fmaf(dpp<kRowShr + 4>(x), yy4, x);
   x = __builtin_fmaf(dpp<kRowShr + 8>(x), yy8, x);
   x = __builtin_fmaf(dpp<kRowBcast15>(x), f15, x);
-  x = __builtin_fmaf(dpp<kRowBcast31>(x), f31, x);
+  if constexpr (G == 64) x = __builtin_fmaf(dpp<kRowBcast31>(x), f31, x);
   return x;
 }
 
@@ -101,93 +104,111 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& tot) {
   return x - v;
 }
 
-__device__ __forceinline__ bool band_eligible(const BArgs& a, const Task& tk, int h, bool gcp_const,
-                                              int2& b) {
+// 0: full matrix, 32 or 64: the banded list the pair goes to.
+__device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h, bool gcp_const,
+                                          int2& b) {
   b = a.band[tk.out0 + h];
   const int hap = tk.hap0 + h;
   const int64_t H = a.hap_off[hap + 1] - a.hap_off[hap];
-  return gcp_const && b.x <= b.y && static_cast<int64_t>(b.y) - b.x + 1 <= kBandLanes &&
-         H <= kBandHapMax && b.x >= -(1 << 26) && b.y <= (1 << 26);
+  if (!gcp_const || b.x > b.y || H > kBandHapMax || b.x < -(1 << 26) || b.y > (1 << 26)) return 0;
+  const int64_t w = static_cast<int64_t>(b.y) - b.x + 1;
+  return w <= kBandNarrow ? kBandNarrow : (w <= kBandLanes ? kBandLanes : 0);
 }
 
 }  // namespace
 
-// Splits every task's pairs into banded items and full-matrix tasks.  One
-// thread per task; appends are reserved once per wavefront.
+// Splits every task's pairs into the two banded lists and full-matrix tasks.
+// One thread per task; appends are reserved once per wavefront.  A task's
+// narrow items are written in pairs (padded with an item of hap -1).
 __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   Task tk{};
   bool gcp_const = false;
-  int nb = 0, nf = 0;
+  int n64 = 0, n32 = 0, nf = 0;
   if (t < a.n_tasks) {
     tk = a.tasks[t];
-    const int64_t ro = a.read_off[tk.read];
-    const int64_t R = a.read_off[tk.read + 1] - ro;
-    const uint8_t c0 = a.gcp[ro] & 127;
-    gcp_const = true;
-    for (int64_t k = 1; k < R; ++k) {
-      if ((a.gcp[ro + k] & 127) != c0) {
-        gcp_const = false;
-        break;
-      }
-    }
+    gcp_const = a.gcp_const[tk.read] != 0;
     for (int h = 0; h < tk.nhap; ++h) {
       int2 b;
-      if (band_eligible(a, tk, h, gcp_const, b)) ++nb; else ++nf;
+      const int c = band_class(a, tk, h, gcp_const, b);
+      n64 += c == kBandLanes;
+      n32 += c == kBandNarrow;
+      nf += c == 0;
     }
   }
-  int tb = 0, tf = 0;
-  const int ob = wave_excl_scan(nb, lane, tb);
+  const int n32p = (n32 + 1) & ~1;
+  int t64 = 0, t32 = 0, tf = 0;
+  const int o64 = wave_excl_scan(n64, lane, t64);
+  const int o32 = wave_excl_scan(n32p, lane, t32);
   const int of = wave_excl_scan(nf, lane, tf);
-  int bb = 0, bf = 0;
+  int b64 = 0, b32 = 0, bf = 0;
   if (lane == 0) {
-    if (tb) bb = atomicAdd(a.item_count, tb);
+    if (t64) b64 = atomicAdd(a.item_count, t64);
+    if (t32) b32 = atomicAdd(a.item32_count, t32);
     if (tf) bf = atomicAdd(a.full_count, tf);
   }
-  bb = __shfl(bb, 0) + ob;
+  b64 = __shfl(b64, 0) + o64;
+  b32 = __shfl(b32, 0) + o32;
   bf = __shfl(bf, 0) + of;
   if (t < a.n_tasks) {
     for (int h = 0; h < tk.nhap; ++h) {
       int2 b;
       const int64_t o = tk.out0 + h;
-      if (band_eligible(a, tk, h, gcp_const, b)) {
+      const int c = band_class(a, tk, h, gcp_const, b);
+      if (c) {
         const int w = b.y - b.x + 1;
-        a.items[bb++] = BandItem{tk.read, tk.hap0 + h, b.x - ((kBandLanes - w) >> 1), 0, o};
+        const BandItem it{tk.read, tk.hap0 + h, b.x - ((c - w) >> 1), 0, o};
+        if (c == kBandLanes) a.items[b64++] = it; else a.items32[b32++] = it;
       } else {
         a.full[bf++] = Task{tk.read, tk.hap0 + h, 1, -1, o};
       }
     }
+    if (n32 & 1) a.items32[b32] = BandItem{tk.read, -1, 0, 0, -1};
   }
 }
 
+// Banded forward over one list.  G = 64: one pair per wavefront.  G = 32: the
+// two pairs 2u, 2u+1 (one read, two haplotypes) in lanes 0-31 and 32-63.
+template <int G>
 __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
+  constexpr int P = kBandLanes / G;
   __shared__ BandLds s;
   const int lane = threadIdx.x;
-  const int n_items = *a.item_count;
-  const bool left_lane = lane < kBandEdge;
-  const bool right_lane = lane >= kBandLanes - kBandEdge;
+  const int grp = lane / G;     // pair slot of this lane
+  const int k = lane % G;       // lane within the slot: diagonal o_lo + k
+  const BandItem* items = G == kBandLanes ? a.items : a.items32;
+  const int n_units = G == kBandLanes ? *a.item_count : (*a.item32_count) / 2;
+  int* queue = G == kBandLanes ? a.counter : a.counter32;
+  const bool left_lane = k < kBandEdge;
+  const bool right_lane = k >= G - kBandEdge;
+  const bool bottom_lane = k == 0;
+  const bool top_lane = k == G - 1;
   const int row16 = lane >> 4;
   int accepted = 0;
   unsigned long long cells = 0;
   for (;;) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(a.counter, 1);
-    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-    if (it >= n_items) break;
-    const BandItem w = a.items[it];
-    const int64_t ro = a.read_off[w.read];
-    const int R = static_cast<int>(a.read_off[w.read + 1] - ro);
-    const int64_t ho = a.hap_off[w.hap];
-    const int H = static_cast<int>(a.hap_off[w.hap + 1] - ho);
-    const int o_lo = w.o_lo;
-    const int d = o_lo + lane;  // this lane's diagonal
+    int u = 0;
+    if (lane == 0) u = atomicAdd(queue, 1);
+    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0));
+    if (u >= n_units) break;
+    const BandItem w = items[P * u + grp];
+    const bool live = w.hap >= 0;
+    const int read = __builtin_amdgcn_readfirstlane(w.read);
+    const int hap = live ? w.hap : items[P * u].hap;  // padding mirrors slot 0
+    const int64_t ro = a.read_off[read];
+    const int R = static_cast<int>(a.read_off[read + 1] - ro);
+    const int64_t ho = a.hap_off[hap];
+    const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+    const int o_lo = live ? w.o_lo : items[P * u].o_lo;
+    const int d = o_lo + k;  // this lane's diagonal
+    uint8_t* hapL = s.hap[grp];
 
-    __syncthreads();  // the previous pair's LDS reads are done
-    for (int c = lane; c < H; c += kBandLanes) s.hap[1 + c] = a.hmask[ho + c];
-    if (lane == 0) {
-      s.hap[0] = 0;
-      s.hap[1 + H] = 0;
+    __syncthreads();  // the previous unit's LDS reads are done
+    for (int c = k; c < H; c += G) hapL[1 + c] = a.hmask[ho + c];
+    if (k == 0) {
+      hapL[0] = 0;
+      hapL[1 + H] = 0;
     }
 
     // Decay factors of the D scan (constant gap continuation along the read).
@@ -199,14 +220,14 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     // Edge zones: the left one from the first row whose left neighbour column
     // is inside the matrix, the right one while the right neighbour column is.
     const int ez_from = left_lane ? 2 - o_lo : INT_MAX;
-    const int ez_to = right_lane ? H - o_lo - kBandLanes : INT_MIN;
+    const int ez_to = right_lane ? H - o_lo - G : INT_MIN;
 
     // Row 0: M = I = 0, D = K/H for columns 0..H (Context.h:109, template :164-175).
     float Mp = 0.f, Ip = 0.f;
     float Dp = (d >= 0 && d <= H) ? kK / static_cast<float>(H) : 0.f;
     float TMp = 0.f, TIp = 0.f, TDp = 0.f;
     {
-      const bool ez = (left_lane && o_lo - 1 >= 0) || (right_lane && o_lo + kBandLanes <= H);
+      const bool ez = (left_lane && o_lo - 1 >= 0) || (right_lane && o_lo + G <= H);
       if (ez) TDp = Dp;
     }
 
@@ -237,13 +258,13 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
       // Software pipeline: the LDS reads for row u+1 are issued before row u
       // is computed.
       RowRec rr_next = s.tab[0];
-      uint32_t hm_next = s.hap[min(max(i0 + d, 0), H + 1)];
-      for (int u = 0; u < n; ++u) {
-        const int i = i0 + u;
+      uint32_t hm_next = hapL[min(max(i0 + d, 0), H + 1)];
+      for (int v = 0; v < n; ++v) {
+        const int i = i0 + v;
         const RowRec rr = rr_next;
         const uint32_t hm = hm_next;
-        rr_next = s.tab[(u + 1) & (kBandLanes - 1)];
-        hm_next = s.hap[min(max(i + 1 + d, 0), H + 1)];
+        rr_next = s.tab[(v + 1) & (kBandLanes - 1)];
+        hm_next = hapL[min(max(i + 1 + d, 0), H + 1)];
         const float prior = (hm & rr.rmask) ? rr.pm : rr.pe;
         const bool ez = i >= ez_from || i <= ez_to;
 
@@ -252,8 +273,14 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
         t = __builtin_fmaf(Dp, rr.g, t);
         t = __builtin_fmaf(Mp, rr.mm, t);
         const float Mn = t * prior;
-        const float In = __builtin_fmaf(dpp<kWaveShl1>(Mp), rr.mx, dpp<kWaveShl1>(Ip) * rr.xx);
-        const float Dn = decay_scan(dpp<kWaveShr1>(Mn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
+        float In = __builtin_fmaf(dpp<kWaveShl1>(Mp), rr.mx, dpp<kWaveShl1>(Ip) * rr.xx);
+        float x = dpp<kWaveShr1>(Mn) * rr.my;
+        if constexpr (G != kBandLanes) {
+          // the wavefront shifts cross the slot boundary: cut them
+          if (top_lane) In = 0.f;
+          if (bottom_lane) x = 0.f;
+        }
+        const float Dn = decay_scan<G>(x, yy, yy2, yy4, yy8, f15, f31);
 
         // touch mass
         float tt = TIp * rr.g;
@@ -265,7 +292,12 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           TMn = Mn;
           TIn = In;
         }
-        float TDn = decay_scan(dpp<kWaveShr1>(TMn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
+        float tx = dpp<kWaveShr1>(TMn) * rr.my;
+        if constexpr (G != kBandLanes) {
+          if (top_lane) TIn = 0.f;
+          if (bottom_lane) tx = 0.f;
+        }
+        float TDn = decay_scan<G>(tx, yy, yy2, yy4, yy8, f15, f31);
         if (ez) TDn = Dn;
 
         Mp = Mn;
@@ -279,23 +311,35 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     // S = sum over the last row of M + I (template :328-343), columns 1..H.
     const int j = R + d;
     const bool in = j >= 1 && j <= H;
-    const double S = wsum(in ? static_cast<double>(Mp) + static_cast<double>(Ip) : 0.0);
-    const double Tm = wsum(in ? static_cast<double>(TMp) + static_cast<double>(TIp) : 0.0);
-    cells += static_cast<unsigned long long>(R) * kBandLanes;
-    if (lane == 0) {
+    double S = in ? static_cast<double>(Mp) + static_cast<double>(Ip) : 0.0;
+    double Tm = in ? static_cast<double>(TMp) + static_cast<double>(TIp) : 0.0;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      S += __shfl_xor(S, o);
+      Tm += __shfl_xor(Tm, o);
+    }
+    if (live) cells += static_cast<unsigned long long>(R) * G;
+    if (k == 0 && live) {
       if (S >= static_cast<double>(kMinAccepted) && Tm <= static_cast<double>(a.tol) * S) {
         a.out[w.out] = log10(S) - kLog10K;
         a.form[w.out] = kFormBandFP32;
         ++accepted;
       } else {
-        const int k = atomicAdd(a.full_count, 1);
-        a.full[k] = Task{w.read, w.hap, 1, -2, w.out};
+        const int q = atomicAdd(a.full_count, 1);
+        a.full[q] = Task{read, hap, 1, -2, w.out};
       }
     }
   }
+  // per-lane tallies live in the slot leaders (lanes 0 and 32)
+  int acc_w = accepted;
+  unsigned long long cells_w = cells;
+  if constexpr (P == 2) {
+    acc_w += __shfl_xor(acc_w, 32);
+    cells_w = __shfl(cells, 0) + __shfl(cells, 32);
+  }
   if (lane == 0) {
-    if (accepted) atomicAdd(a.accepted, accepted);
-    if (cells && a.cells_done) atomicAdd(a.cells_done, cells);
+    if (acc_w) atomicAdd(a.accepted, acc_w);
+    if (cells_w && a.cells_done) atomicAdd(a.cells_done, cells_w);
   }
 }
 
@@ -306,13 +350,15 @@ hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(band_forward_kernel, dim3(grid), dim3(kBandLanes), 0, st, a);
+  hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(grid), dim3(kBandLanes), 0, st, a);
+  hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid), dim3(kBandLanes), 0, st, a);
   return hipGetLastError();
 }
 
 int band_blocks_per_cu() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel, kBandLanes, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel<kBandLanes>, kBandLanes, 0) !=
+      hipSuccess)
     return 8;
   return n > 0 ? n : 1;
 }

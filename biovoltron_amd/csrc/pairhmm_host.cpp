@@ -30,8 +30,11 @@ hipError_t launch_mlp_backward(const float* params, const float* X, const float*
                                hipStream_t st);
 hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, float lr, int t,
                        hipStream_t st);
-hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const float* X, int64_t n,
-                                       int margin, int max_width, int2* band, hipStream_t st);
+hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
+                                       const float* X, int64_t n, int margin, int max_width,
+                                       int2* band, hipStream_t st);
+hipError_t launch_hap_keys(const uint8_t* hmask, const int64_t* hap_off, int64_t n_haps,
+                           uint32_t* hkey, hipStream_t st);
 template <typename T>
 hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st);
 template <typename T>
@@ -245,6 +248,7 @@ struct bv_plan {
   uint8_t* d_ins = nullptr;
   uint8_t* d_del = nullptr;
   uint8_t* d_gcp = nullptr;
+  uint8_t* d_gcp_const = nullptr;  // per read: 1 if its gap continuation is constant
   int64_t* d_read_off = nullptr;
   uint8_t* d_hmask = nullptr;
   int64_t* d_hap_off = nullptr;
@@ -252,10 +256,13 @@ struct bv_plan {
   Task* d_rescue = nullptr;
   Task* d_fallback = nullptr;
   int2* d_band = nullptr;
-  BandItem* d_items = nullptr;  // banded work list, built on the device each run
+  BandItem* d_items = nullptr;    // banded work lists, built on the device each run
+  BandItem* d_items32 = nullptr;
   // learned bands: features and predictions computed on the device each run
   float* d_X = nullptr;
   int* d_anchor = nullptr;
+  int2* d_ends = nullptr;
+  uint32_t* d_hkey = nullptr;  // per haplotype position: 12-mer code
   float* d_Y = nullptr;
   float* d_model = nullptr;  // MLP parameters (copy)
   bool has_model = false;
@@ -263,8 +270,8 @@ struct bv_plan {
   bool has_band = false;
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
   // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
-  // cells (u64), [8..9] swept cells (u64), [10] banded item count, [11] its
-  // queue
+  // cells (u64), [8..9] swept cells (u64), [10] wide banded item count,
+  // [11] its queue, [12] narrow banded item count, [13] its queue
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -280,10 +287,11 @@ namespace {
 void free_plan(bv_plan* p) {
   if (!p) return;
   // d_rescue_cells points into d_counters and is not freed on its own.
-  void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
+  void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_gcp_const, p->d_read_off,
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
-                  p->d_X, p->d_anchor, p->d_Y, p->d_model};
+                  p->d_items32,
+                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_hkey};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
@@ -373,8 +381,20 @@ hipError_t launch_features(bv_plan* p) {
   f.n_tasks = p->n_tasks;
   f.X = p->d_X;
   f.anchor = p->d_anchor;
+  f.ends = p->d_ends;
+  f.hkey = p->d_hkey;
   const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 16));
   return launch_band_features(f, grid, p->stream);
+}
+
+int ensure_band_lists(bv_plan* p) {
+  const int64_t n = std::max<int64_t>(1, p->n_pairs);
+  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
+  // narrow items are padded to an even count per read
+  if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_tasks)) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
+  return BV_OK;
 }
 
 int ensure_feature_buffers(bv_plan* p) {
@@ -383,13 +403,19 @@ int ensure_feature_buffers(bv_plan* p) {
     return fail(BV_ENOMEM, "feature buffer allocation failed");
   if (!p->d_anchor && hipMalloc(&p->d_anchor, sizeof(int) * n) != hipSuccess)
     return fail(BV_ENOMEM, "anchor buffer allocation failed");
+  if (!p->d_ends && hipMalloc(&p->d_ends, sizeof(int2) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "seed end buffer allocation failed");
+  if (!p->d_hkey) {
+    // haplotype k-mer codes: fixed for the plan, computed once
+    if (hipMalloc(&p->d_hkey, sizeof(uint32_t) * std::max<int64_t>(1, p->hap_bytes)) != hipSuccess)
+      return fail(BV_ENOMEM, "haplotype key buffer allocation failed");
+    HIP_TRY(launch_hap_keys(p->d_hmask, p->d_hap_off, p->n_haps, p->d_hkey, p->stream));
+  }
   if (!p->d_Y && hipMalloc(&p->d_Y, sizeof(float) * 2 * n) != hipSuccess)
     return fail(BV_ENOMEM, "prediction buffer allocation failed");
   if (!p->d_band && hipMalloc(&p->d_band, sizeof(int2) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band buffer allocation failed");
-  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
-    return fail(BV_ENOMEM, "band work list allocation failed");
-  return BV_OK;
+  return ensure_band_lists(p);
 }
 
 BArgs band_args(const bv_plan* p) {
@@ -399,6 +425,7 @@ BArgs band_args(const bv_plan* p) {
   b.ins = p->d_ins;
   b.del = p->d_del;
   b.gcp = p->d_gcp;
+  b.gcp_const = p->d_gcp_const;
   b.read_off = p->d_read_off;
   b.hmask = p->d_hmask;
   b.hap_off = p->d_hap_off;
@@ -408,6 +435,9 @@ BArgs band_args(const bv_plan* p) {
   b.items = p->d_items;
   b.item_count = p->d_counters + 10;
   b.counter = p->d_counters + 11;
+  b.items32 = p->d_items32;
+  b.item32_count = p->d_counters + 12;
+  b.counter32 = p->d_counters + 13;
   b.full = p->d_fallback;
   b.full_count = p->d_counters + 3;
   b.ph2pr = g_ctx.d_ph2pr_f;
@@ -421,7 +451,7 @@ BArgs band_args(const bv_plan* p) {
 }
 
 int enqueue_run(bv_plan* p, bool timed) {
-  HIP_TRY(launch_zero_ints(p->d_counters, 12, p->stream));
+  HIP_TRY(launch_zero_ints(p->d_counters, 16, p->stream));
   if (p->n_tasks == 0) return BV_OK;
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   if (timed) HIP_TRY(hipEventRecord(p->ev[0], p->stream));
@@ -429,7 +459,7 @@ int enqueue_run(bv_plan* p, bool timed) {
     // Learned bands: seed features, MLP, band table -- all on the device.
     HIP_TRY(launch_features(p));
     HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
-    HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_X, p->n_pairs, p->band_margin,
+    HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_ends, p->d_X, p->n_pairs, p->band_margin,
                                         p->band_max_width, p->d_band, p->stream));
   }
   if (fp32 && p->has_band) {
@@ -604,6 +634,18 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   parallel_for(hbytes, [&](int64_t a0, int64_t a1) {
     for (int64_t i = a0; i < a1; ++i) hmask[i] = al.hap_mask[b->hap_bases[hb0 + i]];
   });
+  // Reads whose gap-continuation quality is the same at every base (the
+  // banded kernel's deletion scan needs a constant decay along the read).
+  std::vector<uint8_t> gcp_const(static_cast<size_t>(std::max<int64_t>(1, b->n_reads)), 0);
+  parallel_for(b->n_reads, [&](int64_t a0, int64_t a1) {
+    for (int64_t r = a0; r < a1; ++r) {
+      const uint8_t* c = b->read_gcp + b->read_off[r];
+      const int64_t R = b->read_off[r + 1] - b->read_off[r];
+      bool same = true;
+      for (int64_t k = 1; k < R && same; ++k) same = (c[k] & 127) == (c[0] & 127);
+      gcp_const[r] = same ? 1 : 0;
+    }
+  });
   int hmax = 0;
   for (int64_t h = 0; h < b->n_haps; ++h) hmax = std::max<int>(hmax, static_cast<int>(hoff[h + 1] - hoff[h]));
   p->hmax = hmax;
@@ -660,6 +702,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       (rc = dev_upload(&p->d_ins, b->read_ins + rb0, rbytes, st)) ||
       (rc = dev_upload(&p->d_del, b->read_del + rb0, rbytes, st)) ||
       (rc = dev_upload(&p->d_gcp, b->read_gcp + rb0, rbytes, st)) ||
+      (rc = dev_upload(&p->d_gcp_const, gcp_const.data(), b->n_reads, st)) ||
       (rc = dev_upload(&p->d_read_off, roff.data(), b->n_reads + 1, st)) ||
       (rc = dev_upload(&p->d_hmask, hmask.data(), hbytes, st)) ||
       (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st)) ||
@@ -762,7 +805,7 @@ int bv_plan_set_bands(bv_plan* p, const int32_t* bands) {
   }
   const int64_t n = std::max<int64_t>(1, p->n_pairs);
   if (!p->d_band) HIP_TRY(hipMalloc(&p->d_band, sizeof(int2) * n));
-  if (!p->d_items) HIP_TRY(hipMalloc(&p->d_items, sizeof(BandItem) * n));
+  if (int rc = ensure_band_lists(p)) return rc;
   HIP_TRY(hipMemcpyAsync(p->d_band, bands, sizeof(int2) * p->n_pairs, hipMemcpyHostToDevice, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   p->has_model = false;
@@ -1049,7 +1092,7 @@ int bv_band_model_train(bv_band_model* m, const float* X, const float* T, int64_
   return BV_OK;
 }
 
-int bv_plan_features(bv_plan* p, float* X_out, int32_t* anchor_out) {
+int bv_plan_features(bv_plan* p, float* X_out, int32_t* anchor_out, int32_t* ends_out) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
   HIP_TRY(hipSetDevice(g_ctx.device));
   int rc = ensure_feature_buffers(p);
@@ -1059,6 +1102,8 @@ int bv_plan_features(bv_plan* p, float* X_out, int32_t* anchor_out) {
     HIP_TRY(hipMemcpyAsync(X_out, p->d_X, sizeof(float) * kFeat * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
   if (anchor_out && p->n_pairs)
     HIP_TRY(hipMemcpyAsync(anchor_out, p->d_anchor, sizeof(int) * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
+  if (ends_out && p->n_pairs)
+    HIP_TRY(hipMemcpyAsync(ends_out, p->d_ends, sizeof(int2) * p->n_pairs, hipMemcpyDeviceToHost, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   return BV_OK;
 }
@@ -1093,9 +1138,10 @@ int bv_plan_bands(bv_plan* p, int32_t* bands_out) {
   return BV_OK;
 }
 
-// Band labels: the narrowest diagonal span [o_lo, o_hi] of j - r holding at
-// least `mass` of the posterior probability (M, I and D states), from an
-// FP64 forward-backward pass with per-row renormalisation on the host.
+// Band labels: the narrowest diagonal span [o_lo, o_hi] of j - r outside
+// which the alignment path makes at most 1 - mass expected visits (posterior
+// of M, I and D cells summed over rows), from an FP64 forward-backward pass
+// with per-row renormalisation on the host.
 int bv_band_labels(const bv_region_batch* b, double mass, int n_threads, int32_t* lo_hi) {
   int rc = validate(b);
   if (rc) return rc;
@@ -1207,9 +1253,12 @@ int bv_band_labels(const bv_region_batch* b, double mass, int n_threads, int32_t
           all += v;
         }
       }
-      // narrowest window holding `mass` of the posterior
+      // Narrowest window outside which the expected number of path visits is
+      // at most 1 - mass.  A path that leaves a diagonal window visits a
+      // diagonal next to it (every move changes j - r by at most one), so
+      // this bounds the probability that the alignment leaves the window.
       const int nd = R + H - 1;
-      const double need = mass * all;
+      const double need = all - (1.0 - mass);
       int best_lo = 0, best_hi = nd - 1;
       double win = 0.0;
       int lo = 0;

