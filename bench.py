@@ -114,8 +114,8 @@ def main():
     ap.add_argument("--band-model", default=str(ROOT / "biovoltron_amd" / "models" / "band_mlp.npz"),
                     help="trained band model; the learned-band leg is skipped when absent")
     ap.add_argument("--no-band", action="store_true")
-    ap.add_argument("--band-margin", type=int, default=4)
-    ap.add_argument("--band-max-width", type=int, default=48)
+    ap.add_argument("--band-margin", type=int, default=10)
+    ap.add_argument("--band-max-width", type=int, default=68)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run a plain device-to-device copy of this size, so byte "
                          "counters can be scaled against a kernel of known traffic")

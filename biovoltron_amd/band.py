@@ -173,7 +173,12 @@ def set_plan_bands(plan: pairhmm.Plan, bands: np.ndarray | None) -> None:
     _check(_lib().bv_plan_set_bands(plan._h, b.ctypes.data))
 
 
-def set_plan_model(plan: pairhmm.Plan, model: BandModel | None, margin=4, max_width=48) -> None:
+MARGIN = 10      # diagonals added per side of the predicted span (narrow band iff the
+                 # predicted span is at most 4 wide)
+MAX_WIDTH = 68   # widest predicted span (with margins) given a band
+
+
+def set_plan_model(plan: pairhmm.Plan, model: BandModel | None, margin=MARGIN, max_width=MAX_WIDTH) -> None:
     """Bands from the network, computed on the device at every run: 32 or
     64 diagonals centred on the predicted posterior span, chosen by the
     predicted span plus `margin` diagonals per side (32 when that fits between
@@ -224,28 +229,37 @@ def heuristic_bands(X: np.ndarray, anchor: np.ndarray, width: int = BAND_LANES) 
 
 NARROW = 32      # diagonals of the narrow band (two pairs per wavefront)
 EDGE = 4         # edge-zone diagonals per side
-END_SLACK = 8    # read bases the end-anchoring seeds may leave uncovered
+END_SLACK = 8    # read bases the end-anchoring seeds may leave uncovered (narrow)
+END_SLACK_WIDE = 32  # the same for the wide band
 
 
-def narrow_ok(X: np.ndarray, ends: np.ndarray) -> np.ndarray:
-    """The seed conditions of the narrow band (band_kernels.hip,
-    band_from_prediction_kernel): every seed hit within 64 of the anchor
-    inside the narrow band's edge zones, and seeds hitting next to the anchor
-    reaching within END_SLACK bases of both read ends."""
+def seeds_ok(X: np.ndarray, ends: np.ndarray, width: int, slack: int) -> np.ndarray:
+    """The seed conditions of a band of `width` diagonals centred on the
+    anchor (band_kernels.hip, band_from_prediction_kernel): every seed hit
+    within 64 of the anchor inside the band's edge zones, and seeds hitting
+    next to the anchor reaching within `slack` bases of both read ends."""
     R = np.rint(X[:, 0] * 256).astype(np.int64)
-    k_in = NARROW // 2 - EDGE
+    k_in = width // 2 - EDGE
     blo = -64 + 4 * np.arange(32)
     outside = (blo < -k_in) | (blo + 3 > k_in - 1)
     hits_in = ~((X[:, 4:36] > 0) & outside[None, :]).any(1)
-    return hits_in & (ends[:, 0] >= 0) & (ends[:, 0] <= END_SLACK) & (ends[:, 1] >= R - END_SLACK)
+    ok = hits_in & (ends[:, 0] >= 0) & (ends[:, 0] <= slack) & (ends[:, 1] >= R - slack)
+    if width == BAND_LANES:
+        ok &= 2 * (ends[:, 1] - ends[:, 0]) >= R
+    return ok
 
 
-def rule_bands(X: np.ndarray, anchor: np.ndarray, ends: np.ndarray) -> np.ndarray:
-    """The learned band's rules without the network: narrow where the seed
-    conditions hold, otherwise 64 diagonals, both centred on the anchor."""
+def rule_bands(X: np.ndarray, anchor: np.ndarray, ends: np.ndarray, narrow: bool = False) -> np.ndarray:
+    """The learned band's rules without the network (centre = anchor): wide
+    where the wide band's seed conditions hold (with `narrow`, narrow where
+    its own hold), else the full matrix."""
     b = heuristic_bands(X, anchor, BAND_LANES)
-    nar = narrow_ok(X, ends) & (b[:, 0] <= b[:, 1])
-    b[nar] = np.stack([anchor - NARROW // 2, anchor + NARROW // 2 - 1], 1)[nar]
+    ok = b[:, 0] <= b[:, 1]
+    wide = ok & seeds_ok(X, ends, BAND_LANES, END_SLACK_WIDE)
+    b[~wide] = (1, 0)
+    if narrow:
+        nar = ok & seeds_ok(X, ends, NARROW, END_SLACK)
+        b[nar] = np.stack([anchor - NARROW // 2, anchor + NARROW // 2 - 1], 1)[nar]
     return b
 
 

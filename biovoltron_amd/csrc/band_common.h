@@ -15,7 +15,8 @@ constexpr int kParams = kHid * kFeat + kHid + kHid * kHid + kHid + 2 * kHid + 2;
 constexpr int kBandMaxHap = kBandHapMax;  // longest haplotype the feature kernel indexes
 
 constexpr int kSeedK = 12;         // seed length
-constexpr int kEndSlack = 8;       // read bases a seed hit may leave unanchored at either end
+constexpr int kEndSlack = 8;       // read bases the anchoring seeds may leave uncovered (narrow band)
+constexpr int kEndSlackWide = 32;  // the same for the wide band
 constexpr int kEndAnchorTol = 8;   // |hit offset - anchor| for an end-anchoring seed
 
 struct FeatArgs {
@@ -24,7 +25,8 @@ struct FeatArgs {
   const int64_t* read_off;
   const uint8_t* hmask;
   const int64_t* hap_off;
-  const uint32_t* hkey;  // per haplotype position: its 12-mer code, or ~0 (N or past the end)
+  const unsigned long long* htab;  // per haplotype 12-mer tables (hap_table_kernel)
+  const int64_t* htab_off;         // [n_haps + 1] slot offsets
   const Task* tasks;
   int n_tasks;
   float* X;      // [pairs][kFeat]

@@ -22,24 +22,14 @@
 namespace bvp {
 
 constexpr int kK = kSeedK;
-constexpr int kSeedSlots = 128;           // LDS hash slots for one read's <= 64 seeds
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr uint32_t kNoKey = ~0u;
+constexpr int kMaxOcc = 4;    // occurrences of one seed used for the histogram
+constexpr int kHapChunk = 4;  // haplotypes whose lookups are in flight together
 
 struct FeatLds {
-  unsigned long long table[kSeedSlots];  // seed key (24-bit k-mer) | seed lane << 32
-  int first_q[64];                       // per seed: first haplotype position of its k-mer
-  int cnt[64];                           // offset counts (median, histogram)
+  int cnt[64];  // offset counts (median, histogram)
 };
-
-__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  return x;
-}
 
 __device__ __forceinline__ int code_of_mask(uint8_t m) {
   // single-class masks 1,2,4,8 -> 0..3; anything else (N, 0) -> -1
@@ -52,34 +42,62 @@ __device__ __forceinline__ int code_of_mask(uint8_t m) {
   }
 }
 
-__device__ __forceinline__ unsigned hash24(unsigned k) {
-  return (k * 2654435761u) >> (32 - 7);  // 7 bits -> 128 slots
+__device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) {
+  return (key * 2654435761u >> 8) & mask;
 }
 
-// 12-mer code of every haplotype position (one block per haplotype).
-__global__ __launch_bounds__(256) void hap_keys_kernel(const uint8_t* __restrict__ hmask,
-                                                       const int64_t* __restrict__ hap_off,
-                                                       int64_t n_haps, uint32_t* __restrict__ hkey) {
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Per haplotype, an open-addressing table of its 12-mers: entry = key |
+// position << 32, empty = ~0; haplotype h owns slots [tab_off[h],
+// tab_off[h+1]), a power of two at least twice its length.  One block per
+// haplotype, built once per plan.
+__global__ __launch_bounds__(256) void hap_table_kernel(const uint8_t* __restrict__ hmask,
+                                                        const int64_t* __restrict__ hap_off,
+                                                        const int64_t* __restrict__ tab_off,
+                                                        int64_t n_haps,
+                                                        unsigned long long* __restrict__ tab) {
   for (int64_t h = blockIdx.x; h < n_haps; h += gridDim.x) {
     const int64_t ho = hap_off[h];
     const int H = static_cast<int>(hap_off[h + 1] - ho);
-    for (int q = threadIdx.x; q < H; q += blockDim.x) {
+    unsigned long long* t = tab + tab_off[h];
+    const uint32_t mask = static_cast<uint32_t>(tab_off[h + 1] - tab_off[h] - 1);
+    for (uint32_t i = threadIdx.x; i <= mask; i += blockDim.x) t[i] = kEmpty;
+    __syncthreads();
+    // haplotypes longer than the banded kernel stages get no table (one slot)
+    for (int q = threadIdx.x; mask > 0 && q + kK <= H; q += blockDim.x) {
       uint32_t key = 0;
-      bool ok = q + kK <= H;
+      bool ok = true;
       for (int k = 0; ok && k < kK; ++k) {
         const int c = code_of_mask(hmask[ho + q + k]);
         ok = c >= 0;
         key = (key << 2) | static_cast<uint32_t>(c & 3);
       }
-      hkey[ho + q] = ok ? key : kNoKey;
+      if (!ok) continue;
+      const unsigned long long e = static_cast<unsigned long long>(key) |
+                                   (static_cast<unsigned long long>(q) << 32);
+      uint32_t sl = slot_of(key, mask);
+      for (uint32_t probe = 0; probe <= mask; ++probe) {
+        if (atomicCAS(&t[sl], kEmpty, e) == kEmpty) break;
+        sl = (sl + 1) & mask;
+      }
     }
+    __syncthreads();
   }
 }
 
-// One wavefront per task (a read and its region's haplotypes).  The read's
-// seeds (<= 64 12-mers spread over the read) go into a small LDS hash table;
-// every haplotype is scanned once, and each seed keeps the first haplotype
-// position of its k-mer.
+// One wavefront per task (a read and its region's haplotypes).  Lane l holds
+// one seed (a 12-mer, <= 64 spread over the read) and looks it up in each
+// haplotype's table: the first occurrence gives the seed's hit offset (the
+// anchor is their median), and up to kMaxOcc occurrences feed the offset
+// histogram, so repeat copies show as extra offsets.
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
   __shared__ FeatLds s;
   const int lane = threadIdx.x;
@@ -106,7 +124,7 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
     const int nseeds = R >= kK ? min(64, R - kK + 1) : 0;
     const int stride = nseeds > 0 ? max(1, (R - kK + 1 + 63) / 64) : 1;
     int my_p = -1;
-    unsigned my_key = 0;
+    uint32_t my_key = 0;
     if (lane < nseeds) {
       my_p = lane * stride;
       if (my_p + kK > R) my_p = -1;
@@ -122,19 +140,38 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       }
     }
     const int nvalid = __popcll(__ballot(my_p >= 0));
-    __syncthreads();
-    for (int i = lane; i < kSeedSlots; i += 64) s.table[i] = kEmpty;
-    __syncthreads();
-    if (my_p >= 0) {
-      unsigned slot = hash24(my_key);
-      const unsigned long long entry =
-          static_cast<unsigned long long>(my_key) | (static_cast<unsigned long long>(lane) << 32);
-      for (int probe = 0; probe < kSeedSlots; ++probe) {
-        if (atomicCAS(&s.table[slot], kEmpty, entry) == kEmpty) break;
-        slot = (slot + 1) & (kSeedSlots - 1);
+    for (int h0 = 0; h0 < tk.nhap; h0 += kHapChunk) {
+      // Table lookups for a chunk of haplotypes first (independent loads in
+      // flight together), then the per-haplotype reductions.
+      int first_c[kHapChunk];
+      int occ_c[kHapChunk][kMaxOcc];
+      int nocc_c[kHapChunk];
+#pragma unroll
+      for (int c = 0; c < kHapChunk; ++c) {
+        first_c[c] = INT_MAX;
+        nocc_c[c] = 0;
+        const int hap = tk.hap0 + h0 + c;
+        if (h0 + c >= tk.nhap || my_p < 0) continue;
+        const int64_t t0 = a.htab_off[hap], t1 = a.htab_off[hap + 1];
+        if (t1 - t0 < 2) continue;  // no table: haplotype longer than the banded kernel stages
+        const unsigned long long* t = a.htab + t0;
+        const uint32_t mask = static_cast<uint32_t>(t1 - t0 - 1);
+        uint32_t sl = slot_of(my_key, mask);
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+          const unsigned long long e = t[sl];
+          if (e == kEmpty) break;
+          if (static_cast<uint32_t>(e) == my_key) {
+            const int q = static_cast<int>(e >> 32);
+            first_c[c] = min(first_c[c], q);
+            if (nocc_c[c] < kMaxOcc) occ_c[c][nocc_c[c]++] = q - my_p;
+          }
+          sl = (sl + 1) & mask;
+        }
       }
-    }
-    for (int h = 0; h < tk.nhap; ++h) {
+#pragma unroll
+      for (int c = 0; c < kHapChunk; ++c) {
+      const int h = h0 + c;
+      if (h >= tk.nhap) break;
       const int hap = tk.hap0 + h;
       const int64_t o = tk.out0 + h;
       const int64_t ho = a.hap_off[hap];
@@ -148,25 +185,9 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
         X[3] = fq_min;
       }
       int anchor = (H - R) / 2;
-      int found_off = INT_MIN;
-      if (H <= kBandMaxHap && nvalid > 0) {
-        __syncthreads();
-        s.first_q[lane] = INT_MAX;
-        __syncthreads();
-        for (int q = lane; q + kK <= H; q += 64) {
-          const uint32_t key = a.hkey[ho + q];
-          if (key == kNoKey) continue;
-          unsigned slot = hash24(key);
-          for (int probe = 0; probe < kSeedSlots; ++probe) {
-            const unsigned long long e = s.table[slot];
-            if (e == kEmpty) break;
-            if (static_cast<uint32_t>(e) == key) atomicMin(&s.first_q[static_cast<int>(e >> 32)], q);
-            slot = (slot + 1) & (kSeedSlots - 1);
-          }
-        }
-        __syncthreads();
-        if (my_p >= 0 && s.first_q[lane] != INT_MAX) found_off = s.first_q[lane] - my_p;
-      }
+      const int found_off = first_c[c] != INT_MAX ? first_c[c] - my_p : INT_MIN;
+      const int nocc = nocc_c[c];
+      const int* occ = occ_c[c];
       // median of the hit offsets as the anchor
       const bool hit = found_off != INT_MIN;
       const int nhit = __popcll(__ballot(hit));
@@ -180,12 +201,12 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
         const int half = (nhit - 1) / 2;
         if (mx - mn < 64) {
           // counting: offsets mn + [0, 64)
+          __syncthreads();
           s.cnt[lane] = 0;
           __syncthreads();
           if (hit) atomicAdd(&s.cnt[found_off - mn], 1);
           __syncthreads();
-          const int c = s.cnt[lane];
-          const int incl = wave_incl_scan(c, lane);
+          const int incl = wave_incl_scan(s.cnt[lane], lane);
           const unsigned long long past = __ballot(incl > half);
           anchor = mn + static_cast<int>(__ffsll(static_cast<long long>(past)) - 1);
         } else {
@@ -204,12 +225,14 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
         }
       }
       // 32 bins of 4 bp over [anchor - 64, anchor + 64)
-      const int bin = hit ? (found_off - anchor + 64) >> 2 : -1;
       const float inv = nvalid > 0 ? 1.f / static_cast<float>(nvalid) : 0.f;
       __syncthreads();
       if (lane < kBins) s.cnt[lane] = 0;
       __syncthreads();
-      if (bin >= 0 && bin < kBins) atomicAdd(&s.cnt[bin], 1);
+      for (int k = 0; k < nocc; ++k) {
+        const int bin = (occ[k] - anchor + 64) >> 2;
+        if (bin >= 0 && bin < kBins) atomicAdd(&s.cnt[bin], 1);
+      }
       __syncthreads();
       if (lane < kBins) X[4 + lane] = static_cast<float>(s.cnt[lane]) * inv;
       // read ends anchored by seeds that hit next to the anchor
@@ -223,6 +246,7 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       if (lane == 0) {
         a.anchor[o] = anchor;
         a.ends[o] = pl >= 0 ? int2{pf, pl} : int2{-1, -1};
+      }
       }
     }
   }
@@ -406,9 +430,14 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
 //    sweep up to its length away from the alignment, out of sight of the
 //    kernel's edge-zone test);
 //  - otherwise 64 diagonals centred on the predicted centre when the span is
-//    at most `max_width` wide;
+//    at most `max_width` wide, every seed hit lies inside the wide band's edge
+//    zones, and the anchoring seeds cover at least half the read and reach
+//    within kEndSlackWide bases of both read ends (a lone spurious 12-mer hit
+//    can put the anchor far from the alignment);
 //  - otherwise the full matrix, as for pairs without seed hits (no anchor) and
 //    haplotypes longer than the banded kernel stages.
+// Seed hits count every occurrence in the haplotype, so a repeat copy the
+// read could align to shows up as a hit at its offset.
 __device__ __forceinline__ bool hits_inside(const float* x, int lo, int hi) {
 #pragma unroll
   for (int b = 0; b < kBins; ++b) {
@@ -420,7 +449,7 @@ __device__ __forceinline__ bool hits_inside(const float* x, int lo, int hi) {
 
 __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const int* __restrict__ anchor,
                                             const int2* __restrict__ ends, const float* __restrict__ X,
-                                            int64_t n, int margin, int max_width,
+                                            int64_t n, int margin, int max_width, int allow_narrow,
                                             int2* __restrict__ band) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -439,14 +468,19 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
   const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
   const int2 e = ends[i];
   constexpr int kIn = kBandNarrow / 2 - kBandEdge;  // narrow interior: [-kIn, kIn - 1]
-  const bool narrow = width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
+  const bool narrow = allow_narrow && width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
                       hits_inside(x, -kIn, kIn - 1) && e.x >= 0 && e.x <= kEndSlack &&
                       e.y >= R - kEndSlack;
+  constexpr int kInW = kBandLanes / 2 - kBandEdge;  // wide interior: [-kInW, kInW - 1]
+  const bool wide = hits_inside(x, dc - kInW, dc + kInW - 1) && e.x >= 0 && e.x <= kEndSlackWide &&
+                    e.y >= R - kEndSlackWide && 2 * (e.y - e.x) >= R;
   if (narrow) {
     band[i] = int2{anchor[i] - kBandNarrow / 2, anchor[i] + kBandNarrow / 2 - 1};
-  } else {
+  } else if (wide) {
     const int c = anchor[i] + dc;
     band[i] = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
+  } else {
+    band[i] = int2{1, 0};
   }
 }
 
@@ -455,12 +489,12 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
 // ---- launchers ---------------------------------------------------------------
 namespace bvp {
 
-hipError_t launch_hap_keys(const uint8_t* hmask, const int64_t* hap_off, int64_t n_haps,
-                           uint32_t* hkey, hipStream_t st) {
+hipError_t launch_hap_tables(const uint8_t* hmask, const int64_t* hap_off, const int64_t* tab_off,
+                             int64_t n_haps, unsigned long long* tab, hipStream_t st) {
   const int64_t blocks = std::min<int64_t>(std::max<int64_t>(n_haps, 1), 65536);
   if (n_haps > 0)
-    hipLaunchKernelGGL(hap_keys_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, hmask,
-                       hap_off, n_haps, hkey);
+    hipLaunchKernelGGL(hap_table_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, hmask,
+                       hap_off, tab_off, n_haps, tab);
   return hipGetLastError();
 }
 
@@ -501,11 +535,11 @@ hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, f
 
 hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
                                        const float* X, int64_t n, int margin, int max_width,
-                                       int2* band, hipStream_t st) {
+                                       int allow_narrow, int2* band, hipStream_t st) {
   const int64_t blocks = (n + 255) / 256;
   if (blocks > 0)
     hipLaunchKernelGGL(band_from_prediction_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
-                       Y, anchor, ends, X, n, margin, max_width, band);
+                       Y, anchor, ends, X, n, margin, max_width, allow_narrow, band);
   return hipGetLastError();
 }
 

@@ -10,7 +10,8 @@ namespace bvp {
 constexpr int kBandLanes = 64;    // wavefront width; the wide band's diagonals (one per lane)
 constexpr int kBandNarrow = 32;   // the narrow band: two pairs of one read per wavefront
 constexpr int kBandEdge = 4;      // diagonals per side whose touch mass is measured
-constexpr int kBandHapMax = 1024; // longest haplotype the banded kernel stages in LDS
+constexpr int kBandHapMax = 1024; // longest haplotype given a band (seed tables, features)
+constexpr int kBandReadMax = 448; // longest read given a band (LDS haplotype window)
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
 // list items 2u and 2u+1 share a read; hap < 0 marks padding.

@@ -53,20 +53,27 @@ constexpr int kRowBcast15 = 0x142;
 constexpr int kRowBcast31 = 0x143;
 
 struct alignas(16) RowRec {
-  float pm;     // 1 - e (match prior)
+  float dp;     // pm - pe, pm = 1 - e the match prior
   float pe;     // e / 3 (mismatch prior; pm for a wildcard read base)
   float mm;     // match -> match into this row
   float g;      // insert/delete -> match into this row (1 - ph2pr[gcp])
   float mx;     // match -> insert
   float xx;     // insert -> insert
   float my;     // match -> delete (along this row)
-  uint32_t rmask;  // bit of this row's read class in the haplotype masks
+  uint32_t cls; // this row's read base class (selector slot)
 };
 
+// Match selectors of the band's haplotype window, per lane group: column
+// c = i - 1 + k of the window is haplotype column j = o_lo + 1 + c (row i,
+// lane k), and sel[g][cls][c] is 1 where that column matches read class cls
+// (0 outside the haplotype).  The prior is then one mixed-precision FMA,
+// sel * (pm - pe) + pe.  tab has one spare entry so that the next row's
+// record can always be read ahead.
+constexpr int kSelCols = kBandReadMax + kBandLanes;
+template <int P>
 struct BandLds {
-  RowRec tab[kBandLanes];
-  // per lane group: hap[g][j] = class mask of column j for 1 <= j <= H; 0 at j = 0, H+1
-  uint8_t hap[2][kBandHapMax + 8];
+  RowRec tab[kBandLanes + 1];
+  _Float16 sel[P][4][kSelCols];
 };
 
 // Inclusive decayed scan over each G-lane group: returns y with
@@ -110,7 +117,10 @@ __device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h,
   b = a.band[tk.out0 + h];
   const int hap = tk.hap0 + h;
   const int64_t H = a.hap_off[hap + 1] - a.hap_off[hap];
-  if (!gcp_const || b.x > b.y || H > kBandHapMax || b.x < -(1 << 26) || b.y > (1 << 26)) return 0;
+  const int64_t R = a.read_off[tk.read + 1] - a.read_off[tk.read];
+  if (!gcp_const || b.x > b.y || H > kBandHapMax || R > kBandReadMax || b.x < -(1 << 26) ||
+      b.y > (1 << 26))
+    return 0;
   const int64_t w = static_cast<int64_t>(b.y) - b.x + 1;
   return w <= kBandNarrow ? kBandNarrow : (w <= kBandLanes ? kBandLanes : 0);
 }
@@ -173,7 +183,7 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
 template <int G>
 __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
   constexpr int P = kBandLanes / G;
-  __shared__ BandLds s;
+  __shared__ BandLds<P> s;
   const int lane = threadIdx.x;
   const int grp = lane / G;     // pair slot of this lane
   const int k = lane % G;       // lane within the slot: diagonal o_lo + k
@@ -202,13 +212,14 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
     const int o_lo = live ? w.o_lo : items[P * u].o_lo;
     const int d = o_lo + k;  // this lane's diagonal
-    uint8_t* hapL = s.hap[grp];
+    _Float16 (*selL)[kSelCols] = s.sel[grp];
 
     __syncthreads();  // the previous unit's LDS reads are done
-    for (int c = k; c < H; c += G) hapL[1 + c] = a.hmask[ho + c];
-    if (k == 0) {
-      hapL[0] = 0;
-      hapL[1 + H] = 0;
+    for (int c = k; c < R + G; c += G) {
+      const int j = o_lo + 1 + c;
+      const uint32_t m = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) selL[q][c] = ((m >> q) & 1u) ? _Float16(1) : _Float16(0);
     }
 
     // Decay factors of the D scan (constant gap continuation along the read).
@@ -242,14 +253,15 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
           const float e = a.ph2pr[a.q[p] & 127];
           RowRec rr;
-          rr.pm = 1.f - e;
-          rr.pe = (rc & kWildcard) ? rr.pm : e / 3.f;
+          const float pm = 1.f - e;
+          rr.pe = (rc & kWildcard) ? pm : e / 3.f;
+          rr.dp = pm - rr.pe;
           rr.mm = a.mm[((hi * (hi + 1)) >> 1) + lo];
           rr.g = 1.f - a.ph2pr[cq];
           rr.mx = a.ph2pr[iq];
           rr.xx = a.ph2pr[cq];
           rr.my = a.ph2pr[dq];
-          rr.rmask = 1u << (rc & 7);
+          rr.cls = rc & 3;
           s.tab[lane] = rr;
         }
       }
@@ -258,14 +270,15 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
       // Software pipeline: the LDS reads for row u+1 are issued before row u
       // is computed.
       RowRec rr_next = s.tab[0];
-      uint32_t hm_next = hapL[min(max(i0 + d, 0), H + 1)];
+      float sel_next = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr_next.cls)][i0 - 1 + k]);
       for (int v = 0; v < n; ++v) {
         const int i = i0 + v;
         const RowRec rr = rr_next;
-        const uint32_t hm = hm_next;
-        rr_next = s.tab[(v + 1) & (kBandLanes - 1)];
-        hm_next = hapL[min(max(i + 1 + d, 0), H + 1)];
-        const float prior = (hm & rr.rmask) ? rr.pm : rr.pe;
+        const float sel = sel_next;
+        rr_next = s.tab[v + 1];
+        // i + k < R + G <= kSelCols
+        sel_next = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr_next.cls)][i + k]);
+        const float prior = __builtin_fmaf(sel, rr.dp, rr.pe);
         const bool ez = i >= ez_from || i <= ez_to;
 
         // forward

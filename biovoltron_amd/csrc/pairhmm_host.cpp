@@ -32,9 +32,9 @@ hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, f
                        hipStream_t st);
 hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
                                        const float* X, int64_t n, int margin, int max_width,
-                                       int2* band, hipStream_t st);
-hipError_t launch_hap_keys(const uint8_t* hmask, const int64_t* hap_off, int64_t n_haps,
-                           uint32_t* hkey, hipStream_t st);
+                                       int allow_narrow, int2* band, hipStream_t st);
+hipError_t launch_hap_tables(const uint8_t* hmask, const int64_t* hap_off, const int64_t* tab_off,
+                             int64_t n_haps, unsigned long long* tab, hipStream_t st);
 template <typename T>
 hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st);
 template <typename T>
@@ -262,7 +262,9 @@ struct bv_plan {
   float* d_X = nullptr;
   int* d_anchor = nullptr;
   int2* d_ends = nullptr;
-  uint32_t* d_hkey = nullptr;  // per haplotype position: 12-mer code
+  unsigned long long* d_htab = nullptr;  // per haplotype 12-mer hash tables
+  int64_t* d_htab_off = nullptr;
+  std::vector<int64_t> hap_len;          // host copy of haplotype lengths
   float* d_Y = nullptr;
   float* d_model = nullptr;  // MLP parameters (copy)
   bool has_model = false;
@@ -291,7 +293,7 @@ void free_plan(bv_plan* p) {
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items32,
-                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_hkey};
+                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
@@ -382,8 +384,9 @@ hipError_t launch_features(bv_plan* p) {
   f.X = p->d_X;
   f.anchor = p->d_anchor;
   f.ends = p->d_ends;
-  f.hkey = p->d_hkey;
-  const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 16));
+  f.htab = p->d_htab;
+  f.htab_off = p->d_htab_off;
+  const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 32));
   return launch_band_features(f, grid, p->stream);
 }
 
@@ -405,11 +408,21 @@ int ensure_feature_buffers(bv_plan* p) {
     return fail(BV_ENOMEM, "anchor buffer allocation failed");
   if (!p->d_ends && hipMalloc(&p->d_ends, sizeof(int2) * n) != hipSuccess)
     return fail(BV_ENOMEM, "seed end buffer allocation failed");
-  if (!p->d_hkey) {
-    // haplotype k-mer codes: fixed for the plan, computed once
-    if (hipMalloc(&p->d_hkey, sizeof(uint32_t) * std::max<int64_t>(1, p->hap_bytes)) != hipSuccess)
-      return fail(BV_ENOMEM, "haplotype key buffer allocation failed");
-    HIP_TRY(launch_hap_keys(p->d_hmask, p->d_hap_off, p->n_haps, p->d_hkey, p->stream));
+  if (!p->d_htab) {
+    // haplotype 12-mer tables: fixed for the plan, built once
+    std::vector<int64_t> off(static_cast<size_t>(p->n_haps) + 1, 0);
+    for (int64_t h = 0; h < p->n_haps; ++h) {
+      int64_t sz = 64;
+      while (sz < 2 * p->hap_len[h]) sz <<= 1;
+      off[h + 1] = off[h] + (p->hap_len[h] <= kBandMaxHap ? sz : 1);
+    }
+    if (hipMalloc(&p->d_htab, sizeof(unsigned long long) * std::max<int64_t>(1, off.back())) != hipSuccess ||
+        hipMalloc(&p->d_htab_off, sizeof(int64_t) * off.size()) != hipSuccess)
+      return fail(BV_ENOMEM, "haplotype table allocation failed");
+    HIP_TRY(hipMemcpyAsync(p->d_htab_off, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice,
+                           p->stream));
+    HIP_TRY(launch_hap_tables(p->d_hmask, p->d_hap_off, p->d_htab_off, p->n_haps, p->d_htab, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
   }
   if (!p->d_Y && hipMalloc(&p->d_Y, sizeof(float) * 2 * n) != hipSuccess)
     return fail(BV_ENOMEM, "prediction buffer allocation failed");
@@ -460,7 +473,7 @@ int enqueue_run(bv_plan* p, bool timed) {
     HIP_TRY(launch_features(p));
     HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
     HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_ends, p->d_X, p->n_pairs, p->band_margin,
-                                        p->band_max_width, p->d_band, p->stream));
+                                        p->band_max_width, p->opt.band_narrow, p->d_band, p->stream));
   }
   if (fp32 && p->has_band) {
     // Pass 1: pairs with an eligible band on the banded kernel; the rest,
@@ -568,6 +581,7 @@ void bv_options_default(bv_options* opt) {
   opt->mode = BV_MODE_AVX;
   opt->fp64_renorm = 1;
   opt->band_tol = 1e-6f;
+  opt->band_narrow = 1;
 }
 
 int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** out) {
@@ -647,7 +661,11 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     }
   });
   int hmax = 0;
-  for (int64_t h = 0; h < b->n_haps; ++h) hmax = std::max<int>(hmax, static_cast<int>(hoff[h + 1] - hoff[h]));
+  p->hap_len.resize(static_cast<size_t>(b->n_haps));
+  for (int64_t h = 0; h < b->n_haps; ++h) {
+    p->hap_len[h] = hoff[h + 1] - hoff[h];
+    hmax = std::max<int>(hmax, static_cast<int>(p->hap_len[h]));
+  }
   p->hmax = hmax;
 
   // Tasks: one per read, over its region's haplotypes, longest first.

@@ -117,9 +117,16 @@ typedef struct bv_options {
   int32_t force_fp64;  /* skip the FP32 pass */
   int32_t grid_f32;    /* persistent grid override, 0 = occupancy-derived */
   int32_t grid_f64;
-  float band_tol;      /* banded pairs whose edge-mass ratio exceeds this are
-                          re-run on the full matrix (default 1e-6) */
-  int32_t reserved[2];
+  float band_tol;      /* banded pairs whose edge-zone touch mass exceeds this
+                          fraction of the banded likelihood are re-run on the
+                          full matrix (default 1e-6) */
+  int32_t band_narrow; /* learned bands may use the 32-diagonal band (two
+                          pairs per wavefront; default 1).  A read that
+                          aligns partly to an inexact repeat copy 17-40
+                          diagonals away can pass every seed test, so the
+                          narrow band needs the network's width gate: use
+                          margin >= 10 in bv_plan_set_band_model */
+  int32_t reserved;
   double trans[7];     /* BV_MODE_SCALAR: {MM, MI, MD, IM, II, DM, DD}
                           (pairhmm.hpp:15-33 order); all zero = ORIGINAL_DEFAULT */
 } bv_options;
