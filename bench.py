@@ -139,13 +139,14 @@ def main():
         plan.run()
         plan.results()  # waits for the stream
     # timed region: K full evaluations, synchronised on both sides
-    plan.results()
+    plan.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
         plan.run()
-    out, form = plan.results()
+    plan.synchronize()
     t2 = time.perf_counter()
     ms_step = (t2 - t1) * 1e3 / args.steps
+    out, form = plan.results()
     # split of one evaluation between the passes, from HIP events on the stream
     ev_ms = plan.time(1)
     st = plan.stats()
@@ -163,13 +164,14 @@ def main():
         for _ in range(args.warmup):
             plan.run()
             plan.results()
-        plan.results()
+        plan.synchronize()
         t1 = time.perf_counter()
         for _ in range(args.steps):
             plan.run()
-        out_b, form_b = plan.results()
+        plan.synchronize()
         t2 = time.perf_counter()
         ms_band = (t2 - t1) * 1e3 / args.steps
+        out_b, form_b = plan.results()
         plan.time(1)
         sb = plan.stats()
         fin = np.isfinite(out)

@@ -46,6 +46,11 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) {
   return (key * 2654435761u >> 8) & mask;
 }
 
+// The features kernel runs one wavefront per block: its LDS traffic is
+// ordered by the wave's in-order DS queue, so a compiler barrier suffices
+// between phases (no s_barrier, no wait on outstanding global stores).
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
 __device__ __forceinline__ int wave_incl_scan(int x, int lane) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -129,14 +134,16 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       my_p = lane * stride;
       if (my_p + kK > R) my_p = -1;
       if (my_p >= 0) {
+        uint8_t c[kK];
+#pragma unroll
+        for (int k = 0; k < kK; ++k) c[k] = a.rclass[ro + my_p + k];
+        uint8_t wild = 0;
+#pragma unroll
         for (int k = 0; k < kK; ++k) {
-          const uint8_t c = a.rclass[ro + my_p + k];
-          if (c & kWildcard) {
-            my_p = -1;
-            break;
-          }
-          my_key = (my_key << 2) | (c & 3);
+          wild |= c[k];
+          my_key = (my_key << 2) | (c[k] & 3);
         }
+        if (wild & kWildcard) my_p = -1;
       }
     }
     const int nvalid = __popcll(__ballot(my_p >= 0));
@@ -201,11 +208,11 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
         const int half = (nhit - 1) / 2;
         if (mx - mn < 64) {
           // counting: offsets mn + [0, 64)
-          __syncthreads();
+          wave_sync();
           s.cnt[lane] = 0;
-          __syncthreads();
+          wave_sync();
           if (hit) atomicAdd(&s.cnt[found_off - mn], 1);
-          __syncthreads();
+          wave_sync();
           const int incl = wave_incl_scan(s.cnt[lane], lane);
           const unsigned long long past = __ballot(incl > half);
           anchor = mn + static_cast<int>(__ffsll(static_cast<long long>(past)) - 1);
@@ -226,14 +233,14 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       }
       // 32 bins of 4 bp over [anchor - 64, anchor + 64)
       const float inv = nvalid > 0 ? 1.f / static_cast<float>(nvalid) : 0.f;
-      __syncthreads();
+      wave_sync();
       if (lane < kBins) s.cnt[lane] = 0;
-      __syncthreads();
+      wave_sync();
       for (int k = 0; k < nocc; ++k) {
         const int bin = (occ[k] - anchor + 64) >> 2;
         if (bin >= 0 && bin < kBins) atomicAdd(&s.cnt[bin], 1);
       }
-      __syncthreads();
+      wave_sync();
       if (lane < kBins) X[4 + lane] = static_cast<float>(s.cnt[lane]) * inv;
       // read ends anchored by seeds that hit next to the anchor
       const bool near = hit && abs(found_off - anchor) <= kEndAnchorTol;

@@ -247,6 +247,10 @@ class Plan:
         _check(self._lib.bv_plan_results(self._h, out.ctypes.data, form.ctypes.data))
         return out, form
 
+    def synchronize(self) -> None:
+        """Waits for the plan's stream (no copies)."""
+        _check(self._lib.bv_plan_results(self._h, None, None))
+
     def stats(self) -> PlanStats:
         s = PlanStats()
         _check(self._lib.bv_plan_stats_get(self._h, C.byref(s)))
