@@ -26,6 +26,7 @@ constexpr unsigned long long kEmpty = ~0ull;
 constexpr uint32_t kNoKey = ~0u;
 constexpr int kMaxOcc = 4;    // occurrences of one seed used for the histogram
 constexpr int kHapChunk = 4;  // haplotypes whose lookups are in flight together
+constexpr int kXcds = 8;      // MI355X accelerator complex dies
 
 struct FeatLds {
   int cnt[64];  // offset counts (median, histogram)
@@ -106,7 +107,19 @@ __global__ __launch_bounds__(256) void hap_table_kernel(const uint8_t* __restric
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
   __shared__ FeatLds s;
   const int lane = threadIdx.x;
-  for (int ti = blockIdx.x; ti < a.n_tasks; ti += gridDim.x) {
+  // Workgroups go round-robin to the 8 XCDs, each with its own L2.  Tasks are
+  // in region order; giving each XCD a contiguous eighth of them keeps only a
+  // few regions' haplotype tables live in each L2.
+  int t_begin = 0, t_end = a.n_tasks, t_first = blockIdx.x, t_step = gridDim.x;
+  if (gridDim.x % kXcds == 0 && gridDim.x >= kXcds) {
+    const int xcd = blockIdx.x % kXcds;
+    const int chunk = (a.n_tasks + kXcds - 1) / kXcds;
+    t_begin = xcd * chunk;
+    t_end = min(a.n_tasks, t_begin + chunk);
+    t_first = t_begin + blockIdx.x / kXcds;
+    t_step = gridDim.x / kXcds;
+  }
+  for (int ti = t_first; ti < t_end; ti += t_step) {
     const Task tk = a.tasks[ti];
     const int64_t ro = a.read_off[tk.read];
     const int R = static_cast<int>(a.read_off[tk.read + 1] - ro);

@@ -236,7 +236,7 @@ void parallel_for(int64_t n, F&& f) {
 struct bv_plan {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bv_options opt{};
   int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
   int hmax = 0;
@@ -252,7 +252,8 @@ struct bv_plan {
   int64_t* d_read_off = nullptr;
   uint8_t* d_hmask = nullptr;
   int64_t* d_hap_off = nullptr;
-  Task* d_tasks = nullptr;
+  Task* d_tasks = nullptr;         // longest first (forward kernels)
+  Task* d_tasks_region = nullptr;  // region order (seed features: haplotype tables stay in L2)
   Task* d_rescue = nullptr;
   Task* d_fallback = nullptr;
   int2* d_band = nullptr;
@@ -280,7 +281,7 @@ struct bv_plan {
   uint8_t* d_form = nullptr;
   void* d_carry32 = nullptr;
   void* d_carry64 = nullptr;
-  float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f;
+  float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f, last_msband = 0.f;
   bool timed = false;
 };
 
@@ -290,7 +291,7 @@ void free_plan(bv_plan* p) {
   if (!p) return;
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_gcp_const, p->d_read_off,
-                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_rescue, p->d_counters,
+                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items32,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off};
@@ -379,7 +380,7 @@ hipError_t launch_features(bv_plan* p) {
   f.read_off = p->d_read_off;
   f.hmask = p->d_hmask;
   f.hap_off = p->d_hap_off;
-  f.tasks = p->d_tasks;
+  f.tasks = p->d_tasks_region;
   f.n_tasks = p->n_tasks;
   f.X = p->d_X;
   f.anchor = p->d_anchor;
@@ -480,7 +481,9 @@ int enqueue_run(bv_plan* p, bool timed) {
     // and every band it rejects, go to the full-matrix list.
     const BArgs b = band_args(p);
     HIP_TRY(launch_band_dispatch(b, p->stream));
+    if (timed) HIP_TRY(hipEventRecord(p->ev[5], p->stream));
     HIP_TRY(launch_band_forward(b, p->grid_band, p->stream));
+    if (timed) HIP_TRY(hipEventRecord(p->ev[6], p->stream));
   } else if (fp32) {
     // Pass 1: every pair in FP32 on the full matrix.
     KArgs a = base_args(p);
@@ -693,6 +696,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->n_pairs = pair_base;
   p->cells = cells;
   if (tasks.size() >= (1u << 31)) return cleanup(fail(BV_EINVAL, "too many tasks"));
+  std::vector<Task> tasks_region = tasks;
   {
     std::vector<int64_t> idx(tasks.size());
     std::iota(idx.begin(), idx.end(), 0);
@@ -724,7 +728,8 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       (rc = dev_upload(&p->d_read_off, roff.data(), b->n_reads + 1, st)) ||
       (rc = dev_upload(&p->d_hmask, hmask.data(), hbytes, st)) ||
       (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st)) ||
-      (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)))
+      (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)) ||
+      (rc = dev_upload(&p->d_tasks_region, tasks_region.data(), static_cast<int64_t>(tasks_region.size()), st)))
     return cleanup(rc);
   if (hipMalloc(&p->d_rescue, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_counters, sizeof(int) * 16) != hipSuccess ||
@@ -751,7 +756,8 @@ int bv_plan_run(bv_plan* p) {
 int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
   if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
   HIP_TRY(hipSetDevice(g_ctx.device));
-  float sum32 = 0.f, sum64 = 0.f, sumfb = 0.f;
+  float sum32 = 0.f, sum64 = 0.f, sumfb = 0.f, sumband = 0.f;
+  const bool banded = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64 && p->has_band;
   HIP_TRY(hipEventRecord(p->ev[3], p->stream));
   hipEvent_t start = p->ev[3];
   // Per-run split between the FP32 and FP64 passes from events between them.
@@ -766,12 +772,18 @@ int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
     sum32 += a;
     sumfb += fb;
     sum64 += c;
+    if (banded) {
+      float bk = 0.f;
+      HIP_TRY(hipEventElapsedTime(&bk, p->ev[5], p->ev[6]));
+      sumband += bk;
+    }
   }
   float total = 0.f;
   HIP_TRY(hipEventElapsedTime(&total, start, p->ev[2]));
   p->last_ms32 = sum32 / iters;
   p->last_ms64 = sum64 / iters;
   p->last_msfb = sumfb / iters;
+  p->last_msband = sumband / iters;
   if (ms_per_run) *ms_per_run = total / iters;
   return BV_OK;
 }
@@ -810,7 +822,7 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->fallback_pairs = p->has_band ? cnt[3] : 0;
   s->cells_swept = static_cast<int64_t>(swept);
   s->last_ms_fallback = p->last_msfb;
-  s->pad = 0.f;
+  s->last_ms_band = p->last_msband;
   return BV_OK;
 }
 

@@ -63,7 +63,7 @@ class PlanStats(C.Structure):
         ("fp64_cells", C.c_int64), ("tasks", C.c_int64), ("grid_f32", C.c_int32),
         ("grid_f64", C.c_int32), ("last_ms_f32", C.c_float), ("last_ms_f64", C.c_float),
         ("band_pairs", C.c_int64), ("fallback_pairs", C.c_int64), ("cells_swept", C.c_int64),
-        ("last_ms_fallback", C.c_float), ("pad", C.c_float),
+        ("last_ms_fallback", C.c_float), ("last_ms_band", C.c_float),
     ]
 
 

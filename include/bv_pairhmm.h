@@ -147,7 +147,7 @@ typedef struct bv_plan_stats {
   int64_t fallback_pairs; /* banded pairs rejected and re-run on the full matrix */
   int64_t cells_swept;    /* FP32 cells actually computed in the last run */
   float last_ms_fallback;
-  float pad;
+  float last_ms_band;     /* banded kernels alone (both widths), last timed run */
 } bv_plan_stats;
 
 /* Uploads a batch to device memory and builds the length-ordered work list. */
