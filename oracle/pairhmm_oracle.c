@@ -395,3 +395,78 @@ long bvo_region_log10(int n_reads, const int64_t* read_off, const uint8_t* rs,
   }
   return rescued;
 }
+
+/* ---- banded FP64 forward with the edge-zone touch mass ------------------
+ * The band of the banded kernel (biovoltron_amd/csrc/pairhmm_band.hip):
+ * cells with d = j - r in [o_lo, o_lo + width - 1]; outside cells are zero,
+ * row 0 included (D(0, j) = 1/H only for j in the band).  The touch DP T has
+ * the forward recurrence but equals the forward value on edge-zone cells:
+ * the `edge` lowest diagonals on rows where the column left of the band is a
+ * real cell (r + o_lo - 1 >= 1; row 0: o_lo - 1 >= 0), and the `edge` highest
+ * ones while the column right of it is (r + o_lo + width <= H).  Rows are
+ * renormalised by a power of two.  Returns log10 of the banded likelihood
+ * (-inf when it is zero) and, in touch_ratio, T's final sum over S. */
+double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
+                        const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                        const uint8_t* hap, int o_lo, int width, int edge,
+                        double* touch_ratio) {
+  bvo_init();
+  const size_t W = (size_t)H + 2;
+  double* buf = (double*)calloc(12 * W, sizeof(double));
+  double *fM0 = buf, *fI0 = buf + W, *fD0 = buf + 2 * W, *tM0 = buf + 3 * W, *tI0 = buf + 4 * W,
+         *tD0 = buf + 5 * W, *fM1 = buf + 6 * W, *fI1 = buf + 7 * W, *fD1 = buf + 8 * W,
+         *tM1 = buf + 9 * W, *tI1 = buf + 10 * W, *tD1 = buf + 11 * W;
+  const int o_hi = o_lo + width - 1;
+  for (int j = 0; j <= H; ++j) {
+    if (j < o_lo || j > o_hi) continue;
+    fD0[j] = 1.0 / (double)H;
+    int zone = (j < o_lo + edge && o_lo - 1 >= 0) || (j > o_hi - edge && o_lo + width <= H);
+    if (zone) tD0[j] = fD0[j];
+  }
+  double log2scale = 0.0;
+  for (int r = 1; r <= R; ++r) {
+    row_par p;
+    row_params(0, r - 1, rs, q, ins, del, gcp, &p);
+    const int lact = r + o_lo - 1 >= 1, ract = r + o_lo + width <= H;
+    fM1[0] = fI1[0] = fD1[0] = tM1[0] = tI1[0] = tD1[0] = 0.0;
+    double mx = 0.0;
+    for (int j = 1; j <= H; ++j) {
+      const int d = j - r;
+      if (d < o_lo || d > o_hi) {
+        fM1[j] = fI1[j] = fD1[j] = tM1[j] = tI1[j] = tD1[j] = 0.0;
+        continue;
+      }
+      const int zone = (d < o_lo + edge && lact) || (d > o_hi - edge && ract);
+      const double prior = is_match(0, p.code, hap[j - 1]) ? p.pm : p.pmis;
+      fM1[j] = (fM0[j - 1] * p.mm + fI0[j - 1] * p.im + fD0[j - 1] * p.dm) * prior;
+      fI1[j] = fM0[j] * p.mi + fI0[j] * p.ii;
+      fD1[j] = fM1[j - 1] * p.md + fD1[j - 1] * p.dd;
+      if (zone) {
+        tM1[j] = fM1[j];
+        tI1[j] = fI1[j];
+        tD1[j] = fD1[j];
+      } else {
+        tM1[j] = (tM0[j - 1] * p.mm + tI0[j - 1] * p.im + tD0[j - 1] * p.dm) * prior;
+        tI1[j] = tM0[j] * p.mi + tI0[j] * p.ii;
+        tD1[j] = tM1[j - 1] * p.md + tD1[j - 1] * p.dd;
+      }
+      mx = fmax(mx, fmax(fM1[j], fmax(fI1[j], fD1[j])));
+    }
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);
+    for (int j = 0; j <= H; ++j) {
+      fM0[j] = ldexp(fM1[j], -e); fI0[j] = ldexp(fI1[j], -e); fD0[j] = ldexp(fD1[j], -e);
+      tM0[j] = ldexp(tM1[j], -e); tI0[j] = ldexp(tI1[j], -e); tD0[j] = ldexp(tD1[j], -e);
+    }
+    log2scale += e;
+  }
+  double S = 0.0, T = 0.0;
+  for (int j = 1; j <= H; ++j) {
+    S += fM0[j] + fI0[j];
+    T += tM0[j] + tI0[j];
+  }
+  free(buf);
+  if (touch_ratio) *touch_ratio = S > 0.0 ? T / S : 1.0;
+  if (!(S > 0.0)) return -INFINITY;
+  return (log2(S) + log2scale) * log10(2.0);
+}

@@ -26,6 +26,10 @@
  *                                 transition matrix.
  *   bvo_band_posterior            per-row posterior mass sums to 1 (pins the
  *                                 backward pass), band covering everything.
+ *   bvo_band_forward              band covering the matrix equals the full
+ *                                 DP with zero touch; brute-force enumeration
+ *                                 of in-band paths and of those touching an
+ *                                 edge zone (R, H <= 4).
  */
 #ifndef BV_PAIRHMM_ORACLE_H
 #define BV_PAIRHMM_ORACLE_H
@@ -95,6 +99,14 @@ long bvo_region_log10(int n_reads, const int64_t* read_off, const uint8_t* rs,
                       const uint8_t* q, const uint8_t* ins, const uint8_t* del,
                       const uint8_t* gcp, int n_haps, const int64_t* hap_off,
                       const uint8_t* hap, int n_threads, double* out);
+
+/* Banded FP64 forward restricted to diagonals [o_lo, o_lo + width - 1] with
+ * the edge-zone touch mass of the banded kernel (pairhmm_band.hip); returns
+ * log10 of the banded likelihood and the touch ratio T/S. */
+double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
+                        const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                        const uint8_t* hap, int o_lo, int width, int edge,
+                        double* touch_ratio);
 
 #ifdef __cplusplus
 }

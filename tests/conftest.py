@@ -35,6 +35,8 @@ class Oracle:
         lib.bvo_scalar_log10.argtypes = [I, I, P, P, P, C.c_void_p]
         lib.bvo_band_posterior.restype = D
         lib.bvo_band_posterior.argtypes = [I, I] + [P] * 6 + [C.c_void_p, C.c_void_p]
+        lib.bvo_band_forward.restype = D
+        lib.bvo_band_forward.argtypes = [I, I] + [P] * 6 + [I, I, I, C.c_void_p]
         lib.bvo_region_log10.restype = C.c_long
         lib.bvo_region_log10.argtypes = [I] + [C.c_void_p] * 6 + [I, C.c_void_p, C.c_void_p, I, C.c_void_p]
         for n in ("bvo_ph2pr_f32", "bvo_mm_f32"):
@@ -65,6 +67,12 @@ class Oracle:
     def scalar_log10(self, rd, hap, trans):
         t = np.ascontiguousarray(trans, np.float64)
         return self.lib.bvo_scalar_log10(len(rd["seq"]), len(hap), rd["seq"], rd["q"], hap, t.ctypes.data)
+
+    def band_forward(self, rd, hap, o_lo, width, edge=4):
+        """(log10 banded likelihood, touch ratio) of the banded FP64 forward."""
+        t = C.c_double(0.0)
+        v = self.lib.bvo_band_forward(*self._args(rd, hap), int(o_lo), int(width), int(edge), C.byref(t))
+        return v, t.value
 
     def band_posterior(self, rd, hap):
         R, H = len(rd["seq"]), len(hap)

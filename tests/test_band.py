@@ -262,3 +262,32 @@ def test_learned_band_against_heuristic_band(tmp_path):
     assert res["learned"][3] < res["rules_narrow"][3]
     loaded = band.BandModel.load(tmp_path / "m.npz")
     assert np.array_equal(loaded.params(), model.params())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("width", [64, 32])
+def test_band_kernel_matches_oracle_band_forward(oracle, width):
+    """The banded kernel against the oracle's FP64 restatement of the same
+    band (likelihood and the touch-mass decision), on bands placed around and
+    across the alignment."""
+    sel, pb = fixture_batch(step=2)
+    lab = band.labels(pb)
+    rng = np.random.default_rng(width)
+    centre = (lab[:, 0] + lab[:, 1]) // 2 + rng.integers(-width // 2, width // 2 + 1, len(sel))
+    bands = np.stack([centre - width // 2, centre - width // 2 + width - 1], 1).astype(np.int32)
+    out, form, st, _ = run_plan(pb, bands=bands)
+    tol = pairhmm.default_options().band_tol
+    n_acc = n_rej = 0
+    worst = 0.0
+    for k, (rd, hap) in enumerate(sel):
+        v, ratio = oracle.band_forward(rd, hap, bands[k, 0], width, 4)
+        if form[k] == 2:
+            n_acc += 1
+            worst = max(worst, abs(out[k] - v))
+            assert ratio < 10 * tol, k
+        elif form[k] == 3:
+            n_rej += 1
+            assert ratio > 0.1 * tol, k
+    say(f"width {width}: accepted {n_acc}, rejected {n_rej}, max |dlog10| vs oracle band {worst:.3g}")
+    assert n_acc > 50 and n_rej > 10
+    assert worst <= 2e-5

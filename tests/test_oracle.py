@@ -239,6 +239,97 @@ def python_dp(rows, hap, match, mistake=None):
     return math.log10(total) if total > 0 else -math.inf
 
 
+def brute_force_band(rows, hap, match, o_lo, width, edge):
+    """(likelihood, touch) summed over the paths whose every cell lies on
+    diagonals [o_lo, o_lo + width - 1], and over those among them that visit
+    an active edge-zone cell; no DP involved."""
+    R, H = len(rows), len(hap)
+    o_hi = o_lo + width - 1
+    tot = [0.0, 0.0]
+
+    def inside(r, j):
+        return o_lo <= j - r <= o_hi
+
+    def zone(r, j):
+        d = j - r
+        if r == 0:
+            return (d < o_lo + edge and o_lo - 1 >= 0) or (d > o_hi - edge and o_lo + width <= H)
+        return (d < o_lo + edge and r + o_lo - 1 >= 1) or (d > o_hi - edge and r + o_lo + width <= H)
+
+    def prior(r, j):
+        p = rows[r - 1]
+        return p["pm"] if match(p["code"], hap[j - 1]) else p["pmis"]
+
+    def walk(state, r, j, w, touched):
+        if not inside(r, j):
+            return
+        touched = touched or zone(r, j)
+        if state in "MI" and r == R:
+            tot[0] += w
+            tot[1] += w if touched else 0.0
+            return
+        nxt = r + 1
+        if state == "D" and r == 0:
+            if j + 1 <= H:
+                walk("M", 1, j + 1, w * rows[0]["dm"] * prior(1, j + 1), touched)
+            return
+        if state == "M":
+            if nxt <= R and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["mm"] * prior(nxt, j + 1), touched)
+            if nxt <= R:
+                walk("I", nxt, j, w * rows[nxt - 1]["mi"], touched)
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["md"], touched)
+        elif state == "I":
+            if nxt <= R and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["im"] * prior(nxt, j + 1), touched)
+            if nxt <= R:
+                walk("I", nxt, j, w * rows[nxt - 1]["ii"], touched)
+        elif state == "D":
+            if nxt <= R and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["dm"] * prior(nxt, j + 1), touched)
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["dd"], touched)
+
+    for j0 in range(H + 1):
+        walk("D", 0, j0, 1.0 / H, False)
+    return tot[0], tot[1]
+
+
+def test_band_forward_vs_brute_force(oracle):
+    """The banded forward and its touch mass against enumeration of the
+    in-band paths, over random bands (cutting the matrix or not) and edge
+    widths."""
+    rng = np.random.default_rng(17)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = touched = 0
+    for rd, hap in small_cases(23, 300):
+        R, H = len(rd["seq"]), len(hap)
+        o_lo = int(rng.integers(-R - 1, H + 1))
+        width = int(rng.integers(1, R + H + 2))
+        edge = int(rng.integers(1, 3))
+        rows = row_params_avx(rd, t)
+        bf, bt = brute_force_band(rows, hap, match_avx, o_lo, width, edge)
+        v, ratio = oracle.band_forward(rd, hap, o_lo, width, edge)
+        if bf == 0.0:
+            assert v == -np.inf
+            continue
+        assert v == pytest.approx(math.log10(bf), abs=1e-12)
+        assert ratio == pytest.approx(bt / bf, rel=1e-9, abs=1e-15)
+        checked += 1
+        touched += bt > 0.0
+    assert checked > 100 and touched > 30
+
+
+def test_band_forward_covering_band_is_the_full_matrix(oracle, golden):
+    pairs, _ = golden("workload1.npz")
+    for rd, hap in pairs[::29]:
+        R, H = len(rd["seq"]), len(hap)
+        v, ratio = oracle.band_forward(rd, hap, -R, R + H + 1, 4)
+        assert v == pytest.approx(oracle.log10_exact(rd, hap, 0), abs=1e-9)
+        assert ratio == 0.0
+
+
 def test_brute_force_catches_planted_mistakes():
     t = np.load(GOLDEN / "tables.npz")
     cases = list(small_cases(3, 40))
