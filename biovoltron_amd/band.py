@@ -28,6 +28,7 @@ import numpy as np
 
 from . import pairhmm, synth
 
+DEFAULT_MODEL = Path(__file__).resolve().parent / "models" / "band_mlp.npz"
 N_FEAT = 36
 N_PARAMS = 6658
 FORMAT_VERSION = 1

@@ -282,6 +282,9 @@ struct bv_plan {
   void* d_carry32 = nullptr;
   void* d_carry64 = nullptr;
   float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f, last_msband = 0.f;
+  // one run captured as a graph (bv_plan_run), rebuilt when the band setup changes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   bool timed = false;
 };
 
@@ -299,6 +302,8 @@ void free_plan(bv_plan* p) {
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -585,6 +590,7 @@ void bv_options_default(bv_options* opt) {
   opt->fp64_renorm = 1;
   opt->band_tol = 1e-6f;
   opt->band_narrow = 1;
+  opt->use_graph = 1;
 }
 
 int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** out) {
@@ -746,11 +752,35 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   return BV_OK;
 }
 
+static void drop_graph(bv_plan* p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+}
+
 int bv_plan_run(bv_plan* p) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
   HIP_TRY(hipSetDevice(g_ctx.device));
   p->timed = false;
-  return enqueue_run(p, false);
+  if (!p->opt.use_graph) return enqueue_run(p, false);
+  if (!p->graph_exec) {
+    // Capture one run: only kernel launches (counters are cleared by a
+    // kernel, task counts are read on the device), so it replays as is.
+    HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_run(p, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(p->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIP_TRY(ec);
+    p->graph = g;
+    HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+  return BV_OK;
 }
 
 int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
@@ -829,6 +859,7 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
 int bv_plan_set_bands(bv_plan* p, const int32_t* bands) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
   HIP_TRY(hipSetDevice(g_ctx.device));
+  drop_graph(p);
   if (!bands) {
     p->has_band = false;
     return BV_OK;
@@ -1141,6 +1172,7 @@ int bv_plan_features(bv_plan* p, float* X_out, int32_t* anchor_out, int32_t* end
 int bv_plan_set_band_model(bv_plan* p, const bv_band_model* m, int margin, int max_width) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
   HIP_TRY(hipSetDevice(g_ctx.device));
+  drop_graph(p);
   if (!m) {
     p->has_model = false;
     p->has_band = false;

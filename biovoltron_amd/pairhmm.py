@@ -53,7 +53,7 @@ class Options(C.Structure):
     _fields_ = [
         ("mode", C.c_int32), ("fp64_renorm", C.c_int32), ("force_fp64", C.c_int32),
         ("grid_f32", C.c_int32), ("grid_f64", C.c_int32), ("band_tol", C.c_float),
-        ("band_narrow", C.c_int32), ("reserved", C.c_int32), ("trans", C.c_double * 7),
+        ("band_narrow", C.c_int32), ("use_graph", C.c_int32), ("trans", C.c_double * 7),
     ]
 
 

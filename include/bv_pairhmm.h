@@ -126,7 +126,9 @@ typedef struct bv_options {
                           diagonals away can pass every seed test, so the
                           narrow band needs the network's width gate: use
                           margin >= 10 in bv_plan_set_band_model */
-  int32_t reserved;
+  int32_t use_graph;   /* bv_plan_run replays the run as one captured hipGraph
+                          (default 1); the graph is rebuilt after the bands,
+                          the band model or its parameters change */
   double trans[7];     /* BV_MODE_SCALAR: {MM, MI, MD, IM, II, DM, DD}
                           (pairhmm.hpp:15-33 order); all zero = ORIGINAL_DEFAULT */
 } bv_options;

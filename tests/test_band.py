@@ -291,3 +291,37 @@ def test_band_kernel_matches_oracle_band_forward(oracle, width):
     say(f"width {width}: accepted {n_acc}, rejected {n_rej}, max |dlog10| vs oracle band {worst:.3g}")
     assert n_acc > 50 and n_rej > 10
     assert worst <= 2e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("learned", [False, True])
+def test_graph_replays_match_eager_runs(learned):
+    """bv_plan_run replays one captured graph; every replay, with eager runs
+    of another plan and host synchronisations in between, equals the eager
+    run of the same plan."""
+    ev = synth.generate(1, seed=synth.SEED_BASE + 77, count=20)
+    pb = pairhmm.PackedBatch.from_synth(ev)
+    model = band.BandModel.load(band.DEFAULT_MODEL) if learned else None
+    with pairhmm.Plan(pb, pairhmm.default_options(use_graph=0)) as eager, \
+            pairhmm.Plan(pb, pairhmm.default_options(use_graph=1)) as graph:
+        if model is not None:
+            band.set_plan_model(eager, model)
+            band.set_plan_model(graph, model)
+        eager.run()
+        ref, ref_form = eager.results()
+        for k in range(4):
+            graph.run()
+            out, form = graph.results()
+            assert np.array_equal(out, ref) and np.array_equal(form, ref_form), k
+            eager.run()
+            eager.results()
+        if learned:
+            # a new band setup rebuilds the graph
+            band.set_plan_model(graph, model, margin=12)
+            band.set_plan_model(eager, model, margin=12)
+            eager.run()
+            ref2, _ = eager.results()
+            graph.run()
+            out2, _ = graph.results()
+            assert np.array_equal(out2, ref2)
+    say("graph replays equal eager runs, learned =", learned)
