@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--band-model", default=str(ROOT / "biovoltron_amd" / "models" / "band_mlp.npz"),
                     help="trained band model; the learned-band leg is skipped when absent")
     ap.add_argument("--no-band", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the occupancy-derived grids instead of tuning them on the plan")
     ap.add_argument("--band-margin", type=int, default=10)
     ap.add_argument("--band-max-width", type=int, default=68)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -137,6 +139,9 @@ def main():
           f"{b.cells:.4e} cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
     plan = pairhmm.Plan(pairhmm.PackedBatch.from_synth(b))
     print(f"[bench] plan ready ({time.perf_counter() - t0 - t_gen:.1f} s)", file=sys.stderr, flush=True)
+    tuned_full = plan.autotune(1) if args.autotune else None
+    if tuned_full:
+        print(f"[bench] autotuned grids (full matrix): {tuned_full}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         plan.run()
         plan.results()  # waits for the stream
@@ -163,6 +168,7 @@ def main():
         from biovoltron_amd import band
         model = band.BandModel.load(args.band_model)
         band.set_plan_model(plan, model, margin=args.band_margin, max_width=args.band_max_width)
+        tuned = plan.autotune(1) if args.autotune else None
         for _ in range(args.warmup):
             plan.run()
             plan.results()
@@ -195,7 +201,8 @@ def main():
             ms_pass1=round(sb.last_ms_f32, 3), ms_fallback=round(sb.last_ms_fallback, 3),
             ms_fp64=round(sb.last_ms_f64, 3), model=Path(args.band_model).name,
             margin=args.band_margin, max_width=args.band_max_width,
-            band_kernel_ms=round(sb.last_ms_band, 3), band_cells=band_cells)
+            band_kernel_ms=round(sb.last_ms_band, 3), band_cells=band_cells,
+            grids=tuned or dict(band=None, fp32=sb.grid_f32, fp64=sb.grid_f64))
         print(f"[bench] learned band: {ms_band:.2f} ms/step ({band_info['speedup_vs_full']}x), "
               f"max |dlog10| vs full {err:.2e}, fallback {sb.fallback_pairs}, "
               f"cells swept {band_info['cells_swept_frac']}", file=sys.stderr, flush=True)
@@ -217,7 +224,8 @@ def main():
             "workload": (f"learned-band PairHMM (full-matrix fallback, FP64 rescue)"
                          if band_info else "batched full-matrix PairHMM")
                         + f", workload {args.workload} generator, {b.n_regions} regions",
-            "full_matrix": {"ms_per_step": round(ms_step, 3), "gcups": round(gcups, 3)},
+            "full_matrix": {"ms_per_step": round(ms_step, 3), "gcups": round(gcups, 3),
+                            "grids": tuned_full or dict(fp32=st.grid_f32, fp64=st.grid_f64)},
             "learned_band": band_info,
             "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,
             "seed": synth.workload_seed(args.workload),

@@ -242,6 +242,7 @@ struct bv_plan {
   int hmax = 0;
   int n_tasks = 0;
   int grid32 = 0, grid64 = 0, grid_band = 0;
+  int grid32_max = 0, grid64_max = 0, grid_band_max = 0;  // scratch is sized for these
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -721,6 +722,9 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_band, p->n_pairs)));
   const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
   p->grid64 = std::max(1, static_cast<int>(std::min<int64_t>(p->grid64, std::max<int64_t>(1, max_fp64_tasks))));
+  p->grid32_max = p->grid32;
+  p->grid64_max = p->grid64;
+  p->grid_band_max = p->grid_band;
   p->stride32 = hmax + kCarryPad;
   p->stride64 = hmax + kCarryPad;
 
@@ -757,6 +761,44 @@ static void drop_graph(bv_plan* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->graph_exec = nullptr;
   p->graph = nullptr;
+}
+
+int bv_plan_autotune(bv_plan* p, int iters, int32_t* grids_out) {
+  if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  int* knobs[3] = {&p->grid_band, &p->grid32, &p->grid64};
+  const int caps[3] = {p->grid_band_max, p->grid32_max, p->grid64_max};
+  float best_ms = 0.f;
+  int rc = bv_plan_time(p, iters, &best_ms);
+  if (rc) return rc;
+  // Coordinate search: each persistent grid over fractions of its
+  // occupancy-derived maximum, timing whole runs on the plan's own data.
+  for (int k = 0; k < 3; ++k) {
+    const int cur = *knobs[k];
+    int best = cur;
+    for (int num : {2, 4, 6, 8}) {
+      const int g = std::max(g_ctx.cus, caps[k] * num / 8);
+      if (g == cur || g > caps[k]) continue;
+      *knobs[k] = g;
+      float ms = 0.f;
+      if ((rc = bv_plan_time(p, iters, &ms))) {
+        *knobs[k] = cur;
+        return rc;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = g;
+      }
+    }
+    *knobs[k] = best;
+  }
+  drop_graph(p);
+  if (grids_out) {
+    grids_out[0] = p->grid_band;
+    grids_out[1] = p->grid32;
+    grids_out[2] = p->grid64;
+  }
+  return BV_OK;
 }
 
 int bv_plan_run(bv_plan* p) {

@@ -83,6 +83,7 @@ EXPORTS = (
     "bv_pairhmm_init", "bv_pairhmm_shutdown", "bv_pairhmm_last_error", "bv_pairhmm_device_info",
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
+    "bv_plan_autotune",
     "bv_pairhmm_batch",
 )
 
@@ -109,6 +110,7 @@ def library() -> C.CDLL:
         lib.bv_plan_time.argtypes = [vp, i32, C.POINTER(C.c_float)]
         lib.bv_plan_results.argtypes = [vp, vp, vp]
         lib.bv_plan_stats_get.argtypes = [vp, C.POINTER(PlanStats)]
+        lib.bv_plan_autotune.argtypes = [vp, i32, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
         lib.bv_pairhmm_batch.argtypes = [C.POINTER(_RegionBatch), C.POINTER(Options), vp, vp]
@@ -246,6 +248,13 @@ class Plan:
         form = np.empty(n, np.uint8)
         _check(self._lib.bv_plan_results(self._h, out.ctypes.data, form.ctypes.data))
         return out, form
+
+    def autotune(self, iters: int = 1) -> dict:
+        """Picks the persistent grids (banded, FP32, FP64) by timing runs of
+        this plan; returns them."""
+        g = (C.c_int32 * 3)()
+        _check(self._lib.bv_plan_autotune(self._h, int(iters), g))
+        return dict(band=g[0], fp32=g[1], fp64=g[2])
 
     def synchronize(self) -> None:
         """Waits for the plan's stream (no copies)."""

@@ -325,3 +325,19 @@ def test_graph_replays_match_eager_runs(learned):
             out2, _ = graph.results()
             assert np.array_equal(out2, ref2)
     say("graph replays equal eager runs, learned =", learned)
+
+
+@pytest.mark.gpu
+def test_autotune_changes_speed_not_results():
+    ev = synth.generate(1, seed=synth.SEED_BASE + 78, count=10)
+    pb = pairhmm.PackedBatch.from_synth(ev)
+    with pairhmm.Plan(pb) as plan:
+        band.set_plan_model(plan, band.BandModel.load(band.DEFAULT_MODEL))
+        plan.run()
+        ref, ref_form = plan.results()
+        grids = plan.autotune(1)
+        plan.run()
+        out, form = plan.results()
+    say("autotuned grids", grids)
+    assert all(v >= 1 for v in grids.values())
+    assert np.array_equal(out, ref) and np.array_equal(form, ref_form)
