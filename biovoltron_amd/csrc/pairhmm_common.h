@@ -34,6 +34,8 @@ constexpr int kRowsPerStripe = 63;// lane 0 carries the previous stripe's last r
 constexpr int kBlock = 64;        // anti-diagonal steps per staging block
 constexpr int kSlots = 8;         // match classes per haplotype column
 constexpr int kCarryPad = 192;    // extra carry entries past the haplotype end
+constexpr int kSplitRows = 256;   // reads at least this long get one task per haplotype
+constexpr int kFp64FirstRows = 1024;  // reads at least this long go to FP64 first (FP32 mode)
 
 // Rescue threshold of the reference driver (pairhmm_common.h:16).
 constexpr float kMinAccepted = 1e-28f;
@@ -84,6 +86,12 @@ struct KArgs {
   int out_raw;  // write the scaled sum S instead of log10 L (testcase ABI)
   unsigned long long* rescue_cells;  // FP32 pass: sum of R*H over rescued pairs
   unsigned long long* cells_done;    // cells computed (R*H per pair)
+  // FP64 pass over long reads run before any FP32 attempt: pairs whose log10 L
+  // is at least verify_above may not underflow in FP32 and are queued here
+  Task* verify;
+  int* verify_count;
+  double verify_above;
+  int verify_pass;  // FP32 pass over those pairs: keep the FP64 result when S < 1e-28
 };
 
 }  // namespace bvp

@@ -66,6 +66,7 @@ int fail(int code, const std::string& msg) {
 // log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
+constexpr int kCounters = 32;  // device counter block of a plan (see bv_plan)
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -255,6 +256,9 @@ struct bv_plan {
   int64_t* d_hap_off = nullptr;
   Task* d_tasks = nullptr;         // longest first (forward kernels)
   Task* d_tasks_region = nullptr;  // region order (seed features: haplotype tables stay in L2)
+  Task* d_tasks64 = nullptr;       // long reads, FP64 first (FP32 mode), one pair each
+  Task* d_verify = nullptr;        // their pairs that may not underflow FP32
+  int n_tasks64 = 0;
   Task* d_rescue = nullptr;
   Task* d_fallback = nullptr;
   int2* d_band = nullptr;
@@ -275,7 +279,8 @@ struct bv_plan {
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
   // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
   // cells (u64), [8..9] swept cells (u64), [10] wide banded item count,
-  // [11] its queue, [12] narrow banded item count, [13] its queue
+  // [11] its queue, [12] narrow banded item count, [13] its queue, [14]
+  // verify count, [15] its queue, [16] FP64-first queue
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -295,7 +300,8 @@ void free_plan(bv_plan* p) {
   if (!p) return;
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_gcp_const, p->d_read_off,
-                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_rescue, p->d_counters,
+                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
+                  p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items32,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off};
@@ -471,8 +477,8 @@ BArgs band_args(const bv_plan* p) {
 }
 
 int enqueue_run(bv_plan* p, bool timed) {
-  HIP_TRY(launch_zero_ints(p->d_counters, 16, p->stream));
-  if (p->n_tasks == 0) return BV_OK;
+  HIP_TRY(launch_zero_ints(p->d_counters, kCounters, p->stream));
+  if (p->n_tasks == 0 && p->n_tasks64 == 0) return BV_OK;
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   if (timed) HIP_TRY(hipEventRecord(p->ev[0], p->stream));
   if (fp32 && p->has_model) {
@@ -533,6 +539,31 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.carry = p->d_carry64;
     a.carry_stride = p->stride64;
     HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
+  }
+  if (p->n_tasks64 > 0) {
+    // Long reads in FP64 first; pairs within reach of FP32's range are queued
+    // for FP32 and keep the FP32 result when it does not underflow (the
+    // reference's rule, pairhmm_avx.hpp:140-149).  S_f >= 1e-28 means
+    // log10 L >= -28 - log10(2^120); the FP64 value screens with a margin.
+    KArgs a = base_args(p);
+    a.tasks = p->d_tasks64;
+    a.n_tasks = p->n_tasks64;
+    a.counter = p->d_counters + 16;
+    a.carry = p->d_carry64;
+    a.carry_stride = p->stride64;
+    a.verify = p->d_verify;
+    a.verify_count = p->d_counters + 14;
+    a.verify_above = -28.0 - 120.0 * 0.30102999566398119521 - 0.05;
+    HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
+    KArgs v = base_args(p);
+    v.tasks = p->d_verify;
+    v.n_tasks = 0;
+    v.n_tasks_dev = p->d_counters + 14;
+    v.counter = p->d_counters + 15;
+    v.carry = p->d_carry32;
+    v.carry_stride = p->stride32;
+    v.verify_pass = 1;
+    HIP_TRY(launch_forward<float>(v, p->grid32, p->stream));
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[2], p->stream));
   return BV_OK;
@@ -678,10 +709,15 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   }
   p->hmax = hmax;
 
-  // Tasks: one per read, over its region's haplotypes, longest first.
-  std::vector<Task> tasks;
+  // Tasks: a read against its region's haplotypes (one task per haplotype
+  // for reads of kSplitRows or more, so long reads spread over the GPU),
+  // longest first.  In the FP32 mode reads of kFp64FirstRows or more go to
+  // the FP64 list instead: they underflow FP32's range at these lengths, and
+  // the few that would not are re-run in FP32 afterwards (verify pass).
+  const bool fp32_mode = opt.mode == BV_MODE_AVX && !opt.force_fp64;
+  std::vector<Task> tasks, tasks64;
   tasks.reserve(static_cast<size_t>(b->n_reads));
-  std::vector<int64_t> cost;
+  std::vector<int64_t> cost, cost64;
   cost.reserve(static_cast<size_t>(b->n_reads));
   int64_t pair_base = 0, cells = 0;
   for (int64_t g = 0; g < b->n_regions; ++g) {
@@ -693,31 +729,49 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     for (int64_t r = r0; r < r1; ++r) {
       if (nh == 0) continue;
       const int64_t R = roff[r + 1] - roff[r];
-      tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
-                           static_cast<int32_t>(nh), 0, pair_base + (r - r0) * nh});
-      cost.push_back(R * hsum);
+      const int64_t out0 = pair_base + (r - r0) * nh;
       cells += R * hsum;
+      if (fp32_mode && R >= kFp64FirstRows) {
+        for (int64_t h = h0; h < h1; ++h) {
+          tasks64.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
+          cost64.push_back(R * (hoff[h + 1] - hoff[h]));
+        }
+      } else if (R >= kSplitRows) {
+        for (int64_t h = h0; h < h1; ++h) {
+          tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
+          cost.push_back(R * (hoff[h + 1] - hoff[h]));
+        }
+      } else {
+        tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
+                             static_cast<int32_t>(nh), 0, out0});
+        cost.push_back(R * hsum);
+      }
     }
     pair_base += (r1 - r0) * nh;
   }
   p->n_pairs = pair_base;
   p->cells = cells;
-  if (tasks.size() >= (1u << 31)) return cleanup(fail(BV_EINVAL, "too many tasks"));
+  if (tasks.size() >= (1u << 31) || tasks64.size() >= (1u << 31))
+    return cleanup(fail(BV_EINVAL, "too many tasks"));
   std::vector<Task> tasks_region = tasks;
-  {
-    std::vector<int64_t> idx(tasks.size());
+  auto longest_first = [](std::vector<Task>& t, const std::vector<int64_t>& c) {
+    std::vector<int64_t> idx(t.size());
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
-    std::vector<Task> sorted(tasks.size());
-    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = tasks[idx[i]];
-    tasks.swap(sorted);
-  }
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return c[x] > c[y]; });
+    std::vector<Task> sorted(t.size());
+    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = t[idx[i]];
+    t.swap(sorted);
+  };
+  longest_first(tasks, cost);
+  longest_first(tasks64, cost64);
   p->n_tasks = static_cast<int>(tasks.size());
+  p->n_tasks64 = static_cast<int>(tasks64.size());
 
   const int bpc32 = g_ctx.bpc_f32, bpc64 = g_ctx.bpc_f64;
   p->grid32 = opt.grid_f32 > 0 ? opt.grid_f32 : g_ctx.cus * bpc32;
   p->grid64 = opt.grid_f64 > 0 ? opt.grid_f64 : g_ctx.cus * bpc64;
-  p->grid32 = std::max(1, std::min<int>(p->grid32, std::max(1, p->n_tasks)));
+  // the FP32 passes also run the full-matrix and verify lists (up to one task per pair)
+  p->grid32 = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p->grid32, std::max<int64_t>(1, p->n_pairs))));
   p->grid_band = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_band, p->n_pairs)));
   const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
@@ -739,10 +793,12 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       (rc = dev_upload(&p->d_hmask, hmask.data(), hbytes, st)) ||
       (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st)) ||
       (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)) ||
-      (rc = dev_upload(&p->d_tasks_region, tasks_region.data(), static_cast<int64_t>(tasks_region.size()), st)))
+      (rc = dev_upload(&p->d_tasks_region, tasks_region.data(), static_cast<int64_t>(tasks_region.size()), st)) ||
+      (rc = dev_upload(&p->d_tasks64, tasks64.data(), static_cast<int64_t>(tasks64.size()), st)) ||
+      (rc = dev_upload(&p->d_verify, tasks64.data(), static_cast<int64_t>(tasks64.size()), st)))
     return cleanup(rc);
   if (hipMalloc(&p->d_rescue, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
-      hipMalloc(&p->d_counters, sizeof(int) * 16) != hipSuccess ||
+      hipMalloc(&p->d_counters, sizeof(int) * kCounters) != hipSuccess ||
       hipMalloc(&p->d_fallback, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_out, sizeof(double) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_form, std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
@@ -883,7 +939,7 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   s->n_pairs = p->n_pairs;
   s->cells = p->cells;
-  s->fp64_pairs = fp32 ? cnt[2] : p->n_pairs;
+  s->fp64_pairs = fp32 ? cnt[2] + p->n_tasks64 : p->n_pairs;
   s->fp64_cells = fp32 ? static_cast<int64_t>(rcells) : p->cells;
   s->tasks = p->n_tasks;
   s->grid_f32 = p->grid32;

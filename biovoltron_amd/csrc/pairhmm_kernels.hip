@@ -341,7 +341,14 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
           a.out[o] = ldexp(S, -shift);
           a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
         } else if constexpr (sizeof(T) == 4) {
-          if (S < static_cast<double>(kMinAccepted) && a.rescue != nullptr) {
+          if (a.verify_pass) {
+            // the reference rule for a long read run in FP64 first: its FP32
+            // result replaces the FP64 one unless it underflows
+            if (S >= static_cast<double>(kMinAccepted)) {
+              a.out[o] = log10(S) - Num<T>::log10K;
+              a.form[o] = kFormFP32;
+            }
+          } else if (S < static_cast<double>(kMinAccepted) && a.rescue != nullptr) {
             const int k = atomicAdd(a.rescue_count, 1);
             a.rescue[k] = Task{tk.read, tk.hap0 + h, 1, -1, o};
             a.form[o] = kFormPending;
@@ -354,8 +361,13 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
             a.form[o] = tk.band == -2 ? kFormBandFallback : kFormFP32;
           }
         } else {
-          a.out[o] = log10(S) - static_cast<double>(shift) * 0.30102999566398120 - Num<T>::log10K;
+          const double L = log10(S) - static_cast<double>(shift) * 0.30102999566398120 - Num<T>::log10K;
+          a.out[o] = L;
           a.form[o] = kFormFP64;
+          if (a.verify != nullptr && L >= a.verify_above) {
+            const int k = atomicAdd(a.verify_count, 1);
+            a.verify[k] = Task{tk.read, tk.hap0 + h, 1, -1, o};
+          }
         }
       }
     }

@@ -213,3 +213,30 @@ def test_class_api_matches_reference_semantics(oracle):
     assert len(ll) == len(raw) == len(recs)
     for a, b_ in zip(ll, raw):
         assert np.abs(np.array(a) - np.array(b_)).max() <= TOL
+
+
+def test_long_reads_fp64_first_keep_the_reference_rule(oracle):
+    """Reads of 1024+ bases run in FP64 first; those whose FP32 sum would not
+    underflow are re-run in FP32 and keep the FP32 result, as the reference
+    driver would (FP32 unless S_f < 1e-28, pairhmm_avx.hpp:140-149)."""
+    rng = np.random.default_rng(31)
+    hap = bytes(rng.choice(list(b"ACGT"), 1400).astype(np.uint8))
+    pairs = []
+    for k, (R, q, err) in enumerate([(1100, 40, 0.0), (1200, 35, 0.002), (1100, 8, 0.08), (1300, 12, 0.05)]):
+        start = int(rng.integers(0, len(hap) - R))
+        seq = bytearray(hap[start:start + R])
+        for p in np.nonzero(rng.random(R) < err)[0]:
+            seq[p] = b"ACGT"[(b"ACGT".index(seq[p]) + 1) % 4]
+        rd = dict(seq=bytes(seq), q=bytes([q]) * R, i=bytes([45]) * R, d=bytes([45]) * R, c=bytes([10]) * R)
+        pairs.append((rd, hap))
+    out, form = run_pairs(pairs)
+    for k, (rd, hap) in enumerate(pairs):
+        ref, rescued = oracle.log10_reference(rd, hap)
+        exact = oracle.log10_exact(rd, hap, 0)
+        say(f"long read {k}: R={len(rd['seq'])} form {form[k]} gpu {out[k]:.6f} reference rule {ref:.6f} "
+            f"(rescued {rescued}) exact {exact:.6f}")
+        assert form[k] == (1 if rescued else 0)
+        if rescued:
+            assert abs(out[k] - exact) <= TOL_FP64 * max(1.0, abs(exact))
+        else:
+            assert abs(out[k] - ref) <= TOL_FP32
