@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void hap_table_kernel(const uint8_t* __restric
     const uint32_t mask = static_cast<uint32_t>(tab_off[h + 1] - tab_off[h] - 1);
     for (uint32_t i = threadIdx.x; i <= mask; i += blockDim.x) t[i] = kEmpty;
     __syncthreads();
-    // haplotypes longer than the banded kernel stages get no table (one slot)
+    // a haplotype shorter than a seed gets no table (one slot)
     for (int q = threadIdx.x; mask > 0 && q + kK <= H; q += blockDim.x) {
       uint32_t key = 0;
       bool ok = true;
@@ -172,8 +172,9 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
         nocc_c[c] = 0;
         const int hap = tk.hap0 + h0 + c;
         if (h0 + c >= tk.nhap || my_p < 0) continue;
+        if (a.hap_off[hap + 1] - a.hap_off[hap] > kBandMaxHap) continue;  // beyond the banded kernel
         const int64_t t0 = a.htab_off[hap], t1 = a.htab_off[hap + 1];
-        if (t1 - t0 < 2) continue;  // no table: haplotype longer than the banded kernel stages
+        if (t1 - t0 < 2) continue;  // no table
         const unsigned long long* t = a.htab + t0;
         const uint32_t mask = static_cast<uint32_t>(t1 - t0 - 1);
         uint32_t sl = slot_of(my_key, mask);
@@ -504,6 +505,76 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
   }
 }
 
+// Start bands of long reads (one task per pair, run in FP64 on the band that
+// follows the alignment).  One wavefront per pair: lane l looks up the read's
+// 12-mer at position 4l (the first 256 bases) in the haplotype's table; seeds
+// found exactly once give diagonals q - p, and the band starts as the 64
+// diagonals centred on their median.  With fewer than kLongMinHits such seeds
+// the pair keeps the full matrix.
+constexpr int kLongSeedStride = 4;
+constexpr int kLongMinHits = 4;
+
+__global__ __launch_bounds__(64) void long_anchor_kernel(const uint8_t* __restrict__ rclass,
+                                                         const int64_t* __restrict__ read_off,
+                                                         const int64_t* __restrict__ htab_off,
+                                                         const unsigned long long* __restrict__ htab,
+                                                         const Task* __restrict__ tasks, int n_tasks,
+                                                         int2* __restrict__ band) {
+  const int lane = threadIdx.x;
+  for (int ti = blockIdx.x; ti < n_tasks; ti += gridDim.x) {
+    const Task tk = tasks[ti];
+    const int64_t ro = read_off[tk.read];
+    const int R = static_cast<int>(read_off[tk.read + 1] - ro);
+    const int p = lane * kLongSeedStride;
+    int diag = INT_MIN;
+    const int64_t t0 = htab_off[tk.hap0], t1 = htab_off[tk.hap0 + 1];
+    if (p + kK <= R && t1 - t0 >= 2) {
+      uint32_t key = 0;
+      uint8_t wild = 0;
+      for (int k = 0; k < kK; ++k) {
+        const uint8_t c = rclass[ro + p + k];
+        wild |= c;
+        key = (key << 2) | (c & 3);
+      }
+      if (!(wild & kWildcard)) {
+        const unsigned long long* t = htab + t0;
+        const uint32_t mask = static_cast<uint32_t>(t1 - t0 - 1);
+        uint32_t sl = slot_of(key, mask);
+        int found = 0, q = 0;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+          const unsigned long long e = t[sl];
+          if (e == kEmpty) break;
+          if (static_cast<uint32_t>(e) == key) {
+            q = static_cast<int>(e >> 32);
+            if (++found > 1) break;
+          }
+          sl = (sl + 1) & mask;
+        }
+        if (found == 1) diag = q - p;
+      }
+    }
+    const bool hit = diag != INT_MIN;
+    const int nhit = __popcll(__ballot(hit));
+    int2 b{1, 0};
+    if (nhit >= kLongMinHits) {
+      const int half = (nhit - 1) / 2;
+      int below = 0, equal = 0;
+      for (int l = 0; l < 64; ++l) {
+        const int v = __shfl(diag, l);
+        if (v != INT_MIN && hit) {
+          below += v < diag;
+          equal += v == diag;
+        }
+      }
+      const bool is_median = hit && below <= half && half < below + equal;
+      const unsigned long long mb = __ballot(is_median);
+      const int med = __shfl(diag, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
+      b = int2{med - kBandLanes / 2, med + kBandLanes / 2 - 1};
+    }
+    if (lane == 0) band[tk.out0] = b;
+  }
+}
+
 }  // namespace bvp
 
 // ---- launchers ---------------------------------------------------------------
@@ -515,6 +586,15 @@ hipError_t launch_hap_tables(const uint8_t* hmask, const int64_t* hap_off, const
   if (n_haps > 0)
     hipLaunchKernelGGL(hap_table_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, hmask,
                        hap_off, tab_off, n_haps, tab);
+  return hipGetLastError();
+}
+
+hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, const int64_t* htab_off,
+                               const unsigned long long* htab, const Task* tasks, int n_tasks, int2* band,
+                               int grid, hipStream_t st) {
+  if (n_tasks > 0)
+    hipLaunchKernelGGL(long_anchor_kernel, dim3(grid), dim3(64), 0, st, rclass, read_off, htab_off, htab,
+                       tasks, n_tasks, band);
   return hipGetLastError();
 }
 

@@ -12,6 +12,8 @@ constexpr int kBandNarrow = 32;   // the narrow band: two pairs of one read per 
 constexpr int kBandEdge = 4;      // diagonals per side whose touch mass is measured
 constexpr int kBandHapMax = 1024; // longest haplotype given a band (seed tables, features)
 constexpr int kBandReadMax = 448; // longest read given a band (LDS haplotype window)
+constexpr int kLongEvery = 16;    // long-read band: rows between moves
+constexpr int kLongMove = kBandEdge;  // long-read band: diagonals per move, at most
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
 // list items 2u and 2u+1 share a read; hap < 0 marks padding.
@@ -59,6 +61,21 @@ struct BArgs {
   float tol;                       // accept when touch mass <= tol * S
   int* accepted;                   // banded pairs accepted
   unsigned long long* cells_done;  // lane-cells computed
+  // long reads (FP64 first, one task per pair): the moving-band list and the
+  // pairs left to the full-matrix FP64 pass (no band, or band rejected)
+  const Task* tasks64;
+  int n_tasks64;
+  int long_on;  // 0: every long pair takes the full matrix
+  BandItem* long_items;
+  int* long_count;
+  int* long_queue;
+  Task* long_full;
+  int* long_full_count;
+  int* long_accepted;
+  const double* ph2pr_d;  // [128]
+  const double* mm_d;     // [8256]
+  double verify_above;    // log10 L at or above which FP32 might not underflow
+  unsigned long long* fp64_cells;
 };
 
 }  // namespace bvp

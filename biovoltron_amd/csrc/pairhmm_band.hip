@@ -356,6 +356,297 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
   }
 }
 
+// ---- long reads: FP64, a band that follows the alignment -------------------
+//
+// Reads of 1024 bases or more run in FP64 (their FP32 sums underflow).  Their
+// alignment drifts across diagonals (indels), so a fixed band would have to
+// be as wide as the drift; instead the 64-diagonal band moves.  Before every
+// 16th row the state (row i - 1) is summarised by the centroid c of the lane
+// masses M + I + D, and the band moves by s = clamp(floor(c - 31.5 + 0.5),
+// -4, 4) diagonals: lane k takes lane k + s's cells, the lanes that fall out
+// are dropped and the new lanes start at zero.  At most 4 = kBandEdge lanes
+// go per move, all inside the edge zone whose touch mass T the kernel carries
+// as in the FP32 kernel above, so a dropped path's in-band relatives are
+// measured like those of a path that leaves a fixed band.  (The dropped
+// forward mass itself is no estimate of the loss: it includes prefixes with
+// no future, e.g. the spread left by a run of low-quality bases.)  A pair is
+// accepted when T/S <= tol; otherwise, or when its likelihood is within reach
+// of FP32 (the reference would keep an FP32 result), it goes to the
+// full-matrix FP64 pass.
+//
+// Precision: the DP starts from D = 1/H (the reference's K is a power of two
+// and cancels), and the state is rescaled by a power of two every 64 rows,
+// which is exact; log10 L = log10 S + (removed exponent) * log10 2.
+//
+// Per 64-row block the row records (doubles) and a 192-column window of the
+// haplotype's class masks are staged in LDS: with at most three moves of 4
+// inside a block, row i and lane k read column i + o + k of the window
+// [i0 + o0 - 12, i0 + o0 + 180).
+namespace {
+
+constexpr int kLongSlack = (kBandLanes / kLongEvery - 1) * kLongMove;  // moves inside a block
+constexpr int kLongWin = 192;
+static_assert(2 * kBandLanes + 2 * kLongSlack <= kLongWin, "long-band window too small");
+static_assert(kLongMove <= kBandEdge, "a move may only drop edge-zone lanes");
+constexpr double kLog10Two = 0.30102999566398119521;
+
+struct alignas(16) RowRec64 {
+  double dp;  // pm - pe
+  double pe;  // e / 3 (pm for a wildcard read base)
+  double mm;
+  double g;   // 1 - ph2pr[gcp]: insert/delete -> match
+  double mx;
+  double xx;
+  double my;
+  uint32_t cls;
+  uint32_t pad;
+};
+
+struct LongLds {
+  RowRec64 tab[kBandLanes];
+  uint8_t hm[kLongWin];
+};
+
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b)), CTRL, 0xf, 0xf,
+                                             true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b >> 32)), CTRL, 0xf,
+                                             0xf, true);
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo)));
+}
+
+// decay_scan<64> in FP64.
+__device__ __forceinline__ double decay_scan64(double x, double y1, double y2, double y4, double y8,
+                                               double f15, double f31) {
+  x = fma(dpp64<kRowShr + 1>(x), y1, x);
+  x = fma(dpp64<kRowShr + 2>(x), y2, x);
+  x = fma(dpp64<kRowShr + 4>(x), y4, x);
+  x = fma(dpp64<kRowShr + 8>(x), y8, x);
+  x = fma(dpp64<kRowBcast15>(x), f15, x);
+  x = fma(dpp64<kRowBcast31>(x), f31, x);
+  return x;
+}
+
+__device__ __forceinline__ double wmax(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Lane k takes lane k + s's value; lanes past the band edge take 0.
+__device__ __forceinline__ double take(double v, int k, int s) {
+  const int src = k + s;
+  const double x = __shfl(v, src & 63);
+  return (src >= 0 && src < kBandLanes) ? x : 0.0;
+}
+
+}  // namespace
+
+// Splits the long-read tasks (one pair each) into the moving-band list and
+// the full-matrix FP64 list.
+__global__ __launch_bounds__(256) void band_dispatch_long_kernel(BArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_tasks64) return;
+  const Task tk = a.tasks64[t];
+  const int2 b = a.band[tk.out0];
+  const int64_t w = static_cast<int64_t>(b.y) - b.x + 1;
+  const bool ok = a.long_on && a.gcp_const[tk.read] != 0 && w >= 1 && w <= kBandLanes &&
+                  b.x >= -(1 << 26) && b.y <= (1 << 26);
+  if (ok) {
+    const int q = atomicAdd(a.long_count, 1);
+    a.long_items[q] = BandItem{tk.read, tk.hap0, b.x - ((kBandLanes - static_cast<int>(w)) >> 1), 0, tk.out0};
+  } else {
+    const int q = atomicAdd(a.long_full_count, 1);
+    a.long_full[q] = tk;
+  }
+}
+
+__global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
+  __shared__ LongLds s;
+  const int k = threadIdx.x;
+  const bool left_lane = k < kBandEdge;
+  const bool right_lane = k >= kBandLanes - kBandEdge;
+  const int row16 = k >> 4;
+  const int n_items = *a.long_count;
+  int accepted = 0;
+  unsigned long long cells = 0;
+  for (;;) {
+    int u = 0;
+    if (k == 0) u = atomicAdd(a.long_queue, 1);
+    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0));
+    if (u >= n_items) break;
+    const BandItem w = a.long_items[u];
+    const int read = __builtin_amdgcn_readfirstlane(w.read);
+    const int hap = __builtin_amdgcn_readfirstlane(w.hap);
+    int o = __builtin_amdgcn_readfirstlane(w.o_lo);  // lane k holds diagonal o + k
+    const int64_t ro = a.read_off[read];
+    const int R = static_cast<int>(a.read_off[read + 1] - ro);
+    const int64_t ho = a.hap_off[hap];
+    const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+
+    const double yy = a.ph2pr_d[a.gcp[ro] & 127];
+    const double yy2 = yy * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4;
+    double f15 = (row16 & 1) ? 1.0 : 0.0, f31 = (row16 >= 2) ? 1.0 : 0.0;
+    for (int m = 0; m <= (k & 15); ++m) f15 *= yy;
+    for (int m = 31; m < k; ++m) f31 *= yy;
+    int ez_from = 2 - o, ez_to = H - o - kBandLanes;
+
+    // Row 0: D = 1/H on columns 0..H.
+    double Mp = 0.0, Ip = 0.0;
+    double Dp = (o + k >= 0 && o + k <= H) ? 1.0 / static_cast<double>(H) : 0.0;
+    double TMp = 0.0, TIp = 0.0, TDp = 0.0;
+    if ((left_lane && o - 1 >= 0) || (right_lane && o + kBandLanes <= H)) TDp = Dp;
+    int lg = 0;  // exponent removed by rescaling
+    int jb = 0;  // haplotype column of window slot 0
+
+    for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
+      const int n = min(kBandLanes, R - i0 + 1);
+      for (int v0 = 0; v0 < n; v0 += kLongEvery) {
+        const int i = i0 + v0;  // next row; the state holds row i - 1
+        if (i > 1) {
+          // Lanes past the haplotype end hold values that never feed
+          // in-matrix cells; clear them before the row statistics.
+          if ((i - 1) + o + k > H) Mp = Ip = Dp = TMp = TIp = TDp = 0.0;
+          const double wgt = Mp + Ip + Dp;
+          const double W = wsum(wgt);
+          const double C = wsum(static_cast<double>(k) * wgt);
+          int sft = 0;
+          if (W > 0.0) {
+            sft = static_cast<int>(floor(C / W - 0.5 * (kBandLanes - 1) + 0.5));
+            sft = min(kLongMove, max(-kLongMove, sft));
+          }
+          sft = __builtin_amdgcn_readfirstlane(sft);
+          if (sft != 0) {
+            Mp = take(Mp, k, sft);
+            Ip = take(Ip, k, sft);
+            Dp = take(Dp, k, sft);
+            TMp = take(TMp, k, sft);
+            TIp = take(TIp, k, sft);
+            TDp = take(TDp, k, sft);
+            o += sft;
+            ez_from = 2 - o;
+            ez_to = H - o - kBandLanes;
+          }
+        }
+        if (v0 == 0) {
+          if (i0 > 1) {
+            const double m = wmax(fmax(Mp, fmax(Ip, Dp)));
+            if (m > 0.0) {
+              const int e = ilogb(m);
+              const double sc = ldexp(1.0, -e);
+              Mp *= sc; Ip *= sc; Dp *= sc;
+              TMp *= sc; TIp *= sc; TDp *= sc;
+              lg += e;
+            }
+          }
+          __syncthreads();  // the previous block's LDS reads are done
+          if (k < n) {
+            const int64_t p = ro + i0 - 1 + k;
+            const uint8_t rc = a.rclass[p];
+            const int iq = a.ins[p] & 127, dq = a.del[p] & 127, cq = a.gcp[p] & 127;
+            const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
+            const double e = a.ph2pr_d[a.q[p] & 127];
+            RowRec64 rr;
+            const double pm = 1.0 - e;
+            rr.pe = (rc & kWildcard) ? pm : e / 3.0;
+            rr.dp = pm - rr.pe;
+            rr.mm = a.mm_d[((hi * (hi + 1)) >> 1) + lo];
+            rr.g = 1.0 - a.ph2pr_d[cq];
+            rr.mx = a.ph2pr_d[iq];
+            rr.xx = a.ph2pr_d[cq];
+            rr.my = a.ph2pr_d[dq];
+            rr.cls = rc & 3;
+            rr.pad = 0;
+            s.tab[k] = rr;
+          }
+          jb = i0 + o - kLongSlack;
+          for (int c = k; c < kLongWin; c += kBandLanes) {
+            const int j = jb + c;
+            s.hm[c] = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : uint8_t{0};
+          }
+          __syncthreads();
+        }
+        const int v1 = min(n, v0 + kLongEvery);
+        for (int v = v0; v < v1; ++v) {
+          const int ii = i0 + v;
+          const RowRec64 rr = s.tab[v];
+          const uint32_t hmv = s.hm[ii + o + k - jb];  // in [0, 151)
+          const double sel = static_cast<double>((hmv >> rr.cls) & 1u);
+          const double prior = fma(sel, rr.dp, rr.pe);
+          const bool ez = (left_lane && ii >= ez_from) || (right_lane && ii <= ez_to);
+
+          double t = Ip * rr.g;
+          t = fma(Dp, rr.g, t);
+          t = fma(Mp, rr.mm, t);
+          const double Mn = t * prior;
+          const double In = fma(dpp64<kWaveShl1>(Mp), rr.mx, dpp64<kWaveShl1>(Ip) * rr.xx);
+          const double Dn = decay_scan64(dpp64<kWaveShr1>(Mn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
+
+          double tt = TIp * rr.g;
+          tt = fma(TDp, rr.g, tt);
+          tt = fma(TMp, rr.mm, tt);
+          double TMn = tt * prior;
+          double TIn = fma(dpp64<kWaveShl1>(TMp), rr.mx, dpp64<kWaveShl1>(TIp) * rr.xx);
+          if (ez) {
+            TMn = Mn;
+            TIn = In;
+          }
+          double TDn = decay_scan64(dpp64<kWaveShr1>(TMn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
+          if (ez) TDn = Dn;
+
+          Mp = Mn;
+          Ip = In;
+          Dp = Dn;
+          TMp = TMn;
+          TIp = TIn;
+          TDp = TDn;
+        }
+      }
+    }
+    // S = sum over the last row of M + I, columns 1..H.
+    const int j = R + o + k;
+    const bool in = j >= 1 && j <= H;
+    const double S = wsum(in ? Mp + Ip : 0.0);
+    const double Tm = wsum(in ? TMp + TIp : 0.0);
+    cells += static_cast<unsigned long long>(R) * kBandLanes;
+    if (k == 0) {
+      const double L = log10(S) + static_cast<double>(lg) * kLog10Two;
+      if (S > 0.0 && Tm <= static_cast<double>(a.tol) * S && L < a.verify_above) {
+        a.out[w.out] = L;
+        a.form[w.out] = kFormBandFP64;
+        ++accepted;
+      } else {
+        const int q = atomicAdd(a.long_full_count, 1);
+        a.long_full[q] = Task{read, hap, 1, -2, w.out};
+      }
+    }
+  }
+  if (k == 0) {
+    if (accepted) atomicAdd(a.long_accepted, accepted);
+    if (cells) {
+      if (a.cells_done) atomicAdd(a.cells_done, cells);
+      if (a.fp64_cells) atomicAdd(a.fp64_cells, cells);
+    }
+  }
+}
+
+hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st) {
+  if (a.n_tasks64 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(band_dispatch_long_kernel, dim3((a.n_tasks64 + 255) / 256), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(band_long_kernel, dim3(grid), dim3(kBandLanes), 0, st, a);
+  return hipGetLastError();
+}
+
+int band_long_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_long_kernel, kBandLanes, 0) != hipSuccess)
+    return 8;
+  return n > 0 ? n : 1;
+}
+
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st) {
   const int blocks = (a.n_tasks + 255) / 256;
   if (blocks > 0) hipLaunchKernelGGL(band_dispatch_kernel, dim3(blocks), dim3(256), 0, st, a);

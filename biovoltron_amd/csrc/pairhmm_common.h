@@ -22,6 +22,7 @@ enum Form : uint8_t {
   kFormFP64 = 1,        // FP32 underflowed, FP64 re-run
   kFormBandFP32 = 2,    // banded FP32 forward accepted
   kFormBandFallback = 3,// band rejected (boundary mass), full FP32 used
+  kFormBandFP64 = 4,    // long read: moving-band FP64 forward accepted
   kFormPending = 255,   // internal: queued for the FP64 pass
 };
 
@@ -92,6 +93,7 @@ struct KArgs {
   int* verify_count;
   double verify_above;
   int verify_pass;  // FP32 pass over those pairs: keep the FP64 result when S < 1e-28
+  unsigned long long* fp64_cells;  // FP64 passes: sum of R*H over the pairs they compute
 };
 
 }  // namespace bvp

@@ -21,6 +21,11 @@ namespace bvp {
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
 hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st);
 int band_blocks_per_cu();
+hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
+int band_long_blocks_per_cu();
+hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, const int64_t* htab_off,
+                               const unsigned long long* htab, const Task* tasks, int n_tasks, int2* band,
+                               int grid, hipStream_t st);
 hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st);
 hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
                               float* H2, hipStream_t st);
@@ -123,7 +128,7 @@ struct DeviceCtx {
   bool ready = false;
   int device = 0;
   int cus = 0;
-  int bpc_f32 = 1, bpc_f64 = 1, bpc_band = 1;
+  int bpc_f32 = 1, bpc_f64 = 1, bpc_band = 1, bpc_long = 1;
   std::string arch;
   HostTables host{};
   float* d_ph2pr_f = nullptr;
@@ -165,6 +170,7 @@ int ensure_init_locked(int device) {
   g_ctx.bpc_f32 = max_blocks_per_cu<float>();
   g_ctx.bpc_f64 = max_blocks_per_cu<double>();
   g_ctx.bpc_band = band_blocks_per_cu();
+  g_ctx.bpc_long = band_long_blocks_per_cu();
   g_ctx.ready = true;
   return BV_OK;
 }
@@ -237,12 +243,12 @@ void parallel_for(int64_t n, F&& f) {
 struct bv_plan {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bv_options opt{};
   int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
   int hmax = 0;
   int n_tasks = 0;
-  int grid32 = 0, grid64 = 0, grid_band = 0;
+  int grid32 = 0, grid64 = 0, grid_band = 0, grid_long = 1;
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0;  // scratch is sized for these
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
@@ -264,6 +270,8 @@ struct bv_plan {
   int2* d_band = nullptr;
   BandItem* d_items = nullptr;    // banded work lists, built on the device each run
   BandItem* d_items32 = nullptr;
+  BandItem* d_items_long = nullptr;  // long reads on the moving band
+  Task* d_long_full = nullptr;       // long reads left to the full-matrix FP64 pass
   // learned bands: features and predictions computed on the device each run
   float* d_X = nullptr;
   int* d_anchor = nullptr;
@@ -280,14 +288,16 @@ struct bv_plan {
   // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
   // cells (u64), [8..9] swept cells (u64), [10] wide banded item count,
   // [11] its queue, [12] narrow banded item count, [13] its queue, [14]
-  // verify count, [15] its queue, [16] FP64-first queue
+  // verify count, [15] its queue, [16] FP64-first queue, [17] moving-band
+  // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
+  // [21] accepted moving-band pairs, [22..23] FP64 cells (u64)
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
   uint8_t* d_form = nullptr;
   void* d_carry32 = nullptr;
   void* d_carry64 = nullptr;
-  float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f, last_msband = 0.f;
+  float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f, last_msband = 0.f, last_mslong = 0.f;
   // one run captured as a graph (bv_plan_run), rebuilt when the band setup changes
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -303,7 +313,7 @@ void free_plan(bv_plan* p) {
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
-                  p->d_items32,
+                  p->d_items32, p->d_items_long, p->d_long_full,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -360,6 +370,11 @@ int validate(const bv_region_batch* b) {
   return BV_OK;
 }
 
+// Long reads run in FP64 first; a pair whose log10 L is at least this may
+// not underflow FP32 (S_f >= 1e-28 means log10 L >= -28 - log10(2^120)), so
+// the reference would keep its FP32 result.  The margin covers FP32 rounding.
+constexpr double kVerifyAbove = -28.0 - 120.0 * 0.30102999566398119521 - 0.05;
+
 KArgs base_args(const bv_plan* p) {
   KArgs a{};
   a.rclass = p->d_rclass;
@@ -382,6 +397,7 @@ KArgs base_args(const bv_plan* p) {
   a.renorm = p->opt.fp64_renorm;
   a.rescue_cells = p->d_rescue_cells;
   a.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
+  a.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
   return a;
 }
 
@@ -410,6 +426,11 @@ int ensure_band_lists(bv_plan* p) {
   // narrow items are padded to an even count per read
   if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_tasks)) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
+  const int64_t n64 = std::max(1, p->n_tasks64);
+  if (!p->d_items_long && hipMalloc(&p->d_items_long, sizeof(BandItem) * n64) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
+  if (!p->d_long_full && hipMalloc(&p->d_long_full, sizeof(Task) * n64) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
   return BV_OK;
 }
 
@@ -427,7 +448,7 @@ int ensure_feature_buffers(bv_plan* p) {
     for (int64_t h = 0; h < p->n_haps; ++h) {
       int64_t sz = 64;
       while (sz < 2 * p->hap_len[h]) sz <<= 1;
-      off[h + 1] = off[h] + (p->hap_len[h] <= kBandMaxHap ? sz : 1);
+      off[h + 1] = off[h] + (p->hap_len[h] >= kSeedK ? sz : 1);
     }
     if (hipMalloc(&p->d_htab, sizeof(unsigned long long) * std::max<int64_t>(1, off.back())) != hipSuccess ||
         hipMalloc(&p->d_htab_off, sizeof(int64_t) * off.size()) != hipSuccess)
@@ -473,6 +494,19 @@ BArgs band_args(const bv_plan* p) {
   b.tol = p->opt.band_tol;
   b.accepted = p->d_counters + 5;
   b.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
+  b.tasks64 = p->d_tasks64;
+  b.n_tasks64 = p->n_tasks64;
+  b.long_on = p->opt.fp64_renorm != 0;
+  b.long_items = p->d_items_long;
+  b.long_count = p->d_counters + 17;
+  b.long_queue = p->d_counters + 18;
+  b.long_full = p->d_long_full;
+  b.long_full_count = p->d_counters + 19;
+  b.long_accepted = p->d_counters + 21;
+  b.ph2pr_d = g_ctx.d_ph2pr_d;
+  b.mm_d = g_ctx.d_mm_d;
+  b.verify_above = kVerifyAbove;
+  b.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
   return b;
 }
 
@@ -487,6 +521,11 @@ int enqueue_run(bv_plan* p, bool timed) {
     HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
     HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_ends, p->d_X, p->n_pairs, p->band_margin,
                                         p->band_max_width, p->opt.band_narrow, p->d_band, p->stream));
+    // Long reads (outside the network's training range): start bands from
+    // the seeds of their first bases; the moving band follows from there.
+    const int grid_anchor = std::max(1, std::min(p->n_tasks64, g_ctx.cus * 32));
+    HIP_TRY(launch_long_anchors(p->d_rclass, p->d_read_off, p->d_htab_off, p->d_htab, p->d_tasks64,
+                                p->n_tasks64, p->d_band, grid_anchor, p->stream));
   }
   if (fp32 && p->has_band) {
     // Pass 1: pairs with an eligible band on the banded kernel; the rest,
@@ -545,15 +584,29 @@ int enqueue_run(bv_plan* p, bool timed) {
     // for FP32 and keep the FP32 result when it does not underflow (the
     // reference's rule, pairhmm_avx.hpp:140-149).  S_f >= 1e-28 means
     // log10 L >= -28 - log10(2^120); the FP64 value screens with a margin.
+    // With bands, the moving-band kernel takes the long pairs first and
+    // leaves the full-matrix pass those without a band or rejected (and any
+    // within FP32's reach, so that the rule above applies to them).
     KArgs a = base_args(p);
-    a.tasks = p->d_tasks64;
-    a.n_tasks = p->n_tasks64;
-    a.counter = p->d_counters + 16;
+    if (p->has_band) {
+      const BArgs b = band_args(p);
+      if (timed) HIP_TRY(hipEventRecord(p->ev[7], p->stream));
+      HIP_TRY(launch_band_long(b, p->grid_long, p->stream));
+      if (timed) HIP_TRY(hipEventRecord(p->ev[8], p->stream));
+      a.tasks = p->d_long_full;
+      a.n_tasks = 0;
+      a.n_tasks_dev = p->d_counters + 19;
+      a.counter = p->d_counters + 20;
+    } else {
+      a.tasks = p->d_tasks64;
+      a.n_tasks = p->n_tasks64;
+      a.counter = p->d_counters + 16;
+    }
     a.carry = p->d_carry64;
     a.carry_stride = p->stride64;
     a.verify = p->d_verify;
     a.verify_count = p->d_counters + 14;
-    a.verify_above = -28.0 - 120.0 * 0.30102999566398119521 - 0.05;
+    a.verify_above = kVerifyAbove;
     HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
     KArgs v = base_args(p);
     v.tasks = p->d_verify;
@@ -779,6 +832,8 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid32_max = p->grid32;
   p->grid64_max = p->grid64;
   p->grid_band_max = p->grid_band;
+  p->grid_long = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_long, p->n_tasks64)));
   p->stride32 = hmax + kCarryPad;
   p->stride64 = hmax + kCarryPad;
 
@@ -884,8 +939,9 @@ int bv_plan_run(bv_plan* p) {
 int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
   if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
   HIP_TRY(hipSetDevice(g_ctx.device));
-  float sum32 = 0.f, sum64 = 0.f, sumfb = 0.f, sumband = 0.f;
+  float sum32 = 0.f, sum64 = 0.f, sumfb = 0.f, sumband = 0.f, sumlong = 0.f;
   const bool banded = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64 && p->has_band;
+  const bool long_banded = banded && p->n_tasks64 > 0;
   HIP_TRY(hipEventRecord(p->ev[3], p->stream));
   hipEvent_t start = p->ev[3];
   // Per-run split between the FP32 and FP64 passes from events between them.
@@ -905,6 +961,11 @@ int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
       HIP_TRY(hipEventElapsedTime(&bk, p->ev[5], p->ev[6]));
       sumband += bk;
     }
+    if (long_banded) {
+      float lk = 0.f;
+      HIP_TRY(hipEventElapsedTime(&lk, p->ev[7], p->ev[8]));
+      sumlong += lk;
+    }
   }
   float total = 0.f;
   HIP_TRY(hipEventElapsedTime(&total, start, p->ev[2]));
@@ -912,6 +973,7 @@ int bv_plan_time(bv_plan* p, int iters, float* ms_per_run) {
   p->last_ms64 = sum64 / iters;
   p->last_msfb = sumfb / iters;
   p->last_msband = sumband / iters;
+  p->last_mslong = sumlong / iters;
   if (ms_per_run) *ms_per_run = total / iters;
   return BV_OK;
 }
@@ -931,26 +993,30 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   if (!p || !s) return fail(BV_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(g_ctx.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  int cnt[16] = {0};
+  int cnt[24] = {0};
   HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  unsigned long long rcells = 0, swept = 0;
-  std::memcpy(&rcells, cnt + 6, sizeof(rcells));
+  unsigned long long c64 = 0, swept = 0;
+  std::memcpy(&c64, cnt + 22, sizeof(c64));
   std::memcpy(&swept, cnt + 8, sizeof(swept));
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   s->n_pairs = p->n_pairs;
   s->cells = p->cells;
+  const bool long_banded = fp32 && p->has_band && p->n_tasks64 > 0;
+  // FP64 pairs: rescued ones and every long read (full matrix or moving band)
   s->fp64_pairs = fp32 ? cnt[2] + p->n_tasks64 : p->n_pairs;
-  s->fp64_cells = fp32 ? static_cast<int64_t>(rcells) : p->cells;
+  s->fp64_cells = static_cast<int64_t>(c64);
   s->tasks = p->n_tasks;
   s->grid_f32 = p->grid32;
   s->grid_f64 = p->grid64;
   s->last_ms_f32 = p->last_ms32;
   s->last_ms_f64 = p->last_ms64;
-  s->band_pairs = p->has_band ? cnt[5] : 0;
+  s->band_pairs = p->has_band ? cnt[5] + cnt[21] : 0;
   s->fallback_pairs = p->has_band ? cnt[3] : 0;
   s->cells_swept = static_cast<int64_t>(swept);
   s->last_ms_fallback = p->last_msfb;
   s->last_ms_band = p->last_msband;
+  s->last_ms_long = p->last_mslong;
+  s->long_band_pairs = long_banded ? cnt[21] : 0;
   return BV_OK;
 }
 

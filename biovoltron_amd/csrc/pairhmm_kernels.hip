@@ -336,6 +336,7 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
               static_cast<unsigned long long>(a.read_off[tk.read + 1] - a.read_off[tk.read]) *
               static_cast<unsigned long long>(a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h]);
           atomicAdd(a.cells_done, cells);
+          if (sizeof(T) == 8 && a.fp64_cells != nullptr) atomicAdd(a.fp64_cells, cells);
         }
         if (a.out_raw) {
           a.out[o] = ldexp(S, -shift);

@@ -32,7 +32,7 @@ from ._build import HIP_LIB
 BV_OK = 0
 MODE_AVX = 0
 MODE_SCALAR = 1
-FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_fallback"}
+FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_fallback", 4: "band_fp64"}
 
 
 class PairHMMError(RuntimeError):
@@ -64,6 +64,7 @@ class PlanStats(C.Structure):
         ("grid_f64", C.c_int32), ("last_ms_f32", C.c_float), ("last_ms_f64", C.c_float),
         ("band_pairs", C.c_int64), ("fallback_pairs", C.c_int64), ("cells_swept", C.c_int64),
         ("last_ms_fallback", C.c_float), ("last_ms_band", C.c_float),
+        ("long_band_pairs", C.c_int64), ("last_ms_long", C.c_float), ("pad", C.c_float),
     ]
 
 

@@ -51,6 +51,7 @@ extern "C" {
 #define BV_FORM_FP64 1
 #define BV_FORM_BAND_FP32 2
 #define BV_FORM_BAND_FALLBACK 3
+#define BV_FORM_BAND_FP64 4 /* long read (>= 1024 bases): moving-band FP64 forward */
 
 /* Initialise the device context and upload the tables (idempotent).  device
  * is a HIP ordinal; -1 keeps the current device. */
@@ -141,15 +142,18 @@ typedef struct bv_plan_stats {
   int64_t n_pairs;
   int64_t cells;        /* sum over pairs of R*H */
   int64_t fp64_pairs;   /* pairs computed by the FP64 pass in the last run */
-  int64_t fp64_cells;
+  int64_t fp64_cells;   /* cells computed in FP64 (full-matrix and banded) */
   int64_t tasks;
   int32_t grid_f32, grid_f64;
   float last_ms_f32, last_ms_f64; /* kernel times of the last timed run */
-  int64_t band_pairs;     /* pairs given a band in the last run */
+  int64_t band_pairs;     /* banded pairs accepted in the last run */
   int64_t fallback_pairs; /* banded pairs rejected and re-run on the full matrix */
-  int64_t cells_swept;    /* FP32 cells actually computed in the last run */
+  int64_t cells_swept;    /* cells actually computed in the last run, all passes */
   float last_ms_fallback;
   float last_ms_band;     /* banded kernels alone (both widths), last timed run */
+  int64_t long_band_pairs; /* long reads accepted on the moving band (also in band_pairs) */
+  float last_ms_long;      /* moving-band kernel (long reads), last timed run */
+  float pad;
 } bv_plan_stats;
 
 /* Uploads a batch to device memory and builds the length-ordered work list. */
@@ -167,7 +171,10 @@ int bv_plan_stats_get(bv_plan* plan, bv_plan_stats* stats);
 /* Per-pair diagonal bands for the FP32 pass: bands[2k], bands[2k+1] =
  * (o_lo, o_hi) of j - r for pair k (host memory, n_pairs x 2 int32); a pair
  * with o_lo > o_hi is evaluated on the full matrix.  NULL removes all bands.
- * No reference counterpart: the reference computes every cell. */
+ * For a read of 1024 bases or more (run in FP64) the band is where a
+ * 64-diagonal band starts: it then follows the alignment, moving at most 4
+ * diagonals every 16 rows (BV_FORM_BAND_FP64); wider bands take the full
+ * matrix.  No reference counterpart: the reference computes every cell. */
 int bv_plan_set_bands(bv_plan* plan, const int32_t* bands);
 void bv_plan_destroy(bv_plan* plan);
 

@@ -470,3 +470,119 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
   if (!(S > 0.0)) return -INFINITY;
   return (log2(S) + log2scale) * log10(2.0);
 }
+
+/* Banded FP64 forward whose band follows the alignment: `width` diagonals
+ * starting at o_init, one array slot per diagonal of the current band.
+ * Before row r (r > 1, (r - 1) % every == 0) the band moves by
+ * s = clamp(floor(c - (width - 1) / 2 + 0.5), -max_move, max_move) diagonals,
+ * c being the centroid of the band slots weighted by M + I + D of row r - 1;
+ * the slots it leaves are dropped and their share of that row's mass is added
+ * to *lost.  The edge zones and the touch DP are those of bvo_band_forward at
+ * the band's current position.  offs (optional, R + 1 entries) receives the
+ * band offset used on each row. */
+double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
+                               const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                               const uint8_t* hap, int o_init, int width, int edge, int every,
+                               int max_move, double* touch_ratio, double* lost, int* offs) {
+  bvo_init();
+  const size_t W = (size_t)width;
+  double* buf = (double*)calloc(12 * W, sizeof(double));
+  double *fM = buf, *fI = buf + W, *fD = buf + 2 * W, *tM = buf + 3 * W, *tI = buf + 4 * W,
+         *tD = buf + 5 * W, *nM = buf + 6 * W, *nI = buf + 7 * W, *nD = buf + 8 * W,
+         *uM = buf + 9 * W, *uI = buf + 10 * W, *uD = buf + 11 * W;
+  int o = o_init;
+  double lost_sum = 0.0, log2scale = 0.0;
+  /* row 0: D = 1/H on columns 0..H (the scale K is carried as log2scale = 0) */
+  for (int k = 0; k < width; ++k) {
+    const int j = o + k;
+    if (j < 0 || j > H) continue;
+    fD[k] = 1.0 / (double)H;
+    const int zone = (k < edge && o - 1 >= 0) || (k >= width - edge && o + width <= H);
+    if (zone) tD[k] = fD[k];
+  }
+  if (offs) offs[0] = o;
+  for (int r = 1; r <= R; ++r) {
+    if (r > 1 && (r - 1) % every == 0) {
+      double Wt = 0.0, C = 0.0;
+      for (int k = 0; k < width; ++k) {
+        const double w = fM[k] + fI[k] + fD[k];
+        Wt += w;
+        C += (double)k * w;
+      }
+      int s = 0;
+      if (Wt > 0.0) {
+        s = (int)floor(C / Wt - 0.5 * (double)(width - 1) + 0.5);
+        if (s > max_move) s = max_move;
+        if (s < -max_move) s = -max_move;
+      }
+      if (s != 0) {
+        double dropped = 0.0;
+        for (int k = 0; k < width; ++k) {
+          const int kept = (k - s >= 0 && k - s < width);  /* slot k lands on k - s */
+          if (!kept) dropped += fM[k] + fI[k] + fD[k];
+        }
+        lost_sum += dropped / Wt;
+        double* arrs[6] = {fM, fI, fD, tM, tI, tD};
+        for (int a = 0; a < 6; ++a) {
+          double* x = arrs[a];
+          if (s > 0) {
+            for (int k = 0; k < width; ++k) x[k] = k + s < width ? x[k + s] : 0.0;
+          } else {
+            for (int k = width - 1; k >= 0; --k) x[k] = k + s >= 0 ? x[k + s] : 0.0;
+          }
+        }
+        o += s;
+      }
+    }
+    if (offs) offs[r] = o;
+    row_par p;
+    row_params(0, r - 1, rs, q, ins, del, gcp, &p);
+    const int lact = r + o - 1 >= 1, ract = r + o + width <= H;
+    double mx = 0.0;
+    for (int k = 0; k < width; ++k) {
+      const int j = r + o + k;
+      if (j < 1 || j > H) {
+        nM[k] = nI[k] = nD[k] = uM[k] = uI[k] = uD[k] = 0.0;
+        continue;
+      }
+      const double prior = is_match(0, p.code, hap[j - 1]) ? p.pm : p.pmis;
+      const double upM = k + 1 < width ? fM[k + 1] : 0.0, upI = k + 1 < width ? fI[k + 1] : 0.0;
+      const double lfM = k >= 1 ? nM[k - 1] : 0.0, lfD = k >= 1 ? nD[k - 1] : 0.0;
+      nM[k] = (fM[k] * p.mm + fI[k] * p.im + fD[k] * p.dm) * prior;
+      nI[k] = upM * p.mi + upI * p.ii;
+      nD[k] = lfM * p.md + lfD * p.dd;
+      const int zone = (k < edge && lact) || (k >= width - edge && ract);
+      if (zone) {
+        uM[k] = nM[k];
+        uI[k] = nI[k];
+        uD[k] = nD[k];
+      } else {
+        const double tuM = k + 1 < width ? tM[k + 1] : 0.0, tuI = k + 1 < width ? tI[k + 1] : 0.0;
+        const double tlM = k >= 1 ? uM[k - 1] : 0.0, tlD = k >= 1 ? uD[k - 1] : 0.0;
+        uM[k] = (tM[k] * p.mm + tI[k] * p.im + tD[k] * p.dm) * prior;
+        uI[k] = tuM * p.mi + tuI * p.ii;
+        uD[k] = tlM * p.md + tlD * p.dd;
+      }
+      mx = fmax(mx, fmax(nM[k], fmax(nI[k], nD[k])));
+    }
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);
+    for (int k = 0; k < width; ++k) {
+      fM[k] = ldexp(nM[k], -e); fI[k] = ldexp(nI[k], -e); fD[k] = ldexp(nD[k], -e);
+      tM[k] = ldexp(uM[k], -e); tI[k] = ldexp(uI[k], -e); tD[k] = ldexp(uD[k], -e);
+    }
+    log2scale += e;
+  }
+  double S = 0.0, T = 0.0;
+  for (int k = 0; k < width; ++k) {
+    const int j = R + o + k;
+    if (j < 1 || j > H) continue;
+    S += fM[k] + fI[k];
+    T += tM[k] + tI[k];
+  }
+  free(buf);
+  if (touch_ratio) *touch_ratio = S > 0.0 ? T / S : 1.0;
+  if (lost) *lost = lost_sum;
+  if (!(S > 0.0)) return -INFINITY;
+  return (log2(S) + log2scale) * log10(2.0);
+}

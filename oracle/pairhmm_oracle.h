@@ -30,6 +30,13 @@
  *                                 DP with zero touch; brute-force enumeration
  *                                 of in-band paths and of those touching an
  *                                 edge zone (R, H <= 4).
+ *   bvo_band_follow_forward       no moves: equals bvo_band_forward; with
+ *                                 moves: brute-force enumeration of the paths
+ *                                 that stay in each row's band (from the
+ *                                 returned offsets), touch mass included;
+ *                                 dropped share against a full-matrix DP with
+ *                                 the same row masks; a planted deletion and an
+ *                                 off-centre start are followed.
  */
 #ifndef BV_PAIRHMM_ORACLE_H
 #define BV_PAIRHMM_ORACLE_H
@@ -107,6 +114,19 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
                         const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
                         const uint8_t* hap, int o_lo, int width, int edge,
                         double* touch_ratio);
+
+/* Banded FP64 forward of the long-read kernel (band_long_kernel in
+ * pairhmm_band.hip): a band
+ * of `width` diagonals starting at o_init that moves by at most max_move
+ * diagonals before every `every`-th row, towards the centroid of the previous
+ * row's mass.  Returns log10 of the banded likelihood, the touch ratio T/S
+ * (the kernel's accept test), the forward-mass share dropped by the moves
+ * (*lost, a diagnostic: it counts prefixes with no future too) and, if offs is
+ * not NULL, the band offset used on each row 0..R. */
+double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
+                               const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
+                               const uint8_t* hap, int o_init, int width, int edge, int every,
+                               int max_move, double* touch_ratio, double* lost, int* offs);
 
 #ifdef __cplusplus
 }

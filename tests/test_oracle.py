@@ -330,6 +330,199 @@ def test_band_forward_covering_band_is_the_full_matrix(oracle, golden):
         assert ratio == 0.0
 
 
+# ---- the band that follows the alignment (long reads) -----------------------
+
+def brute_force_moving_band(rows, hap, match, offs, width, edge):
+    """(likelihood, touch) over the paths that stay in each row's band
+    [offs[r], offs[r] + width - 1] and whose row-to-row steps start on a
+    diagonal the next row's band still holds (a move drops the others)."""
+    R, H = len(rows), len(hap)
+    tot = [0.0, 0.0]
+
+    def inb(r, d):
+        return offs[r] <= d <= offs[r] + width - 1
+
+    def zone(r, j):
+        d, o = j - r, offs[r]
+        lo = o - 1 >= 0 if r == 0 else r + o - 1 >= 1
+        return (d < o + edge and lo) or (d > o + width - 1 - edge and r + o + width <= H)
+
+    def prior(r, j):
+        p = rows[r - 1]
+        return p["pm"] if match(p["code"], hap[j - 1]) else p["pmis"]
+
+    def walk(state, r, j, w, touched):
+        if not inb(r, j - r):
+            return
+        touched = touched or zone(r, j)
+        if state in "MI" and r == R:
+            tot[0] += w
+            tot[1] += w if touched else 0.0
+            return
+        nxt, d = r + 1, j - r
+        down = nxt <= R and inb(nxt, d)  # this cell's slot survives into row r + 1
+        if state == "D" and r == 0:
+            if j + 1 <= H and down:
+                walk("M", 1, j + 1, w * rows[0]["dm"] * prior(1, j + 1), touched)
+            return
+        if state == "M":
+            if down and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["mm"] * prior(nxt, j + 1), touched)
+            if down:
+                walk("I", nxt, j, w * rows[nxt - 1]["mi"], touched)
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["md"], touched)
+        elif state == "I":
+            if down and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["im"] * prior(nxt, j + 1), touched)
+            if down:
+                walk("I", nxt, j, w * rows[nxt - 1]["ii"], touched)
+        elif state == "D":
+            if down and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["dm"] * prior(nxt, j + 1), touched)
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["dd"], touched)
+
+    for j0 in range(H + 1):
+        walk("D", 0, j0, 1.0 / H, False)
+    return tot[0], tot[1]
+
+
+def masked_dp_moves(rows, hap, match, o_init, width, every, max_move):
+    """Full-matrix DP (columns 0..H) with cells outside the current band set
+    to zero, moving the band by the centroid rule; returns (log10 L, dropped
+    share, offsets).  No renormalisation: for short pairs only."""
+    R, H = len(rows), len(hap)
+    o = o_init
+    M = np.zeros(H + 1); I = np.zeros(H + 1); D = np.zeros(H + 1)
+    for j in range(H + 1):
+        if o <= j <= o + width - 1:
+            D[j] = 1.0 / H
+    offs, lost = [o], 0.0
+    for r in range(1, R + 1):
+        if r > 1 and (r - 1) % every == 0:
+            cols = [j for j in range(H + 1) if o <= j - (r - 1) <= o + width - 1]
+            w = {j: M[j] + I[j] + D[j] for j in cols}
+            tot = sum(w.values())
+            s = 0
+            if tot > 0:
+                c = sum((j - (r - 1) - o) * w[j] for j in cols) / tot
+                s = max(-max_move, min(max_move, math.floor(c - (width - 1) / 2 + 0.5)))
+            if s:
+                gone = [j for j in cols if not (o + s <= j - (r - 1) <= o + s + width - 1)]
+                lost += sum(w[j] for j in gone) / tot
+                for j in gone:
+                    M[j] = I[j] = D[j] = 0.0
+                o += s
+        offs.append(o)
+        p = rows[r - 1]
+        nM = np.zeros(H + 1); nI = np.zeros(H + 1); nD = np.zeros(H + 1)
+        for j in range(1, H + 1):
+            if not (o <= j - r <= o + width - 1):
+                continue
+            pr = p["pm"] if match(p["code"], hap[j - 1]) else p["pmis"]
+            nM[j] = (M[j - 1] * p["mm"] + I[j - 1] * p["im"] + D[j - 1] * p["dm"]) * pr
+            nI[j] = M[j] * p["mi"] + I[j] * p["ii"]
+            nD[j] = nM[j - 1] * p["md"] + nD[j - 1] * p["dd"]
+        M, I, D = nM, nI, nD
+    total = (M[1:] + I[1:]).sum()
+    return (math.log10(total) if total > 0 else -math.inf), lost, np.array(offs)
+
+
+def test_band_follow_without_moves_is_the_static_band(oracle, golden):
+    pairs, _ = golden("workload1.npz")
+    rng = np.random.default_rng(5)
+    for rd, hap in pairs[::31]:
+        R, H = len(rd["seq"]), len(hap)
+        for width in (8, 64):
+            o = int(rng.integers(-R // 2, H - R // 2))
+            v, ratio, lost, offs = oracle.band_follow(rd, hap, o, width, 4, 16, 0)
+            sv, sratio = oracle.band_forward(rd, hap, o, width, 4)
+            assert (offs == o).all() and lost == 0.0
+            if np.isinf(sv):
+                assert np.isinf(v)
+                continue
+            assert v == pytest.approx(sv, abs=1e-11)
+            assert ratio == pytest.approx(sratio, rel=1e-9, abs=1e-15)
+
+
+def test_band_follow_vs_brute_force(oracle):
+    """Moving bands on tiny pairs: likelihood and touch mass against the
+    enumeration of the paths the moves keep, for the offsets the oracle chose."""
+    rng = np.random.default_rng(29)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = moved = 0
+    for rd, hap in small_cases(31, 400):
+        R, H = len(rd["seq"]), len(hap)
+        width = int(rng.integers(1, 4))
+        o = int(rng.integers(-2, H))
+        max_move = int(rng.integers(1, 3))
+        v, ratio, lost, offs = oracle.band_follow(rd, hap, o, width, 1, 1, max_move)
+        assert np.abs(np.diff(offs)).max(initial=0) <= max_move
+        rows = row_params_avx(rd, t)
+        bf, bt = brute_force_moving_band(rows, hap, match_avx, offs, width, 1)
+        if bf == 0.0:
+            assert v == -np.inf
+            continue
+        assert v == pytest.approx(math.log10(bf), abs=1e-12)
+        assert ratio == pytest.approx(bt / bf, rel=1e-9, abs=1e-15)
+        checked += 1
+        moved += bool((np.diff(offs) != 0).any())
+    assert checked > 150 and moved > 40
+
+
+def test_band_follow_moves_and_drops_as_a_masked_dp(oracle):
+    """The move rule (centroid of the previous row, clamped) and the dropped
+    share, against a full-matrix DP with row masks, on pairs of 30-60 bases."""
+    rng = np.random.default_rng(41)
+    t = np.load(GOLDEN / "tables.npz")
+    moved = 0
+    for _ in range(25):
+        H = int(rng.integers(40, 60))
+        hap = bytes(rng.choice(list(b"ACGT"), H).astype(np.uint8))
+        a = int(rng.integers(0, 10))
+        seq = bytearray(hap[a:a + int(rng.integers(25, 30))])
+        for k in rng.choice(len(seq), 3, replace=False):
+            seq[k] = int(rng.choice(list(b"ACGT")))
+        R = len(seq)
+        rd = dict(seq=bytes(seq), q=rng.integers(10, 30, R).astype(np.uint8).tobytes(),
+                  i=np.full(R, 20, np.uint8).tobytes(), d=np.full(R, 20, np.uint8).tobytes(),
+                  c=np.full(R, 10, np.uint8).tobytes())
+        width = int(rng.integers(6, 12))
+        o = a + int(rng.integers(-6, 7)) - width // 2
+        v, ratio, lost, offs = oracle.band_follow(rd, hap, o, width, 2, 4, 2)
+        lv, llost, loffs = masked_dp_moves(row_params_avx(rd, t), hap, match_avx, o, width, 4, 2)
+        assert (offs == loffs).all()
+        assert v == pytest.approx(lv, abs=1e-11)
+        assert lost == pytest.approx(llost, rel=1e-9, abs=1e-300)
+        moved += bool((np.diff(offs) != 0).any())
+    assert moved >= 10
+
+
+def _long_pair(rng, hap, pieces, q=30):
+    seq = b"".join(hap[a:b] for a, b in pieces)
+    R = len(seq)
+    return dict(seq=seq, q=np.full(R, q, np.uint8).tobytes(), i=np.full(R, 45, np.uint8).tobytes(),
+                d=np.full(R, 45, np.uint8).tobytes(), c=np.full(R, 10, np.uint8).tobytes())
+
+
+def test_band_follow_tracks_deletions_and_an_off_centre_start(oracle):
+    """Three 20-base deletions (the alignment drifts by 60 diagonals, more
+    than the band is wide) and a start 12 diagonals off the band centre: the
+    band ends centred on the final diagonal and the likelihood is the full
+    matrix's.  Without moves the same band loses the alignment."""
+    rng = np.random.default_rng(3)
+    hap = bytes(rng.choice(list(b"ACGT"), 1400).astype(np.uint8))
+    rd = _long_pair(rng, hap, [(100, 400), (420, 700), (720, 1000), (1020, 1300)])
+    v, ratio, lost, offs = oracle.band_follow(rd, hap, 100 - 32 + 12)
+    assert abs(offs[-1] + 31.5 - 160) <= 2
+    assert abs(offs[250] + 31.5 - 100) <= 2
+    assert v == pytest.approx(oracle.log10_exact(rd, hap, 0), abs=1e-6)
+    assert ratio < 1e-6 and lost < 1e-6
+    v0, ratio0, _, _ = oracle.band_follow(rd, hap, 100 - 32 + 12, max_move=0)
+    assert v0 < v - 10 and ratio0 > 1e-6
+
+
 def test_brute_force_catches_planted_mistakes():
     t = np.load(GOLDEN / "tables.npz")
     cases = list(small_cases(3, 40))
