@@ -29,6 +29,8 @@ FLOPS_PER_CELL = 12  # avx-pairhmm-template.h:188-197: 8 mul + 4 add
 BAND_FLOPS_PER_CELL = 2 * FLOPS_PER_CELL
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) datasheet peak
 PEAK_FP32_NOPACK_TFLOPS = 78.6
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector datasheet peak
+LONG_ROWS = 1024  # reads this long run in FP64 (the moving band when banded)
 
 
 def profiled_traffic(profile: str, kernel: str):
@@ -191,6 +193,10 @@ def main():
         R = np.diff(b.read_off)[rr]
         ran = (form_b == 2) | (form_b == 3)
         band_cells = int((R[ran] * (used[ran, 1] - used[ran, 0] + 1)).sum())
+        # long reads with a start band ran on the moving band (64 diagonals)
+        w = used[:, 1] - used[:, 0] + 1
+        ran_long = (R >= LONG_ROWS) & (w >= 1) & (w <= 64)
+        long_cells = int(R[ran_long].sum()) * 64
         band_info = dict(
             ms_per_step=round(ms_band, 3), gcups=round(b.cells / (ms_band / 1e3) / 1e9, 3),
             speedup_vs_full=round(ms_step / ms_band, 3),
@@ -202,10 +208,14 @@ def main():
             ms_fp64=round(sb.last_ms_f64, 3), model=Path(args.band_model).name,
             margin=args.band_margin, max_width=args.band_max_width,
             band_kernel_ms=round(sb.last_ms_band, 3), band_cells=band_cells,
+            long_pairs=int(ran_long.sum()), long_band_pairs=int(sb.long_band_pairs),
+            long_band_kernel_ms=round(sb.last_ms_long, 3), long_band_cells=long_cells,
             grids=tuned or dict(band=None, fp32=sb.grid_f32, fp64=sb.grid_f64))
         print(f"[bench] learned band: {ms_band:.2f} ms/step ({band_info['speedup_vs_full']}x), "
               f"max |dlog10| vs full {err:.2e}, fallback {sb.fallback_pairs}, "
-              f"cells swept {band_info['cells_swept_frac']}", file=sys.stderr, flush=True)
+              f"cells swept {band_info['cells_swept_frac']}, band kernels {sb.last_ms_band:.2f} ms, "
+              f"long reads {int(ran_long.sum())} ({sb.long_band_pairs} accepted, "
+              f"{sb.last_ms_long:.2f} ms), fp64 {sb.last_ms_f64:.2f} ms", file=sys.stderr, flush=True)
         out = out_b
     line = {
         "metric": METRIC,
@@ -229,7 +239,7 @@ def main():
             "learned_band": band_info,
             "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,
             "seed": synth.workload_seed(args.workload),
-            "fp64_rescued_pairs": st.fp64_pairs, "fp64_rescued_cells": st.fp64_cells,
+            "fp64_pairs": st.fp64_pairs, "fp64_cells": st.fp64_cells,
             "device": info.arch.decode(), "compute_units": info.compute_units,
             "grid_fp32": st.grid_f32, "grid_fp64": st.grid_f64,
         },
@@ -263,6 +273,13 @@ def main():
             roof["traffic_source"] = f"profiles/{tb[1]} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
         line["roofline"] = roof
         line["config"]["full_matrix"]["roofline"] = full_roof
+        if band_info["long_band_kernel_ms"] > 0:
+            ach64 = (BAND_FLOPS_PER_CELL * band_info["long_band_cells"]
+                     / (band_info["long_band_kernel_ms"] / 1e3) / 1e12)
+            band_info["long_roofline"] = {
+                "bound": "valu", "kernel": "band_long_kernel (FP64)", "achieved": round(ach64, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach64 / PEAK_FP64_TFLOPS, 4),
+                "flops_per_cell": BAND_FLOPS_PER_CELL, "kernel_ms": band_info["long_band_kernel_ms"]}
     else:
         line["roofline"] = full_roof
     if not args.no_cpu:

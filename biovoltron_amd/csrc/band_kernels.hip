@@ -508,11 +508,15 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
 // Start bands of long reads (one task per pair, run in FP64 on the band that
 // follows the alignment).  One wavefront per pair: lane l looks up the read's
 // 12-mer at position 4l (the first 256 bases) in the haplotype's table; seeds
-// found exactly once give diagonals q - p, and the band starts as the 64
-// diagonals centred on their median.  With fewer than kLongMinHits such seeds
-// the pair keeps the full matrix.
+// found exactly once give diagonals q - p; the start diagonal is their median
+// over the first 64 bases (at least kLongMinFirst seeds; else over all 256),
+// and the band starts as the 64 diagonals from it - kLongStartLift (room
+// above the alignment for a read start explained as a run of insertions).
+// With fewer than kLongMinHits such seeds the pair keeps the full matrix.
 constexpr int kLongSeedStride = 4;
 constexpr int kLongMinHits = 4;
+constexpr int kLongFirstLanes = 16;  // seeds of the first 64 bases
+constexpr int kLongMinFirst = 3;
 
 __global__ __launch_bounds__(64) void long_anchor_kernel(const uint8_t* __restrict__ rclass,
                                                          const int64_t* __restrict__ read_off,
@@ -555,21 +559,30 @@ __global__ __launch_bounds__(64) void long_anchor_kernel(const uint8_t* __restri
     }
     const bool hit = diag != INT_MIN;
     const int nhit = __popcll(__ballot(hit));
-    int2 b{1, 0};
-    if (nhit >= kLongMinHits) {
-      const int half = (nhit - 1) / 2;
+    // the median over the lanes in `use`
+    auto median = [&](bool use, int n_use) {
+      const int half = (n_use - 1) / 2;
       int below = 0, equal = 0;
       for (int l = 0; l < 64; ++l) {
         const int v = __shfl(diag, l);
-        if (v != INT_MIN && hit) {
+        const bool lu = __shfl(static_cast<int>(use), l) != 0;
+        if (lu && use) {
           below += v < diag;
           equal += v == diag;
         }
       }
-      const bool is_median = hit && below <= half && half < below + equal;
+      const bool is_median = use && below <= half && half < below + equal;
       const unsigned long long mb = __ballot(is_median);
-      const int med = __shfl(diag, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
-      b = int2{med - kBandLanes / 2, med + kBandLanes / 2 - 1};
+      return __shfl(diag, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
+    };
+    int2 b{1, 0};
+    if (nhit >= kLongMinHits) {
+      // the start diagonal: from the seeds of the first 64 bases when there
+      // are enough of them (the alignment drifts along a long read)
+      const bool first = hit && lane < kLongFirstLanes;
+      const int nfirst = __popcll(__ballot(first));
+      const int med = nfirst >= kLongMinFirst ? median(first, nfirst) : median(hit, nhit);
+      b = int2{med - kLongStartLift, med - kLongStartLift + kBandLanes - 1};
     }
     if (lane == 0) band[tk.out0] = b;
   }

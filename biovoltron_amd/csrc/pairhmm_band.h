@@ -12,8 +12,12 @@ constexpr int kBandNarrow = 32;   // the narrow band: two pairs of one read per 
 constexpr int kBandEdge = 4;      // diagonals per side whose touch mass is measured
 constexpr int kBandHapMax = 1024; // longest haplotype given a band (seed tables, features)
 constexpr int kBandReadMax = 448; // longest read given a band (LDS haplotype window)
+constexpr int kBandSlide = 8;     // wide band: start slide (diagonals)
+constexpr float kBandHold = 1e-9f;// a move that would drop more of a row's mass is held
 constexpr int kLongEvery = 16;    // long-read band: rows between moves
 constexpr int kLongMove = kBandEdge;  // long-read band: diagonals per move, at most
+constexpr int kLongStartLift = 16;// long-read start band: the anchor sits this far above lane 0
+                                  // less than the centre (room above the alignment)
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
 // list items 2u and 2u+1 share a read; hap < 0 marks padding.

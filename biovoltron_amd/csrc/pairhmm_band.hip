@@ -63,13 +63,14 @@ struct alignas(16) RowRec {
   uint32_t cls; // this row's read base class (selector slot)
 };
 
-// Match selectors of the band's haplotype window, per lane group: column
-// c = i - 1 + k of the window is haplotype column j = o_lo + 1 + c (row i,
-// lane k), and sel[g][cls][c] is 1 where that column matches read class cls
-// (0 outside the haplotype).  The prior is then one mixed-precision FMA,
-// sel * (pm - pe) + pe.  tab has one spare entry so that the next row's
+// Match selectors of the band's haplotype window, per lane group: slot c of
+// the window is haplotype column j = wb + 1 + c, wb being the band's lowest
+// offset, so row i, lane k (column i + o(i) + k) reads slot
+// i - 1 + k + o(i) - wb; sel[g][cls][c] is 1 where that column matches read
+// class cls (0 outside the haplotype).  The prior is then one mixed-precision
+// FMA, sel * (pm - pe) + pe.  tab has one spare entry so that the next row's
 // record can always be read ahead.
-constexpr int kSelCols = kBandReadMax + kBandLanes;
+constexpr int kSelCols = kBandReadMax + kBandLanes + 2 * kBandSlide;
 template <int P>
 struct BandLds {
   RowRec tab[kBandLanes + 1];
@@ -91,6 +92,12 @@ __device__ __forceinline__ float decay_scan(float x, float yy1, float yy2, float
   x = __builtin_fmaf(dpp<kRowBcast15>(x), f15, x);
   if constexpr (G == 64) x = __builtin_fmaf(dpp<kRowBcast31>(x), f31, x);
   return x;
+}
+
+__device__ __forceinline__ float wsumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
 __device__ __forceinline__ double wsum(double v) {
@@ -205,18 +212,31 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     const BandItem w = items[P * u + grp];
     const bool live = w.hap >= 0;
     const int read = __builtin_amdgcn_readfirstlane(w.read);
-    const int hap = live ? w.hap : items[P * u].hap;  // padding mirrors slot 0
+    int hap = live ? w.hap : items[P * u].hap;  // padding mirrors slot 0
+    int o_lo = live ? w.o_lo : items[P * u].o_lo;
+    if constexpr (G == kBandLanes) {  // one pair: wave-uniform
+      hap = __builtin_amdgcn_readfirstlane(hap);
+      o_lo = __builtin_amdgcn_readfirstlane(o_lo);
+    }
     const int64_t ro = a.read_off[read];
     const int R = static_cast<int>(a.read_off[read + 1] - ro);
     const int64_t ho = a.hap_off[hap];
     const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
-    const int o_lo = live ? w.o_lo : items[P * u].o_lo;
-    const int d = o_lo + k;  // this lane's diagonal
+    // Start slide (wide band only): the band starts Pt diagonals higher and
+    // steps down one diagonal per row to o_lo, so that a read start explained
+    // as a run of insertions (which sweeps down one diagonal per row from
+    // above) stays inside; capped where the matrix ends.  A step is held
+    // while the top lane it would drop carries more than kBandHold of the
+    // row's mass.  off(r) is the offset the schedule asks for on row r.
+    constexpr int kSlideG = G == kBandLanes ? kBandSlide : 0;
+    const int Pt = min(kSlideG, max(0, (H - 1) - (o_lo + G - 1)));
+    auto off = [&](int r) { return o_lo + max(0, Pt + 1 - r); };
+    const int wb = o_lo;  // lowest band offset: window slot c is column wb + 1 + c
     _Float16 (*selL)[kSelCols] = s.sel[grp];
 
     __syncthreads();  // the previous unit's LDS reads are done
-    for (int c = k; c < R + G; c += G) {
-      const int j = o_lo + 1 + c;
+    for (int c = k; c < R + G + Pt; c += G) {
+      const int j = wb + 1 + c;
       const uint32_t m = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : 0u;
 #pragma unroll
       for (int q = 0; q < 4; ++q) selL[q][c] = ((m >> q) & 1u) ? _Float16(1) : _Float16(0);
@@ -228,17 +248,24 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     float f15 = (row16 & 1) ? 1.f : 0.f, f31 = (row16 >= 2) ? 1.f : 0.f;
     for (int m = 0; m <= (lane & 15); ++m) f15 *= yy;  // distance to lane 15 of the row below
     for (int m = 31; m < lane; ++m) f31 *= yy;        // distance to lane 31
-    // Edge zones: the left one from the first row whose left neighbour column
-    // is inside the matrix, the right one while the right neighbour column is.
-    const int ez_from = left_lane ? 2 - o_lo : INT_MAX;
-    const int ez_to = right_lane ? H - o_lo - G : INT_MIN;
+    // Edge zones (T = F there): the left one where the band's left
+    // neighbour column is inside the matrix, the right one where its right
+    // neighbour column is.
 
     // Row 0: M = I = 0, D = K/H for columns 0..H (Context.h:109, template :164-175).
+    int o_cur = off(1);
     float Mp = 0.f, Ip = 0.f;
-    float Dp = (d >= 0 && d <= H) ? kK / static_cast<float>(H) : 0.f;
+    // Insertion runs across the band's long edges, added to the touch mass:
+    // ent = K/H * dm(1) * mx(2) * yy^(i-2) is a run M(1, c), I down column c,
+    // that enters the top lane at row i (times the row-1 prior at c); xg (in
+    // the bottom lane) carries the I mass that left through the bottom lane,
+    // continued as pure runs to the last row.
+    float ent = 0.f, xg = 0.f, pe1 = 0.f, dp1 = 0.f;
+    int cls1 = 0;
+    float Dp = (o_cur + k >= 0 && o_cur + k <= H) ? kK / static_cast<float>(H) : 0.f;
     float TMp = 0.f, TIp = 0.f, TDp = 0.f;
     {
-      const bool ez = (left_lane && o_lo - 1 >= 0) || (right_lane && o_lo + G <= H);
+      const bool ez = (left_lane && o_cur - 1 >= 0) || (right_lane && o_cur + G <= H);
       if (ez) TDp = Dp;
     }
 
@@ -267,19 +294,52 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
       }
       __syncthreads();
       const int n = min(kBandLanes, R - i0 + 1);
-      // Software pipeline: the LDS reads for row u+1 are issued before row u
-      // is computed.
+      if (i0 == 1) {
+        const RowRec r1 = s.tab[0];
+        pe1 = r1.pe;
+        dp1 = r1.dp;
+        cls1 = __builtin_amdgcn_readfirstlane(r1.cls);
+        ent = R >= 2 ? kK / static_cast<float>(H) * r1.g * s.tab[1].mx : 0.f;
+      }
+      // Software pipeline: the next row's record is read before this row is
+      // computed.
       RowRec rr_next = s.tab[0];
-      float sel_next = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr_next.cls)][i0 - 1 + k]);
       for (int v = 0; v < n; ++v) {
         const int i = i0 + v;
         const RowRec rr = rr_next;
-        const float sel = sel_next;
+        const int oi = off(i);
         rr_next = s.tab[v + 1];
-        // i + k < R + G <= kSelCols
-        sel_next = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr_next.cls)][i + k]);
+        if constexpr (kSlideG > 0) {
+          if (oi < o_cur) {
+            // slide down one diagonal: lane k takes lane k - 1's cells and
+            // the top lane is dropped, unless it holds more than kBandHold
+            // of the row's mass
+            const float wl = Mp + Ip + Dp;
+            const float wtop = __shfl(wl, kBandLanes - 1);
+            if (wtop <= kBandHold * wsumf(wl)) {
+              Mp = dpp<kWaveShr1>(Mp);
+              Ip = dpp<kWaveShr1>(Ip);
+              Dp = dpp<kWaveShr1>(Dp);
+              TMp = dpp<kWaveShr1>(TMp);
+              TIp = dpp<kWaveShr1>(TIp);
+              TDp = dpp<kWaveShr1>(TDp);
+              --o_cur;
+            }
+          }
+        }
+        const int oc = kSlideG > 0 ? o_cur : oi;
+        const int col = i + oc + k;  // this lane's column on row i
+        // window slot i - 1 + k + oc - wb is in [0, R + G + Pt) for i <= R
+        const float sel = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr.cls)][i - 1 + k + oc - wb]);
         const float prior = __builtin_fmaf(sel, rr.dp, rr.pe);
-        const bool ez = i >= ez_from || i <= ez_to;
+        const bool ez = (left_lane && i + oc >= 2) || (right_lane && i + oc + G <= H);
+        // insertion runs: the I mass leaving the bottom lane (from row i - 1)
+        // and the pure run entering the top lane on row i
+        xg = __builtin_fmaf(xg, yy, (bottom_lane && col - 1 >= 1 && col - 1 <= H)
+                                        ? __builtin_fmaf(Mp, rr.mx, Ip * rr.xx) : 0.f);
+        const float sel1 = static_cast<float>(selL[cls1][i - 1 + k + oc - wb]);
+        const float enter = (top_lane && i >= 2 && col <= H) ? ent * __builtin_fmaf(sel1, dp1, pe1) : 0.f;
+        if (i >= 2) ent *= yy;
 
         // forward
         float t = Ip * rr.g;
@@ -310,6 +370,7 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           if (top_lane) TIn = 0.f;
           if (bottom_lane) tx = 0.f;
         }
+        TIn += enter;
         float TDn = decay_scan<G>(tx, yy, yy2, yy4, yy8, f15, f31);
         if (ez) TDn = Dn;
 
@@ -322,10 +383,10 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
       }
     }
     // S = sum over the last row of M + I (template :328-343), columns 1..H.
-    const int j = R + d;
+    const int j = R + (kSlideG > 0 ? o_cur : off(R)) + k;
     const bool in = j >= 1 && j <= H;
     double S = in ? static_cast<double>(Mp) + static_cast<double>(Ip) : 0.0;
-    double Tm = in ? static_cast<double>(TMp) + static_cast<double>(TIp) : 0.0;
+    double Tm = (in ? static_cast<double>(TMp) + static_cast<double>(TIp) : 0.0) + static_cast<double>(xg);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
       S += __shfl_xor(S, o);
@@ -363,16 +424,19 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
 // be as wide as the drift; instead the 64-diagonal band moves.  Before every
 // 16th row the state (row i - 1) is summarised by the centroid c of the lane
 // masses M + I + D, and the band moves by s = clamp(floor(c - 31.5 + 0.5),
-// -4, 4) diagonals: lane k takes lane k + s's cells, the lanes that fall out
+// -4, 4) diagonals (the start band, from the seeds, leaves 16 extra
+// diagonals above the alignment for a read start explained as a run of
+// insertions, which sweeps down one diagonal per row): lane k takes lane
+// k + s's cells, the lanes that fall out
 // are dropped and the new lanes start at zero.  At most 4 = kBandEdge lanes
 // go per move, all inside the edge zone whose touch mass T the kernel carries
-// as in the FP32 kernel above, so a dropped path's in-band relatives are
-// measured like those of a path that leaves a fixed band.  (The dropped
-// forward mass itself is no estimate of the loss: it includes prefixes with
-// no future, e.g. the spread left by a run of low-quality bases.)  A pair is
-// accepted when T/S <= tol; otherwise, or when its likelihood is within reach
-// of FP32 (the reference would keep an FP32 result), it goes to the
-// full-matrix FP64 pass.
+// as in the FP32 kernel above.  A move is held when the lanes it would drop
+// carry more than kBandHold of the row's forward mass (a cut mode can still
+// have a large future; forward mass that small rarely does).  The touch mass
+// also counts the insertion runs across the long edges, as in the FP32
+// kernel.  A pair is accepted when T/S <= tol; otherwise, or when its
+// likelihood is within reach of FP32 (the reference would keep an FP32
+// result), it goes to the full-matrix FP64 pass.
 //
 // Precision: the DP starts from D = 1/H (the reference's K is a power of two
 // and cancels), and the state is rescaled by a power of two every 64 rows,
@@ -501,6 +565,10 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
     if ((left_lane && o - 1 >= 0) || (right_lane && o + kBandLanes <= H)) TDp = Dp;
     int lg = 0;  // exponent removed by rescaling
     int jb = 0;  // haplotype column of window slot 0
+    // insertion runs across the long edges (as in band_forward_kernel):
+    // ent enters the top lane, xg (bottom lane) carries what left below
+    double ent = 0.0, xg = 0.0, pe1 = 0.0, dp1 = 0.0;
+    uint32_t cls1 = 0;
 
     for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
       const int n = min(kBandLanes, R - i0 + 1);
@@ -519,6 +587,11 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
             sft = min(kLongMove, max(-kLongMove, sft));
           }
           sft = __builtin_amdgcn_readfirstlane(sft);
+          if (sft != 0) {
+            // hold: no move that drops more than kBandHold of the row's mass
+            const bool gone = k - sft < 0 || k - sft >= kBandLanes;
+            if (wsum(gone ? wgt : 0.0) > static_cast<double>(kBandHold) * W) sft = 0;
+          }
           if (sft != 0) {
             Mp = take(Mp, k, sft);
             Ip = take(Ip, k, sft);
@@ -539,6 +612,8 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
               const double sc = ldexp(1.0, -e);
               Mp *= sc; Ip *= sc; Dp *= sc;
               TMp *= sc; TIp *= sc; TDp *= sc;
+              ent *= sc;
+              xg *= sc;
               lg += e;
             }
           }
@@ -568,6 +643,13 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
             s.hm[c] = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : uint8_t{0};
           }
           __syncthreads();
+          if (i0 == 1) {
+            const RowRec64 r1 = s.tab[0];
+            pe1 = r1.pe;
+            dp1 = r1.dp;
+            cls1 = __builtin_amdgcn_readfirstlane(r1.cls);
+            ent = R >= 2 ? r1.g * s.tab[1].mx / static_cast<double>(H) : 0.0;
+          }
         }
         const int v1 = min(n, v0 + kLongEvery);
         for (int v = v0; v < v1; ++v) {
@@ -577,6 +659,12 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
           const double sel = static_cast<double>((hmv >> rr.cls) & 1u);
           const double prior = fma(sel, rr.dp, rr.pe);
           const bool ez = (left_lane && ii >= ez_from) || (right_lane && ii <= ez_to);
+          const int col = ii + o + k;
+          xg = fma(xg, yy, (k == 0 && col - 1 >= 1 && col - 1 <= H) ? fma(Mp, rr.mx, Ip * rr.xx) : 0.0);
+          const double enter = (k == kBandLanes - 1 && ii >= 2 && col <= H)
+                                   ? ent * fma(static_cast<double>((hmv >> cls1) & 1u), dp1, pe1)
+                                   : 0.0;
+          if (ii >= 2) ent *= yy;
 
           double t = Ip * rr.g;
           t = fma(Dp, rr.g, t);
@@ -594,6 +682,7 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
             TMn = Mn;
             TIn = In;
           }
+          TIn += enter;
           double TDn = decay_scan64(dpp64<kWaveShr1>(TMn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
           if (ez) TDn = Dn;
 
@@ -610,7 +699,7 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
     const int j = R + o + k;
     const bool in = j >= 1 && j <= H;
     const double S = wsum(in ? Mp + Ip : 0.0);
-    const double Tm = wsum(in ? TMp + TIp : 0.0);
+    const double Tm = wsum((in ? TMp + TIp : 0.0) + xg);
     cells += static_cast<unsigned long long>(R) * kBandLanes;
     if (k == 0) {
       const double L = log10(S) + static_cast<double>(lg) * kLog10Two;

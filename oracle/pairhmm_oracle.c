@@ -471,26 +471,33 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
   return (log2(S) + log2scale) * log10(2.0);
 }
 
-/* Banded FP64 forward whose band follows the alignment: `width` diagonals
- * starting at o_init, one array slot per diagonal of the current band.
- * Before row r (r > 1, (r - 1) % every == 0) the band moves by
- * s = clamp(floor(c - (width - 1) / 2 + 0.5), -max_move, max_move) diagonals,
- * c being the centroid of the band slots weighted by M + I + D of row r - 1;
- * the slots it leaves are dropped and their share of that row's mass is added
- * to *lost.  The edge zones and the touch DP are those of bvo_band_forward at
- * the band's current position.  offs (optional, R + 1 entries) receives the
- * band offset used on each row. */
+/* Banded FP64 forward whose band moves: `width` diagonals, one array slot per
+ * diagonal of the current band.  Before row r the band may move by s
+ * diagonals (slot k then holds the cells of slot k + s, the slots that fall
+ * out are dropped, the new ones start at zero):
+ *  - sched == NULL: when r > 1 and (r - 1) % every == 0, towards the
+ *    centroid c of the slot masses M + I + D of row r - 1:
+ *    s = clamp(floor(c - t + 0.5), -max_move, max_move), t = (width - 1) / 2,
+ *    plus end_bias on rows r > R - end_rows; the band starts at o_init;
+ *  - sched != NULL: towards the scheduled offset, s = clamp(sched[r] - o,
+ *    -max_move, max_move); the band starts at sched[0].
+ * Hold rule (hold_eps > 0): a move whose dropped slots carry more than
+ * hold_eps of row r - 1's mass is not made.  The edge zones and the touch DP
+ * are those of bvo_band_forward at the band's current position.  offs
+ * (optional, R + 1 entries) receives the offset used on each row. */
 double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
                                const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
                                const uint8_t* hap, int o_init, int width, int edge, int every,
-                               int max_move, double* touch_ratio, double* lost, int* offs) {
+                               int max_move, int end_rows, int end_bias, double hold_eps,
+                               int irun, const int* sched, double* touch_ratio, double* lost,
+                               int* offs) {
   bvo_init();
   const size_t W = (size_t)width;
   double* buf = (double*)calloc(12 * W, sizeof(double));
   double *fM = buf, *fI = buf + W, *fD = buf + 2 * W, *tM = buf + 3 * W, *tI = buf + 4 * W,
          *tD = buf + 5 * W, *nM = buf + 6 * W, *nI = buf + 7 * W, *nD = buf + 8 * W,
          *uM = buf + 9 * W, *uI = buf + 10 * W, *uD = buf + 11 * W;
-  int o = o_init;
+  int o = sched ? sched[0] : o_init;
   double lost_sum = 0.0, log2scale = 0.0;
   /* row 0: D = 1/H on columns 0..H (the scale K is carried as log2scale = 0) */
   for (int k = 0; k < width; ++k) {
@@ -501,43 +508,67 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
     if (zone) tD[k] = fD[k];
   }
   if (offs) offs[0] = o;
+  /* irun: insertion runs across the band's long edges.  A read start
+   * explained as insertions is a path M(1, c) then I down column c; where c
+   * lies above the band it enters at the top slot (row r, column c): that
+   * pure run is added to the touch DP there.  A read end explained as
+   * insertions leaves through the bottom slot: the I mass that leaves is
+   * carried on as pure runs (x, times ii per row) and added to T at the end. */
+  row_par p1, p2;
+  row_params(0, 0, rs, q, ins, del, gcp, &p1);
+  row_params(0, R >= 2 ? 1 : 0, rs, q, ins, del, gcp, &p2);
+  double run_prod = 1.0, x_tail = 0.0;
   for (int r = 1; r <= R; ++r) {
-    if (r > 1 && (r - 1) % every == 0) {
-      double Wt = 0.0, C = 0.0;
-      for (int k = 0; k < width; ++k) {
-        const double w = fM[k] + fI[k] + fD[k];
-        Wt += w;
-        C += (double)k * w;
-      }
-      int s = 0;
-      if (Wt > 0.0) {
-        s = (int)floor(C / Wt - 0.5 * (double)(width - 1) + 0.5);
-        if (s > max_move) s = max_move;
-        if (s < -max_move) s = -max_move;
-      }
-      if (s != 0) {
-        double dropped = 0.0;
-        for (int k = 0; k < width; ++k) {
-          const int kept = (k - s >= 0 && k - s < width);  /* slot k lands on k - s */
-          if (!kept) dropped += fM[k] + fI[k] + fD[k];
+    int s = 0;
+    double Wt = 0.0, C = 0.0;
+    for (int k = 0; k < width; ++k) {
+      const double w = fM[k] + fI[k] + fD[k];
+      Wt += w;
+      C += (double)k * w;
+    }
+    if (sched) {
+      s = sched[r] - o;
+    } else if (r > 1 && (r - 1) % every == 0 && Wt > 0.0) {
+      const double target = 0.5 * (double)(width - 1) + (r > R - end_rows ? (double)end_bias : 0.0);
+      s = (int)floor(C / Wt - target + 0.5);
+    }
+    if (s > max_move) s = max_move;
+    if (s < -max_move) s = -max_move;
+    if (s != 0) {
+      double dropped = 0.0;
+      for (int k = 0; k < width; ++k)
+        if (!(k - s >= 0 && k - s < width)) dropped += fM[k] + fI[k] + fD[k];  /* slot k lands on k - s */
+      if (hold_eps > 0.0 && dropped > hold_eps * Wt) s = 0;
+      else if (Wt > 0.0) lost_sum += dropped / Wt;
+    }
+    if (s != 0) {
+      double* arrs[6] = {fM, fI, fD, tM, tI, tD};
+      for (int a = 0; a < 6; ++a) {
+        double* x = arrs[a];
+        if (s > 0) {
+          for (int k = 0; k < width; ++k) x[k] = k + s < width ? x[k + s] : 0.0;
+        } else {
+          for (int k = width - 1; k >= 0; --k) x[k] = k + s >= 0 ? x[k + s] : 0.0;
         }
-        lost_sum += dropped / Wt;
-        double* arrs[6] = {fM, fI, fD, tM, tI, tD};
-        for (int a = 0; a < 6; ++a) {
-          double* x = arrs[a];
-          if (s > 0) {
-            for (int k = 0; k < width; ++k) x[k] = k + s < width ? x[k + s] : 0.0;
-          } else {
-            for (int k = width - 1; k >= 0; --k) x[k] = k + s >= 0 ? x[k + s] : 0.0;
-          }
-        }
-        o += s;
       }
+      o += s;
     }
     if (offs) offs[r] = o;
     row_par p;
     row_params(0, r - 1, rs, q, ins, del, gcp, &p);
     const int lact = r + o - 1 >= 1, ract = r + o + width <= H;
+    double top_in = 0.0;
+    if (irun) {
+      if (r >= 3) run_prod *= p.ii;
+      const int c = r + o + width - 1;
+      if (r >= 2 && c >= 1 && c <= H) {
+        const double pr1 = is_match(0, p1.code, hap[c - 1]) ? p1.pm : p1.pmis;
+        top_in = ldexp(p1.dm * pr1 * p2.mi * run_prod / (double)H, -(int)log2scale);
+      }
+      x_tail *= p.ii;
+      const int c0 = r - 1 + o;
+      if (c0 >= 1 && c0 <= H) x_tail += fM[0] * p.mi + fI[0] * p.ii;
+    }
     double mx = 0.0;
     for (int k = 0; k < width; ++k) {
       const int j = r + o + k;
@@ -565,8 +596,10 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
       }
       mx = fmax(mx, fmax(nM[k], fmax(nI[k], nD[k])));
     }
+    uI[width - 1] += top_in;
     int e = 0;
     if (mx > 0.0) frexp(mx, &e);
+    x_tail = ldexp(x_tail, -e);
     for (int k = 0; k < width; ++k) {
       fM[k] = ldexp(nM[k], -e); fI[k] = ldexp(nI[k], -e); fD[k] = ldexp(nD[k], -e);
       tM[k] = ldexp(uM[k], -e); tI[k] = ldexp(uI[k], -e); tD[k] = ldexp(uD[k], -e);
@@ -580,6 +613,7 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
     S += fM[k] + fI[k];
     T += tM[k] + tI[k];
   }
+  T += x_tail;
   free(buf);
   if (touch_ratio) *touch_ratio = S > 0.0 ? T / S : 1.0;
   if (lost) *lost = lost_sum;

@@ -115,18 +115,26 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
                         const uint8_t* hap, int o_lo, int width, int edge,
                         double* touch_ratio);
 
-/* Banded FP64 forward of the long-read kernel (band_long_kernel in
- * pairhmm_band.hip): a band
- * of `width` diagonals starting at o_init that moves by at most max_move
- * diagonals before every `every`-th row, towards the centroid of the previous
- * row's mass.  Returns log10 of the banded likelihood, the touch ratio T/S
- * (the kernel's accept test), the forward-mass share dropped by the moves
- * (*lost, a diagnostic: it counts prefixes with no future too) and, if offs is
- * not NULL, the band offset used on each row 0..R. */
+/* Banded FP64 forward whose band moves (the long-read kernel's band, and the
+ * wide band with its end slides, both in pairhmm_band.hip): `width` diagonals
+ * starting at o_init (or sched[0]); before a row the band moves towards the
+ * centroid of the previous row's mass every `every` rows (by at most
+ * max_move, aiming end_bias slots above the centre on the last end_rows
+ * rows), or, when sched is not NULL, towards the offsets sched[0..R] (by at
+ * most max_move per row).  A move is not made when the slots it would drop
+ * hold more than hold_eps of the row's mass (hold_eps > 0).  irun != 0 adds to
+ * the touch mass the insertion runs that enter through the top edge from a
+ * read start above the band and that leave through the bottom edge to end
+ * the read below it.  Returns log10 of
+ * the banded likelihood, the touch ratio T/S (the accept test), the mass share
+ * the moves dropped (*lost) and, if offs is not NULL, the band offset used on
+ * each row 0..R. */
 double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
                                const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
                                const uint8_t* hap, int o_init, int width, int edge, int every,
-                               int max_move, double* touch_ratio, double* lost, int* offs);
+                               int max_move, int end_rows, int end_bias, double hold_eps,
+                               int irun, const int* sched, double* touch_ratio, double* lost,
+                               int* offs);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,7 @@ class Oracle:
         lib.bvo_band_forward.restype = D
         lib.bvo_band_forward.argtypes = [I, I] + [P] * 6 + [I, I, I, C.c_void_p]
         lib.bvo_band_follow_forward.restype = D
-        lib.bvo_band_follow_forward.argtypes = [I, I] + [P] * 6 + [I] * 5 + [C.c_void_p] * 3
+        lib.bvo_band_follow_forward.argtypes = [I, I] + [P] * 6 + [I] * 7 + [D, I] + [C.c_void_p] * 4
         lib.bvo_region_log10.restype = C.c_long
         lib.bvo_region_log10.argtypes = [I] + [C.c_void_p] * 6 + [I, C.c_void_p, C.c_void_p, I, C.c_void_p]
         for n in ("bvo_ph2pr_f32", "bvo_mm_f32"):
@@ -76,13 +76,20 @@ class Oracle:
         v = self.lib.bvo_band_forward(*self._args(rd, hap), int(o_lo), int(width), int(edge), C.byref(t))
         return v, t.value
 
-    def band_follow(self, rd, hap, o_init, width=64, edge=4, every=16, max_move=4):
+    def band_follow(self, rd, hap, o_init, width=64, edge=4, every=16, max_move=4, end_rows=0,
+                    end_bias=0, hold_eps=0.0, irun=0, sched=None):
         """(log10 banded likelihood, touch ratio, dropped share, per-row band
-        offsets) of the FP64 forward whose band follows the alignment."""
+        offsets) of the FP64 forward whose band moves (towards the row mass
+        centroid, or along sched)."""
         t, lost = C.c_double(0.0), C.c_double(0.0)
         offs = np.zeros(len(rd["seq"]) + 1, np.int32)
+        sp = None
+        if sched is not None:
+            sched = np.ascontiguousarray(sched, np.int32)
+            sp = sched.ctypes.data
         v = self.lib.bvo_band_follow_forward(*self._args(rd, hap), int(o_init), int(width), int(edge),
-                                             int(every), int(max_move), C.byref(t), C.byref(lost),
+                                             int(every), int(max_move), int(end_rows), int(end_bias),
+                                             float(hold_eps), int(irun), sp, C.byref(t), C.byref(lost),
                                              offs.ctypes.data)
         return v, t.value, lost.value, offs
 

@@ -264,12 +264,44 @@ def test_learned_band_against_heuristic_band(tmp_path):
     assert np.array_equal(loaded.params(), model.params())
 
 
+SLIDE = 8  # the wide band's start slide (diagonals)
+HOLD = 1e-9  # a move that drops more of its row's mass is held
+
+
+def slide_schedule(R, H, o_lo, width):
+    """Per-row offsets the wide band's start slide asks for: up to SLIDE
+    diagonals higher on rows 0-1, one lower per row down to o_lo; capped
+    where the matrix ends.  The narrow band does not slide."""
+    P = SLIDE if width == 64 else 0
+    pt = min(P, max(0, (H - 1) - (o_lo + width - 1)))
+    off = [o_lo + max(0, pt + 1 - r) for r in range(1, R + 1)]
+    return np.array([off[0]] + off, np.int32)
+
+
+def oracle_band(oracle, rd, hap, o_lo, width):
+    """The oracle's restatement of the banded kernels: the slide (held by the
+    hold rule) and the insertion runs across the long edges."""
+    R, H = len(rd["seq"]), len(hap)
+    v, ratio, _, _ = oracle.band_follow(rd, hap, o_lo, width, 4, max_move=1, hold_eps=HOLD, irun=1,
+                                        sched=slide_schedule(R, H, o_lo, width))
+    return v, ratio
+
+
+def test_slide_schedule_shape():
+    s = slide_schedule(100, 400, 50, 64)
+    assert s[0] == s[1] == 58 and s[9] == 50 and s[100] == 50
+    assert np.abs(np.diff(s)).max() == 1
+    # a band already past the matrix's top does not slide
+    assert (slide_schedule(20, 30, -22, 64) == -22).all()
+    assert (slide_schedule(100, 400, 50, 32) == 50).all()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("width", [64, 32])
 def test_band_kernel_matches_oracle_band_forward(oracle, width):
     """The banded kernel against the oracle's FP64 restatement of the same
     band (likelihood and the touch-mass decision), on bands placed around and
-    across the alignment."""
+    across the alignment; the wide band with its end slides."""
     sel, pb = fixture_batch(step=2)
     lab = band.labels(pb)
     rng = np.random.default_rng(width)
@@ -280,7 +312,7 @@ def test_band_kernel_matches_oracle_band_forward(oracle, width):
     n_acc = n_rej = 0
     worst = 0.0
     for k, (rd, hap) in enumerate(sel):
-        v, ratio = oracle.band_forward(rd, hap, bands[k, 0], width, 4)
+        v, ratio = oracle_band(oracle, rd, hap, int(bands[k, 0]), width)
         if form[k] == 2:
             n_acc += 1
             worst = max(worst, abs(out[k] - v))

@@ -16,6 +16,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-3
 FORM_FP64, FORM_BAND_FP64 = 1, 4
 VERIFY_ABOVE = -28.0 - 120.0 * np.log10(2.0) - 0.05
+# the moving band's rules: moves that would drop more than 1e-9 of a row's
+# mass are held; the touch mass counts the insertion runs across the edges
+RULES = dict(hold_eps=1e-9, irun=1)
 
 
 def say(*a):
@@ -55,7 +58,7 @@ def check_against_oracle(oracle, sel, out, form, used, tol):
             assert form[k] in (0, FORM_FP64), k
             continue
         o = b0 - (64 - (b1 - b0 + 1)) // 2
-        v, ratio, lost, _ = oracle.band_follow(rd, hap, o)
+        v, ratio, lost, _ = oracle.band_follow(rd, hap, o, **RULES)
         if form[k] == FORM_BAND_FP64:
             n_acc += 1
             worst = max(worst, abs(out[k] - v))
@@ -143,7 +146,7 @@ def test_long_band_follows_planted_drift(oracle):
     bands = np.array([[s - 32 + 12, s + 31 + 12] for s in starts], np.int32)
     out, form, st, used = run_plan(pb, bands=bands)
     for k, (rd, h) in enumerate(sel):
-        v, ratio, lost, offs = oracle.band_follow(rd, h, int(bands[k, 0]))
+        v, ratio, lost, offs = oracle.band_follow(rd, h, int(bands[k, 0]), **RULES)
         ex = oracle.log10_exact(rd, h, 0)
         say(f"drift read {k}: R={len(rd['seq'])} form {form[k]} gpu {out[k]:.9f} band {v:.9f} "
             f"exact {ex:.9f} ratio {ratio:.2e} lost {lost:.2e} final offset {offs[-1]}")

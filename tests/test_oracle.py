@@ -332,10 +332,15 @@ def test_band_forward_covering_band_is_the_full_matrix(oracle, golden):
 
 # ---- the band that follows the alignment (long reads) -----------------------
 
-def brute_force_moving_band(rows, hap, match, offs, width, edge):
+def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
     """(likelihood, touch) over the paths that stay in each row's band
     [offs[r], offs[r] + width - 1] and whose row-to-row steps start on a
-    diagonal the next row's band still holds (a move drops the others)."""
+    diagonal the next row's band still holds (a move drops the others).
+    irun adds to the touch sum (a) the pure insertion runs M(1, c), I down
+    column c, that enter the top slot on row r >= 2 (c = r + offs[r] +
+    width - 1), continued as touch-only paths that end where they enter an
+    edge-zone cell; (b) the in-band paths that step from the bottom slot into
+    an I below the band and stay in I to the last row."""
     R, H = len(rows), len(hap)
     tot = [0.0, 0.0]
 
@@ -351,40 +356,59 @@ def brute_force_moving_band(rows, hap, match, offs, width, edge):
         p = rows[r - 1]
         return p["pm"] if match(p["code"], hap[j - 1]) else p["pmis"]
 
-    def walk(state, r, j, w, touched):
+    def ii_prod(a, b):  # product of rows a..b's ii (1-based rows)
+        out = 1.0
+        for k in range(a, b + 1):
+            out *= rows[k - 1]["ii"]
+        return out
+
+    def walk(state, r, j, w, touched, ghost=False, entry=False):
         if not inb(r, j - r):
             return
-        touched = touched or zone(r, j)
+        z = zone(r, j)
+        if ghost and z and not entry:
+            return  # T = F on zone cells
+        touched = touched or z
         if state in "MI" and r == R:
-            tot[0] += w
-            tot[1] += w if touched else 0.0
+            if not ghost:
+                tot[0] += w
+            tot[1] += w if (touched or ghost) else 0.0
             return
         nxt, d = r + 1, j - r
         down = nxt <= R and inb(nxt, d)  # this cell's slot survives into row r + 1
+        if irun and not ghost and down and state in "MI" and d == offs[nxt] and 1 <= j <= H:
+            step = rows[nxt - 1]["mi"] if state == "M" else rows[nxt - 1]["ii"]
+            tot[1] += w * step * ii_prod(nxt + 1, R)
         if state == "D" and r == 0:
             if j + 1 <= H and down:
-                walk("M", 1, j + 1, w * rows[0]["dm"] * prior(1, j + 1), touched)
+                walk("M", 1, j + 1, w * rows[0]["dm"] * prior(1, j + 1), touched, ghost)
             return
         if state == "M":
             if down and j + 1 <= H:
-                walk("M", nxt, j + 1, w * rows[nxt - 1]["mm"] * prior(nxt, j + 1), touched)
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["mm"] * prior(nxt, j + 1), touched, ghost)
             if down:
-                walk("I", nxt, j, w * rows[nxt - 1]["mi"], touched)
+                walk("I", nxt, j, w * rows[nxt - 1]["mi"], touched, ghost)
             if j + 1 <= H:
-                walk("D", r, j + 1, w * rows[r - 1]["md"], touched)
+                walk("D", r, j + 1, w * rows[r - 1]["md"], touched, ghost)
         elif state == "I":
             if down and j + 1 <= H:
-                walk("M", nxt, j + 1, w * rows[nxt - 1]["im"] * prior(nxt, j + 1), touched)
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["im"] * prior(nxt, j + 1), touched, ghost)
             if down:
-                walk("I", nxt, j, w * rows[nxt - 1]["ii"], touched)
+                walk("I", nxt, j, w * rows[nxt - 1]["ii"], touched, ghost)
         elif state == "D":
             if down and j + 1 <= H:
-                walk("M", nxt, j + 1, w * rows[nxt - 1]["dm"] * prior(nxt, j + 1), touched)
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["dm"] * prior(nxt, j + 1), touched, ghost)
             if j + 1 <= H:
-                walk("D", r, j + 1, w * rows[r - 1]["dd"], touched)
+                walk("D", r, j + 1, w * rows[r - 1]["dd"], touched, ghost)
 
     for j0 in range(H + 1):
         walk("D", 0, j0, 1.0 / H, False)
+    if irun:
+        for r in range(2, R + 1):
+            c = r + offs[r] + width - 1
+            if 1 <= c <= H:
+                w = rows[0]["dm"] * prior(1, c) * rows[1]["mi"] * ii_prod(3, r) / H
+                walk("I", r, c, w, True, ghost=True, entry=True)
     return tot[0], tot[1]
 
 
@@ -469,6 +493,68 @@ def test_band_follow_vs_brute_force(oracle):
         checked += 1
         moved += bool((np.diff(offs) != 0).any())
     assert checked > 150 and moved > 40
+
+
+def test_band_follow_given_offsets_vs_brute_force(oracle):
+    """The same band driven by given per-row offsets (the wide band's end
+    slides are such a schedule): enumeration of the paths it keeps."""
+    rng = np.random.default_rng(37)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = 0
+    for rd, hap in small_cases(43, 600):
+        R, H = len(rd["seq"]), len(hap)
+        width = int(rng.integers(1, 4))
+        steps = rng.integers(-2, 1, R)  # moves of 0, -1 or -2 between rows
+        sched = np.concatenate([[0, 0], np.cumsum(steps[1:])]).astype(np.int32) + int(rng.integers(-1, H + 2))
+        v, ratio, _, offs = oracle.band_follow(rd, hap, 0, width, 1, max_move=2, sched=sched)
+        assert (offs == sched).all()
+        bf, bt = brute_force_moving_band(row_params_avx(rd, t), hap, match_avx, sched, width, 1)
+        if bf == 0.0:
+            assert v == -np.inf
+            continue
+        assert v == pytest.approx(math.log10(bf), abs=1e-12)
+        assert ratio == pytest.approx(bt / bf, rel=1e-9, abs=1e-15)
+        checked += 1
+    assert checked > 100
+
+
+def test_band_follow_insertion_runs_vs_brute_force(oracle):
+    """The touch mass with the insertion runs across the long edges (irun):
+    enumeration of the pure runs entering the top slot and of the paths
+    leaving the bottom slot as pure runs, on static and moving bands."""
+    rng = np.random.default_rng(53)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = changed = 0
+    for rd, hap in small_cases(59, 500):
+        R, H = len(rd["seq"]), len(hap)
+        width = int(rng.integers(1, 4))
+        steps = rng.integers(-1, 1, R) if rng.random() < 0.5 else np.zeros(R, int)
+        sched = np.concatenate([[0, 0], np.cumsum(steps[1:])]).astype(np.int32) + int(rng.integers(-2, H + 1))
+        v, ratio, _, _ = oracle.band_follow(rd, hap, 0, width, 1, max_move=1, irun=1, sched=sched)
+        _, ratio0, _, _ = oracle.band_follow(rd, hap, 0, width, 1, max_move=1, irun=0, sched=sched)
+        bf, bt = brute_force_moving_band(row_params_avx(rd, t), hap, match_avx, sched, width, 1, irun=True)
+        if bf == 0.0:
+            continue
+        assert v == pytest.approx(math.log10(bf), abs=1e-12)
+        assert ratio == pytest.approx(bt / bf, rel=1e-9, abs=1e-15)
+        checked += 1
+        changed += ratio > ratio0 * (1 + 1e-9)
+    assert checked > 150 and changed > 50
+
+
+def test_band_follow_hold_rule(oracle, golden):
+    """With hold_eps no move drops more than hold_eps of its row's mass, and
+    without moves the hold changes nothing."""
+    pairs, _ = golden("workload1.npz")
+    held = 0
+    for rd, hap in pairs[::41]:
+        R, H = len(rd["seq"]), len(hap)
+        v, ratio, lost, offs = oracle.band_follow(rd, hap, -10, 32, 4, 8, 4, hold_eps=1e-9)
+        v0, _, lost0, offs0 = oracle.band_follow(rd, hap, -10, 32, 4, 8, 4)
+        moves = int((np.diff(offs) != 0).sum())
+        assert lost <= 1e-9 * max(moves, 1)
+        held += not np.array_equal(offs, offs0)
+    assert held > 0
 
 
 def test_band_follow_moves_and_drops_as_a_masked_dp(oracle):
