@@ -1,11 +1,17 @@
 """PairHMM throughput on one MI355X.
 
-A step is one full evaluation of the workload: the FP32 forward pass over all
-read x haplotype pairs of 10,000 synthetic regions, the FP64 re-run of pairs
-whose FP32 sum falls below 1e-28, and the final log10, with inputs resident in
-device memory.  Prints one JSON line.
+The default workload is the combined one: 100,000 synthetic regions, 90 % of
+them short-read regions (reads of 50-250 bases against 2-32 haplotypes of
+100-600 bases) and 10 % long-read regions (reads of 2-10 kb against 4-8
+haplotypes of 3-12 kb), about 4.04e8 read x haplotype pairs.  A step is one
+full evaluation of it with inputs resident in device memory, as one captured
+graph: learned bands for the short reads (full-matrix FP32 fallback), the
+moving FP64 band for the long reads (full-matrix FP64 fallback), the FP64
+re-run of pairs whose FP32 sum falls below 1e-28, and the final log10.  The
+full-matrix evaluation of the same workload is timed beside it.  Prints one
+JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--regions N] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 4] [--regions N] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -31,6 +37,12 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) datasheet peak
 PEAK_FP32_NOPACK_TFLOPS = 78.6
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector datasheet peak
 LONG_ROWS = 1024  # reads this long run in FP64 (the moving band when banded)
+WORKLOADS = {
+    0: "one region of 64 reads x 8 haplotypes",
+    1: "short-read regions (reads 50-250 bp, 2-32 haplotypes of 100-600 bp)",
+    2: "long-read regions (reads 2-10 kb, 4-8 haplotypes of 3-12 kb, 5-10 % errors)",
+    4: "combined: 90 % short-read and 10 % long-read regions",
+}
 
 
 def profiled_traffic(profile: str, kernel: str):
@@ -109,10 +121,14 @@ def cpu_baseline(batch, out_gpu, budget_s: float):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed steps of the learned-band leg")
+    ap.add_argument("--full-steps", type=int, default=2, help="timed steps of the full-matrix leg")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--regions", type=int, default=10000)
-    ap.add_argument("--workload", type=int, default=1)
+    ap.add_argument("--regions", type=int, default=100000)
+    ap.add_argument("--workload", type=int, default=4)
+    ap.add_argument("--tune-regions", type=int, default=2000,
+                    help="the autotuner times runs of a plan of this many regions of the same "
+                         "workload and the full plan takes its grids")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--band-model", default=str(ROOT / "biovoltron_amd" / "models" / "band_mlp.npz"),
@@ -141,20 +157,31 @@ def main():
           f"{b.cells:.4e} cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
     plan = pairhmm.Plan(pairhmm.PackedBatch.from_synth(b))
     print(f"[bench] plan ready ({time.perf_counter() - t0 - t_gen:.1f} s)", file=sys.stderr, flush=True)
-    tuned_full = plan.autotune(1) if args.autotune else None
-    if tuned_full:
-        print(f"[bench] autotuned grids (full matrix): {tuned_full}", file=sys.stderr, flush=True)
+    use_band = not args.no_band and Path(args.band_model).exists()
+    tuned_full = tuned = None
+    if args.autotune:
+        from biovoltron_amd import band
+        bt = synth.generate(args.workload, count=min(args.tune_regions, args.regions))
+        with pairhmm.Plan(pairhmm.PackedBatch.from_synth(bt)) as tp:
+            tuned_full = tp.autotune(1)
+            if use_band:
+                band.set_plan_model(tp, band.BandModel.load(args.band_model), margin=args.band_margin,
+                                    max_width=args.band_max_width)
+                tuned = tp.autotune(1)
+        plan.set_grids(tuned_full)
+        print(f"[bench] autotuned grids on {bt.n_regions} regions: full matrix {tuned_full}, "
+              f"learned band {tuned}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         plan.run()
         plan.results()  # waits for the stream
     # timed region: K full evaluations, synchronised on both sides
     plan.synchronize()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.full_steps):
         plan.run()
     plan.synchronize()
     t2 = time.perf_counter()
-    ms_step = (t2 - t1) * 1e3 / args.steps
+    ms_step = (t2 - t1) * 1e3 / args.full_steps
     out, form = plan.results()
     # split of one evaluation between the passes, from HIP events on the stream
     ev_ms = plan.time(1)
@@ -164,13 +191,19 @@ def main():
           f"fp64 {st.last_ms_f64:.2f} ms ({st.fp64_pairs} pairs, {st.fp64_cells:.3e} cells)",
           file=sys.stderr, flush=True)
     gcups = b.cells / (ms_step / 1e3) / 1e9
-    achieved = FLOPS_PER_CELL * b.cells / (st.last_ms_f32 / 1e3) / 1e12
+    rr, hh = b.pair_table()
+    R = np.diff(b.read_off)[rr]
+    Hl = np.diff(b.hap_off)[hh]
+    short = R < LONG_ROWS
+    cells_fp32 = int((R[short] * Hl[short]).sum())  # the FP32 full-matrix pass (short reads)
+    achieved = FLOPS_PER_CELL * cells_fp32 / (st.last_ms_f32 / 1e3) / 1e12
     band_info = None
-    if not args.no_band and Path(args.band_model).exists():
+    if use_band:
         from biovoltron_amd import band
         model = band.BandModel.load(args.band_model)
         band.set_plan_model(plan, model, margin=args.band_margin, max_width=args.band_max_width)
-        tuned = plan.autotune(1) if args.autotune else None
+        if tuned:
+            plan.set_grids(tuned)
         for _ in range(args.warmup):
             plan.run()
             plan.results()
@@ -189,8 +222,6 @@ def main():
         # band cells the banded kernels computed: R x width for every pair
         # they ran (accepted, form 2, or rejected, form 3)
         used = band.plan_bands(plan)
-        rr, _ = b.pair_table()
-        R = np.diff(b.read_off)[rr]
         ran = (form_b == 2) | (form_b == 3)
         band_cells = int((R[ran] * (used[ran, 1] - used[ran, 0] + 1)).sum())
         # long reads with a start band ran on the moving band (64 diagonals)
@@ -228,13 +259,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64-rescue",
+        "dtype": "fp32+fp64",
         "data": "synthetic (seeded generator, biovoltron_amd/csrc/synth.cpp)",
         "config": {
-            "workload": (f"learned-band PairHMM (full-matrix fallback, FP64 rescue)"
-                         if band_info else "batched full-matrix PairHMM")
-                        + f", workload {args.workload} generator, {b.n_regions} regions",
+            "workload": (("learned-band PairHMM (short reads: learned band, full-matrix FP32 "
+                          "fallback; long reads: moving FP64 band, full-matrix FP64 fallback; "
+                          "FP64 rescue)") if band_info else "batched full-matrix PairHMM")
+                        + f", {WORKLOADS.get(args.workload, 'workload %d' % args.workload)}, "
+                          f"{b.n_regions} regions",
             "full_matrix": {"ms_per_step": round(ms_step, 3), "gcups": round(gcups, 3),
+                            "steps": args.full_steps,
                             "grids": tuned_full or dict(fp32=st.grid_f32, fp64=st.grid_f64)},
             "learned_band": band_info,
             "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,

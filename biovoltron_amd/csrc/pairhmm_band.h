@@ -14,6 +14,7 @@ constexpr int kBandHapMax = 1024; // longest haplotype given a band (seed tables
 constexpr int kBandReadMax = 448; // longest read given a band (LDS haplotype window)
 constexpr int kBandSlide = 8;     // wide band: start slide (diagonals)
 constexpr float kBandHold = 1e-9f;// a move that would drop more of a row's mass is held
+constexpr int kEdgeRunRows = 48;  // insertion runs leaving the band bottom: last rows counted
 constexpr int kLongEvery = 16;    // long-read band: rows between moves
 constexpr int kLongMove = kBandEdge;  // long-read band: diagonals per move, at most
 constexpr int kLongStartLift = 16;// long-read start band: the anchor sits this far above lane 0

@@ -250,18 +250,19 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     for (int m = 31; m < lane; ++m) f31 *= yy;        // distance to lane 31
     // Edge zones (T = F there): the left one where the band's left
     // neighbour column is inside the matrix, the right one where its right
-    // neighbour column is.
+    // neighbour column is.  Without a slide the rows where each is active
+    // are fixed per lane.
+    const int ez_from = left_lane ? 2 - o_lo : INT_MAX;
+    const int ez_to = right_lane ? H - o_lo - G : INT_MIN;
 
     // Row 0: M = I = 0, D = K/H for columns 0..H (Context.h:109, template :164-175).
     int o_cur = off(1);
     float Mp = 0.f, Ip = 0.f;
-    // Insertion runs across the band's long edges, added to the touch mass:
-    // ent = K/H * dm(1) * mx(2) * yy^(i-2) is a run M(1, c), I down column c,
-    // that enters the top lane at row i (times the row-1 prior at c); xg (in
-    // the bottom lane) carries the I mass that left through the bottom lane,
-    // continued as pure runs to the last row.
-    float ent = 0.f, xg = 0.f, pe1 = 0.f, dp1 = 0.f;
-    int cls1 = 0;
+    // A read end explained as a run of insertions leaves the band through
+    // its bottom lane and sweeps down one diagonal per row; xg (in the bottom
+    // lane) carries the I mass that left, continued as pure runs to the last
+    // row, and is added to the touch mass.
+    float xg = 0.f;
     float Dp = (o_cur + k >= 0 && o_cur + k <= H) ? kK / static_cast<float>(H) : 0.f;
     float TMp = 0.f, TIp = 0.f, TDp = 0.f;
     {
@@ -294,21 +295,13 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
       }
       __syncthreads();
       const int n = min(kBandLanes, R - i0 + 1);
-      if (i0 == 1) {
-        const RowRec r1 = s.tab[0];
-        pe1 = r1.pe;
-        dp1 = r1.dp;
-        cls1 = __builtin_amdgcn_readfirstlane(r1.cls);
-        ent = R >= 2 ? kK / static_cast<float>(H) * r1.g * s.tab[1].mx : 0.f;
-      }
-      // Software pipeline: the next row's record is read before this row is
-      // computed.
-      RowRec rr_next = s.tab[0];
       for (int v = 0; v < n; ++v) {
         const int i = i0 + v;
-        const RowRec rr = rr_next;
+        const RowRec rr = s.tab[v];
         const int oi = off(i);
-        rr_next = s.tab[v + 1];
+        // the selector at the offset the schedule asks for (window slot
+        // i - 1 + k + off(i) - wb is in [0, R + G + Pt))
+        float sel = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr.cls)][i - 1 + k + oi - wb]);
         if constexpr (kSlideG > 0) {
           if (oi < o_cur) {
             // slide down one diagonal: lane k takes lane k - 1's cells and
@@ -316,7 +309,7 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
             // of the row's mass
             const float wl = Mp + Ip + Dp;
             const float wtop = __shfl(wl, kBandLanes - 1);
-            if (wtop <= kBandHold * wsumf(wl)) {
+            if (__builtin_amdgcn_readfirstlane(static_cast<int>(wtop <= kBandHold * wsumf(wl)))) {
               Mp = dpp<kWaveShr1>(Mp);
               Ip = dpp<kWaveShr1>(Ip);
               Dp = dpp<kWaveShr1>(Dp);
@@ -328,18 +321,26 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           }
         }
         const int oc = kSlideG > 0 ? o_cur : oi;
-        const int col = i + oc + k;  // this lane's column on row i
-        // window slot i - 1 + k + oc - wb is in [0, R + G + Pt) for i <= R
-        const float sel = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr.cls)][i - 1 + k + oc - wb]);
+        // a held slide leaves the band above the schedule: read again
+        // (window slot i - 1 + k + oc - wb is in [0, R + G + Pt))
+        if (kSlideG > 0 && oc != oi)
+          sel = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr.cls)][i - 1 + k + oc - wb]);
         const float prior = __builtin_fmaf(sel, rr.dp, rr.pe);
-        const bool ez = (left_lane && i + oc >= 2) || (right_lane && i + oc + G <= H);
-        // insertion runs: the I mass leaving the bottom lane (from row i - 1)
-        // and the pure run entering the top lane on row i
-        xg = __builtin_fmaf(xg, yy, (bottom_lane && col - 1 >= 1 && col - 1 <= H)
-                                        ? __builtin_fmaf(Mp, rr.mx, Ip * rr.xx) : 0.f);
-        const float sel1 = static_cast<float>(selL[cls1][i - 1 + k + oc - wb]);
-        const float enter = (top_lane && i >= 2 && col <= H) ? ent * __builtin_fmaf(sel1, dp1, pe1) : 0.f;
-        if (i >= 2) ent *= yy;
+        bool ez;
+        if constexpr (kSlideG > 0) {
+          const bool lon = i + oc >= 2, ron = i + oc + G <= H;  // wave-uniform
+          ez = (left_lane && lon) || (right_lane && ron);
+        } else {
+          ez = i >= ez_from || i <= ez_to;
+        }
+        // the I mass leaving the bottom lane from row i - 1 (counted on the
+        // last kEdgeRunRows rows; before them a run's factor yy^rows to the
+        // last row is below 1e-46)
+        if (i > R - kEdgeRunRows) {
+          const int c0 = i - 1 + oc;  // the bottom lane's column on row i - 1
+          xg = __builtin_fmaf(xg, yy, (bottom_lane && c0 >= 1 && c0 <= H)
+                                          ? __builtin_fmaf(Mp, rr.mx, Ip * rr.xx) : 0.f);
+        }
 
         // forward
         float t = Ip * rr.g;
@@ -370,7 +371,6 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           if (top_lane) TIn = 0.f;
           if (bottom_lane) tx = 0.f;
         }
-        TIn += enter;
         float TDn = decay_scan<G>(tx, yy, yy2, yy4, yy8, f15, f31);
         if (ez) TDn = Dn;
 
@@ -433,8 +433,8 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
 // as in the FP32 kernel above.  A move is held when the lanes it would drop
 // carry more than kBandHold of the row's forward mass (a cut mode can still
 // have a large future; forward mass that small rarely does).  The touch mass
-// also counts the insertion runs across the long edges, as in the FP32
-// kernel.  A pair is accepted when T/S <= tol; otherwise, or when its
+// also counts the insertion runs that leave through the bottom lane, as in
+// the FP32 kernel.  A pair is accepted when T/S <= tol; otherwise, or when its
 // likelihood is within reach of FP32 (the reference would keep an FP32
 // result), it goes to the full-matrix FP64 pass.
 //
@@ -565,10 +565,9 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
     if ((left_lane && o - 1 >= 0) || (right_lane && o + kBandLanes <= H)) TDp = Dp;
     int lg = 0;  // exponent removed by rescaling
     int jb = 0;  // haplotype column of window slot 0
-    // insertion runs across the long edges (as in band_forward_kernel):
-    // ent enters the top lane, xg (bottom lane) carries what left below
-    double ent = 0.0, xg = 0.0, pe1 = 0.0, dp1 = 0.0;
-    uint32_t cls1 = 0;
+    // a read end explained as insertions, leaving through the bottom lane
+    // (as in band_forward_kernel)
+    double xg = 0.0;
 
     for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
       const int n = min(kBandLanes, R - i0 + 1);
@@ -612,7 +611,6 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
               const double sc = ldexp(1.0, -e);
               Mp *= sc; Ip *= sc; Dp *= sc;
               TMp *= sc; TIp *= sc; TDp *= sc;
-              ent *= sc;
               xg *= sc;
               lg += e;
             }
@@ -643,13 +641,6 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
             s.hm[c] = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : uint8_t{0};
           }
           __syncthreads();
-          if (i0 == 1) {
-            const RowRec64 r1 = s.tab[0];
-            pe1 = r1.pe;
-            dp1 = r1.dp;
-            cls1 = __builtin_amdgcn_readfirstlane(r1.cls);
-            ent = R >= 2 ? r1.g * s.tab[1].mx / static_cast<double>(H) : 0.0;
-          }
         }
         const int v1 = min(n, v0 + kLongEvery);
         for (int v = v0; v < v1; ++v) {
@@ -660,11 +651,8 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
           const double prior = fma(sel, rr.dp, rr.pe);
           const bool ez = (left_lane && ii >= ez_from) || (right_lane && ii <= ez_to);
           const int col = ii + o + k;
-          xg = fma(xg, yy, (k == 0 && col - 1 >= 1 && col - 1 <= H) ? fma(Mp, rr.mx, Ip * rr.xx) : 0.0);
-          const double enter = (k == kBandLanes - 1 && ii >= 2 && col <= H)
-                                   ? ent * fma(static_cast<double>((hmv >> cls1) & 1u), dp1, pe1)
-                                   : 0.0;
-          if (ii >= 2) ent *= yy;
+          if (ii > R - kEdgeRunRows)
+            xg = fma(xg, yy, (k == 0 && col - 1 >= 1 && col - 1 <= H) ? fma(Mp, rr.mx, Ip * rr.xx) : 0.0);
 
           double t = Ip * rr.g;
           t = fma(Dp, rr.g, t);
@@ -682,7 +670,6 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
             TMn = Mn;
             TIn = In;
           }
-          TIn += enter;
           double TDn = decay_scan64(dpp64<kWaveShr1>(TMn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
           if (ez) TDn = Dn;
 

@@ -249,7 +249,7 @@ struct bv_plan {
   int hmax = 0;
   int n_tasks = 0;
   int grid32 = 0, grid64 = 0, grid_band = 0, grid_long = 1;
-  int grid32_max = 0, grid64_max = 0, grid_band_max = 0;  // scratch is sized for these
+  int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -834,6 +834,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid_band_max = p->grid_band;
   p->grid_long = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_long, p->n_tasks64)));
+  p->grid_long_max = p->grid_long;
   p->stride32 = hmax + kCarryPad;
   p->stride64 = hmax + kCarryPad;
 
@@ -877,14 +878,14 @@ static void drop_graph(bv_plan* p) {
 int bv_plan_autotune(bv_plan* p, int iters, int32_t* grids_out) {
   if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
   HIP_TRY(hipSetDevice(g_ctx.device));
-  int* knobs[3] = {&p->grid_band, &p->grid32, &p->grid64};
-  const int caps[3] = {p->grid_band_max, p->grid32_max, p->grid64_max};
+  int* knobs[4] = {&p->grid_band, &p->grid32, &p->grid64, &p->grid_long};
+  const int caps[4] = {p->grid_band_max, p->grid32_max, p->grid64_max, p->grid_long_max};
   float best_ms = 0.f;
   int rc = bv_plan_time(p, iters, &best_ms);
   if (rc) return rc;
   // Coordinate search: each persistent grid over fractions of its
   // occupancy-derived maximum, timing whole runs on the plan's own data.
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     const int cur = *knobs[k];
     int best = cur;
     for (int num : {2, 4, 6, 8}) {
@@ -908,7 +909,20 @@ int bv_plan_autotune(bv_plan* p, int iters, int32_t* grids_out) {
     grids_out[0] = p->grid_band;
     grids_out[1] = p->grid32;
     grids_out[2] = p->grid64;
+    grids_out[3] = p->grid_long;
   }
+  return BV_OK;
+}
+
+int bv_plan_set_grids(bv_plan* p, const int32_t* grids) {
+  if (!p || !grids) return fail(BV_EINVAL, "NULL argument");
+  int* knobs[4] = {&p->grid_band, &p->grid32, &p->grid64, &p->grid_long};
+  const int caps[4] = {p->grid_band_max, p->grid32_max, p->grid64_max, p->grid_long_max};
+  for (int k = 0; k < 4; ++k) {
+    if (grids[k] < 0) return fail(BV_EINVAL, "negative grid");
+    if (grids[k] > 0) *knobs[k] = std::min(grids[k], std::max(1, caps[k]));
+  }
+  drop_graph(p);
   return BV_OK;
 }
 

@@ -84,7 +84,7 @@ EXPORTS = (
     "bv_pairhmm_init", "bv_pairhmm_shutdown", "bv_pairhmm_last_error", "bv_pairhmm_device_info",
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
-    "bv_plan_autotune",
+    "bv_plan_autotune", "bv_plan_set_grids",
     "bv_pairhmm_batch",
 )
 
@@ -112,6 +112,7 @@ def library() -> C.CDLL:
         lib.bv_plan_results.argtypes = [vp, vp, vp]
         lib.bv_plan_stats_get.argtypes = [vp, C.POINTER(PlanStats)]
         lib.bv_plan_autotune.argtypes = [vp, i32, vp]
+        lib.bv_plan_set_grids.argtypes = [vp, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
         lib.bv_pairhmm_batch.argtypes = [C.POINTER(_RegionBatch), C.POINTER(Options), vp, vp]
@@ -251,11 +252,17 @@ class Plan:
         return out, form
 
     def autotune(self, iters: int = 1) -> dict:
-        """Picks the persistent grids (banded, FP32, FP64) by timing runs of
-        this plan; returns them."""
-        g = (C.c_int32 * 3)()
+        """Picks the persistent grids (banded, FP32, FP64, long-read band) by
+        timing runs of this plan; returns them."""
+        g = (C.c_int32 * 4)()
         _check(self._lib.bv_plan_autotune(self._h, int(iters), g))
-        return dict(band=g[0], fp32=g[1], fp64=g[2])
+        return dict(band=g[0], fp32=g[1], fp64=g[2], long=g[3])
+
+    def set_grids(self, grids: dict) -> None:
+        """Sets persistent grids (keys band, fp32, fp64, long; missing or 0
+        keeps a grid), clamped to this plan's maxima."""
+        g = (C.c_int32 * 4)(*(int(grids.get(k) or 0) for k in ("band", "fp32", "fp64", "long")))
+        _check(self._lib.bv_plan_set_grids(self._h, g))
 
     def synchronize(self) -> None:
         """Waits for the plan's stream (no copies)."""

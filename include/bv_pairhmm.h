@@ -214,12 +214,17 @@ int bv_plan_features(bv_plan* plan, float* X_out, int32_t* anchor_out, int32_t* 
  * without seed hits or with haplotypes longer than 1024).  NULL model turns
  * learned bands off. */
 int bv_plan_set_band_model(bv_plan* plan, const bv_band_model* model, int margin, int max_width);
-/* Autotuner: picks the persistent grid of the banded kernels, the FP32 pass
- * and the FP64 pass (a coordinate search over fractions of each kernel's
- * occupancy-derived grid, timing whole runs of this plan, `iters` each) and
- * keeps the fastest for later runs.  grids_out (optional) receives
- * {banded, fp32, fp64}.  Tune after setting bands or a band model. */
+/* Autotuner: picks the persistent grid of the banded kernels, the FP32 pass,
+ * the FP64 pass and the long-read moving band (a coordinate search over
+ * fractions of each kernel's occupancy-derived grid, timing whole runs of
+ * this plan, `iters` each) and keeps the fastest for later runs.  grids_out
+ * (optional) receives {banded, fp32, fp64, long}.  Tune after setting bands
+ * or a band model. */
 int bv_plan_autotune(bv_plan* plan, int iters, int32_t* grids_out);
+/* Sets the persistent grids {banded, fp32, fp64, long} (0 keeps a grid; each
+ * is clamped to the plan's occupancy-derived maximum), e.g. to reuse grids
+ * tuned on a smaller plan of the same workload. */
+int bv_plan_set_grids(bv_plan* plan, const int32_t* grids);
 /* The band table of the last run (n_pairs x 2 int32). */
 int bv_plan_bands(bv_plan* plan, int32_t* bands_out);
 /* Training labels (FP64 forward-backward on the host), lo_hi [n_pairs][2]:

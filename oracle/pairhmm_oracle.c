@@ -559,15 +559,19 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
     const int lact = r + o - 1 >= 1, ract = r + o + width <= H;
     double top_in = 0.0;
     if (irun) {
+      /* counted on rows 2..48 (entries) and from the last 48 rows (exits):
+       * beyond them a run's factor ii^rows is below 1e-46 */
       if (r >= 3) run_prod *= p.ii;
       const int c = r + o + width - 1;
-      if (r >= 2 && c >= 1 && c <= H) {
+      if ((irun & 1) && r >= 2 && r <= 48 && c >= 1 && c <= H) {
         const double pr1 = is_match(0, p1.code, hap[c - 1]) ? p1.pm : p1.pmis;
         top_in = ldexp(p1.dm * pr1 * p2.mi * run_prod / (double)H, -(int)log2scale);
       }
-      x_tail *= p.ii;
-      const int c0 = r - 1 + o;
-      if (c0 >= 1 && c0 <= H) x_tail += fM[0] * p.mi + fI[0] * p.ii;
+      if ((irun & 2) && r > R - 48) {
+        x_tail *= p.ii;
+        const int c0 = r - 1 + o;
+        if (c0 >= 1 && c0 <= H) x_tail += fM[0] * p.mi + fI[0] * p.ii;
+      }
     }
     double mx = 0.0;
     for (int k = 0; k < width; ++k) {

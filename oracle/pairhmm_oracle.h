@@ -122,10 +122,11 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
  * max_move, aiming end_bias slots above the centre on the last end_rows
  * rows), or, when sched is not NULL, towards the offsets sched[0..R] (by at
  * most max_move per row).  A move is not made when the slots it would drop
- * hold more than hold_eps of the row's mass (hold_eps > 0).  irun != 0 adds to
- * the touch mass the insertion runs that enter through the top edge from a
- * read start above the band and that leave through the bottom edge to end
- * the read below it.  Returns log10 of
+ * hold more than hold_eps of the row's mass (hold_eps > 0).  irun adds to the
+ * touch mass the pure insertion runs that enter through the top edge from a
+ * read start above the band (bit 0, rows 2..48) and those that leave through
+ * the bottom edge to end the read below it (bit 1, the last 48 rows; the
+ * kernels count these).  Returns log10 of
  * the banded likelihood, the touch ratio T/S (the accept test), the mass share
  * the moves dropped (*lost) and, if offs is not NULL, the band offset used on
  * each row 0..R. */

@@ -280,9 +280,9 @@ def slide_schedule(R, H, o_lo, width):
 
 def oracle_band(oracle, rd, hap, o_lo, width):
     """The oracle's restatement of the banded kernels: the slide (held by the
-    hold rule) and the insertion runs across the long edges."""
+    hold rule) and the insertion runs that leave through the bottom edge."""
     R, H = len(rd["seq"]), len(hap)
-    v, ratio, _, _ = oracle.band_follow(rd, hap, o_lo, width, 4, max_move=1, hold_eps=HOLD, irun=1,
+    v, ratio, _, _ = oracle.band_follow(rd, hap, o_lo, width, 4, max_move=1, hold_eps=HOLD, irun=2,
                                         sched=slide_schedule(R, H, o_lo, width))
     return v, ratio
 

@@ -340,7 +340,8 @@ def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
     column c, that enter the top slot on row r >= 2 (c = r + offs[r] +
     width - 1), continued as touch-only paths that end where they enter an
     edge-zone cell; (b) the in-band paths that step from the bottom slot into
-    an I below the band and stay in I to the last row."""
+    an I below the band and stay in I to the last row -- entries on rows
+    2..48 and exits from the last 48 rows."""
     R, H = len(rows), len(hap)
     tot = [0.0, 0.0]
 
@@ -376,7 +377,8 @@ def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
             return
         nxt, d = r + 1, j - r
         down = nxt <= R and inb(nxt, d)  # this cell's slot survives into row r + 1
-        if irun and not ghost and down and state in "MI" and d == offs[nxt] and 1 <= j <= H:
+        if irun and not ghost and down and state in "MI" and d == offs[nxt] and 1 <= j <= H \
+                and nxt > R - 48:
             step = rows[nxt - 1]["mi"] if state == "M" else rows[nxt - 1]["ii"]
             tot[1] += w * step * ii_prod(nxt + 1, R)
         if state == "D" and r == 0:
@@ -404,7 +406,7 @@ def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
     for j0 in range(H + 1):
         walk("D", 0, j0, 1.0 / H, False)
     if irun:
-        for r in range(2, R + 1):
+        for r in range(2, min(R, 48) + 1):
             c = r + offs[r] + width - 1
             if 1 <= c <= H:
                 w = rows[0]["dm"] * prior(1, c) * rows[1]["mi"] * ii_prod(3, r) / H
@@ -530,7 +532,7 @@ def test_band_follow_insertion_runs_vs_brute_force(oracle):
         width = int(rng.integers(1, 4))
         steps = rng.integers(-1, 1, R) if rng.random() < 0.5 else np.zeros(R, int)
         sched = np.concatenate([[0, 0], np.cumsum(steps[1:])]).astype(np.int32) + int(rng.integers(-2, H + 1))
-        v, ratio, _, _ = oracle.band_follow(rd, hap, 0, width, 1, max_move=1, irun=1, sched=sched)
+        v, ratio, _, _ = oracle.band_follow(rd, hap, 0, width, 1, max_move=1, irun=3, sched=sched)
         _, ratio0, _, _ = oracle.band_follow(rd, hap, 0, width, 1, max_move=1, irun=0, sched=sched)
         bf, bt = brute_force_moving_band(row_params_avx(rd, t), hap, match_avx, sched, width, 1, irun=True)
         if bf == 0.0:
