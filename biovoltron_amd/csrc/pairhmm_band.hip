@@ -94,6 +94,43 @@ __device__ __forceinline__ float decay_scan(float x, float yy1, float yy2, float
   return x;
 }
 
+// Both decayed scans of a row (the forward D and the touch D), as
+// decay_scan<G>, with each level's DPP operand folded into one v_fmac_f32_dpp
+// (the compiler does not combine DPP into a multiply-accumulate and emits a
+// separate v_mov_b32_dpp per level).  The two scans are interleaved so that
+// every DPP read of the previous level's result is two wait states after its
+// write (one instruction plus one s_nop), as gfx9 requires; the block starts
+// and ends with s_nop 1 for the writes and reads around it.
+template <int G>
+__device__ __forceinline__ void decay_scan2(float& x, float& t, float y1, float y2, float y4, float y8,
+                                            float f15, float f31) {
+#define BVP_LVL(CTRL, Y)                                                           \
+  "v_fmac_f32_dpp %0, %0, " Y " " CTRL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+  "v_fmac_f32_dpp %1, %1, " Y " " CTRL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+  if constexpr (G == 64) {
+    asm volatile("s_nop 1\n\t"
+                 BVP_LVL("row_shr:1", "%2") "s_nop 0\n\t"
+                 BVP_LVL("row_shr:2", "%3") "s_nop 0\n\t"
+                 BVP_LVL("row_shr:4", "%4") "s_nop 0\n\t"
+                 BVP_LVL("row_shr:8", "%5") "s_nop 0\n\t"
+                 BVP_LVL("row_bcast:15", "%6") "s_nop 0\n\t"
+                 BVP_LVL("row_bcast:31", "%7") "s_nop 1"
+                 : "+v"(x), "+v"(t)
+                 : "v"(y1), "v"(y2), "v"(y4), "v"(y8), "v"(f15), "v"(f31));
+  } else {
+    asm volatile("s_nop 1\n\t"
+                 BVP_LVL("row_shr:1", "%2") "s_nop 0\n\t"
+                 BVP_LVL("row_shr:2", "%3") "s_nop 0\n\t"
+                 BVP_LVL("row_shr:4", "%4") "s_nop 0\n\t"
+                 BVP_LVL("row_shr:8", "%5") "s_nop 0\n\t"
+                 BVP_LVL("row_bcast:15", "%6") "s_nop 1"
+                 : "+v"(x), "+v"(t)
+                 : "v"(y1), "v"(y2), "v"(y4), "v"(y8), "v"(f15));
+    (void)f31;
+  }
+#undef BVP_LVL
+}
+
 __device__ __forceinline__ float wsumf(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -354,7 +391,6 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           if (top_lane) In = 0.f;
           if (bottom_lane) x = 0.f;
         }
-        const float Dn = decay_scan<G>(x, yy, yy2, yy4, yy8, f15, f31);
 
         // touch mass
         float tt = TIp * rr.g;
@@ -371,7 +407,10 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           if (top_lane) TIn = 0.f;
           if (bottom_lane) tx = 0.f;
         }
-        float TDn = decay_scan<G>(tx, yy, yy2, yy4, yy8, f15, f31);
+        // the D scans of both DPs (x becomes Dn, tx becomes TDn)
+        decay_scan2<G>(x, tx, yy, yy2, yy4, yy8, f15, f31);
+        const float Dn = x;
+        float TDn = tx;
         if (ez) TDn = Dn;
 
         Mp = Mn;
