@@ -131,6 +131,17 @@ __device__ __forceinline__ void decay_scan2(float& x, float& t, float y1, float 
 #undef BVP_LVL
 }
 
+// I of the next row: dpp_shl1(m) * mx + dpp_shl1(i) * xx, the second DPP
+// operand folded into a v_fmac_f32_dpp (s_nop 1 first: the DPP read of m must
+// be two wait states after its write).
+__device__ __forceinline__ float insert_next(float m, float i, float mx, float xx) {
+  float r = dpp<kWaveShl1>(i) * xx;
+  asm volatile("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+               : "+v"(r)
+               : "v"(m), "v"(mx));
+  return r;
+}
+
 __device__ __forceinline__ float wsumf(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -379,12 +390,9 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
                                           ? __builtin_fmaf(Mp, rr.mx, Ip * rr.xx) : 0.f);
         }
 
-        // forward
-        float t = Ip * rr.g;
-        t = __builtin_fmaf(Dp, rr.g, t);
-        t = __builtin_fmaf(Mp, rr.mm, t);
-        const float Mn = t * prior;
-        float In = __builtin_fmaf(dpp<kWaveShl1>(Mp), rr.mx, dpp<kWaveShl1>(Ip) * rr.xx);
+        // forward (I and D into M share the factor g = 1 - ph2pr[gcp])
+        const float Mn = __builtin_fmaf(Mp, rr.mm, (Ip + Dp) * rr.g) * prior;
+        float In = insert_next(Mp, Ip, rr.mx, rr.xx);
         float x = dpp<kWaveShr1>(Mn) * rr.my;
         if constexpr (G != kBandLanes) {
           // the wavefront shifts cross the slot boundary: cut them
@@ -393,11 +401,8 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
         }
 
         // touch mass
-        float tt = TIp * rr.g;
-        tt = __builtin_fmaf(TDp, rr.g, tt);
-        tt = __builtin_fmaf(TMp, rr.mm, tt);
-        float TMn = tt * prior;
-        float TIn = __builtin_fmaf(dpp<kWaveShl1>(TMp), rr.mx, dpp<kWaveShl1>(TIp) * rr.xx);
+        float TMn = __builtin_fmaf(TMp, rr.mm, (TIp + TDp) * rr.g) * prior;
+        float TIn = insert_next(TMp, TIp, rr.mx, rr.xx);
         if (ez) {
           TMn = Mn;
           TIn = In;
