@@ -37,7 +37,7 @@ struct BArgs {
   const uint8_t* ins;
   const uint8_t* del;
   const uint8_t* gcp;
-  const uint8_t* gcp_const;  // per read: 1 if the gap continuation is constant along it
+  const uint8_t* gcp_const;  // per read: 1 if the gap continuation is constant along it, quality not 0
   const int64_t* read_off;
   // haplotypes
   const uint8_t* hmask;
@@ -81,6 +81,15 @@ struct BArgs {
   const double* mm_d;     // [8256]
   double verify_above;    // log10 L at or above which FP32 might not underflow
   unsigned long long* fp64_cells;
+  // short-read kernel: selector window columns per read class (dynamic LDS),
+  // at least the longest banded read + kBandLanes + 2 * kBandSlide
+  int sel_cols;
 };
+
+// Dynamic LDS bytes of the short-read banded kernel for `pairs` pairs per
+// wavefront (1 or 2): the row table, then 4 selector rows of sel_cols per pair.
+constexpr int band_sel_cols(int read_max) {
+  return ((read_max + kBandLanes + 2 * kBandSlide) + 7) & ~7;
+}
 
 }  // namespace bvp

@@ -52,15 +52,19 @@ constexpr int kRowShr = 0x110;    // + n: lane k <- lane k-n inside a 16-lane ro
 constexpr int kRowBcast15 = 0x142;
 constexpr int kRowBcast31 = 0x143;
 
+// The gap-to-match factor g = 1 - ph2pr[gcp] is constant along a banded read
+// (the dispatch admits only reads with a constant, nonzero-quality gap
+// continuation), so the kernel carries I and D times g: M then takes
+// mm * M + (I + D) with no multiply by g, the insert and delete openings are
+// mx * g and my * g, and the final sums divide I by g once.
 struct alignas(16) RowRec {
-  float dp;     // pm - pe, pm = 1 - e the match prior
-  float pe;     // e / 3 (mismatch prior; pm for a wildcard read base)
-  float mm;     // match -> match into this row
-  float g;      // insert/delete -> match into this row (1 - ph2pr[gcp])
-  float mx;     // match -> insert
-  float xx;     // insert -> insert
-  float my;     // match -> delete (along this row)
-  uint32_t cls; // this row's read base class (selector slot)
+  float dp;   // pm - pe, pm = 1 - e the match prior
+  float pe;   // e / 3 (mismatch prior; pm for a wildcard read base)
+  float mm;   // match -> match into this row
+  float mx;   // g * (match -> insert)
+  float xx;   // insert -> insert
+  float my;   // g * (match -> delete, along this row)
+  float pad[2];
 };
 
 // Match selectors of the band's haplotype window, per lane group: slot c of
@@ -68,14 +72,14 @@ struct alignas(16) RowRec {
 // offset, so row i, lane k (column i + o(i) + k) reads slot
 // i - 1 + k + o(i) - wb; sel[g][cls][c] is 1 where that column matches read
 // class cls (0 outside the haplotype).  The prior is then one mixed-precision
-// FMA, sel * (pm - pe) + pe.  tab has one spare entry so that the next row's
-// record can always be read ahead.
-constexpr int kSelCols = kBandReadMax + kBandLanes + 2 * kBandSlide;
-template <int P>
-struct BandLds {
-  RowRec tab[kBandLanes + 1];
-  _Float16 sel[P][4][kSelCols];
-};
+// FMA, sel * (pm - pe) + pe.  A lane's address is its part k + o(i) - wb
+// (changed only by a slide) plus the row's part 2 * (cls * C + i - 1), which
+// the lane that built row i's record keeps in a register: read with
+// v_readlane, it does not wait for the record's LDS load.  The window has
+// C = a.sel_cols columns per class, sized by the plan's longest banded read
+// (dynamic LDS: shorter reads leave room for more wavefronts per CU).
+constexpr int kTabBytes = static_cast<int>(sizeof(RowRec)) * (kBandLanes + 1);
+extern __shared__ __align__(16) unsigned char band_lds[];
 
 // Inclusive decayed scan over each G-lane group: returns y with
 // y_k = sum_{m>=0} yy^m x_{k-m}, k and k-m in the same group.  Four row_shr
@@ -142,10 +146,17 @@ __device__ __forceinline__ float insert_next(float m, float i, float mx, float x
   return r;
 }
 
-__device__ __forceinline__ float wsumf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+// Sum over the 64 lanes, wave-uniform: row_shr scans of each 16-lane row,
+// row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3; lane 63
+// then holds the total.  (DPP only: no lane-address registers.)
+__device__ __forceinline__ float wtotalf(float v) {
+  v += dpp<kRowShr + 1>(v);
+  v += dpp<kRowShr + 2>(v);
+  v += dpp<kRowShr + 4>(v);
+  v += dpp<kRowShr + 8>(v);
+  v += dpp<kRowBcast15, 0xa>(v);
+  v += dpp<kRowBcast31, 0xc>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 __device__ __forceinline__ double wsum(double v) {
@@ -236,9 +247,10 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
 // Banded forward over one list.  G = 64: one pair per wavefront.  G = 32: the
 // two pairs 2u, 2u+1 (one read, two haplotypes) in lanes 0-31 and 32-63.
 template <int G>
-__global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void band_forward_kernel(BArgs a) {
   constexpr int P = kBandLanes / G;
-  __shared__ BandLds<P> s;
+  RowRec* tab = reinterpret_cast<RowRec*>(band_lds);
+  const int C = a.sel_cols;
   const int lane = threadIdx.x;
   const int grp = lane / G;     // pair slot of this lane
   const int k = lane % G;       // lane within the slot: diagonal o_lo + k
@@ -280,22 +292,34 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
     const int Pt = min(kSlideG, max(0, (H - 1) - (o_lo + G - 1)));
     auto off = [&](int r) { return o_lo + max(0, Pt + 1 - r); };
     const int wb = o_lo;  // lowest band offset: window slot c is column wb + 1 + c
-    _Float16 (*selL)[kSelCols] = s.sel[grp];
+    _Float16* selL = reinterpret_cast<_Float16*>(band_lds + kTabBytes) + grp * 4 * C;  // [4][C]
 
     __syncthreads();  // the previous unit's LDS reads are done
     for (int c = k; c < R + G + Pt; c += G) {
       const int j = wb + 1 + c;
       const uint32_t m = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : 0u;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) selL[q][c] = ((m >> q) & 1u) ? _Float16(1) : _Float16(0);
+      for (int q = 0; q < 4; ++q) selL[q * C + c] = ((m >> q) & 1u) ? _Float16(1) : _Float16(0);
     }
 
-    // Decay factors of the D scan (constant gap continuation along the read).
-    const float yy = a.ph2pr[a.gcp[ro] & 127];
-    const float yy2 = yy * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4;
-    float f15 = (row16 & 1) ? 1.f : 0.f, f31 = (row16 >= 2) ? 1.f : 0.f;
-    for (int m = 0; m <= (lane & 15); ++m) f15 *= yy;  // distance to lane 15 of the row below
-    for (int m = 31; m < lane; ++m) f31 *= yy;        // distance to lane 31
+    // Decay factors of the D scan (constant gap continuation along the read)
+    // and the gap-to-match factor g that I and D are carried times.
+    const int cq0 = a.gcp[ro] & 127;
+    const float yy = a.ph2pr[cq0];
+    const float gx = 1.f - a.ph2pr[cq0];
+    const double igx = 1.0 / static_cast<double>(gx);
+    const float yy2 = yy * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4, yy16 = yy8 * yy8;
+    auto ypow = [&](int e) {  // yy^e for 1 <= e <= 32
+      float p = (e & 1) ? yy : 1.f;
+      if (e & 2) p *= yy2;
+      if (e & 4) p *= yy4;
+      if (e & 8) p *= yy8;
+      if (e & 16) p *= yy16;
+      if (e & 32) p *= yy16 * yy16;
+      return p;
+    };
+    const float f15 = (row16 & 1) ? ypow((lane & 15) + 1) : 0.f;  // distance to lane 15 of the row below
+    const float f31 = (row16 >= 2) ? ypow(lane - 31) : 0.f;       // distance to lane 31
     // Edge zones (T = F there): the left one where the band's left
     // neighbour column is inside the matrix, the right one where its right
     // neighbour column is.  Without a slide the rows where each is active
@@ -305,13 +329,15 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
 
     // Row 0: M = I = 0, D = K/H for columns 0..H (Context.h:109, template :164-175).
     int o_cur = off(1);
+    // this lane's selector slot k + o_cur - wb (the row adds its part)
+    const _Float16* sl = selL + (k + o_cur - wb);
     float Mp = 0.f, Ip = 0.f;
     // A read end explained as a run of insertions leaves the band through
-    // its bottom lane and sweeps down one diagonal per row; xg (in the bottom
-    // lane) carries the I mass that left, continued as pure runs to the last
-    // row, and is added to the touch mass.
+    // its bottom lane and sweeps down one diagonal per row; xg (the bottom
+    // lane's value counts) carries the I mass that left, continued as pure
+    // runs to the last row, and is added to the touch mass.
     float xg = 0.f;
-    float Dp = (o_cur + k >= 0 && o_cur + k <= H) ? kK / static_cast<float>(H) : 0.f;
+    float Dp = (o_cur + k >= 0 && o_cur + k <= H) ? gx * (kK / static_cast<float>(H)) : 0.f;
     float TMp = 0.f, TIp = 0.f, TDp = 0.f;
     {
       const bool ez = (left_lane && o_cur - 1 >= 0) || (right_lane && o_cur + G <= H);
@@ -320,12 +346,13 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
 
     for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
       __syncthreads();
+      uint32_t selo = 0;  // row i0 + lane's selector offset (bytes)
       {
         const int r = i0 + lane;
         if (r <= R) {
           const int64_t p = ro + r - 1;
           const uint8_t rc = a.rclass[p];
-          const int iq = a.ins[p] & 127, dq = a.del[p] & 127, cq = a.gcp[p] & 127;
+          const int iq = a.ins[p] & 127, dq = a.del[p] & 127;
           const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
           const float e = a.ph2pr[a.q[p] & 127];
           RowRec rr;
@@ -333,31 +360,34 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
           rr.pe = (rc & kWildcard) ? pm : e / 3.f;
           rr.dp = pm - rr.pe;
           rr.mm = a.mm[((hi * (hi + 1)) >> 1) + lo];
-          rr.g = 1.f - a.ph2pr[cq];
-          rr.mx = a.ph2pr[iq];
-          rr.xx = a.ph2pr[cq];
-          rr.my = a.ph2pr[dq];
-          rr.cls = rc & 3;
-          s.tab[lane] = rr;
+          selo = static_cast<uint32_t>(2 * ((rc & 3) * C + r - 1));
+          rr.mx = gx * a.ph2pr[iq];
+          rr.xx = yy;
+          rr.my = gx * a.ph2pr[dq];
+          rr.pad[0] = rr.pad[1] = 0.f;
+          tab[lane] = rr;
         }
       }
       __syncthreads();
-      const int n = min(kBandLanes, R - i0 + 1);
-      for (int v = 0; v < n; ++v) {
+      const int n = __builtin_amdgcn_readfirstlane(min(kBandLanes, R - i0 + 1));
+      // window slot i - 1 + k + o_cur - wb (in [0, R + G + Pt)): so is the
+      // row's part in bytes
+      auto sel_at = [&](uint32_t so) {
+        return *reinterpret_cast<const _Float16*>(reinterpret_cast<const char*>(sl) + so);
+      };
+      // Row v of the block with its record, selector offset and selector
+      // (read before the row starts); a slide re-reads the selector, and the
+      // one read ahead for the next row (nsel, at offset nso) if given.
+      auto row = [&](int v, const RowRec& rr, uint32_t so, _Float16 selh, _Float16* nsel, uint32_t nso) {
         const int i = i0 + v;
-        const RowRec rr = s.tab[v];
-        const int oi = off(i);
-        // the selector at the offset the schedule asks for (window slot
-        // i - 1 + k + off(i) - wb is in [0, R + G + Pt))
-        float sel = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr.cls)][i - 1 + k + oi - wb]);
         if constexpr (kSlideG > 0) {
-          if (oi < o_cur) {
+          if (off(i) < o_cur) {
             // slide down one diagonal: lane k takes lane k - 1's cells and
             // the top lane is dropped, unless it holds more than kBandHold
             // of the row's mass
-            const float wl = Mp + Ip + Dp;
-            const float wtop = __shfl(wl, kBandLanes - 1);
-            if (__builtin_amdgcn_readfirstlane(static_cast<int>(wtop <= kBandHold * wsumf(wl)))) {
+            const float wl = __builtin_fmaf(Ip + Dp, static_cast<float>(igx), Mp);
+            const float wtop = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wl), kBandLanes - 1));
+            if (wtop <= kBandHold * wtotalf(wl)) {
               Mp = dpp<kWaveShr1>(Mp);
               Ip = dpp<kWaveShr1>(Ip);
               Dp = dpp<kWaveShr1>(Dp);
@@ -365,14 +395,14 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
               TIp = dpp<kWaveShr1>(TIp);
               TDp = dpp<kWaveShr1>(TDp);
               --o_cur;
+              --sl;
+              selh = sel_at(so);
+              if (nsel) *nsel = sel_at(nso);
             }
           }
         }
-        const int oc = kSlideG > 0 ? o_cur : oi;
-        // a held slide leaves the band above the schedule: read again
-        // (window slot i - 1 + k + oc - wb is in [0, R + G + Pt))
-        if (kSlideG > 0 && oc != oi)
-          sel = static_cast<float>(selL[__builtin_amdgcn_readfirstlane(rr.cls)][i - 1 + k + oc - wb]);
+        const int oc = kSlideG > 0 ? o_cur : o_lo;  // without a slide off(i) = o_lo
+        const float sel = static_cast<float>(selh);
         const float prior = __builtin_fmaf(sel, rr.dp, rr.pe);
         bool ez;
         if constexpr (kSlideG > 0) {
@@ -381,17 +411,17 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
         } else {
           ez = i >= ez_from || i <= ez_to;
         }
-        // the I mass leaving the bottom lane from row i - 1 (counted on the
-        // last kEdgeRunRows rows; before them a run's factor yy^rows to the
-        // last row is below 1e-46)
+        // the I mass leaving the bottom lane from row i - 1, times g (counted
+        // on the last kEdgeRunRows rows; before them a run's factor yy^rows
+        // to the last row is below 1e-46)
         if (i > R - kEdgeRunRows) {
           const int c0 = i - 1 + oc;  // the bottom lane's column on row i - 1
-          xg = __builtin_fmaf(xg, yy, (bottom_lane && c0 >= 1 && c0 <= H)
-                                          ? __builtin_fmaf(Mp, rr.mx, Ip * rr.xx) : 0.f);
+          const float xe = __builtin_fmaf(Mp, rr.mx, Ip * rr.xx);
+          xg = __builtin_fmaf(xg, yy, (c0 >= 1 && c0 <= H) ? xe : 0.f);
         }
 
-        // forward (I and D into M share the factor g = 1 - ph2pr[gcp])
-        const float Mn = __builtin_fmaf(Mp, rr.mm, (Ip + Dp) * rr.g) * prior;
+        // forward (I and D are carried times g)
+        const float Mn = __builtin_fmaf(Mp, rr.mm, Ip + Dp) * prior;
         float In = insert_next(Mp, Ip, rr.mx, rr.xx);
         float x = dpp<kWaveShr1>(Mn) * rr.my;
         if constexpr (G != kBandLanes) {
@@ -401,7 +431,7 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
         }
 
         // touch mass
-        float TMn = __builtin_fmaf(TMp, rr.mm, (TIp + TDp) * rr.g) * prior;
+        float TMn = __builtin_fmaf(TMp, rr.mm, TIp + TDp) * prior;
         float TIn = insert_next(TMp, TIp, rr.mx, rr.xx);
         if (ez) {
           TMn = Mn;
@@ -424,13 +454,30 @@ __global__ __launch_bounds__(64) void band_forward_kernel(BArgs a) {
         TMp = TMn;
         TIp = TIn;
         TDp = TDn;
+      };
+      // two rows per step, both rows' operands read at its start
+      int v = 0;
+      for (; v + 1 < n; v += 2) {
+        const RowRec ra = tab[v], rb = tab[v + 1];
+        const uint32_t sa = __builtin_amdgcn_readlane(selo, v), sb = __builtin_amdgcn_readlane(selo, v + 1);
+        const _Float16 ha = sel_at(sa);
+        _Float16 hb = sel_at(sb);
+        row(v, ra, sa, ha, &hb, sb);
+        row(v + 1, rb, sb, hb, nullptr, 0u);
+      }
+      if (v < n) {
+        const RowRec ra = tab[v];
+        const uint32_t sa = __builtin_amdgcn_readlane(selo, v);
+        row(v, ra, sa, sel_at(sa), nullptr, 0u);
       }
     }
-    // S = sum over the last row of M + I (template :328-343), columns 1..H.
-    const int j = R + (kSlideG > 0 ? o_cur : off(R)) + k;
+    // S = sum over the last row of M + I (template :328-343), columns 1..H;
+    // I is carried times g.
+    const int j = R + (kSlideG > 0 ? o_cur : o_lo) + k;
     const bool in = j >= 1 && j <= H;
-    double S = in ? static_cast<double>(Mp) + static_cast<double>(Ip) : 0.0;
-    double Tm = (in ? static_cast<double>(TMp) + static_cast<double>(TIp) : 0.0) + static_cast<double>(xg);
+    double S = in ? __builtin_fma(static_cast<double>(Ip), igx, static_cast<double>(Mp)) : 0.0;
+    double Tm = (in ? __builtin_fma(static_cast<double>(TIp), igx, static_cast<double>(TMp)) : 0.0) +
+                (bottom_lane ? static_cast<double>(xg) * igx : 0.0);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
       S += __shfl_xor(S, o);
@@ -773,16 +820,23 @@ hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+static size_t band_lds_bytes(int pairs, int sel_cols) {
+  return static_cast<size_t>(kTabBytes) + static_cast<size_t>(pairs) * 4 * sel_cols * sizeof(_Float16);
+}
+
 hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(grid), dim3(kBandLanes), 0, st, a);
-  hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid), dim3(kBandLanes), 0, st, a);
+  if (a.sel_cols < band_sel_cols(0) || a.sel_cols > band_sel_cols(kBandReadMax)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(grid), dim3(kBandLanes),
+                     band_lds_bytes(1, a.sel_cols), st, a);
+  hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid), dim3(kBandLanes),
+                     band_lds_bytes(2, a.sel_cols), st, a);
   return hipGetLastError();
 }
 
-int band_blocks_per_cu() {
+int band_blocks_per_cu(int sel_cols) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel<kBandLanes>, kBandLanes, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel<kBandLanes>, kBandLanes,
+                                                   band_lds_bytes(1, sel_cols)) != hipSuccess)
     return 8;
   return n > 0 ? n : 1;
 }

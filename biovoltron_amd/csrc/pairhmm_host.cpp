@@ -20,7 +20,7 @@
 namespace bvp {
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
 hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st);
-int band_blocks_per_cu();
+int band_blocks_per_cu(int sel_cols);
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
 int band_long_blocks_per_cu();
 hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, const int64_t* htab_off,
@@ -128,7 +128,7 @@ struct DeviceCtx {
   bool ready = false;
   int device = 0;
   int cus = 0;
-  int bpc_f32 = 1, bpc_f64 = 1, bpc_band = 1, bpc_long = 1;
+  int bpc_f32 = 1, bpc_f64 = 1, bpc_long = 1;
   std::string arch;
   HostTables host{};
   float* d_ph2pr_f = nullptr;
@@ -169,7 +169,6 @@ int ensure_init_locked(int device) {
   HIP_TRY(hipMemcpy(g_ctx.d_err_scalar, h.err_scalar, sizeof(h.err_scalar), hipMemcpyHostToDevice));
   g_ctx.bpc_f32 = max_blocks_per_cu<float>();
   g_ctx.bpc_f64 = max_blocks_per_cu<double>();
-  g_ctx.bpc_band = band_blocks_per_cu();
   g_ctx.bpc_long = band_long_blocks_per_cu();
   g_ctx.ready = true;
   return BV_OK;
@@ -250,6 +249,7 @@ struct bv_plan {
   int n_tasks = 0;
   int grid32 = 0, grid64 = 0, grid_band = 0, grid_long = 1;
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
+  int sel_cols = 0;  // short-read banded kernel: selector columns per class
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -507,6 +507,7 @@ BArgs band_args(const bv_plan* p) {
   b.mm_d = g_ctx.d_mm_d;
   b.verify_above = kVerifyAbove;
   b.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
+  b.sel_cols = p->sel_cols;
   return b;
 }
 
@@ -742,18 +743,27 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   parallel_for(hbytes, [&](int64_t a0, int64_t a1) {
     for (int64_t i = a0; i < a1; ++i) hmask[i] = al.hap_mask[b->hap_bases[hb0 + i]];
   });
-  // Reads whose gap-continuation quality is the same at every base (the
-  // banded kernel's deletion scan needs a constant decay along the read).
+  // Reads whose gap-continuation quality is the same at every base and not 0
+  // (the banded kernels' deletion scan needs a constant decay along the read,
+  // and the short-read kernel carries I and D times the gap-to-match factor
+  // 1 - ph2pr[q], which is 0 for q = 0).
   std::vector<uint8_t> gcp_const(static_cast<size_t>(std::max<int64_t>(1, b->n_reads)), 0);
   parallel_for(b->n_reads, [&](int64_t a0, int64_t a1) {
     for (int64_t r = a0; r < a1; ++r) {
       const uint8_t* c = b->read_gcp + b->read_off[r];
       const int64_t R = b->read_off[r + 1] - b->read_off[r];
-      bool same = true;
+      bool same = R == 0 || (c[0] & 127) != 0;
       for (int64_t k = 1; k < R && same; ++k) same = (c[k] & 127) == (c[0] & 127);
       gcp_const[r] = same ? 1 : 0;
     }
   });
+  // The short-read banded kernel's selector window: the longest read it may get.
+  int rmax_band = 0;
+  for (int64_t r = 0; r < b->n_reads; ++r) {
+    const int64_t R = b->read_off[r + 1] - b->read_off[r];
+    if (R <= kBandReadMax) rmax_band = std::max(rmax_band, static_cast<int>(R));
+  }
+  p->sel_cols = band_sel_cols(rmax_band);
   int hmax = 0;
   p->hap_len.resize(static_cast<size_t>(b->n_haps));
   for (int64_t h = 0; h < b->n_haps; ++h) {
@@ -826,7 +836,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   // the FP32 passes also run the full-matrix and verify lists (up to one task per pair)
   p->grid32 = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p->grid32, std::max<int64_t>(1, p->n_pairs))));
   p->grid_band = static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_band, p->n_pairs)));
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * band_blocks_per_cu(p->sel_cols), p->n_pairs)));
   const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
   p->grid64 = std::max(1, static_cast<int>(std::min<int64_t>(p->grid64, std::max<int64_t>(1, max_fp64_tasks))));
   p->grid32_max = p->grid32;
