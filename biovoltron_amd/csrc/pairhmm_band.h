@@ -17,6 +17,7 @@ constexpr float kBandHold = 1e-9f;// a move that would drop more of a row's mass
 constexpr int kEdgeRunRows = 48;  // insertion runs leaving the band bottom: last rows counted
 constexpr int kLongEvery = 16;    // long-read band: rows between moves
 constexpr int kLongMove = kBandEdge;  // long-read band: diagonals per move, at most
+constexpr int kBandChunk = 8;  // consecutive units per wavefront assignment (short-read kernel)
 constexpr int kLongStartLift = 16;// long-read start band: the anchor sits this far above lane 0
                                   // less than the centre (room above the alignment)
 
@@ -50,10 +51,8 @@ struct BArgs {
   // and 32-wide bands in read-sharing pairs
   BandItem* items;
   int* item_count;
-  int* counter;       // work queue of the 64-wide list
   BandItem* items32;
   int* item32_count;
-  int* counter32;     // work queue of the 32-wide list (counts item pairs)
   // pairs for the full-matrix FP32 pass (ineligible or rejected)
   Task* full;
   int* full_count;

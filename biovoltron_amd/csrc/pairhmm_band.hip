@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "pairhmm_band.h"
 
@@ -256,7 +257,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   const int k = lane % G;       // lane within the slot: diagonal o_lo + k
   const BandItem* items = G == kBandLanes ? a.items : a.items32;
   const int n_units = G == kBandLanes ? *a.item_count : (*a.item32_count) / 2;
-  int* queue = G == kBandLanes ? a.counter : a.counter32;
   const bool left_lane = k < kBandEdge;
   const bool right_lane = k >= G - kBandEdge;
   const bool bottom_lane = k == 0;
@@ -264,11 +264,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   const int row16 = lane >> 4;
   int accepted = 0;
   unsigned long long cells = 0;
-  for (;;) {
-    int u = 0;
-    if (lane == 0) u = atomicAdd(queue, 1);
-    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0));
-    if (u >= n_units) break;
+  // Units are assigned statically, kBandChunk consecutive units (typically
+  // the pairs of one read) at a time, chunk c to wavefront c mod gridDim: the
+  // lists are in task order, longest tasks first, so every wavefront gets the
+  // same mix.  (A shared work counter costs one same-address atomic per unit,
+  // and those serialise at the L2 slower than the units run.)
+  auto next_unit = [&](int u) {
+    return (u + 1) % kBandChunk != 0 ? u + 1 : u + 1 + static_cast<int>(gridDim.x - 1) * kBandChunk;
+  };
+  for (int u = blockIdx.x * kBandChunk; u < n_units; u = next_unit(u)) {
     const BandItem w = items[P * u + grp];
     const bool live = w.hap >= 0;
     const int read = __builtin_amdgcn_readfirstlane(w.read);
@@ -376,11 +380,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
         return *reinterpret_cast<const _Float16*>(reinterpret_cast<const char*>(sl) + so);
       };
       // Row v of the block with its record, selector offset and selector
-      // (read before the row starts); a slide re-reads the selector, and the
-      // one read ahead for the next row (nsel, at offset nso) if given.
-      auto row = [&](int v, const RowRec& rr, uint32_t so, _Float16 selh, _Float16* nsel, uint32_t nso) {
+      // (read before the row starts).  kSlide: the row may take a step of
+      // the start slide (a slide re-reads the selector, and the one read
+      // ahead for the next row, nsel at offset nso, if given).  kExit: 0 no
+      // exit-run term, 1 the row is in the exit zone, 2 check.
+      auto row = [&](auto kSlide, auto kExit, int v, const RowRec& rr, uint32_t so, _Float16 selh,
+                     _Float16* nsel, uint32_t nso) {
         const int i = i0 + v;
-        if constexpr (kSlideG > 0) {
+        if constexpr (decltype(kSlide)::value) {
           if (off(i) < o_cur) {
             // slide down one diagonal: lane k takes lane k - 1's cells and
             // the top lane is dropped, unless it holds more than kBandHold
@@ -414,7 +421,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
         // the I mass leaving the bottom lane from row i - 1, times g (counted
         // on the last kEdgeRunRows rows; before them a run's factor yy^rows
         // to the last row is below 1e-46)
-        if (i > R - kEdgeRunRows) {
+        constexpr int kX = decltype(kExit)::value;
+        if (kX == 1 || (kX == 2 && i > R - kEdgeRunRows)) {
           const int c0 = i - 1 + oc;  // the bottom lane's column on row i - 1
           const float xe = __builtin_fmaf(Mp, rr.mx, Ip * rr.xx);
           xg = __builtin_fmaf(xg, yy, (c0 >= 1 && c0 <= H) ? xe : 0.f);
@@ -455,21 +463,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
         TIp = TIn;
         TDp = TDn;
       };
-      // two rows per step, both rows' operands read at its start
+      // Rows [v, end) two per step, both rows' operands read at its start.
+      auto rows = [&](auto kSlide, auto kExit, int v, int end) {
+        for (; v + 1 < end; v += 2) {
+          const RowRec ra = tab[v], rb = tab[v + 1];
+          const uint32_t sa = __builtin_amdgcn_readlane(selo, v), sb = __builtin_amdgcn_readlane(selo, v + 1);
+          const _Float16 ha = sel_at(sa);
+          _Float16 hb = sel_at(sb);
+          row(kSlide, kExit, v, ra, sa, ha, &hb, sb);
+          row(kSlide, kExit, v + 1, rb, sb, hb, nullptr, 0u);
+        }
+        if (v < end) {
+          const RowRec ra = tab[v];
+          const uint32_t sa = __builtin_amdgcn_readlane(selo, v);
+          row(kSlide, kExit, v, ra, sa, sel_at(sa), nullptr, 0u);
+        }
+      };
+      using T_ = std::true_type;
+      using F_ = std::false_type;
+      using X0 = std::integral_constant<int, 0>;
+      using X1 = std::integral_constant<int, 1>;
+      using X2 = std::integral_constant<int, 2>;
+      // Phases, so that the common rows carry no branch: the start slide
+      // (rows up to 1 + the rows the band still stands above o_lo), the rows
+      // before the exit zone, the rows in it.
       int v = 0;
-      for (; v + 1 < n; v += 2) {
-        const RowRec ra = tab[v], rb = tab[v + 1];
-        const uint32_t sa = __builtin_amdgcn_readlane(selo, v), sb = __builtin_amdgcn_readlane(selo, v + 1);
-        const _Float16 ha = sel_at(sa);
-        _Float16 hb = sel_at(sb);
-        row(v, ra, sa, ha, &hb, sb);
-        row(v + 1, rb, sb, hb, nullptr, 0u);
+      if constexpr (kSlideG > 0) {
+        while (v < n && (i0 + v == 1 || o_cur > o_lo)) {
+          const RowRec ra = tab[v];
+          const uint32_t sa = __builtin_amdgcn_readlane(selo, v);
+          row(T_{}, X2{}, v, ra, sa, sel_at(sa), nullptr, 0u);
+          ++v;
+        }
       }
-      if (v < n) {
-        const RowRec ra = tab[v];
-        const uint32_t sa = __builtin_amdgcn_readlane(selo, v);
-        row(v, ra, sa, sel_at(sa), nullptr, 0u);
-      }
+      const int vx = min(n, max(v, R - kEdgeRunRows + 1 - i0));  // first row of the exit zone
+      rows(F_{}, X0{}, v, vx);
+      rows(F_{}, X1{}, vx, n);
     }
     // S = sum over the last row of M + I (template :328-343), columns 1..H;
     // I is carried times g.

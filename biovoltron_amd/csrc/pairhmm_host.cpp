@@ -287,7 +287,7 @@ struct bv_plan {
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
   // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
   // cells (u64), [8..9] swept cells (u64), [10] wide banded item count,
-  // [11] its queue, [12] narrow banded item count, [13] its queue, [14]
+  // [11] unused, [12] narrow banded item count, [13] unused, [14]
   // verify count, [15] its queue, [16] FP64-first queue, [17] moving-band
   // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
   // [21] accepted moving-band pairs, [22..23] FP64 cells (u64)
@@ -481,10 +481,8 @@ BArgs band_args(const bv_plan* p) {
   b.band = p->d_band;
   b.items = p->d_items;
   b.item_count = p->d_counters + 10;
-  b.counter = p->d_counters + 11;
   b.items32 = p->d_items32;
   b.item32_count = p->d_counters + 12;
-  b.counter32 = p->d_counters + 13;
   b.full = p->d_fallback;
   b.full_count = p->d_counters + 3;
   b.ph2pr = g_ctx.d_ph2pr_f;

@@ -320,6 +320,9 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
   const int lane = threadIdx.x;
   Pair2<T>* carry = static_cast<Pair2<T>*>(a.carry) + static_cast<int64_t>(blockIdx.x) * a.carry_stride;
   const int n_tasks = a.n_tasks_dev ? *a.n_tasks_dev : a.n_tasks;
+  // Cell counts are summed per wavefront and added once at the end (a
+  // same-address atomic per pair would queue at the L2).
+  unsigned long long cells_w = 0;
   for (;;) {
     int ti = 0;
     if (lane == 0) ti = atomicAdd(a.counter, 1);
@@ -331,13 +334,8 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
       const int64_t o = tk.out0 + h;
       const double S = pair_forward<T>(a, s, carry, lane, tk.read, tk.hap0 + h, shift);
       if (lane == 0) {
-        if (a.cells_done != nullptr) {
-          const unsigned long long cells =
-              static_cast<unsigned long long>(a.read_off[tk.read + 1] - a.read_off[tk.read]) *
-              static_cast<unsigned long long>(a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h]);
-          atomicAdd(a.cells_done, cells);
-          if (sizeof(T) == 8 && a.fp64_cells != nullptr) atomicAdd(a.fp64_cells, cells);
-        }
+        cells_w += static_cast<unsigned long long>(a.read_off[tk.read + 1] - a.read_off[tk.read]) *
+                   static_cast<unsigned long long>(a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h]);
         if (a.out_raw) {
           a.out[o] = ldexp(S, -shift);
           a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
@@ -372,6 +370,10 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
         }
       }
     }
+  }
+  if (lane == 0 && cells_w != 0 && a.cells_done != nullptr) {
+    atomicAdd(a.cells_done, cells_w);
+    if (sizeof(T) == 8 && a.fp64_cells != nullptr) atomicAdd(a.fp64_cells, cells_w);
   }
 }
 
