@@ -28,10 +28,6 @@ constexpr int kMaxOcc = 4;    // occurrences of one seed used for the histogram
 constexpr int kHapChunk = 4;  // haplotypes whose lookups are in flight together
 constexpr int kXcds = 8;      // MI355X accelerator complex dies
 
-struct FeatLds {
-  int cnt[64];  // offset counts (median, histogram)
-};
-
 __device__ __forceinline__ int code_of_mask(uint8_t m) {
   // single-class masks 1,2,4,8 -> 0..3; anything else (N, 0) -> -1
   switch (m) {
@@ -47,18 +43,28 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) {
   return (key * 2654435761u >> 8) & mask;
 }
 
-// The features kernel runs one wavefront per block: its LDS traffic is
-// ordered by the wave's in-order DS queue, so a compiler barrier suffices
-// between phases (no s_barrier, no wait on outstanding global stores).
-__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+// Wave-wide minimum and maximum by DPP (no LDS round trips): row_shr 1, 2,
+// 4, 8 inside each 16-lane row, then row_bcast:15 into rows 1 and 3 and
+// row_bcast:31 into rows 2 and 3.  Lanes a move does not reach keep the
+// identity (bound_ctrl off, `old` = identity).
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
 
-__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  return x;
+template <typename Op>
+__device__ __forceinline__ int wave_scan_dpp(int x, int id, Op op) {
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 1, 0xf, 0xf, false));
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 2, 0xf, 0xf, false));
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 4, 0xf, 0xf, false));
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, kDppRowShr + 8, 0xf, 0xf, false));
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, kDppRowBcast15, 0xa, 0xf, false));
+  x = op(x, __builtin_amdgcn_update_dpp(id, x, kDppRowBcast31, 0xc, 0xf, false));
+  return x;  // inclusive scan; lane 63 holds the total
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+  return __builtin_amdgcn_readlane(wave_scan_dpp(x, INT_MAX, [](int u, int v) { return min(u, v); }), 63);
+}
+__device__ __forceinline__ int wave_max(int x) {
+  return __builtin_amdgcn_readlane(wave_scan_dpp(x, INT_MIN, [](int u, int v) { return max(u, v); }), 63);
 }
 
 // Per haplotype, an open-addressing table of its 12-mers: entry = key |
@@ -105,7 +111,6 @@ __global__ __launch_bounds__(256) void hap_table_kernel(const uint8_t* __restric
 // anchor is their median), and up to kMaxOcc occurrences feed the offset
 // histogram, so repeat copies show as extra offsets.
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
-  __shared__ FeatLds s;
   const int lane = threadIdx.x;
   // Workgroups go round-robin to the 8 XCDs, each with its own L2.  Tasks are
   // in region order; giving each XCD a contiguous eighth of them keeps only a
@@ -209,61 +214,51 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       const int found_off = first_c[c] != INT_MAX ? first_c[c] - my_p : INT_MIN;
       const int nocc = nocc_c[c];
       const int* occ = occ_c[c];
-      // median of the hit offsets as the anchor
+      // Median of the hit offsets as the anchor: walk the distinct offsets in
+      // ascending order (wave minimum, then the lanes holding it by ballot)
+      // until more than half of the hits are behind -- one or two steps as a
+      // rule, at most one per hit.
       const bool hit = found_off != INT_MIN;
       const int nhit = __popcll(__ballot(hit));
       if (nhit > 0) {
-        int mn = hit ? found_off : INT_MAX, mx = hit ? found_off : INT_MIN;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          mn = min(mn, __shfl_xor(mn, off));
-          mx = max(mx, __shfl_xor(mx, off));
-        }
         const int half = (nhit - 1) / 2;
-        if (mx - mn < 64) {
-          // counting: offsets mn + [0, 64)
-          wave_sync();
-          s.cnt[lane] = 0;
-          wave_sync();
-          if (hit) atomicAdd(&s.cnt[found_off - mn], 1);
-          wave_sync();
-          const int incl = wave_incl_scan(s.cnt[lane], lane);
-          const unsigned long long past = __ballot(incl > half);
-          anchor = mn + static_cast<int>(__ffsll(static_cast<long long>(past)) - 1);
-        } else {
-          int below = 0, equal = 0;
-          for (int l = 0; l < 64; ++l) {
-            const int v = __shfl(found_off, l);
-            const bool vh = __shfl(static_cast<int>(hit), l) != 0;
-            if (vh && hit) {
-              below += v < found_off;
-              equal += v == found_off;
-            }
+        int rest = hit ? found_off : INT_MAX;
+        int behind = 0;
+        for (;;) {
+          const int v = wave_min(rest);
+          behind += __popcll(__ballot(rest == v));
+          if (behind > half) {
+            anchor = v;
+            break;
           }
-          const bool is_median = hit && below <= half && half < below + equal;
-          const unsigned long long mb = __ballot(is_median);
-          anchor = __shfl(found_off, static_cast<int>(__ffsll(static_cast<long long>(mb)) - 1));
+          if (rest == v) rest = INT_MAX;
         }
       }
-      // 32 bins of 4 bp over [anchor - 64, anchor + 64)
+      // 32 bins of 4 bp over [anchor - 64, anchor + 64): lane b < 32 counts
+      // bin b.  Each round counts the lanes whose occurrence (slot k) falls in
+      // one bin, by ballot -- as many rounds as distinct bins.
       const float inv = nvalid > 0 ? 1.f / static_cast<float>(nvalid) : 0.f;
-      wave_sync();
-      if (lane < kBins) s.cnt[lane] = 0;
-      wave_sync();
-      for (int k = 0; k < nocc; ++k) {
-        const int bin = (occ[k] - anchor + 64) >> 2;
-        if (bin >= 0 && bin < kBins) atomicAdd(&s.cnt[bin], 1);
+      int my_cnt = 0;
+#pragma unroll
+      for (int k = 0; k < kMaxOcc; ++k) {
+        int bin = -1;
+        if (k < nocc) {
+          const int b = (occ[k] - anchor + 64) >> 2;
+          if (b >= 0 && b < kBins) bin = b;
+        }
+        for (;;) {
+          const unsigned long long live = __ballot(bin >= 0);
+          if (live == 0) break;
+          const int bb = __builtin_amdgcn_readlane(bin, __ffsll(static_cast<long long>(live)) - 1);
+          const unsigned long long same = __ballot(bin == bb);
+          if (lane == bb) my_cnt += __popcll(same);
+          if (bin == bb) bin = -1;
+        }
       }
-      wave_sync();
-      if (lane < kBins) X[4 + lane] = static_cast<float>(s.cnt[lane]) * inv;
+      if (lane < kBins) X[4 + lane] = static_cast<float>(my_cnt) * inv;
       // read ends anchored by seeds that hit next to the anchor
       const bool near = hit && abs(found_off - anchor) <= kEndAnchorTol;
-      int pf = near ? my_p : INT_MAX, pl = near ? my_p + kK : -1;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        pf = min(pf, __shfl_xor(pf, off));
-        pl = max(pl, __shfl_xor(pl, off));
-      }
+      const int pf = wave_min(near ? my_p : INT_MAX), pl = wave_max(near ? my_p + kK : -1);
       if (lane == 0) {
         a.anchor[o] = anchor;
         a.ends[o] = pl >= 0 ? int2{pf, pl} : int2{-1, -1};

@@ -186,6 +186,82 @@ def torch_loss_grad(params, X, T, w):
     return float(loss), p.grad.numpy()
 
 
+SEED_K, MAX_OCC, END_TOL, FEAT_MAX_HAP = 12, 4, 8, 1024
+
+
+def host_features(read, hap):
+    """The seed features of one pair restated on the host (band_kernels.hip,
+    band_features_kernel): lengths, quality summary, the lower median of the
+    seeds' first-hit offsets (the anchor), the 32-bin offset histogram over
+    every occurrence, and the read span of the seeds hitting near the anchor.
+    Returns (x [36], anchor, ends) and whether a seed had more occurrences than
+    the kernel counts (their order then depends on the table layout)."""
+    seq, q = read["seq"], np.frombuffer(read["q"], np.uint8) & 127
+    R, H = len(seq), len(hap)
+    x = np.zeros(36, np.float32)
+    x[0], x[1] = np.float32(R) / 256, np.float32(H) / 512
+    x[2] = np.float32(np.float32(q.sum() / R) / np.float32(40))
+    x[3] = np.float32(q.min()) / np.float32(40)
+    nseeds = min(64, R - SEED_K + 1) if R >= SEED_K else 0
+    stride = max(1, (R - SEED_K + 1 + 63) // 64) if nseeds else 1
+    seeds = [p for p in (l * stride for l in range(nseeds))
+             if p + SEED_K <= R and b"N" not in seq[p:p + SEED_K]]
+    where = {}
+    if 12 <= H <= FEAT_MAX_HAP:
+        for j in range(H - SEED_K + 1):
+            kmer = hap[j:j + SEED_K]
+            if b"N" not in kmer:
+                where.setdefault(kmer, []).append(j)
+    hits, occs, ambiguous = {}, [], False
+    for p in seeds:
+        js = where.get(seq[p:p + SEED_K], [])
+        if js:
+            hits[p] = min(js) - p
+            ambiguous |= len(js) > MAX_OCC
+            occs += [j - p for j in js]
+    anchor = int((H - R) / 2)
+    if hits:
+        anchor = sorted(hits.values())[(len(hits) - 1) // 2]
+    cnt = np.zeros(32, np.int64)
+    for o in occs:
+        b = (o - anchor + 64) >> 2
+        if 0 <= b < 32:
+            cnt[b] += 1
+    inv = np.float32(1) / np.float32(len(seeds)) if seeds else np.float32(0)
+    x[4:] = cnt.astype(np.float32) * inv
+    near = [p for p, o in hits.items() if abs(o - anchor) <= END_TOL]
+    ends = (min(near), max(near) + SEED_K) if near else (-1, -1)
+    return x, anchor, ends, ambiguous
+
+
+@pytest.mark.gpu
+def test_features_match_host_restatement():
+    """Device seed features (hash-table lookups, wave-wide median, histogram
+    and end reductions) against the host restatement, pair by pair."""
+    b = synth.generate(1, count=3)
+    plan = pairhmm.Plan(pairhmm.PackedBatch.from_synth(b))
+    try:
+        X, anc, ends = band.plan_features(plan, with_ends=True)
+    finally:
+        plan.close()
+    o, checked, skipped = 0, 0, 0
+    for g in range(b.n_regions):
+        reads, haps = b.region(g)
+        for rd in reads:
+            for hp in haps:
+                x, a, e, amb = host_features(rd, hp)
+                if amb:
+                    skipped += 1
+                else:
+                    assert anc[o] == a, (g, o)
+                    assert tuple(ends[o]) == e, (g, o)
+                    assert np.allclose(X[o], x, rtol=1e-6, atol=1e-7), (g, o, X[o] - x)
+                    checked += 1
+                o += 1
+    say("features checked", checked, "skipped (seeds with > 4 occurrences)", skipped)
+    assert o == len(X) and checked > 0.99 * o
+
+
 @pytest.mark.gpu
 def test_mlp_gradients_match_torch_reference():
     rng = np.random.default_rng(4)
