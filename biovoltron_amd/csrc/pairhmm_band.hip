@@ -65,7 +65,8 @@ struct alignas(16) RowRec {
   float mx;   // g * (match -> insert)
   float xx;   // insert -> insert
   float my;   // g * (match -> delete, along this row)
-  float pad[2];
+  uint32_t selo;  // byte offset of this row's selectors: 2 * (cls * C + i - 1)
+  float pad;
 };
 
 // Match selectors of the band's haplotype window, per lane group: slot c of
@@ -74,9 +75,8 @@ struct alignas(16) RowRec {
 // i - 1 + k + o(i) - wb; sel[g][cls][c] is 1 where that column matches read
 // class cls (0 outside the haplotype).  The prior is then one mixed-precision
 // FMA, sel * (pm - pe) + pe.  A lane's address is its part k + o(i) - wb
-// (changed only by a slide) plus the row's part 2 * (cls * C + i - 1), which
-// the lane that built row i's record keeps in a register: read with
-// v_readlane, it does not wait for the record's LDS load.  The window has
+// (changed only by a slide) plus the row's part from the row record.  The
+// window has
 // C = a.sel_cols columns per class, sized by the plan's longest banded read
 // (dynamic LDS: shorter reads leave room for more wavefronts per CU).
 constexpr int kTabBytes = static_cast<int>(sizeof(RowRec)) * (kBandLanes + 1);
@@ -350,7 +350,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
 
     for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
       __syncthreads();
-      uint32_t selo = 0;  // row i0 + lane's selector offset (bytes)
       {
         const int r = i0 + lane;
         if (r <= R) {
@@ -364,11 +363,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
           rr.pe = (rc & kWildcard) ? pm : e / 3.f;
           rr.dp = pm - rr.pe;
           rr.mm = a.mm[((hi * (hi + 1)) >> 1) + lo];
-          selo = static_cast<uint32_t>(2 * ((rc & 3) * C + r - 1));
           rr.mx = gx * a.ph2pr[iq];
           rr.xx = yy;
           rr.my = gx * a.ph2pr[dq];
-          rr.pad[0] = rr.pad[1] = 0.f;
+          rr.selo = static_cast<uint32_t>(2 * ((rc & 3) * C + r - 1));
+          rr.pad = 0.f;
           tab[lane] = rr;
         }
       }
@@ -467,7 +466,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
       auto rows = [&](auto kSlide, auto kExit, int v, int end) {
         for (; v + 1 < end; v += 2) {
           const RowRec ra = tab[v], rb = tab[v + 1];
-          const uint32_t sa = __builtin_amdgcn_readlane(selo, v), sb = __builtin_amdgcn_readlane(selo, v + 1);
+          const uint32_t sa = ra.selo, sb = rb.selo;
           const _Float16 ha = sel_at(sa);
           _Float16 hb = sel_at(sb);
           row(kSlide, kExit, v, ra, sa, ha, &hb, sb);
@@ -475,8 +474,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
         }
         if (v < end) {
           const RowRec ra = tab[v];
-          const uint32_t sa = __builtin_amdgcn_readlane(selo, v);
-          row(kSlide, kExit, v, ra, sa, sel_at(sa), nullptr, 0u);
+          row(kSlide, kExit, v, ra, ra.selo, sel_at(ra.selo), nullptr, 0u);
         }
       };
       using T_ = std::true_type;
@@ -491,8 +489,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
       if constexpr (kSlideG > 0) {
         while (v < n && (i0 + v == 1 || o_cur > o_lo)) {
           const RowRec ra = tab[v];
-          const uint32_t sa = __builtin_amdgcn_readlane(selo, v);
-          row(T_{}, X2{}, v, ra, sa, sel_at(sa), nullptr, 0u);
+          row(T_{}, X2{}, v, ra, ra.selo, sel_at(ra.selo), nullptr, 0u);
           ++v;
         }
       }
