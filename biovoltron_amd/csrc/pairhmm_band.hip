@@ -166,6 +166,35 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
+// Sum over each G-lane slot, returned to every lane of the slot: row_shr
+// scans of each 16-lane row, row_bcast:15 into rows 1 and 3 and (G = 64)
+// row_bcast:31 into rows 2 and 3, after which the slot's top lane holds the
+// total.
+template <int G>
+__device__ __forceinline__ float slot_total(float v, int grp) {
+  v += dpp<kRowShr + 1>(v);
+  v += dpp<kRowShr + 2>(v);
+  v += dpp<kRowShr + 4>(v);
+  v += dpp<kRowShr + 8>(v);
+  v += dpp<kRowBcast15, 0xa>(v);
+  if constexpr (G == kBandLanes) {
+    v += dpp<kRowBcast31, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  } else {
+    const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return grp ? s1 : s0;
+  }
+}
+
+// log10 of a positive finite float: the exponent exactly, the mantissa's
+// log2 in [-1, 0) by v_log_f32 (absolute error below 1e-7).
+__device__ __forceinline__ double log10_pos(float x) {
+  int e;
+  const float m = frexpf(x, &e);
+  return (static_cast<double>(e) + static_cast<double>(__log2f(m))) * 0.30102999566398119521;
+}
+
 // Wave-wide exclusive prefix sum of v; total in `tot`.
 __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& tot) {
   int x = v;
@@ -501,18 +530,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
     // I is carried times g.
     const int j = R + (kSlideG > 0 ? o_cur : o_lo) + k;
     const bool in = j >= 1 && j <= H;
-    double S = in ? __builtin_fma(static_cast<double>(Ip), igx, static_cast<double>(Mp)) : 0.0;
-    double Tm = (in ? __builtin_fma(static_cast<double>(TIp), igx, static_cast<double>(TMp)) : 0.0) +
-                (bottom_lane ? static_cast<double>(xg) * igx : 0.0);
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-      S += __shfl_xor(S, o);
-      Tm += __shfl_xor(Tm, o);
-    }
+    const float ig = static_cast<float>(igx);
+    const float S = slot_total<G>(in ? __builtin_fmaf(Ip, ig, Mp) : 0.f, grp);
+    const float Tm = slot_total<G>((in ? __builtin_fmaf(TIp, ig, TMp) : 0.f) + (bottom_lane ? xg * ig : 0.f), grp);
     if (live) cells += static_cast<unsigned long long>(R) * G;
     if (k == 0 && live) {
-      if (S >= static_cast<double>(kMinAccepted) && Tm <= static_cast<double>(a.tol) * S) {
-        a.out[w.out] = log10(S) - kLog10K;
+      if (S >= kMinAccepted && Tm <= static_cast<float>(a.tol) * S) {
+        a.out[w.out] = log10_pos(S) - kLog10K;
         a.form[w.out] = kFormBandFP32;
         ++accepted;
       } else {
