@@ -118,6 +118,58 @@ def cpu_baseline(batch, out_gpu, budget_s: float):
                        f"{os.cpu_count()} host CPUs visible, {cores} in affinity"), worst
 
 
+FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_rejected_full_fp32", 4: "band_fp64_long"}
+
+
+def oracle_form_errors(batch, legs, rr, hh, short_per_form: int = 600, long_per_form: int = 6):
+    """Max |dlog10 L| of each result form against the FP64 full-matrix oracle
+    (oracle/liboracle_pairhmm.so, bvo_log10_exact: per-row renormalised FP64
+    forward, never underflows) on a seeded sample of pairs per form.  `legs`
+    maps a leg name to its (log10 L, form) arrays; rr, hh give each pair's
+    read and haplotype; forms are split by read length (reads of 1024 bases
+    or more run FP64 first)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    lib = C.CDLL(str(ROOT / "oracle" / "liboracle_pairhmm.so"))
+    lib.bvo_init()
+    ex = lib.bvo_log10_exact
+    ex.restype = C.c_double
+    ex.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_int]
+    b = batch
+    rlen = np.diff(b.read_off)[rr]
+    rng = np.random.default_rng(7)
+    base = lambda a, o: a.ctypes.data + int(o)  # noqa: E731
+
+    def exact(p):
+        r, h = int(rr[p]), int(hh[p])
+        ro, ho = int(b.read_off[r]), int(b.hap_off[h])
+        return ex(int(b.read_off[r + 1]) - ro, int(b.hap_off[h + 1]) - ho,
+                  base(b.read_bases, ro), base(b.read_qual, ro), base(b.read_ins, ro),
+                  base(b.read_del, ro), base(b.read_gcp, ro), base(b.hap_bases, ho), 0)
+
+    jobs = []
+    for leg, (out, form) in legs.items():
+        for code, name in FORM_NAMES.items():
+            for long_reads in (False, True):
+                sel = np.flatnonzero((form == code) & ((rlen >= LONG_ROWS) == long_reads))
+                if sel.size == 0:
+                    continue
+                k = min(sel.size, long_per_form if long_reads else short_per_form)
+                pick = np.sort(rng.choice(sel, size=k, replace=False))
+                jobs.append((f"{leg}/{name}/{'long' if long_reads else 'short'}", out, pick, sel.size))
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    report = {}
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as pool:
+        for key, out, pick, total in jobs:
+            ref = np.array(list(pool.map(exact, pick.tolist())))
+            d = np.abs(out[pick] - ref)
+            report[key] = dict(pairs_in_form=int(total), sampled=int(pick.size),
+                               max_abs_dlog10=float(d.max()),
+                               mean_abs_dlog10=float(d.mean()))
+    return report, round(time.perf_counter() - t0, 1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,7 +181,10 @@ def main():
     ap.add_argument("--tune-regions", type=int, default=2000,
                     help="the autotuner times runs of a plan of this many regions of the same "
                          "workload and the full plan takes its grids")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU legs (reference timing and the FP64-oracle error per form)")
+    ap.add_argument("--no-oracle", action="store_true",
+                    help="skip the FP64-oracle error per form")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--band-model", default=str(ROOT / "biovoltron_amd" / "models" / "band_mlp.npz"),
                     help="trained band model; the learned-band leg is skipped when absent")
@@ -183,6 +238,7 @@ def main():
     t2 = time.perf_counter()
     ms_step = (t2 - t1) * 1e3 / args.full_steps
     out, form = plan.results()
+    legs = {"full": (out, form)}
     # split of one evaluation between the passes, from HIP events on the stream
     ev_ms = plan.time(1)
     st = plan.stats()
@@ -215,6 +271,7 @@ def main():
         t2 = time.perf_counter()
         ms_band = (t2 - t1) * 1e3 / args.steps
         out_b, form_b = plan.results()
+        legs["learned_band"] = (out_b, form_b)
         plan.time(1)
         sb = plan.stats()
         fin = np.isfinite(out)
@@ -248,6 +305,24 @@ def main():
               f"long reads {int(ran_long.sum())} ({sb.long_band_pairs} accepted, "
               f"{sb.last_ms_long:.2f} ms), fp64 {sb.last_ms_f64:.2f} ms", file=sys.stderr, flush=True)
         out = out_b
+    # Device post-processing of the last run (normalise, filter, marginalise
+    # over a two-allele map: haplotype h of a region -> allele h % 2), timed
+    # with HIP events around its kernel; bytes a caller copies back with it
+    # against copying the whole matrix.
+    nh = np.diff(b.region_hap_off)
+    n_all = np.minimum(nh, 2).astype(np.int32)
+    ha = np.concatenate([np.arange(k) % 2 for k in nh]).astype(np.int32) if b.n_regions else np.zeros(0, np.int32)
+    keep, _, A = plan.postprocess(ha, n_all)
+    post_ms = plan.stats().last_ms_post
+    post_bytes = b.n_pairs * 8 + b.n_reads * 9 + 2 * A.size * 8  # read L, write keep + A
+    post_info = dict(
+        kernel_ms=round(post_ms, 3), reads=int(b.n_reads), reads_kept=int(keep.sum()),
+        allele_entries=int(A.size), hbm_bytes=int(post_bytes),
+        hbm_gbps=round(post_bytes / (post_ms / 1e3) / 1e9, 1) if post_ms > 0 else None,
+        d2h_bytes=int(b.n_reads + A.size * 8), d2h_bytes_full_matrix=int(b.n_pairs * 9))
+    print(f"[bench] post-processing: {post_ms:.3f} ms, {post_info['hbm_gbps']} GB/s, "
+          f"copy-back {post_info['d2h_bytes'] / 1e9:.2f} GB vs {post_info['d2h_bytes_full_matrix'] / 1e9:.2f} GB",
+          file=sys.stderr, flush=True)
     line = {
         "metric": METRIC,
         "value": round(band_info["gcups"] if band_info else gcups, 3),
@@ -271,6 +346,7 @@ def main():
                             "steps": args.full_steps,
                             "grids": tuned_full or dict(fp32=st.grid_f32, fp64=st.grid_f64)},
             "learned_band": band_info,
+            "postprocess": post_info,
             "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,
             "seed": synth.workload_seed(args.workload),
             "fp64_pairs": st.fp64_pairs, "fp64_cells": st.fp64_cells,
@@ -320,6 +396,13 @@ def main():
         cb, worst = cpu_baseline(b, out, args.cpu_seconds)
         line["cpu_baseline"] = cb
         line["config"]["max_abs_dlog10_vs_cpu_sample"] = worst
+        if not args.no_oracle:
+            rep, secs = oracle_form_errors(b, legs, rr, hh)
+            line["config"]["vs_fp64_oracle"] = dict(
+                sample="seeded per form: up to 600 short-read and 6 long-read pairs per form and leg",
+                seconds=secs, forms=rep)
+            print(f"[bench] vs FP64 oracle ({secs} s): " + ", ".join(
+                f"{k} {v['max_abs_dlog10']:.2e}" for k, v in rep.items()), file=sys.stderr, flush=True)
     plan.close()
     if args.copy_calibration > 0:
         # A device-to-device copy of known size through the HIP runtime's own

@@ -96,4 +96,23 @@ struct KArgs {
   unsigned long long* fp64_cells;  // FP64 passes: sum of R*H over the pairs they compute
 };
 
+// Post-processing of one run's results (post_kernel): per read, the
+// normalised row, the filter decision and the per-allele maxima.
+struct PostArgs {
+  const double* L;                // log10 L per pair, region by region, read-major
+  int64_t read0, read_end;        // the reads the regions cover: [read0, read_end)
+  int64_t n_regions;
+  const int64_t* region_read_off; // n_regions + 1 (read indices)
+  const int64_t* region_hap_off;  // n_regions + 1
+  const int64_t* region_pair_off; // n_regions + 1
+  const int64_t* read_off;        // n_reads + 1 (read lengths)
+  int normalize;                  // cap rows at best - 4.5 before writing Lnorm / A
+  uint8_t* keep;                  // n_reads, or null
+  double* Lnorm;                  // n_pairs, or null
+  const int32_t* hap_allele;      // per haplotype: allele index in its region
+  const int32_t* n_alleles;       // per region
+  const int64_t* allele_off;      // per region: first entry of its block in A
+  double* A;                      // sum over regions of reads * alleles, or null
+};
+
 }  // namespace bvp

@@ -45,6 +45,7 @@ hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st);
 template <typename T>
 int max_blocks_per_cu();
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
+hipError_t launch_post(const PostArgs& a, hipStream_t st);
 }  // namespace bvp
 
 namespace {
@@ -298,6 +299,20 @@ struct bv_plan {
   void* d_carry32 = nullptr;
   void* d_carry64 = nullptr;
   float last_ms32 = 0.f, last_ms64 = 0.f, last_msfb = 0.f, last_msband = 0.f, last_mslong = 0.f;
+  // post-processing (bv_plan_postprocess): region tables (host copies kept
+  // from bv_plan_create, device copies made on first use) and its buffers
+  int64_t n_regions = 0;
+  std::vector<int64_t> region_read_off, region_hap_off, region_pair_off;
+  int64_t* d_region_tab = nullptr;  // [read_off | hap_off | pair_off], 3 x (n_regions + 1)
+  int32_t* d_hap_allele = nullptr;  // n_haps
+  int32_t* d_n_alleles = nullptr;   // n_regions
+  int64_t* d_allele_off = nullptr;  // n_regions + 1
+  double* d_lnorm = nullptr;        // n_pairs
+  uint8_t* d_keep = nullptr;        // n_reads
+  double* d_allele = nullptr;
+  int64_t allele_cap = 0;
+  hipEvent_t ev_post[2] = {nullptr, nullptr};
+  float last_mspost = 0.f;
   // one run captured as a graph (bv_plan_run), rebuilt when the band setup changes
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -314,10 +329,14 @@ void free_plan(bv_plan* p) {
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items32, p->d_items_long, p->d_long_full,
-                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off};
+                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off,
+                  p->d_region_tab, p->d_hap_allele, p->d_n_alleles, p->d_allele_off, p->d_lnorm,
+                  p->d_keep, p->d_allele};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : p->ev_post)
     if (e) (void)hipEventDestroy(e);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -672,7 +691,7 @@ void bv_options_default(bv_options* opt) {
   std::memset(opt, 0, sizeof(*opt));
   opt->mode = BV_MODE_AVX;
   opt->fp64_renorm = 1;
-  opt->band_tol = 1e-6f;
+  opt->band_tol = 5e-7f;
   opt->band_narrow = 1;
   opt->use_graph = 1;
 }
@@ -781,7 +800,12 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   std::vector<int64_t> cost, cost64;
   cost.reserve(static_cast<size_t>(b->n_reads));
   int64_t pair_base = 0, cells = 0;
+  p->n_regions = b->n_regions;
+  p->region_read_off.assign(b->region_read_off, b->region_read_off + (b->n_regions ? b->n_regions + 1 : 0));
+  p->region_hap_off.assign(b->region_hap_off, b->region_hap_off + (b->n_regions ? b->n_regions + 1 : 0));
+  p->region_pair_off.assign(static_cast<size_t>(b->n_regions ? b->n_regions + 1 : 0), 0);
   for (int64_t g = 0; g < b->n_regions; ++g) {
+    p->region_pair_off[g] = pair_base;
     const int64_t r0 = b->region_read_off[g], r1 = b->region_read_off[g + 1];
     const int64_t h0 = b->region_hap_off[g], h1 = b->region_hap_off[g + 1];
     const int64_t nh = h1 - h0;
@@ -810,6 +834,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     }
     pair_base += (r1 - r0) * nh;
   }
+  if (b->n_regions) p->region_pair_off[b->n_regions] = pair_base;
   p->n_pairs = pair_base;
   p->cells = cells;
   if (tasks.size() >= (1u << 31) || tasks64.size() >= (1u << 31))
@@ -1039,6 +1064,101 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->last_ms_band = p->last_msband;
   s->last_ms_long = p->last_mslong;
   s->long_band_pairs = long_banded ? cnt[21] : 0;
+  s->last_ms_post = p->last_mspost;
+  return BV_OK;
+}
+
+int bv_plan_postprocess(bv_plan* p, int32_t flags, const int32_t* hap_allele, const int32_t* n_alleles,
+                        uint8_t* keep_out, double* normalized_out, double* allele_out) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  if ((flags & ~BV_POST_NORMALIZE) != 0) return fail(BV_EINVAL, "unknown post-processing flags");
+  if ((hap_allele == nullptr) != (n_alleles == nullptr))
+    return fail(BV_EINVAL, "hap_allele and n_alleles are given together");
+  if (allele_out && !hap_allele) return fail(BV_EINVAL, "allele_out needs hap_allele and n_alleles");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  const int64_t G = p->n_regions;
+  if (G == 0 || p->n_pairs == 0) return BV_OK;
+  // the haplotype -> allele map must name an allele of the haplotype's region
+  // (genotyper.hpp:200-209 maps every haplotype; unmapped ones go to allele 0)
+  std::vector<int64_t> aoff;
+  if (hap_allele) {
+    aoff.assign(static_cast<size_t>(G + 1), 0);
+    for (int64_t g = 0; g < G; ++g) {
+      const int32_t na = n_alleles[g];
+      if (na < 0) return fail(BV_EINVAL, "region " + std::to_string(g) + " has a negative allele count");
+      for (int64_t h = p->region_hap_off[g]; h < p->region_hap_off[g + 1]; ++h)
+        if (hap_allele[h] < 0 || hap_allele[h] >= na)
+          return fail(BV_EINVAL, "haplotype " + std::to_string(h) + " maps to allele " +
+                                     std::to_string(hap_allele[h]) + ", region " + std::to_string(g) +
+                                     " has " + std::to_string(na));
+      aoff[g + 1] = aoff[g] + (p->region_read_off[g + 1] - p->region_read_off[g]) * na;
+    }
+  }
+  hipStream_t st = p->stream;
+  if (!p->d_region_tab) {
+    const size_t n1 = static_cast<size_t>(G + 1);
+    std::vector<int64_t> tab(3 * n1);
+    std::copy(p->region_read_off.begin(), p->region_read_off.end(), tab.begin());
+    std::copy(p->region_hap_off.begin(), p->region_hap_off.end(), tab.begin() + n1);
+    std::copy(p->region_pair_off.begin(), p->region_pair_off.end(), tab.begin() + 2 * n1);
+    if (int rc = dev_upload(&p->d_region_tab, tab.data(), static_cast<int64_t>(tab.size()), st)) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  if (!p->ev_post[0]) {
+    HIP_TRY(hipEventCreate(&p->ev_post[0]));
+    HIP_TRY(hipEventCreate(&p->ev_post[1]));
+  }
+  if (keep_out && !p->d_keep) HIP_TRY(hipMalloc(&p->d_keep, std::max<int64_t>(1, p->n_reads)));
+  if (normalized_out && !p->d_lnorm) HIP_TRY(hipMalloc(&p->d_lnorm, sizeof(double) * p->n_pairs));
+  if (hap_allele) {
+    if (!p->d_hap_allele) {
+      HIP_TRY(hipMalloc(&p->d_hap_allele, sizeof(int32_t) * std::max<int64_t>(1, p->n_haps)));
+      HIP_TRY(hipMalloc(&p->d_n_alleles, sizeof(int32_t) * G));
+      HIP_TRY(hipMalloc(&p->d_allele_off, sizeof(int64_t) * (G + 1)));
+    }
+    if (allele_out && aoff[G] > p->allele_cap) {
+      if (p->d_allele) HIP_TRY(hipFree(p->d_allele));
+      p->d_allele = nullptr;
+      p->allele_cap = 0;
+      HIP_TRY(hipMalloc(&p->d_allele, sizeof(double) * aoff[G]));
+      p->allele_cap = aoff[G];
+    }
+    if (p->n_haps)
+      HIP_TRY(hipMemcpyAsync(p->d_hap_allele, hap_allele, sizeof(int32_t) * p->n_haps, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->d_n_alleles, n_alleles, sizeof(int32_t) * G, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->d_allele_off, aoff.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, st));
+  }
+  PostArgs a{};
+  a.L = p->d_out;
+  a.read0 = p->region_read_off[0];
+  a.read_end = p->region_read_off[G];
+  a.n_regions = G;
+  a.region_read_off = p->d_region_tab;
+  a.region_hap_off = p->d_region_tab + (G + 1);
+  a.region_pair_off = p->d_region_tab + 2 * (G + 1);
+  a.read_off = p->d_read_off;
+  a.normalize = (flags & BV_POST_NORMALIZE) ? 1 : 0;
+  a.keep = keep_out ? p->d_keep : nullptr;
+  a.Lnorm = normalized_out ? p->d_lnorm : nullptr;
+  a.hap_allele = p->d_hap_allele;
+  a.n_alleles = p->d_n_alleles;
+  a.allele_off = p->d_allele_off;
+  a.A = (allele_out && aoff[G] > 0) ? p->d_allele : nullptr;
+  if (keep_out) {
+    // reads outside every region get no decision: 0
+    HIP_TRY(hipMemsetAsync(p->d_keep, 0, std::max<int64_t>(1, p->n_reads), st));
+  }
+  HIP_TRY(hipEventRecord(p->ev_post[0], st));
+  HIP_TRY(launch_post(a, st));
+  HIP_TRY(hipEventRecord(p->ev_post[1], st));
+  if (keep_out && p->n_reads)
+    HIP_TRY(hipMemcpyAsync(keep_out, p->d_keep, p->n_reads, hipMemcpyDeviceToHost, st));
+  if (normalized_out)
+    HIP_TRY(hipMemcpyAsync(normalized_out, p->d_lnorm, sizeof(double) * p->n_pairs, hipMemcpyDeviceToHost, st));
+  if (allele_out && aoff[G] > 0)
+    HIP_TRY(hipMemcpyAsync(allele_out, p->d_allele, sizeof(double) * aoff[G], hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipEventElapsedTime(&p->last_mspost, p->ev_post[0], p->ev_post[1]));
   return BV_OK;
 }
 

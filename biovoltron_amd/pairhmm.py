@@ -64,7 +64,7 @@ class PlanStats(C.Structure):
         ("grid_f64", C.c_int32), ("last_ms_f32", C.c_float), ("last_ms_f64", C.c_float),
         ("band_pairs", C.c_int64), ("fallback_pairs", C.c_int64), ("cells_swept", C.c_int64),
         ("last_ms_fallback", C.c_float), ("last_ms_band", C.c_float),
-        ("long_band_pairs", C.c_int64), ("last_ms_long", C.c_float), ("pad", C.c_float),
+        ("long_band_pairs", C.c_int64), ("last_ms_long", C.c_float), ("last_ms_post", C.c_float),
     ]
 
 
@@ -84,9 +84,11 @@ EXPORTS = (
     "bv_pairhmm_init", "bv_pairhmm_shutdown", "bv_pairhmm_last_error", "bv_pairhmm_device_info",
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
-    "bv_plan_autotune", "bv_plan_set_grids",
+    "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_postprocess",
     "bv_pairhmm_batch",
 )
+
+BV_POST_NORMALIZE = 1
 
 _lib = None
 
@@ -113,6 +115,7 @@ def library() -> C.CDLL:
         lib.bv_plan_stats_get.argtypes = [vp, C.POINTER(PlanStats)]
         lib.bv_plan_autotune.argtypes = [vp, i32, vp]
         lib.bv_plan_set_grids.argtypes = [vp, vp]
+        lib.bv_plan_postprocess.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
         lib.bv_pairhmm_batch.argtypes = [C.POINTER(_RegionBatch), C.POINTER(Options), vp, vp]
@@ -263,6 +266,29 @@ class Plan:
         keeps a grid), clamped to this plan's maxima."""
         g = (C.c_int32 * 4)(*(int(grids.get(k) or 0) for k in ("band", "fp32", "fp64", "long")))
         _check(self._lib.bv_plan_set_grids(self._h, g))
+
+    def postprocess(self, hap_allele=None, n_alleles=None, normalize: bool = True,
+                    keep: bool = True, normalized: bool = False):
+        """Device post-processing of the last run (bv_plan_postprocess):
+        normalize_likelihoods / filter_poorly_modeled_reads per read
+        (pairhmm.hpp:83-114) and, given a haplotype -> allele map (global
+        haplotype index -> allele index in its region) and the allele count
+        per region, Genotyper::marginal_likelihoods (genotyper.hpp:222-241).
+        Returns (keep[n_reads] or None, normalised log10 L[n_pairs] or None,
+        allele likelihoods (region by region, reads x alleles) or None)."""
+        b = self.batch
+        k = np.zeros(b.n_reads, np.uint8) if keep else None
+        nz = np.empty(b.n_pairs, np.float64) if normalized else None
+        A = ha = na = None
+        if hap_allele is not None:
+            ha = np.ascontiguousarray(hap_allele, np.int32)
+            na = np.ascontiguousarray(n_alleles, np.int32)
+            rro = b.arrays["region_read_off"]
+            A = np.empty(int((np.diff(rro) * na).sum()), np.float64)
+        ptr = lambda x: None if x is None else x.ctypes.data  # noqa: E731
+        _check(self._lib.bv_plan_postprocess(self._h, BV_POST_NORMALIZE if normalize else 0,
+                                             ptr(ha), ptr(na), ptr(k), ptr(nz), ptr(A)))
+        return k, nz, A
 
     def synchronize(self) -> None:
         """Waits for the plan's stream (no copies)."""

@@ -9,6 +9,9 @@
 //   PairHMM::normalize_likelihoods / filter_poorly_modeled_reads
 //       pairhmm.hpp:83-114 (reproduced as written: reads whose best
 //       likelihood is *below* the threshold are the ones kept)
+//   HaplotypeCaller's one call per window (haplotypecaller.hpp:90, inside
+//       the window loop of :126-151): compute_likelihoods_batch takes every
+//       window at once -- one device run, normalise and filter on the device
 //
 // Types are duck-typed so the header needs none of the reference's headers:
 // a read provides .seq and .qual (string-like) and, for the AVX model,
@@ -111,9 +114,9 @@ inline void append(std::vector<uint8_t>& dst, std::string_view s, std::size_t n,
              reinterpret_cast<const uint8_t*>(s.data()) + n);
 }
 
+// Appends one region (a window's haplotypes and reads) to a batch.
 template <class Hap, class Read>
-Packed pack_region(const std::vector<Hap>& haps, const std::vector<Read>& reads, bool gaps) {
-  Packed p;
+void append_region(Packed& p, const std::vector<Hap>& haps, const std::vector<Read>& reads, bool gaps) {
   for (const auto& r : reads) {
     const auto seq = view(r.seq);
     const std::size_t n = seq.size();
@@ -136,9 +139,66 @@ Packed pack_region(const std::vector<Hap>& haps, const std::vector<Read>& reads,
                  reinterpret_cast<const uint8_t*>(s.data()) + s.size());
     p.hap_off.push_back(static_cast<int64_t>(p.hap.size()));
   }
-  p.region_read_off.push_back(static_cast<int64_t>(reads.size()));
-  p.region_hap_off.push_back(static_cast<int64_t>(haps.size()));
+  p.region_read_off.push_back(static_cast<int64_t>(p.read_off.size()) - 1);
+  p.region_hap_off.push_back(static_cast<int64_t>(p.hap_off.size()) - 1);
+}
+
+template <class Hap, class Read>
+Packed pack_region(const std::vector<Hap>& haps, const std::vector<Read>& reads, bool gaps) {
+  Packed p;
+  append_region(p, haps, reads, gaps);
   return p;
+}
+
+struct PlanGuard {
+  bv_plan* plan = nullptr;
+  ~PlanGuard() {
+    if (plan) bv_plan_destroy(plan);
+  }
+};
+
+// Every window as one region of one plan: one run, then normalise and
+// filter on the device (bv_plan_postprocess); each window's reads are
+// filtered in place and its kept rows returned, in input order.
+template <class Hap, class Read>
+std::vector<std::vector<std::vector<double>>> run_windows(
+    const std::vector<std::pair<const std::vector<Hap>*, std::vector<Read>*>>& windows,
+    const bv_options& opt) {
+  Packed p;
+  for (const auto& [haps, reads] : windows) {
+    if (!haps || !reads) throw PairHMMError(BV_EINVAL, "a window has a null haplotype or read list");
+    if (!reads->empty() && haps->empty())
+      throw PairHMMError(BV_EINVAL, "a window has reads but no haplotypes");
+    append_region(p, *haps, *reads, opt.mode == BV_MODE_AVX);
+  }
+  std::vector<std::vector<std::vector<double>>> out(windows.size());
+  const auto batch = p.batch();
+  if (batch.n_reads == 0) return out;
+  PlanGuard g;
+  check(bv_plan_create(&batch, &opt, &g.plan));
+  check(bv_plan_run(g.plan));
+  bv_plan_stats st{};
+  check(bv_plan_stats_get(g.plan, &st));
+  std::vector<uint8_t> keep(static_cast<std::size_t>(batch.n_reads));
+  std::vector<double> norm(static_cast<std::size_t>(st.n_pairs));
+  check(bv_plan_postprocess(g.plan, BV_POST_NORMALIZE, nullptr, nullptr, keep.data(), norm.data(), nullptr));
+  std::size_t r_glob = 0, pair = 0;
+  for (std::size_t k = 0; k < windows.size(); ++k) {
+    auto& reads = *windows[k].second;
+    const std::size_t nh = windows[k].first->size();
+    std::vector<Read> kept;
+    for (std::size_t i = 0; i < reads.size(); ++i) {
+      if (keep[r_glob + i]) {
+        const auto row = norm.begin() + static_cast<std::ptrdiff_t>(pair + i * nh);
+        out[k].emplace_back(row, row + static_cast<std::ptrdiff_t>(nh));
+        kept.push_back(std::move(reads[i]));
+      }
+    }
+    pair += reads.size() * nh;
+    r_glob += reads.size();
+    reads.swap(kept);
+  }
+  return out;
 }
 
 }  // namespace detail
@@ -192,6 +252,32 @@ class GpuPairHMM {
     normalize_likelihoods(ll);
     filter_poorly_modeled_reads(reads, ll);
     return ll;
+  }
+
+  // Every window of a HaplotypeCaller run at once: windows[k] = {&haplotypes,
+  // &reads}; returns for each window what compute_likelihoods returns for it
+  // and filters its reads the same way, from one batched device run.
+  template <class Hap, class Read>
+  std::vector<std::vector<std::vector<double>>> compute_likelihoods_batch(
+      const std::vector<std::pair<const std::vector<Hap>*, std::vector<Read>*>>& windows) const {
+    bv_options opt;
+    bv_options_default(&opt);
+    opt.mode = BV_MODE_AVX;
+    opt.fp64_renorm = renorm_ ? 1 : 0;
+    return detail::run_windows(windows, opt);
+  }
+
+  // The same for the scalar model with a transition matrix.
+  template <class Hap, class Read, class Trans>
+  std::vector<std::vector<std::vector<double>>> compute_likelihoods_batch(
+      const std::vector<std::pair<const std::vector<Hap>*, std::vector<Read>*>>& windows,
+      const Trans& trans) const {
+    bv_options opt;
+    bv_options_default(&opt);
+    opt.mode = BV_MODE_SCALAR;
+    opt.fp64_renorm = 0;
+    for (int k = 0; k < 7; ++k) opt.trans[k] = trans[k];
+    return detail::run_windows(windows, opt);
   }
 
  private:

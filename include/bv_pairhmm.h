@@ -120,7 +120,9 @@ typedef struct bv_options {
   int32_t grid_f64;
   float band_tol;      /* banded pairs whose edge-zone touch mass exceeds this
                           fraction of the banded likelihood are re-run on the
-                          full matrix (default 1e-6) */
+                          full matrix (default 5e-7: at 1e-6, one short-read
+                          pair in 4.04e8 with a touch ratio of 5.8e-7 lost
+                          2.4e-3 of its likelihood, |dlog10 L| = 1.05e-3) */
   int32_t band_narrow; /* learned bands may use the 32-diagonal band (two
                           pairs per wavefront; default 1).  A read that
                           aligns partly to an inexact repeat copy 17-40
@@ -153,7 +155,7 @@ typedef struct bv_plan_stats {
   float last_ms_band;     /* banded kernels alone (both widths), last timed run */
   int64_t long_band_pairs; /* long reads accepted on the moving band (also in band_pairs) */
   float last_ms_long;      /* moving-band kernel (long reads), last timed run */
-  float pad;
+  float last_ms_post;      /* post-processing kernel, last bv_plan_postprocess */
 } bv_plan_stats;
 
 /* Uploads a batch to device memory and builds the length-ordered work list. */
@@ -177,6 +179,34 @@ int bv_plan_stats_get(bv_plan* plan, bv_plan_stats* stats);
  * matrix.  No reference counterpart: the reference computes every cell. */
 int bv_plan_set_bands(bv_plan* plan, const int32_t* bands);
 void bv_plan_destroy(bv_plan* plan);
+
+/* Post-processing of the last run's results on the device, so that only
+ * per-read decisions and per-allele likelihoods need to leave it.  Replaces,
+ * per region:
+ *   PairHMM::normalize_likelihoods (pairhmm.hpp:83-93): with
+ *     BV_POST_NORMALIZE every entry below (row best - 4.5) is raised to it;
+ *   PairHMM::filter_poorly_modeled_reads (pairhmm.hpp:95-114): keep_out[r] =
+ *     1 when the row best is below min(2, ceil(0.02 * read length)) * -4,
+ *     the reference's test as written (it keeps the poorly modelled reads);
+ *     reads outside every region get 0.  The caller drops rows with 0;
+ *   Genotyper::marginal_likelihoods (genotyper.hpp:222-241, map of :200-209):
+ *     allele_out[allele_off(g) + r_local * n_alleles[g] + a] = max over the
+ *     region's haplotypes h with hap_allele[h] == a of the (normalised)
+ *     entry, or -DBL_MAX (std::numeric_limits<double>::lowest()) when none
+ *     maps to a; allele_off(g) = sum over earlier regions of reads *
+ *     n_alleles.  Rows are computed for every read; the reference computes
+ *     them for the reads the genotyper keeps, a subset of these rows.
+ * hap_allele (n_haps, indexed by global haplotype) and n_alleles (n_regions)
+ * are given together or both NULL; hap_allele[h] must lie in
+ * [0, n_alleles[g]) for h in region g (the reference maps an unmapped
+ * haplotype to allele 0).  keep_out (n_reads), normalized_out (n_pairs) and
+ * allele_out may each be NULL.  Synchronous; call after bv_plan_run.  No
+ * reference counterpart as one call: the reference runs these loops on the
+ * host over the whole matrix. */
+#define BV_POST_NORMALIZE 1
+int bv_plan_postprocess(bv_plan* plan, int32_t flags, const int32_t* hap_allele,
+                        const int32_t* n_alleles, uint8_t* keep_out, double* normalized_out,
+                        double* allele_out);
 
 /* ---- learned bands ----------------------------------------------------------
  * None of these has a reference counterpart: the reference computes every

@@ -4,6 +4,7 @@
  */
 #include "pairhmm_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <omp.h>
 #include <stdlib.h>
@@ -623,4 +624,36 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
   if (lost) *lost = lost_sum;
   if (!(S > 0.0)) return -INFINITY;
   return (log2(S) + log2scale) * log10(2.0);
+}
+
+/* pairhmm.hpp:27-36 constants; the loops follow pairhmm.hpp:83-114 and
+ * genotyper.hpp:222-241 (std::ranges::max over the row, the cap, the filter
+ * as written, then the per-allele maximum starting from lowest()). */
+void bvo_postprocess_region(int n_reads, int n_haps, const double* L, const int64_t* read_len,
+                            int normalize, const int32_t* hap_allele, int n_alleles,
+                            uint8_t* keep, double* Lnorm, double* A) {
+  for (int r = 0; r < n_reads; ++r) {
+    const double* row = L + (size_t)r * (size_t)n_haps;
+    double best = row[0];
+    for (int h = 1; h < n_haps; ++h)
+      if (best < row[h]) best = row[h];
+    const double cap = best + -4.5;
+    if (keep) {
+      const double e = ceil((double)read_len[r] * 0.02);
+      const double thr = (2.0 < e ? 2.0 : e) * -4.0;
+      keep[r] = best < thr ? 1 : 0;
+    }
+    for (int h = 0; h < n_haps; ++h) {
+      const double v = (normalize && row[h] < cap) ? cap : row[h];
+      if (Lnorm) Lnorm[(size_t)r * (size_t)n_haps + (size_t)h] = v;
+    }
+    if (A) {
+      double* out = A + (size_t)r * (size_t)n_alleles;
+      for (int a = 0; a < n_alleles; ++a) out[a] = -DBL_MAX;
+      for (int h = 0; h < n_haps; ++h) {
+        const double v = (normalize && row[h] < cap) ? cap : row[h];
+        if (v > out[hap_allele[h]]) out[hap_allele[h]] = v;
+      }
+    }
+  }
 }

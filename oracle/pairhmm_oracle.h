@@ -137,6 +137,20 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
                                int irun, const int* sched, double* touch_ratio, double* lost,
                                int* offs);
 
+/* Post-processing of one region's read x haplotype matrix L (n_reads rows of
+ * n_haps, read-major), in the order the reference applies it:
+ *   normalize_likelihoods (pairhmm.hpp:83-93), when normalize != 0: every
+ *     entry below the row's best - 4.5 is raised to it (written to Lnorm);
+ *   filter_poorly_modeled_reads (pairhmm.hpp:95-114): keep[r] = 1 when the
+ *     row's best is below min(2, ceil(read_len[r] * 0.02)) * -4;
+ *   marginal_likelihoods (genotyper.hpp:222-241): A[r * n_alleles + a] = the
+ *     largest (normalised) entry of row r over the haplotypes h with
+ *     hap_allele[h] == a, std::numeric_limits<double>::lowest() if none.
+ * Any output may be NULL. */
+void bvo_postprocess_region(int n_reads, int n_haps, const double* L, const int64_t* read_len,
+                            int normalize, const int32_t* hap_allele, int n_alleles,
+                            uint8_t* keep, double* Lnorm, double* A);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,20 @@ int cpu_tests() {
     threw = e.status == BV_EINVAL;
   }
   EXPECT(threw);
+  // batched windows: a window with reads but no haplotypes is an error
+  // (raised before any device call), windows without reads give no rows
+  std::vector<Hap> no_haps;
+  std::vector<Read> some{make_read("ACGTACGT")}, none;
+  using W = std::pair<const std::vector<Hap>*, std::vector<Read>*>;
+  threw = false;
+  try {
+    hmm.compute_likelihoods_batch(std::vector<W>{{&haps, &none}, {&no_haps, &some}});
+  } catch (const PairHMMError& e) {
+    threw = e.status == BV_EINVAL && std::strstr(e.what(), "no haplotypes") != nullptr;
+  }
+  EXPECT(threw);
+  const auto nothing = hmm.compute_likelihoods_batch(std::vector<W>{{&haps, &none}, {&no_haps, &none}});
+  EXPECT(nothing.size() == 2 && nothing[0].empty() && nothing[1].empty());
   std::printf("cpu tests: %s\n", fails ? "FAILED" : "ok");
   return fails ? 1 : 0;
 }
@@ -92,6 +106,41 @@ int gpu_run() {
   std::printf("kept %zu\n", ll.size());
   const auto sc = hmm.compute_likelihoods(haps, reads, kOriginalDefault);
   std::printf("scalar kept %zu\n", sc.size());
+
+  // HaplotypeCaller batching: windows of different shapes in one device
+  // run equal one compute_likelihoods call per window, reads filtered alike
+  // (both models; rows compared bit for bit).
+  std::vector<std::vector<Hap>> whaps;
+  std::vector<std::vector<Read>> wreads;
+  for (int w = 0; w < 5; ++w) {
+    const std::string b = base.substr(static_cast<std::size_t>(w), 60);
+    std::vector<Hap> hs{{b}, {b.substr(0, 25) + "A" + b.substr(26)}, {b.substr(0, 40) + b.substr(43)}};
+    hs.resize(static_cast<std::size_t>(1 + w % 3));
+    std::vector<Read> rs;
+    for (int i = 0; i < 2 + w; ++i)
+      rs.push_back(make_read(b.substr(static_cast<std::size_t>(i * 3), 30 + 4 * i), static_cast<char>(15 + 5 * i)));
+    rs.push_back(make_read(std::string(20, 'T'), 40));  // poorly modelled: its row is kept (filter as written)
+    whaps.push_back(hs);
+    wreads.push_back(rs);
+  }
+  for (int model = 0; model < 2; ++model) {
+    auto batch_reads = wreads;
+    std::vector<std::pair<const std::vector<Hap>*, std::vector<Read>*>> windows;
+    for (std::size_t w = 0; w < whaps.size(); ++w) windows.emplace_back(&whaps[w], &batch_reads[w]);
+    const auto all = model == 0 ? hmm.compute_likelihoods_batch(windows)
+                                : hmm.compute_likelihoods_batch(windows, kOriginalDefault);
+    bool same = all.size() == whaps.size();
+    std::size_t rows = 0;
+    for (std::size_t w = 0; same && w < whaps.size(); ++w) {
+      auto one_reads = wreads[w];
+      const auto one = model == 0 ? hmm.compute_likelihoods(whaps[w], one_reads)
+                                  : hmm.compute_likelihoods(whaps[w], one_reads, kOriginalDefault);
+      same = one == all[w] && one_reads.size() == batch_reads[w].size();
+      for (std::size_t i = 0; same && i < one_reads.size(); ++i) same = one_reads[i].seq == batch_reads[w][i].seq;
+      rows += one.size();
+    }
+    std::printf("%s batch %s (%zu rows)\n", model == 0 ? "avx" : "scalar", same ? "equal" : "DIFFERENT", rows);
+  }
   return 0;
 }
 

@@ -37,3 +37,5 @@ def test_adapter_gpu_matches_oracle(oracle):
             ref, _ = oracle.log10_reference(rd, h.encode())
             assert abs(mat[i, j] - ref) <= 1e-3
     assert out[3].startswith("kept") and out[4].startswith("scalar kept")
+    # HaplotypeCaller batching: all windows in one run equal one call per window
+    assert out[5].startswith("avx batch equal") and out[6].startswith("scalar batch equal"), out[5:]
