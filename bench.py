@@ -45,24 +45,30 @@ WORKLOADS = {
 }
 
 
-def profiled_traffic(profile: str, kernel: str):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE passes, each scaled by a copy of known size),
-    or None when the summary is absent."""
+def profiled_traffic(profile: str, leg: str, kernels):
+    """HBM bytes per launch of the given kernel roles (summed) in one leg of
+    the bench, from a committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE
+    passes, each scaled by a copy of known size), or None when the summary or
+    a role is absent."""
     path = ROOT / "profiles" / profile
     if not path.exists():
         return None
+    found = {}
     in_table = False
     for line in path.read_text().splitlines():
-        if line.startswith("| kernel | FETCH_SIZE"):
+        if line.startswith("| leg | kernel (role) | FETCH_SIZE"):
             in_table = True
             continue
-        if in_table and line.startswith(f"| `{kernel}"):
+        if in_table and line.startswith("|") and not line.startswith("|---"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            return int((float(cells[3]) + float(cells[4])) * 1e9), path.name
-        if in_table and not line.startswith("|"):
+            name = cells[1].strip("`")
+            if cells[0] == leg and name in kernels:
+                found[name] = (float(cells[4]) + float(cells[5])) * 1e9
+        elif in_table and not line.startswith("|"):
             in_table = False
-    return None
+    if len(found) != len(kernels):
+        return None
+    return int(sum(found.values())), path.name
 
 
 def cpu_baseline(batch, out_gpu, budget_s: float):
@@ -361,8 +367,8 @@ def main():
         "frac_of_unpacked_peak": round(achieved / PEAK_FP32_NOPACK_TFLOPS, 4),
         "flops_per_cell": FLOPS_PER_CELL, "kernel_ms": round(st.last_ms_f32, 3), "traffic": None,
     }
-    tr = profiled_traffic(f"pairhmm_workload{args.workload}.md",
-                          "void bvp::pairhmm_forward_kernel<float>(bvp::KArgs) (full-matrix pass)")
+    tr = profiled_traffic(f"pairhmm_workload{args.workload}.md", "full",
+                          ["full-matrix FP32 pass (short reads)"])
     if tr is not None:
         full_roof["traffic"] = tr[0]
         full_roof["traffic_source"] = f"profiles/{tr[1]} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
@@ -377,7 +383,8 @@ def main():
             "flops_per_cell": BAND_FLOPS_PER_CELL, "kernel_ms": band_info["band_kernel_ms"],
             "traffic": None,
         }
-        tb = profiled_traffic(f"pairhmm_workload{args.workload}.md", "void bvp::band_forward_kernel<64>")
+        tb = profiled_traffic(f"pairhmm_workload{args.workload}.md", "learned",
+                              ["band_forward_kernel<64>", "band_forward_kernel<32>"])
         if tb is not None:
             roof["traffic"] = tb[0]
             roof["traffic_source"] = f"profiles/{tb[1]} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
