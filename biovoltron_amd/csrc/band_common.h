@@ -33,6 +33,7 @@ struct FeatArgs {
   int* anchor;   // [pairs]
   int2* ends;    // [pairs]: first seed start, last seed end (read positions) among
                  // hits within kEndAnchorTol of the anchor; (-1, -1) without hits
+  int* next;     // [8] task claim counters, one per XCD, zero at launch
 };
 
 }  // namespace bvp

@@ -27,6 +27,7 @@ constexpr uint32_t kNoKey = ~0u;
 constexpr int kMaxOcc = 4;    // occurrences of one seed used for the histogram
 constexpr int kHapChunk = 4;  // haplotypes whose lookups are in flight together
 constexpr int kXcds = 8;      // MI355X accelerator complex dies
+constexpr int kFeatClaim = 2; // tasks a features wavefront claims at once
 
 __device__ __forceinline__ int code_of_mask(uint8_t m) {
   // single-class masks 1,2,4,8 -> 0..3; anything else (N, 0) -> -1
@@ -113,18 +114,28 @@ __global__ __launch_bounds__(256) void hap_table_kernel(const uint8_t* __restric
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
   const int lane = threadIdx.x;
   // Workgroups go round-robin to the 8 XCDs, each with its own L2.  Tasks are
-  // in region order; giving each XCD a contiguous eighth of them keeps only a
-  // few regions' haplotype tables live in each L2.
-  int t_begin = 0, t_end = a.n_tasks, t_first = blockIdx.x, t_step = gridDim.x;
+  // in region order; each XCD takes a contiguous eighth of them, and its
+  // wavefronts claim the next kFeatClaim tasks of that eighth from a counter
+  // of their own XCD, so that the tasks in flight on an XCD stay a few
+  // regions wide and those regions' haplotype tables stay in its L2.  (A
+  // fixed stride over the eighth lets fast wavefronts run ahead of slow
+  // ones: the tasks in flight then span dozens of regions and nearly every
+  // table probe missed the L2.)
+  int t_begin = 0, t_end = a.n_tasks, ctr = 0;
   if (gridDim.x % kXcds == 0 && gridDim.x >= kXcds) {
-    const int xcd = blockIdx.x % kXcds;
+    ctr = blockIdx.x % kXcds;
     const int chunk = (a.n_tasks + kXcds - 1) / kXcds;
-    t_begin = xcd * chunk;
+    t_begin = min(a.n_tasks, ctr * chunk);
     t_end = min(a.n_tasks, t_begin + chunk);
-    t_first = t_begin + blockIdx.x / kXcds;
-    t_step = gridDim.x / kXcds;
   }
-  for (int ti = t_first; ti < t_end; ti += t_step) {
+  for (;;) {
+    int c0 = 0;
+    if (lane == 0) c0 = atomicAdd(a.next + ctr, kFeatClaim);
+    c0 = __builtin_amdgcn_readfirstlane(__shfl(c0, 0));
+    const int tc0 = t_begin + c0;
+    if (c0 >= t_end - t_begin) break;
+    const int tc1 = min(t_end, tc0 + kFeatClaim);
+  for (int ti = tc0; ti < tc1; ++ti) {
     const Task tk = a.tasks[ti];
     const int64_t ro = a.read_off[tk.read];
     const int R = static_cast<int>(a.read_off[tk.read + 1] - ro);
@@ -265,6 +276,7 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
       }
       }
     }
+  }
   }
 }
 

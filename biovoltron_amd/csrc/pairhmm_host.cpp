@@ -73,6 +73,7 @@ int fail(int code, const std::string& msg) {
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
 constexpr int kCounters = 32;  // device counter block of a plan (see bv_plan)
+constexpr int kFeatCounters = 24;  // [24..31]: the features kernel's per-XCD task claims
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -291,7 +292,8 @@ struct bv_plan {
   // [11] unused, [12] narrow banded item count, [13] unused, [14]
   // verify count, [15] its queue, [16] FP64-first queue, [17] moving-band
   // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
-  // [21] accepted moving-band pairs, [22..23] FP64 cells (u64)
+  // [21] accepted moving-band pairs, [22..23] FP64 cells (u64), [24..31]
+  // features task claims (one per XCD)
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -434,6 +436,9 @@ hipError_t launch_features(bv_plan* p) {
   f.ends = p->d_ends;
   f.htab = p->d_htab;
   f.htab_off = p->d_htab_off;
+  f.next = p->d_counters + kFeatCounters;
+  // the claim counters are cleared by a kernel (graph-safe, no memset node)
+  if (hipError_t e = launch_zero_ints(f.next, 8, p->stream); e != hipSuccess) return e;
   const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 32));
   return launch_band_features(f, grid, p->stream);
 }
