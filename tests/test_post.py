@@ -175,3 +175,49 @@ def test_device_postprocess_rejects_bad_maps():
     with pytest.raises(pairhmm.PairHMMError, match="flags"):
         pairhmm._check(plan._lib.bv_plan_postprocess(plan._h, 6, None, None, None, None, None))
     plan.close()
+
+
+@pytest.mark.gpu
+def test_device_postprocess_empty_regions(oracle):
+    """Regions without reads (with or without haplotypes) between regions
+    with reads: the binary search over region offsets must skip them, and
+    the allele blocks of read-less regions are empty."""
+    from biovoltron_amd import pairhmm
+
+    rng = np.random.default_rng(3)
+
+    def rd(n, q=30):
+        s = bytes(rng.choice(list(b"ACGT"), n).astype(np.uint8))
+        return dict(seq=s, q=bytes([q]) * n, i=bytes([45]) * n, d=bytes([45]) * n, c=bytes([10]) * n)
+
+    def hap(n):
+        return bytes(rng.choice(list(b"ACGT"), n).astype(np.uint8))
+
+    regions = [([], [hap(40)]), ([rd(30), rd(35)], [hap(60), hap(61), hap(59)]), ([], []),
+               ([rd(20)], [hap(45)]), ([], [hap(50), hap(52)]), ([rd(25), rd(26), rd(60)], [hap(80), hap(81)])]
+    pb = pairhmm.PackedBatch.from_regions(regions)
+    plan = pairhmm.Plan(pb)
+    plan.run()
+    out, _ = plan.results()
+    nh = np.diff(pb.arrays["region_hap_off"])
+    na = np.array([max(1, k) for k in nh], np.int32)
+    ha = np.concatenate([np.arange(k) % max(1, k) for k in nh]).astype(np.int32)
+    keep, Ln, A = plan.postprocess(ha, na, normalized=True)
+    rro = pb.arrays["region_read_off"]
+    read_len = np.diff(pb.arrays["read_off"])
+    p0 = a0 = 0
+    for g, (reads, haps) in enumerate(regions):
+        n, m = len(reads), len(haps)
+        if n == 0:
+            continue
+        r0 = int(rro[g])
+        k_o, Ln_o, A_o = post_oracle(oracle, out[p0:p0 + n * m].reshape(n, m), read_len[r0:r0 + n], True,
+                                     ha[int(pb.arrays["region_hap_off"][g]):int(pb.arrays["region_hap_off"][g + 1])],
+                                     int(na[g]))
+        np.testing.assert_array_equal(keep[r0:r0 + n], k_o)
+        np.testing.assert_array_equal(Ln[p0:p0 + n * m], Ln_o.ravel())
+        np.testing.assert_array_equal(A[a0:a0 + n * int(na[g])], A_o.ravel())
+        p0 += n * m
+        a0 += n * int(na[g])
+    assert p0 == len(out) and a0 == len(A)
+    plan.close()
