@@ -184,6 +184,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--regions", type=int, default=100000)
     ap.add_argument("--workload", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=None,
+                    help="generator seed (default: the workload's own, 20261203 + workload)")
     ap.add_argument("--tune-regions", type=int, default=2000,
                     help="the autotuner times runs of a plan of this many regions of the same "
                          "workload and the full plan takes its grids")
@@ -212,7 +214,7 @@ def main():
     from biovoltron_amd import pairhmm, synth
 
     t0 = time.perf_counter()
-    b = synth.generate(args.workload, count=args.regions)
+    b = synth.generate(args.workload, seed=args.seed, count=args.regions)
     t_gen = time.perf_counter() - t0
     print(f"[bench] generated workload {args.workload}: {b.n_regions} regions, {b.n_pairs} pairs, "
           f"{b.cells:.4e} cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
@@ -222,7 +224,7 @@ def main():
     tuned_full = tuned = None
     if args.autotune:
         from biovoltron_amd import band
-        bt = synth.generate(args.workload, count=min(args.tune_regions, args.regions))
+        bt = synth.generate(args.workload, seed=args.seed, count=min(args.tune_regions, args.regions))
         with pairhmm.Plan(pairhmm.PackedBatch.from_synth(bt)) as tp:
             tuned_full = tp.autotune(1)
             if use_band:
@@ -354,7 +356,7 @@ def main():
             "learned_band": band_info,
             "postprocess": post_info,
             "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,
-            "seed": synth.workload_seed(args.workload),
+            "seed": args.seed if args.seed is not None else synth.workload_seed(args.workload),
             "fp64_pairs": st.fp64_pairs, "fp64_cells": st.fp64_cells,
             "device": info.arch.decode(), "compute_units": info.compute_units,
             "grid_fp32": st.grid_f32, "grid_fp64": st.grid_f64,
