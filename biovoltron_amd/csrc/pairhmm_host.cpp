@@ -315,6 +315,12 @@ struct bv_plan {
   int64_t allele_cap = 0;
   hipEvent_t ev_post[2] = {nullptr, nullptr};
   float last_mspost = 0.f;
+  // long reads on the band run on a second stream beside the short-read
+  // passes that follow the banded kernels, with their own carry scratch
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void* d_carry32_long = nullptr;
+  void* d_carry64_long = nullptr;
   // one run captured as a graph (bv_plan_run), rebuilt when the band setup changes
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -333,13 +339,16 @@ void free_plan(bv_plan* p) {
                   p->d_items32, p->d_items_long, p->d_long_full,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off,
                   p->d_region_tab, p->d_hap_allele, p->d_n_alleles, p->d_allele_off, p->d_lnorm,
-                  p->d_keep, p->d_allele};
+                  p->d_keep, p->d_allele, p->d_carry32_long, p->d_carry64_long};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : p->ev_post)
     if (e) (void)hipEventDestroy(e);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -455,6 +464,14 @@ int ensure_band_lists(bv_plan* p) {
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_long_full && hipMalloc(&p->d_long_full, sizeof(Task) * n64) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
+  if (p->n_tasks64 > 0) {
+    if (!p->d_carry32_long &&
+        hipMalloc(&p->d_carry32_long, sizeof(float) * 2 * p->stride32 * p->grid32_max) != hipSuccess)
+      return fail(BV_ENOMEM, "carry scratch allocation failed");
+    if (!p->d_carry64_long &&
+        hipMalloc(&p->d_carry64_long, sizeof(double) * 2 * p->stride64 * p->grid64_max) != hipSuccess)
+      return fail(BV_ENOMEM, "carry scratch allocation failed");
+  }
   return BV_OK;
 }
 
@@ -533,6 +550,47 @@ BArgs band_args(const bv_plan* p) {
   return b;
 }
 
+// Long reads in FP64 first; pairs within reach of FP32's range are queued
+// for FP32 and keep the FP32 result when it does not underflow (the
+// reference's rule, pairhmm_avx.hpp:140-149).  S_f >= 1e-28 means
+// log10 L >= -28 - log10(2^120); the FP64 value screens with a margin.
+// With bands, the moving-band kernel takes the long pairs first and leaves
+// the full-matrix pass those without a band or rejected (and any within
+// FP32's reach, so that the rule above applies to them).
+int enqueue_long(bv_plan* p, bool timed, hipStream_t st, void* carry32, void* carry64) {
+  KArgs a = base_args(p);
+  if (p->has_band) {
+    const BArgs b = band_args(p);
+    if (timed) HIP_TRY(hipEventRecord(p->ev[7], st));
+    HIP_TRY(launch_band_long(b, p->grid_long, st));
+    if (timed) HIP_TRY(hipEventRecord(p->ev[8], st));
+    a.tasks = p->d_long_full;
+    a.n_tasks = 0;
+    a.n_tasks_dev = p->d_counters + 19;
+    a.counter = p->d_counters + 20;
+  } else {
+    a.tasks = p->d_tasks64;
+    a.n_tasks = p->n_tasks64;
+    a.counter = p->d_counters + 16;
+  }
+  a.carry = carry64;
+  a.carry_stride = p->stride64;
+  a.verify = p->d_verify;
+  a.verify_count = p->d_counters + 14;
+  a.verify_above = kVerifyAbove;
+  HIP_TRY(launch_forward<double>(a, p->grid64, st));
+  KArgs v = base_args(p);
+  v.tasks = p->d_verify;
+  v.n_tasks = 0;
+  v.n_tasks_dev = p->d_counters + 14;
+  v.counter = p->d_counters + 15;
+  v.carry = carry32;
+  v.carry_stride = p->stride32;
+  v.verify_pass = 1;
+  HIP_TRY(launch_forward<float>(v, p->grid32, st));
+  return BV_OK;
+}
+
 int enqueue_run(bv_plan* p, bool timed) {
   HIP_TRY(launch_zero_ints(p->d_counters, kCounters, p->stream));
   if (p->n_tasks == 0 && p->n_tasks64 == 0) return BV_OK;
@@ -550,6 +608,12 @@ int enqueue_run(bv_plan* p, bool timed) {
     HIP_TRY(launch_long_anchors(p->d_rclass, p->d_read_off, p->d_htab_off, p->d_htab, p->d_tasks64,
                                 p->n_tasks64, p->d_band, grid_anchor, p->stream));
   }
+  // Long reads with bands: moving band, full-matrix FP64 for the pairs it
+  // leaves, FP32 verify.  They run on the second stream from the end of the
+  // banded kernels (which take every wave slot) beside the short-read
+  // fallback and rescue passes, with their own carry scratch; the run joins
+  // the two streams before its end.
+  const bool long_side = fp32 && p->has_band && p->n_tasks64 > 0 && p->d_carry32_long && p->d_carry64_long;
   if (fp32 && p->has_band) {
     // Pass 1: pairs with an eligible band on the banded kernel; the rest,
     // and every band it rejects, go to the full-matrix list.
@@ -569,6 +633,12 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.carry = p->d_carry32;
     a.carry_stride = p->stride32;
     HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+  }
+  if (long_side) {
+    HIP_TRY(hipEventRecord(p->ev_fork, p->stream));
+    HIP_TRY(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    if (int rc = enqueue_long(p, timed, p->side, p->d_carry32_long, p->d_carry64_long)) return rc;
+    HIP_TRY(hipEventRecord(p->ev_join, p->side));
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
   if (fp32 && p->has_band) {
@@ -602,45 +672,10 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.carry_stride = p->stride64;
     HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
   }
-  if (p->n_tasks64 > 0) {
-    // Long reads in FP64 first; pairs within reach of FP32's range are queued
-    // for FP32 and keep the FP32 result when it does not underflow (the
-    // reference's rule, pairhmm_avx.hpp:140-149).  S_f >= 1e-28 means
-    // log10 L >= -28 - log10(2^120); the FP64 value screens with a margin.
-    // With bands, the moving-band kernel takes the long pairs first and
-    // leaves the full-matrix pass those without a band or rejected (and any
-    // within FP32's reach, so that the rule above applies to them).
-    KArgs a = base_args(p);
-    if (p->has_band) {
-      const BArgs b = band_args(p);
-      if (timed) HIP_TRY(hipEventRecord(p->ev[7], p->stream));
-      HIP_TRY(launch_band_long(b, p->grid_long, p->stream));
-      if (timed) HIP_TRY(hipEventRecord(p->ev[8], p->stream));
-      a.tasks = p->d_long_full;
-      a.n_tasks = 0;
-      a.n_tasks_dev = p->d_counters + 19;
-      a.counter = p->d_counters + 20;
-    } else {
-      a.tasks = p->d_tasks64;
-      a.n_tasks = p->n_tasks64;
-      a.counter = p->d_counters + 16;
-    }
-    a.carry = p->d_carry64;
-    a.carry_stride = p->stride64;
-    a.verify = p->d_verify;
-    a.verify_count = p->d_counters + 14;
-    a.verify_above = kVerifyAbove;
-    HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
-    KArgs v = base_args(p);
-    v.tasks = p->d_verify;
-    v.n_tasks = 0;
-    v.n_tasks_dev = p->d_counters + 14;
-    v.counter = p->d_counters + 15;
-    v.carry = p->d_carry32;
-    v.carry_stride = p->stride32;
-    v.verify_pass = 1;
-    HIP_TRY(launch_forward<float>(v, p->grid32, p->stream));
+  if (p->n_tasks64 > 0 && !long_side) {
+    if (int rc = enqueue_long(p, timed, p->stream, p->d_carry32, p->d_carry64)) return rc;
   }
+  if (long_side) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_join, 0));
   if (timed) HIP_TRY(hipEventRecord(p->ev[2], p->stream));
   return BV_OK;
 }
@@ -743,10 +778,14 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     free_plan(p);
     return code;
   };
-  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(BV_EHIP, "hipStreamCreate failed"));
   for (auto& e : p->ev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BV_EHIP, "hipEventCreate failed"));
+  if (hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess)
+    return cleanup(fail(BV_EHIP, "hipEventCreate failed"));
 
   // Rebase read and hap byte ranges to start at zero.
   const int64_t rb0 = b->n_reads ? b->read_off[0] : 0;

@@ -436,6 +436,34 @@ def test_graph_replays_match_eager_runs(learned):
 
 
 @pytest.mark.gpu
+def test_graph_with_long_reads_on_second_stream():
+    """With long reads and a band model, the long-read passes run on a
+    second stream joined back before the run ends (a fork and join in the
+    captured graph).  Graph replays, with eager runs of another plan in
+    between, equal the eager run; long pairs took the moving band."""
+    ev = synth.generate(4, count=24)
+    pb = pairhmm.PackedBatch.from_synth(ev)
+    R = np.diff(pb.arrays["read_off"])
+    assert (R >= 1024).any() and (R < 1024).any()
+    model = band.BandModel.load(band.DEFAULT_MODEL)
+    with pairhmm.Plan(pb, pairhmm.default_options(use_graph=0)) as eager, \
+            pairhmm.Plan(pb, pairhmm.default_options(use_graph=1)) as graph:
+        band.set_plan_model(eager, model)
+        band.set_plan_model(graph, model)
+        eager.run()
+        ref, ref_form = eager.results()
+        say("long-read pairs on the moving band", int((ref_form == 4).sum()), "of", len(ref))
+        assert (ref_form == 4).sum() > 0
+        for k in range(3):
+            graph.run()
+            out, form = graph.results()
+            assert np.array_equal(out, ref) and np.array_equal(form, ref_form), k
+            eager.run()
+            eager.results()
+        assert graph.stats().long_band_pairs == int((ref_form == 4).sum())
+
+
+@pytest.mark.gpu
 def test_autotune_changes_speed_not_results():
     ev = synth.generate(1, seed=synth.SEED_BASE + 78, count=10)
     pb = pairhmm.PackedBatch.from_synth(ev)
