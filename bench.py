@@ -218,8 +218,10 @@ def main():
     t_gen = time.perf_counter() - t0
     print(f"[bench] generated workload {args.workload}: {b.n_regions} regions, {b.n_pairs} pairs, "
           f"{b.cells:.4e} cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    t_plan0 = time.perf_counter()
     plan = pairhmm.Plan(pairhmm.PackedBatch.from_synth(b))
-    print(f"[bench] plan ready ({time.perf_counter() - t0 - t_gen:.1f} s)", file=sys.stderr, flush=True)
+    t_plan = time.perf_counter() - t_plan0
+    print(f"[bench] plan ready ({t_plan:.1f} s)", file=sys.stderr, flush=True)
     use_band = not args.no_band and Path(args.band_model).exists()
     tuned_full = tuned = None
     if args.autotune:
@@ -278,7 +280,9 @@ def main():
         plan.synchronize()
         t2 = time.perf_counter()
         ms_band = (t2 - t1) * 1e3 / args.steps
+        t_copy0 = time.perf_counter()
         out_b, form_b = plan.results()
+        t_copy = time.perf_counter() - t_copy0
         legs["learned_band"] = (out_b, form_b)
         plan.time(1)
         sb = plan.stats()
@@ -307,6 +311,13 @@ def main():
             long_pairs=int(ran_long.sum()), long_band_pairs=int(sb.long_band_pairs),
             long_band_kernel_ms=round(sb.last_ms_long, 3), long_band_cells=long_cells,
             grids=tuned or dict(band=None, fp32=sb.grid_f32, fp64=sb.grid_f64))
+        # End to end for one evaluation from host buffers: plan set-up (host
+        # work lists, input upload), one run, copy of the results and form
+        # bytes back.  The band model's set-up (haplotype tables) is left out.
+        e2e_s = t_plan + ms_band / 1e3 + t_copy
+        band_info["end_to_end"] = dict(
+            plan_create_s=round(t_plan, 3), run_s=round(ms_band / 1e3, 3), result_copy_s=round(t_copy, 3),
+            gcups=round(b.cells / e2e_s / 1e9, 1))
         print(f"[bench] learned band: {ms_band:.2f} ms/step ({band_info['speedup_vs_full']}x), "
               f"max |dlog10| vs full {err:.2e}, fallback {sb.fallback_pairs}, "
               f"cells swept {band_info['cells_swept_frac']}, band kernels {sb.last_ms_band:.2f} ms, "

@@ -679,6 +679,43 @@ int enqueue_run(bv_plan* p, bool timed) {
   if (timed) HIP_TRY(hipEventRecord(p->ev[2], p->stream));
   return BV_OK;
 }
+// Orders tasks by descending cost, ties in input order (the order a stable
+// sort gives, so the work list is deterministic).  Chunks are sorted on
+// separate threads and merged pairwise, each round's merges in parallel: a
+// single-threaded sort of the 2.4e7 tasks of a 100,000-region plan took
+// seconds of the plan's set-up.
+void longest_first(std::vector<Task>& t, const std::vector<int64_t>& c) {
+  using Key = std::pair<int64_t, int64_t>;  // (-cost, position): all distinct
+  const int64_t n = static_cast<int64_t>(t.size());
+  std::vector<Key> k(static_cast<size_t>(n));
+  parallel_for(n, [&](int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) k[i] = Key{-c[i], i};
+  });
+  const int nt = n < (1 << 16) ? 1 : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<int64_t> bound(static_cast<size_t>(nt) + 1);
+  for (int i = 0; i <= nt; ++i) bound[i] = n * i / nt;
+  auto run = [](std::vector<std::thread>& th) {
+    for (auto& x : th) x.join();
+    th.clear();
+  };
+  std::vector<std::thread> th;
+  for (int i = 0; i < nt; ++i)
+    th.emplace_back([&, i] { std::sort(k.begin() + bound[i], k.begin() + bound[i + 1]); });
+  run(th);
+  for (int w = 1; w < nt; w *= 2) {
+    for (int i = 0; i + w < nt; i += 2 * w) {
+      const int64_t lo = bound[i], mid = bound[i + w], hi = bound[std::min(nt, i + 2 * w)];
+      th.emplace_back([&k, lo, mid, hi] { std::inplace_merge(k.begin() + lo, k.begin() + mid, k.begin() + hi); });
+    }
+    run(th);
+  }
+  std::vector<Task> sorted(static_cast<size_t>(n));
+  parallel_for(n, [&](int64_t a, int64_t b) {
+    for (int64_t i = a; i < b; ++i) sorted[i] = t[k[i].second];
+  });
+  t.swap(sorted);
+}
+
 }  // namespace
 
 extern "C" {
@@ -884,14 +921,6 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   if (tasks.size() >= (1u << 31) || tasks64.size() >= (1u << 31))
     return cleanup(fail(BV_EINVAL, "too many tasks"));
   std::vector<Task> tasks_region = tasks;
-  auto longest_first = [](std::vector<Task>& t, const std::vector<int64_t>& c) {
-    std::vector<int64_t> idx(t.size());
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return c[x] > c[y]; });
-    std::vector<Task> sorted(t.size());
-    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = t[idx[i]];
-    t.swap(sorted);
-  };
   longest_first(tasks, cost);
   longest_first(tasks64, cost64);
   p->n_tasks = static_cast<int>(tasks.size());
