@@ -17,6 +17,7 @@ constexpr float kBandHold = 1e-9f;// a move that would drop more of a row's mass
 constexpr int kEdgeRunRows = 48;  // insertion runs leaving the band bottom: last rows counted
 constexpr int kLongEvery = 16;    // long-read band: rows between moves
 constexpr int kLongMove = kBandEdge;  // long-read band: diagonals per move, at most
+constexpr int kXcdCount = 8;   // MI355X accelerator complex dies (workgroups go round-robin over them)
 constexpr int kBandChunk = 8;  // consecutive units per wavefront assignment (short-read kernel)
 constexpr int kLongStartLift = 16;// long-read start band: the anchor sits this far above lane 0
                                   // less than the centre (room above the alignment)
@@ -64,6 +65,7 @@ struct BArgs {
   uint8_t* form;
   float tol;                       // accept when touch mass <= tol * S
   int* accepted;                   // banded pairs accepted
+  int* claims;                     // [16] chunk claim counters: per XCD, wide then narrow kernel
   unsigned long long* cells_done;  // lane-cells computed
   // long reads (FP64 first, one task per pair): the moving-band list and the
   // pairs left to the full-matrix FP64 pass (no band, or band rejected)

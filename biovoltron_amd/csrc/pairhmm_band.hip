@@ -293,15 +293,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   const int row16 = lane >> 4;
   int accepted = 0;
   unsigned long long cells = 0;
-  // Units are assigned statically, kBandChunk consecutive units (typically
-  // the pairs of one read) at a time, chunk c to wavefront c mod gridDim: the
-  // lists are in task order, longest tasks first, so every wavefront gets the
-  // same mix.  (A shared work counter costs one same-address atomic per unit,
-  // and those serialise at the L2 slower than the units run.)
-  auto next_unit = [&](int u) {
-    return (u + 1) % kBandChunk != 0 ? u + 1 : u + 1 + static_cast<int>(gridDim.x - 1) * kBandChunk;
-  };
-  for (int u = blockIdx.x * kBandChunk; u < n_units; u = next_unit(u)) {
+  // Units are claimed kBandChunk consecutive units (typically the pairs of
+  // one read) at a time.  The wavefronts of XCD x take chunks x, x + 8,
+  // x + 16, ... from a counter of their own, so each claim is one atomic on
+  // one of eight addresses, and a wavefront that starts late (its slot held
+  // by a kernel of the other stream) takes fewer chunks instead of delaying
+  // the kernel's end.  (One shared counter with one claim per unit
+  // serialised at the L2 at ~82 M units/s, slower than the units run.)
+  const bool by_xcd = gridDim.x % kXcdCount == 0;
+  const int xcd = by_xcd ? static_cast<int>(blockIdx.x % kXcdCount) : 0;
+  int* const claim = a.claims + (G == kBandLanes ? 0 : kXcdCount) + xcd;
+  for (;;) {
+    int c = 0;
+    if (lane == 0) c = atomicAdd(claim, 1);
+    c = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+    const int chunk = by_xcd ? xcd + kXcdCount * c : c;
+    if (chunk >= (n_units + kBandChunk - 1) / kBandChunk) break;
+    const int u_end = min(n_units, (chunk + 1) * kBandChunk);
+  for (int u = chunk * kBandChunk; u < u_end; ++u) {
     const BandItem w = items[P * u + grp];
     const bool live = w.hap >= 0;
     const int read = __builtin_amdgcn_readfirstlane(w.read);
@@ -544,6 +553,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
         a.full[q] = Task{read, hap, 1, -2, w.out};
       }
     }
+  }
   }
   // per-lane tallies live in the slot leaders (lanes 0 and 32)
   int acc_w = accepted;

@@ -72,8 +72,9 @@ int fail(int code, const std::string& msg) {
 // log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
-constexpr int kCounters = 32;  // device counter block of a plan (see bv_plan)
+constexpr int kCounters = 48;  // device counter block of a plan (see bv_plan)
 constexpr int kFeatCounters = 24;  // [24..31]: the features kernel's per-XCD task claims
+constexpr int kBandClaims = 32;    // [32..47]: the banded kernels' per-XCD chunk claims
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -293,7 +294,8 @@ struct bv_plan {
   // verify count, [15] its queue, [16] FP64-first queue, [17] moving-band
   // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
   // [21] accepted moving-band pairs, [22..23] FP64 cells (u64), [24..31]
-  // features task claims (one per XCD)
+  // features task claims (one per XCD), [32..47] banded chunk claims (per
+  // XCD, wide then narrow kernel)
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -532,6 +534,7 @@ BArgs band_args(const bv_plan* p) {
   b.form = p->d_form;
   b.tol = p->opt.band_tol;
   b.accepted = p->d_counters + 5;
+  b.claims = p->d_counters + kBandClaims;
   b.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
   b.tasks64 = p->d_tasks64;
   b.n_tasks64 = p->n_tasks64;
@@ -609,11 +612,17 @@ int enqueue_run(bv_plan* p, bool timed) {
                                 p->n_tasks64, p->d_band, grid_anchor, p->stream));
   }
   // Long reads with bands: moving band, full-matrix FP64 for the pairs it
-  // leaves, FP32 verify.  They run on the second stream from the end of the
-  // banded kernels (which take every wave slot) beside the short-read
-  // fallback and rescue passes, with their own carry scratch; the run joins
-  // the two streams before its end.
+  // leaves, FP32 verify.  They run on the second stream, beside every
+  // short-read pass (the banded kernels claim their work per XCD, so waves
+  // that start late take less of it), with their own carry scratch; the run
+  // joins the two streams before its end.
   const bool long_side = fp32 && p->has_band && p->n_tasks64 > 0 && p->d_carry32_long && p->d_carry64_long;
+  if (long_side) {
+    HIP_TRY(hipEventRecord(p->ev_fork, p->stream));
+    HIP_TRY(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    if (int rc = enqueue_long(p, timed, p->side, p->d_carry32_long, p->d_carry64_long)) return rc;
+    HIP_TRY(hipEventRecord(p->ev_join, p->side));
+  }
   if (fp32 && p->has_band) {
     // Pass 1: pairs with an eligible band on the banded kernel; the rest,
     // and every band it rejects, go to the full-matrix list.
@@ -633,12 +642,6 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.carry = p->d_carry32;
     a.carry_stride = p->stride32;
     HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
-  }
-  if (long_side) {
-    HIP_TRY(hipEventRecord(p->ev_fork, p->stream));
-    HIP_TRY(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-    if (int rc = enqueue_long(p, timed, p->side, p->d_carry32_long, p->d_carry64_long)) return rc;
-    HIP_TRY(hipEventRecord(p->ev_join, p->side));
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
   if (fp32 && p->has_band) {
