@@ -46,6 +46,8 @@ template <typename T>
 int max_blocks_per_cu();
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
 hipError_t launch_post(const PostArgs& a, hipStream_t st);
+hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, const uint8_t* triplets,
+                                    uint8_t* ins, uint8_t* del, uint8_t* gcp, hipStream_t st);
 }  // namespace bvp
 
 namespace {
@@ -357,9 +359,10 @@ void free_plan(bv_plan* p) {
   delete p;
 }
 
+// Allocates max(n, capacity, 1) elements and copies n of them from src.
 template <typename T>
-int dev_upload(T** dst, const T* src, int64_t n, hipStream_t st) {
-  const size_t bytes = sizeof(T) * static_cast<size_t>(std::max<int64_t>(n, 1));
+int dev_upload(T** dst, const T* src, int64_t n, hipStream_t st, int64_t capacity = 0) {
+  const size_t bytes = sizeof(T) * static_cast<size_t>(std::max<int64_t>(std::max<int64_t>(n, capacity), 1));
   HIP_TRY(hipMalloc(dst, bytes));
   if (n > 0) HIP_TRY(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
   return BV_OK;
@@ -858,6 +861,25 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       gcp_const[r] = same ? 1 : 0;
     }
   });
+  // Reads whose insertion, deletion and gap-continuation bytes are each
+  // constant along the read (every SamRecord's GOP/GCP strings, sam.hpp:
+  // 451-475): when all reads are, they travel as one triplet per read and a
+  // kernel writes the per-base bytes on the device.
+  std::vector<uint8_t> triplets(static_cast<size_t>(3 * std::max<int64_t>(1, b->n_reads)), 0);
+  std::atomic<bool> all_uniform{true};
+  parallel_for(b->n_reads, [&](int64_t a0, int64_t a1) {
+    bool ok = true;
+    for (int64_t r = a0; r < a1 && ok; ++r) {
+      const int64_t o = b->read_off[r], R = b->read_off[r + 1] - o;
+      const uint8_t *i = b->read_ins + o, *d = b->read_del + o, *c = b->read_gcp + o;
+      for (int64_t k = 1; k < R && ok; ++k) ok = i[k] == i[0] && d[k] == d[0] && c[k] == c[0];
+      triplets[3 * r] = i[0];
+      triplets[3 * r + 1] = d[0];
+      triplets[3 * r + 2] = c[0];
+    }
+    if (!ok) all_uniform.store(false, std::memory_order_relaxed);
+  });
+  const bool gaps_per_read = b->n_reads > 0 && all_uniform.load();
   // The short-read banded kernel's selector window: the longest read it may get.
   int rmax_band = 0;
   for (int64_t r = 0; r < b->n_reads; ++r) {
@@ -950,9 +972,12 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   hipStream_t st = p->stream;
   if ((rc = dev_upload(&p->d_rclass, rclass.data(), rbytes, st)) ||
       (rc = dev_upload(&p->d_q, b->read_qual + rb0, rbytes, st)) ||
-      (rc = dev_upload(&p->d_ins, b->read_ins + rb0, rbytes, st)) ||
-      (rc = dev_upload(&p->d_del, b->read_del + rb0, rbytes, st)) ||
-      (rc = dev_upload(&p->d_gcp, b->read_gcp + rb0, rbytes, st)) ||
+      (rc = dev_upload(&p->d_ins, gaps_per_read ? nullptr : b->read_ins + rb0, gaps_per_read ? 0 : rbytes, st,
+                       rbytes)) ||
+      (rc = dev_upload(&p->d_del, gaps_per_read ? nullptr : b->read_del + rb0, gaps_per_read ? 0 : rbytes, st,
+                       rbytes)) ||
+      (rc = dev_upload(&p->d_gcp, gaps_per_read ? nullptr : b->read_gcp + rb0, gaps_per_read ? 0 : rbytes, st,
+                       rbytes)) ||
       (rc = dev_upload(&p->d_gcp_const, gcp_const.data(), b->n_reads, st)) ||
       (rc = dev_upload(&p->d_read_off, roff.data(), b->n_reads + 1, st)) ||
       (rc = dev_upload(&p->d_hmask, hmask.data(), hbytes, st)) ||
@@ -970,6 +995,15 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       hipMalloc(&p->d_carry32, sizeof(float) * 2 * p->stride32 * p->grid32) != hipSuccess ||
       hipMalloc(&p->d_carry64, sizeof(double) * 2 * p->stride64 * p->grid64) != hipSuccess)
     return cleanup(fail(BV_ENOMEM, "device allocation failed"));
+  if (gaps_per_read) {
+    uint8_t* d_trip = nullptr;
+    if ((rc = dev_upload(&d_trip, triplets.data(), 3 * b->n_reads, st))) return cleanup(rc);
+    const hipError_t e = launch_expand_read_bytes(p->d_read_off, b->n_reads, d_trip, p->d_ins, p->d_del,
+                                                  p->d_gcp, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipFree(d_trip);
+    if (e != hipSuccess || es != hipSuccess) return cleanup(fail(BV_EHIP, "read byte expansion failed"));
+  }
   // The 64-bit rescue cell counter shares the zeroed counter block.
   p->d_rescue_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 6);
   if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(BV_EHIP, "upload failed"));

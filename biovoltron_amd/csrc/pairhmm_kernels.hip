@@ -29,6 +29,8 @@
 // one fused multiply-add: prior = sel * (pm - pmis) + pmis.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pairhmm_common.h"
 
 namespace bvp {
@@ -382,6 +384,27 @@ __global__ void zero_ints_kernel(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// Per-base insertion / deletion / gap-continuation bytes from one triplet per
+// read (reads whose three quality strings are each constant along the read,
+// as SamRecord's GOP/GCP strings are, sam.hpp:451-475): the host uploads 3
+// bytes per read instead of 3 per base.  One workgroup per read at a time.
+__global__ __launch_bounds__(64) void expand_read_bytes_kernel(const int64_t* __restrict__ read_off,
+                                                                int64_t n_reads,
+                                                                const uint8_t* __restrict__ triplets,
+                                                                uint8_t* __restrict__ ins,
+                                                                uint8_t* __restrict__ del,
+                                                                uint8_t* __restrict__ gcp) {
+  for (int64_t r = blockIdx.x; r < n_reads; r += gridDim.x) {
+    const int64_t a = read_off[r], b = read_off[r + 1];
+    const uint8_t i = triplets[3 * r], d = triplets[3 * r + 1], c = triplets[3 * r + 2];
+    for (int64_t p = a + threadIdx.x; p < b; p += blockDim.x) {
+      ins[p] = i;
+      del[p] = d;
+      gcp[p] = c;
+    }
+  }
+}
+
 }  // namespace bvp
 
 // ---- launchers (host) ----------------------------------------------------
@@ -390,6 +413,15 @@ namespace bvp {
 
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st) {
   hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, const uint8_t* triplets,
+                                    uint8_t* ins, uint8_t* del, uint8_t* gcp, hipStream_t st) {
+  if (n_reads <= 0) return hipSuccess;
+  const int64_t grid = std::min<int64_t>(n_reads, 65536);
+  hipLaunchKernelGGL(expand_read_bytes_kernel, dim3(static_cast<unsigned>(grid)), dim3(64), 0, st, read_off,
+                     n_reads, triplets, ins, del, gcp);
   return hipGetLastError();
 }
 
