@@ -124,6 +124,56 @@ def cpu_baseline(batch, out_gpu, budget_s: float):
                        f"{os.cpu_count()} host CPUs visible, {cores} in affinity"), worst
 
 
+def sw_pairs(batch, n_regions: int):
+    """Smith-Waterman pairs of the haplotype caller's shape from the first
+    regions of the workload: every haplotype of a region aligned to the
+    region's first haplotype (standing in for the reference window), as
+    haplotype_graph.hpp:219 aligns each haplotype to the reference."""
+    b = batch
+    pairs = []
+    for g in range(min(n_regions, b.n_regions)):
+        h0, h1 = int(b.region_hap_off[g]), int(b.region_hap_off[g + 1])
+        if h1 - h0 < 2:
+            continue
+        ref = b.hap_bases[int(b.hap_off[h0]):int(b.hap_off[h0 + 1])].tobytes()
+        if len(ref) > 4096:
+            continue
+        for h in range(h0 + 1, h1):
+            alt = b.hap_bases[int(b.hap_off[h]):int(b.hap_off[h + 1])].tobytes()
+            if len(alt) <= 4096:
+                pairs.append((ref, alt))
+    return pairs
+
+
+def sw_cpu_baseline(pairs, device_results, budget_s: float):
+    """SmithWaterman::align built from the reference header (oracle/_ref,
+    one thread) on a bounded prefix of the same pairs; also checks the
+    device results against it pair by pair."""
+    path = ROOT / "oracle" / "_ref" / "libref_sw.so"
+    if not path.exists():
+        return None
+    from biovoltron_amd.sw import SmithWaterman, cigar_string
+    f = C.CDLL(str(path)).ref_sw_align
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+    p = np.array(SmithWaterman.NEW_SW_PARAMETERS, np.int32)
+    cells = done = mismatches = 0
+    t = 0.0
+    for (r, a), (off, el) in zip(pairs, device_results):
+        buf = C.create_string_buffer(4 * (len(r) + len(a)) + 64)
+        o = C.c_int32()
+        t0 = time.perf_counter()
+        f(len(r), r, len(a), a, p.ctypes.data, C.byref(o), buf, len(buf))
+        t += time.perf_counter() - t0
+        mismatches += (o.value, buf.value.decode()) != (off, cigar_string(el))
+        cells += len(r) * len(a)
+        done += 1
+        if t >= budget_s:
+            break
+    return dict(value=round(cells / t / 1e9, 4), unit="GCUPS", cores=1, kind="reference",
+                sample=f"{done} of {len(pairs)} pairs, {cells:.3e} cells, {t:.1f} s"), done, mismatches
+
+
 FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_rejected_full_fp32", 4: "band_fp64_long"}
 
 
@@ -191,6 +241,8 @@ def main():
                          "workload and the full plan takes its grids")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU legs (reference timing and the FP64-oracle error per form)")
+    ap.add_argument("--sw-regions", type=int, default=10000,
+                    help="regions whose haplotypes the Smith-Waterman line aligns")
     ap.add_argument("--no-oracle", action="store_true",
                     help="skip the FP64-oracle error per form")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -339,6 +391,22 @@ def main():
         allele_entries=int(A.size), hbm_bytes=int(post_bytes),
         hbm_gbps=round(post_bytes / (post_ms / 1e3) / 1e9, 1) if post_ms > 0 else None,
         d2h_bytes=int(b.n_reads + A.size * 8), d2h_bytes_full_matrix=int(b.n_pairs * 9))
+    # Smith-Waterman (haplotype to reference window) on pairs of the same
+    # workload's first regions
+    from biovoltron_amd import sw
+    swp = sw_pairs(b, args.sw_regions)
+    sw_info = None
+    if swp:
+        timing = {}
+        t_sw = time.perf_counter()
+        sw_res = sw.align_batch(swp, timing=timing)
+        t_sw = time.perf_counter() - t_sw
+        sw_cells = sum(len(r) * len(a) for r, a in swp)
+        sw_info = dict(pairs=len(swp), cells=sw_cells, kernel_ms=round(timing["kernel_ms"], 3),
+                       gcups=round(sw_cells / (timing["kernel_ms"] / 1e3) / 1e9, 2), call_s=round(t_sw, 3),
+                       params="NEW_SW_PARAMETERS (200, -150, -260, -11)")
+        print(f"[bench] smith-waterman: {len(swp)} pairs, {timing['kernel_ms']:.2f} ms kernel, "
+              f"{sw_info['gcups']} GCUPS", file=sys.stderr, flush=True)
     print(f"[bench] post-processing: {post_ms:.3f} ms, {post_info['hbm_gbps']} GB/s, "
           f"copy-back {post_info['d2h_bytes'] / 1e9:.2f} GB vs {post_info['d2h_bytes_full_matrix'] / 1e9:.2f} GB",
           file=sys.stderr, flush=True)
@@ -366,6 +434,7 @@ def main():
                             "grids": tuned_full or dict(fp32=st.grid_f32, fp64=st.grid_f64)},
             "learned_band": band_info,
             "postprocess": post_info,
+            "smith_waterman": sw_info,
             "regions": b.n_regions, "pairs": b.n_pairs, "cells": b.cells,
             "seed": args.seed if args.seed is not None else synth.workload_seed(args.workload),
             "fp64_pairs": st.fp64_pairs, "fp64_cells": st.fp64_cells,
@@ -416,6 +485,11 @@ def main():
         cb, worst = cpu_baseline(b, out, args.cpu_seconds)
         line["cpu_baseline"] = cb
         line["config"]["max_abs_dlog10_vs_cpu_sample"] = worst
+        if sw_info is not None:
+            r = sw_cpu_baseline(swp, sw_res, 3.0)
+            if r is not None:
+                sw_info["cpu_baseline"], n_chk, n_bad = r
+                sw_info["checked_vs_reference"] = dict(pairs=n_chk, different=n_bad)
         if not args.no_oracle:
             rep, secs = oracle_form_errors(b, legs, rr, hh)
             line["config"]["vs_fp64_oracle"] = dict(

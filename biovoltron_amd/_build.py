@@ -19,7 +19,7 @@ HIP_LIB = PKG / "libbvpairhmm.so"
 SYNTH_LIB = PKG / "libbv_synth.so"
 
 HIP_SOURCES = ["pairhmm_kernels.hip", "pairhmm_band.hip", "band_kernels.hip", "post_kernels.hip",
-               "pairhmm_host.cpp"]
+               "sw_kernels.hip", "pairhmm_host.cpp"]
 
 # Packed FP32 ops are disabled for all device code ("platform
 # constraints"); hipcc's host pass warns that it does not know the feature.

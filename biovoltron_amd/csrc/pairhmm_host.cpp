@@ -16,6 +16,7 @@
 #include "bv_pairhmm.h"
 #include "pairhmm_band.h"
 #include "pairhmm_common.h"
+#include "sw_common.h"
 
 namespace bvp {
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
@@ -1306,6 +1307,112 @@ int bv_pairhmm_batch(const bv_region_batch* b, const bv_options* opt, double* lo
   if (!rc) rc = bv_plan_results(p, log10_out, form_out);
   bv_plan_destroy(p);
   return rc;
+}
+
+int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out, int32_t* n_elems_out,
+                char* ops_out, int32_t* lens_out, float* kernel_ms) {
+  if (!b || !params || !offset_out || !n_elems_out || !ops_out || !lens_out)
+    return fail(BV_EINVAL, "NULL argument");
+  if (b->n_pairs < 0) return fail(BV_EINVAL, "negative pair count");
+  if (b->n_pairs == 0) return BV_OK;
+  if (!b->ref_off || !b->ref || !b->alt_off || !b->alt) return fail(BV_EINVAL, "sequence arrays are NULL");
+  for (int k = 0; k < 4; ++k)
+    if (params[k] > kSwMaxWeight || params[k] < -kSwMaxWeight)
+      return fail(BV_EINVAL, "Smith-Waterman weight beyond +-65536");
+  const int64_t n = b->n_pairs;
+  int rmax = 0, hmax = 0;
+  std::vector<int64_t> out_off(static_cast<size_t>(n) + 1, 0);
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t R = b->ref_off[k + 1] - b->ref_off[k], H = b->alt_off[k + 1] - b->alt_off[k];
+    // SmithWaterman::align asserts non-empty sequences (smithwaterman.hpp:236)
+    if (R < 1 || H < 1) return fail(BV_EINVAL, "pair " + std::to_string(k) + " has an empty sequence");
+    if (R > kSwMaxLen || H > kSwMaxLen)
+      return fail(BV_EINVAL, "pair " + std::to_string(k) + " is longer than 4096 bases");
+    rmax = std::max(rmax, static_cast<int>(R));
+    hmax = std::max(hmax, static_cast<int>(H));
+    out_off[k + 1] = out_off[k] + R + H + 3;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init_locked(-1)) return rc;
+  }
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  const int64_t rb0 = b->ref_off[0], ab0 = b->alt_off[0];
+  const int64_t rbytes = b->ref_off[n] - rb0, abytes = b->alt_off[n] - ab0;
+  std::vector<int64_t> roff(static_cast<size_t>(n) + 1), aoff(static_cast<size_t>(n) + 1);
+  for (int64_t k = 0; k <= n; ++k) {
+    roff[k] = b->ref_off[k] - rb0;
+    aoff[k] = b->alt_off[k] - ab0;
+  }
+  // persistent grid: at most 32 wavefronts per CU (lane 0's carry loads and
+  // the trace stores need other wavefronts to hide behind), and at most
+  // 16 GiB of per-wavefront scratch (trace matrix, carry row, last row and
+  // column)
+  const int64_t stride = sw_scratch_bytes(rmax, hmax);
+  const int64_t budget = int64_t(16) << 30;
+  const int grid = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>({n, static_cast<int64_t>(g_ctx.cus) * 32, budget / stride})));
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  int64_t *d_roff = nullptr, *d_aoff = nullptr, *d_out = nullptr;
+  uint8_t *d_ref = nullptr, *d_alt = nullptr;
+  int32_t *d_offset = nullptr, *d_n = nullptr, *d_lens = nullptr;
+  char *d_ops = nullptr, *d_scr = nullptr;
+  int* d_queue = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&](int code) {
+    void* bufs[] = {d_roff, d_aoff, d_out, d_ref, d_alt, d_offset, d_n, d_lens, d_ops, d_scr, d_queue};
+    for (void* x : bufs)
+      if (x) (void)hipFree(x);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return code;
+  };
+  int rc = 0;
+  const int64_t slots = out_off[n];
+  if ((rc = dev_upload(&d_roff, roff.data(), n + 1, st)) || (rc = dev_upload(&d_aoff, aoff.data(), n + 1, st)) ||
+      (rc = dev_upload(&d_out, out_off.data(), n + 1, st)) ||
+      (rc = dev_upload(&d_ref, b->ref + rb0, rbytes, st)) || (rc = dev_upload(&d_alt, b->alt + ab0, abytes, st)))
+    return cleanup(rc);
+  if (hipMalloc(&d_offset, sizeof(int32_t) * n) != hipSuccess || hipMalloc(&d_n, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc(&d_lens, sizeof(int32_t) * slots) != hipSuccess || hipMalloc(&d_ops, slots) != hipSuccess ||
+      hipMalloc(&d_scr, static_cast<size_t>(stride) * grid) != hipSuccess ||
+      hipMalloc(&d_queue, sizeof(int)) != hipSuccess)
+    return cleanup(fail(BV_ENOMEM, "Smith-Waterman device allocation failed"));
+  if (hipMemsetAsync(d_queue, 0, sizeof(int), st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    return cleanup(fail(BV_EHIP, "Smith-Waterman set-up failed"));
+  SwArgs a{};
+  a.n_pairs = n;
+  a.ref_off = d_roff;
+  a.ref = d_ref;
+  a.alt_off = d_aoff;
+  a.alt = d_alt;
+  a.wm = params[0];
+  a.wx = params[1];
+  a.wo = params[2];
+  a.we = params[3];
+  a.queue = d_queue;
+  a.scratch = d_scr;
+  a.scratch_stride = stride;
+  a.out_off = d_out;
+  a.offset = d_offset;
+  a.n_elems = d_n;
+  a.ops = d_ops;
+  a.lens = d_lens;
+  if (hipEventRecord(e0, st) != hipSuccess || launch_sw(a, grid, st) != hipSuccess ||
+      hipEventRecord(e1, st) != hipSuccess)
+    return cleanup(fail(BV_EHIP, "Smith-Waterman launch failed"));
+  if (hipMemcpyAsync(offset_out, d_offset, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(n_elems_out, d_n, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(lens_out, d_lens, sizeof(int32_t) * slots, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(ops_out, d_ops, slots, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return cleanup(fail(BV_EHIP, "Smith-Waterman run failed"));
+  if (kernel_ms && hipEventElapsedTime(kernel_ms, e0, e1) != hipSuccess)
+    return cleanup(fail(BV_EHIP, "Smith-Waterman timing failed"));
+  return cleanup(BV_OK);
 }
 
 int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* out_scaled) {

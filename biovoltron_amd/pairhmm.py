@@ -84,7 +84,7 @@ EXPORTS = (
     "bv_pairhmm_init", "bv_pairhmm_shutdown", "bv_pairhmm_last_error", "bv_pairhmm_device_info",
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
-    "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_postprocess",
+    "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_postprocess", "bv_sw_align",
     "bv_pairhmm_batch",
 )
 

@@ -203,6 +203,30 @@ void bv_plan_destroy(bv_plan* plan);
  * allele_out may each be NULL.  Synchronous; call after bv_plan_run.  No
  * reference counterpart as one call: the reference runs these loops on the
  * host over the whole matrix. */
+/* Smith-Waterman alignment of alternate sequences (haplotypes) to reference
+ * sequences, the integer DP upstream of PairHMM in the haplotype caller.
+ * Replaces SmithWaterman::align(ref, alt, params)
+ * (include/biovoltron/algo/align/inexact_match/smithwaterman.hpp:229-255),
+ * called from haplotype_graph.hpp:219, for many pairs at once; same results
+ * (offset and CIGAR elements, including the reference's zero-length 'M'
+ * element when the walk starts with a gap).  Pair k aligns alt bytes
+ * [alt_off[k], alt_off[k+1]) to ref bytes [ref_off[k], ref_off[k+1]);
+ * params = {w_match, w_mismatch, w_open, w_extend} (|w| <= 65536; the
+ * reference's default is NEW_SW_PARAMETERS = {200, -150, -260, -11}).
+ * Outputs: offset_out[k], n_elems_out[k], and the CIGAR elements of pair k
+ * in ops_out / lens_out from slot sum_{m<k} (R_m + H_m + 3) on.  Sequences
+ * of 1..4096 bases (the reference asserts non-empty ones).  kernel_ms
+ * (optional) receives the kernel time.  Synchronous. */
+typedef struct bv_sw_batch {
+  int64_t n_pairs;
+  const int64_t* ref_off;
+  const uint8_t* ref;
+  const int64_t* alt_off;
+  const uint8_t* alt;
+} bv_sw_batch;
+int bv_sw_align(const bv_sw_batch* batch, const int32_t* params, int32_t* offset_out, int32_t* n_elems_out,
+                char* ops_out, int32_t* lens_out, float* kernel_ms);
+
 #define BV_POST_NORMALIZE 1
 int bv_plan_postprocess(bv_plan* plan, int32_t flags, const int32_t* hap_allele,
                         const int32_t* n_alleles, uint8_t* keep_out, double* normalized_out,

@@ -151,6 +151,14 @@ void bvo_postprocess_region(int n_reads, int n_haps, const double* L, const int6
                             int normalize, const int32_t* hap_allele, int n_alleles,
                             uint8_t* keep, double* Lnorm, double* A);
 
+/* Smith-Waterman of SmithWaterman::align (smithwaterman.hpp:54-258; see
+ * sw_oracle.c): reference bases are rows, alternate bases columns;
+ * params = {w_match, w_mismatch, w_open, w_extend}.  Writes the CIGAR
+ * elements in order to ops/lens (capacity R + H + 3), the alignment offset
+ * to *offset, and returns the element count. */
+int bvo_sw_align(int R, const uint8_t* ref, int H, const uint8_t* alt, const int32_t* params,
+                 int32_t* offset, char* ops, int32_t* lens);
+
 #ifdef __cplusplus
 }
 #endif
