@@ -1320,7 +1320,7 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
     if (params[k] > kSwMaxWeight || params[k] < -kSwMaxWeight)
       return fail(BV_EINVAL, "Smith-Waterman weight beyond +-65536");
   const int64_t n = b->n_pairs;
-  int rmax = 0, hmax = 0;
+  if (n >= (int64_t(1) << 31)) return fail(BV_EINVAL, "more than 2^31 pairs");
   std::vector<int64_t> out_off(static_cast<size_t>(n) + 1, 0);
   for (int64_t k = 0; k < n; ++k) {
     const int64_t R = b->ref_off[k + 1] - b->ref_off[k], H = b->alt_off[k + 1] - b->alt_off[k];
@@ -1328,8 +1328,6 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
     if (R < 1 || H < 1) return fail(BV_EINVAL, "pair " + std::to_string(k) + " has an empty sequence");
     if (R > kSwMaxLen || H > kSwMaxLen)
       return fail(BV_EINVAL, "pair " + std::to_string(k) + " is longer than 4096 bases");
-    rmax = std::max(rmax, static_cast<int>(R));
-    hmax = std::max(hmax, static_cast<int>(H));
     out_off[k + 1] = out_off[k] + R + H + 3;
   }
   {
@@ -1344,24 +1342,52 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
     roff[k] = b->ref_off[k] - rb0;
     aoff[k] = b->alt_off[k] - ab0;
   }
-  // persistent grid: at most 32 wavefronts per CU (lane 0's carry loads and
-  // the trace stores need other wavefronts to hide behind), and at most
-  // 16 GiB of per-wavefront scratch (trace matrix, carry row, last row and
-  // column)
-  const int64_t stride = sw_scratch_bytes(rmax, hmax);
+  // Size classes (longest side up to 512, 1024, 2048, 4096 bases), one
+  // launch each: a launch's persistent grid is at most 32 wavefronts per CU
+  // (lane 0's carry loads and the trace stores need other wavefronts to hide
+  // behind) and at most 16 GiB of per-wavefront scratch (trace matrix, carry
+  // row, last row and column) sized for the class's longest pair.
+  constexpr int kClasses = 4;
+  std::vector<int32_t> order;
+  order.reserve(static_cast<size_t>(n));
+  int64_t cls_begin[kClasses + 1] = {0};
+  int cls_r[kClasses] = {0}, cls_h[kClasses] = {0};
+  for (int c = 0; c < kClasses; ++c) {
+    const int lim = 512 << c;
+    for (int64_t k = 0; k < n; ++k) {
+      const int R = static_cast<int>(b->ref_off[k + 1] - b->ref_off[k]);
+      const int H = static_cast<int>(b->alt_off[k + 1] - b->alt_off[k]);
+      const int m = std::max(R, H);
+      if (m <= lim && (c == 0 || m > (256 << c))) {
+        order.push_back(static_cast<int32_t>(k));
+        cls_r[c] = std::max(cls_r[c], R);
+        cls_h[c] = std::max(cls_h[c], H);
+      }
+    }
+    cls_begin[c + 1] = static_cast<int64_t>(order.size());
+  }
   const int64_t budget = int64_t(16) << 30;
-  const int grid = static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>({n, static_cast<int64_t>(g_ctx.cus) * 32, budget / stride})));
+  int64_t scratch_bytes = 0;
+  int cls_grid[kClasses] = {0};
+  int64_t cls_stride[kClasses] = {0};
+  for (int c = 0; c < kClasses; ++c) {
+    const int64_t nc = cls_begin[c + 1] - cls_begin[c];
+    if (nc == 0) continue;
+    cls_stride[c] = sw_scratch_bytes(cls_r[c], cls_h[c]);
+    cls_grid[c] = static_cast<int>(std::max<int64_t>(
+        1, std::min<int64_t>({nc, static_cast<int64_t>(g_ctx.cus) * 32, budget / cls_stride[c]})));
+    scratch_bytes = std::max(scratch_bytes, cls_stride[c] * cls_grid[c]);
+  }
   hipStream_t st = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   int64_t *d_roff = nullptr, *d_aoff = nullptr, *d_out = nullptr;
   uint8_t *d_ref = nullptr, *d_alt = nullptr;
-  int32_t *d_offset = nullptr, *d_n = nullptr, *d_lens = nullptr;
+  int32_t *d_offset = nullptr, *d_n = nullptr, *d_lens = nullptr, *d_order = nullptr;
   char *d_ops = nullptr, *d_scr = nullptr;
   int* d_queue = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&](int code) {
-    void* bufs[] = {d_roff, d_aoff, d_out, d_ref, d_alt, d_offset, d_n, d_lens, d_ops, d_scr, d_queue};
+    void* bufs[] = {d_roff, d_aoff, d_out, d_ref, d_alt, d_offset, d_n, d_lens, d_ops, d_scr, d_queue, d_order};
     for (void* x : bufs)
       if (x) (void)hipFree(x);
     if (e0) (void)hipEventDestroy(e0);
@@ -1373,14 +1399,15 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
   const int64_t slots = out_off[n];
   if ((rc = dev_upload(&d_roff, roff.data(), n + 1, st)) || (rc = dev_upload(&d_aoff, aoff.data(), n + 1, st)) ||
       (rc = dev_upload(&d_out, out_off.data(), n + 1, st)) ||
-      (rc = dev_upload(&d_ref, b->ref + rb0, rbytes, st)) || (rc = dev_upload(&d_alt, b->alt + ab0, abytes, st)))
+      (rc = dev_upload(&d_ref, b->ref + rb0, rbytes, st)) || (rc = dev_upload(&d_alt, b->alt + ab0, abytes, st)) ||
+      (rc = dev_upload(&d_order, order.data(), n, st)))
     return cleanup(rc);
   if (hipMalloc(&d_offset, sizeof(int32_t) * n) != hipSuccess || hipMalloc(&d_n, sizeof(int32_t) * n) != hipSuccess ||
       hipMalloc(&d_lens, sizeof(int32_t) * slots) != hipSuccess || hipMalloc(&d_ops, slots) != hipSuccess ||
-      hipMalloc(&d_scr, static_cast<size_t>(stride) * grid) != hipSuccess ||
-      hipMalloc(&d_queue, sizeof(int)) != hipSuccess)
+      hipMalloc(&d_scr, static_cast<size_t>(scratch_bytes)) != hipSuccess ||
+      hipMalloc(&d_queue, sizeof(int) * kClasses) != hipSuccess)
     return cleanup(fail(BV_ENOMEM, "Smith-Waterman device allocation failed"));
-  if (hipMemsetAsync(d_queue, 0, sizeof(int), st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+  if (hipMemsetAsync(d_queue, 0, sizeof(int) * kClasses, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess)
     return cleanup(fail(BV_EHIP, "Smith-Waterman set-up failed"));
   SwArgs a{};
@@ -1393,17 +1420,24 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
   a.wx = params[1];
   a.wo = params[2];
   a.we = params[3];
-  a.queue = d_queue;
   a.scratch = d_scr;
-  a.scratch_stride = stride;
   a.out_off = d_out;
   a.offset = d_offset;
   a.n_elems = d_n;
   a.ops = d_ops;
   a.lens = d_lens;
-  if (hipEventRecord(e0, st) != hipSuccess || launch_sw(a, grid, st) != hipSuccess ||
-      hipEventRecord(e1, st) != hipSuccess)
-    return cleanup(fail(BV_EHIP, "Smith-Waterman launch failed"));
+  if (hipEventRecord(e0, st) != hipSuccess) return cleanup(fail(BV_EHIP, "Smith-Waterman launch failed"));
+  for (int c = 0; c < kClasses; ++c) {
+    // the classes run one after another on the stream and share the scratch
+    const int64_t nc = cls_begin[c + 1] - cls_begin[c];
+    if (nc == 0) continue;
+    a.n_pairs = nc;
+    a.order = d_order + cls_begin[c];
+    a.queue = d_queue + c;
+    a.scratch_stride = cls_stride[c];
+    if (launch_sw(a, cls_grid[c], st) != hipSuccess) return cleanup(fail(BV_EHIP, "Smith-Waterman launch failed"));
+  }
+  if (hipEventRecord(e1, st) != hipSuccess) return cleanup(fail(BV_EHIP, "Smith-Waterman launch failed"));
   if (hipMemcpyAsync(offset_out, d_offset, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipMemcpyAsync(n_elems_out, d_n, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipMemcpyAsync(lens_out, d_lens, sizeof(int32_t) * slots, hipMemcpyDeviceToHost, st) != hipSuccess ||

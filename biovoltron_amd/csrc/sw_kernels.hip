@@ -16,7 +16,9 @@
 // reference's shortcut (:54-60, :238-239).
 //
 // Layout: one wavefront per pair (persistent grid, pairs from an atomic
-// queue).  The wavefront sweeps 64-row stripes by anti-diagonals: lane l holds
+// queue over the launch's size class: the host launches pairs of up to 512,
+// 1024, 2048 and 4096 bases separately, so that the per-wavefront scratch,
+// sized for the class's longest pair, leaves room for a full grid).  The wavefront sweeps 64-row stripes by anti-diagonals: lane l holds
 // row r0 + l and at step t computes column t - l + 1.  The row above comes from
 // the lane above by a DPP shift of its previous step's (S, down, down length);
 // lane 0 reads the previous stripe's last row from a carry row in the
@@ -41,10 +43,11 @@ __global__ __launch_bounds__(64) void sw_align_kernel(SwArgs a) {
   const int lane = threadIdx.x;
   char* scr = a.scratch + static_cast<int64_t>(blockIdx.x) * a.scratch_stride;
   for (;;) {
-    int pk = 0;
-    if (lane == 0) pk = atomicAdd(a.queue, 1);
-    pk = __builtin_amdgcn_readfirstlane(__shfl(pk, 0));
-    if (pk >= a.n_pairs) break;
+    int q = 0;
+    if (lane == 0) q = atomicAdd(a.queue, 1);
+    q = __builtin_amdgcn_readfirstlane(__shfl(q, 0));
+    if (q >= a.n_pairs) break;
+    const int pk = a.order[q];
     const int64_t ro = a.ref_off[pk], ao = a.alt_off[pk];
     const int R = static_cast<int>(a.ref_off[pk + 1] - ro);
     const int H = static_cast<int>(a.alt_off[pk + 1] - ao);
