@@ -124,11 +124,13 @@ def cpu_baseline(batch, out_gpu, budget_s: float):
                        f"{os.cpu_count()} host CPUs visible, {cores} in affinity"), worst
 
 
-def sw_pairs(batch, n_regions: int):
+def sw_pairs(batch, n_regions: int, max_len: int = 1024):
     """Smith-Waterman pairs of the haplotype caller's shape from the first
     regions of the workload: every haplotype of a region aligned to the
     region's first haplotype (standing in for the reference window), as
-    haplotype_graph.hpp:219 aligns each haplotype to the reference."""
+    haplotype_graph.hpp:219 aligns each haplotype to the reference.  Regions
+    with haplotypes longer than max_len (the 3-12 kb long-read regions) are
+    left out: a caller's active-region haplotypes are a few hundred bases."""
     b = batch
     pairs = []
     for g in range(min(n_regions, b.n_regions)):
@@ -136,11 +138,11 @@ def sw_pairs(batch, n_regions: int):
         if h1 - h0 < 2:
             continue
         ref = b.hap_bases[int(b.hap_off[h0]):int(b.hap_off[h0 + 1])].tobytes()
-        if len(ref) > 4096:
+        if len(ref) > max_len:
             continue
         for h in range(h0 + 1, h1):
             alt = b.hap_bases[int(b.hap_off[h]):int(b.hap_off[h + 1])].tobytes()
-            if len(alt) <= 4096:
+            if len(alt) <= max_len:
                 pairs.append((ref, alt))
     return pairs
 
