@@ -998,7 +998,10 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     return cleanup(fail(BV_ENOMEM, "device allocation failed"));
   if (gaps_per_read) {
     uint8_t* d_trip = nullptr;
-    if ((rc = dev_upload(&d_trip, triplets.data(), 3 * b->n_reads, st))) return cleanup(rc);
+    if ((rc = dev_upload(&d_trip, triplets.data(), 3 * b->n_reads, st))) {
+      if (d_trip) (void)hipFree(d_trip);
+      return cleanup(rc);
+    }
     const hipError_t e = launch_expand_read_bytes(p->d_read_off, b->n_reads, d_trip, p->d_ins, p->d_del,
                                                   p->d_gcp, st);
     const hipError_t es = hipStreamSynchronize(st);
