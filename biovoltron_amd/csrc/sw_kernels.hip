@@ -24,8 +24,9 @@
 // lane 0 reads the previous stripe's last row from a carry row in the
 // wavefront's scratch.  Trace values go to scratch in anti-diagonal-major order
 // (cell (i, j) at (i + j) * (R + 1) + i), so a step's stores are contiguous.
-// Lane 0 then finds the end cell and walks the trace.  Integer work: results
-// are bit-identical to the reference's.
+// Lane 0 then finds the end cell and the wavefront walks the trace, 64 cells
+// of a diagonal per load.  Integer work: results are bit-identical to the
+// reference's.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -173,9 +174,10 @@ __global__ __launch_bounds__(64) void sw_align_kernel(SwArgs a) {
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
+    // end cell (calculate_cigar :141-172), by lane 0, then shared
+    int pi = 0, pj = H, seg = 0;
     if (lane == 0) {
-      // end cell (calculate_cigar :141-172)
-      int best = INT_MIN, pi = 0, pj = H, seg = 0;
+      int best = INT_MIN;
       for (int r = 1; r <= R; ++r)
         if (lastcol[r] >= best) {
           best = lastcol[r];
@@ -191,50 +193,78 @@ __global__ __launch_bounds__(64) void sw_align_kernel(SwArgs a) {
           seg = H - c;
         }
       }
-      // walk (:174-210), elements written in reverse order, then reversed
-      char* ops = a.ops + slot;
-      int32_t* lens = a.lens + slot;
-      int n = 0;
-      if (seg > 0) {
+    }
+    pi = __builtin_amdgcn_readfirstlane(__shfl(pi, 0));
+    pj = __builtin_amdgcn_readfirstlane(__shfl(pj, 0));
+    seg = __builtin_amdgcn_readfirstlane(__shfl(seg, 0));
+    // Walk (:174-210) by the whole wavefront in step: lane k holds the trace
+    // of cell (bi - k, bj - k), 64 cells down the current diagonal loaded in
+    // one round trip, and the walk reads them by v_readlane until a gap
+    // takes it off that diagonal.  Lane 0 writes the elements, in reverse
+    // order, then reverses them.
+    char* ops = a.ops + slot;
+    int32_t* lens = a.lens + slot;
+    int n = 0;
+    if (seg > 0) {
+      if (lane == 0) {
         ops[n] = 'S';
-        lens[n++] = seg;
-        seg = 0;
+        lens[n] = seg;
       }
-      char state = 'M';
-      do {
-        const int tv = trace[static_cast<int64_t>(pi + pj) * (R + 1) + pi];
-        char ns;
-        int step;
-        if (tv > 0) {
-          ns = 'D';
-          step = tv;
-          pi -= step;
-        } else if (tv < 0) {
-          ns = 'I';
-          step = -tv;
-          pj -= step;
-        } else {
-          ns = 'M';
-          step = 1;
-          --pi;
-          --pj;
-        }
-        if (ns == state) {
-          seg += step;
-        } else {
+      ++n;
+      seg = 0;
+    }
+    char state = 'M';
+    int bi = -1, bj = -1, win = 0;
+    do {
+      if (pi - pj != bi - bj || bi - pi < 0 || bi - pi >= 64) {
+        const int ci = pi - lane, cj = pj - lane;
+        win = (ci >= 1 && cj >= 1) ? trace[static_cast<int64_t>(ci + cj) * (R + 1) + ci] : 0;
+        bi = pi;
+        bj = pj;
+      }
+      const int tv = __builtin_amdgcn_readlane(win, bi - pi);
+      char ns;
+      int step;
+      if (tv > 0) {
+        ns = 'D';
+        step = tv;
+        pi -= step;
+      } else if (tv < 0) {
+        ns = 'I';
+        step = -tv;
+        pj -= step;
+      } else {
+        ns = 'M';
+        step = 1;
+        --pi;
+        --pj;
+      }
+      if (ns == state) {
+        seg += step;
+      } else {
+        if (lane == 0) {
           ops[n] = state;
-          lens[n++] = seg;
-          seg = step;
-          state = ns;
+          lens[n] = seg;
         }
-      } while (pi > 0 && pj > 0);
-      ops[n] = state;
-      lens[n++] = seg;
-      a.offset[pk] = pi;
-      if (pj > 0) {
-        ops[n] = 'S';
-        lens[n++] = pj;
+        ++n;
+        seg = step;
+        state = ns;
       }
+    } while (pi > 0 && pj > 0);
+    if (lane == 0) {
+      ops[n] = state;
+      lens[n] = seg;
+    }
+    ++n;
+    if (pj > 0) {
+      if (lane == 0) {
+        ops[n] = 'S';
+        lens[n] = pj;
+      }
+      ++n;
+    }
+    if (lane == 0) {
+      a.offset[pk] = pi;
       for (int k = 0; k < n / 2; ++k) {
         const char o = ops[k];
         ops[k] = ops[n - 1 - k];
