@@ -174,29 +174,39 @@ __global__ __launch_bounds__(64) void sw_align_kernel(SwArgs a) {
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    // end cell (calculate_cigar :141-172), by lane 0, then shared
+    // End cell (calculate_cigar :141-172), in the reference's order: the
+    // last column and then the last row are read 64 entries per load (one
+    // per lane) and scanned in step by every lane through v_readlane.
     int pi = 0, pj = H, seg = 0;
-    if (lane == 0) {
+    {
       int best = INT_MIN;
-      for (int r = 1; r <= R; ++r)
-        if (lastcol[r] >= best) {
-          best = lastcol[r];
-          pi = r;
+      for (int r0 = 1; r0 <= R; r0 += 64) {
+        const int v = r0 + lane <= R ? lastcol[r0 + lane] : 0;
+        const int m = min(64, R - r0 + 1);
+        for (int k = 0; k < m; ++k) {
+          const int x = __builtin_amdgcn_readlane(v, k);
+          if (x >= best) {
+            best = x;
+            pi = r0 + k;
+          }
         }
-      for (int c = 1; c <= H; ++c) {
-        const int s = lastrow[c];
-        const int dn = R > c ? R - c : c - R, dc = pi > pj ? pi - pj : pj - pi;
-        if (s > best || (s == best && dn < dc)) {
-          best = s;
-          pi = R;
-          pj = c;
-          seg = H - c;
+      }
+      for (int c0 = 1; c0 <= H; c0 += 64) {
+        const int v = c0 + lane <= H ? lastrow[c0 + lane] : 0;
+        const int m = min(64, H - c0 + 1);
+        for (int k = 0; k < m; ++k) {
+          const int x = __builtin_amdgcn_readlane(v, k);
+          const int c = c0 + k;
+          const int dn = R > c ? R - c : c - R, dc = pi > pj ? pi - pj : pj - pi;
+          if (x > best || (x == best && dn < dc)) {
+            best = x;
+            pi = R;
+            pj = c;
+            seg = H - c;
+          }
         }
       }
     }
-    pi = __builtin_amdgcn_readfirstlane(__shfl(pi, 0));
-    pj = __builtin_amdgcn_readfirstlane(__shfl(pj, 0));
-    seg = __builtin_amdgcn_readfirstlane(__shfl(seg, 0));
     // Walk (:174-210) by the whole wavefront in step: lane k holds the trace
     // of cell (bi - k, bj - k), 64 cells down the current diagonal loaded in
     // one round trip, and the walk reads them by v_readlane until a gap
