@@ -1191,7 +1191,11 @@ int bv_plan_postprocess(bv_plan* p, int32_t flags, const int32_t* hap_allele, co
   if (allele_out && !hap_allele) return fail(BV_EINVAL, "allele_out needs hap_allele and n_alleles");
   HIP_TRY(hipSetDevice(g_ctx.device));
   const int64_t G = p->n_regions;
-  if (G == 0 || p->n_pairs == 0) return BV_OK;
+  if (G == 0 || p->n_pairs == 0) {
+    // no pair to decide on: every read gets 0, as reads outside every region do
+    if (keep_out && p->n_reads > 0) std::memset(keep_out, 0, static_cast<size_t>(p->n_reads));
+    return BV_OK;
+  }
   // the haplotype -> allele map must name an allele of the haplotype's region
   // (genotyper.hpp:200-209 maps every haplotype; unmapped ones go to allele 0)
   std::vector<int64_t> aoff;

@@ -35,20 +35,6 @@
 
 namespace bvp {
 
-#ifdef BVP_DEBUG_DUMP
-__device__ float bvp_debug[16][8][6];  // [step][lane][M, zn, jn, prior, D, sel]
-#define BVP_DUMP(U)                                                            \
-  if (blockIdx.x == 0 && st == 0 && b == 0 && (U) < 16 && lane < 8) {         \
-    bvp_debug[(U)][lane][0] = static_cast<float>(M);                           \
-    bvp_debug[(U)][lane][1] = static_cast<float>(zn);                          \
-    bvp_debug[(U)][lane][2] = static_cast<float>(jn);                          \
-    bvp_debug[(U)][lane][3] = static_cast<float>(prior);                       \
-    bvp_debug[(U)][lane][4] = static_cast<float>(D);                           \
-    bvp_debug[(U)][lane][5] = static_cast<float>(sel);                         \
-  }
-#else
-#define BVP_DUMP(U)
-#endif
 
 template <typename T>
 struct alignas(2 * sizeof(T)) Pair2 {
@@ -178,7 +164,6 @@ __device__ __forceinline__ void row_params(const KArgs& a, int64_t ro, int R, in
       jn = c.j;                                                                \
     }                                                                          \
     if (lane == L) outp[(U)] = Pair2<T>{zn, jn};                               \
-    BVP_DUMP(U)                                                                \
     ZS = zn;                                                                   \
     J1 = jn;                                                                   \
     M1 = M;                                                                    \
@@ -443,11 +428,6 @@ int max_blocks_per_cu() {
 }
 template int max_blocks_per_cu<float>();
 
-#ifdef BVP_DEBUG_DUMP
-extern "C" int bv_debug_read(float* out) {
-  return hipMemcpyFromSymbol(out, HIP_SYMBOL(bvp_debug), sizeof(bvp_debug)) == hipSuccess ? 0 : -1;
-}
-#endif
 template int max_blocks_per_cu<double>();
 
 }  // namespace bvp

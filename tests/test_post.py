@@ -221,3 +221,24 @@ def test_device_postprocess_empty_regions(oracle):
         a0 += n * int(na[g])
     assert p0 == len(out) and a0 == len(A)
     plan.close()
+
+
+@pytest.mark.gpu
+def test_device_postprocess_reads_without_pairs():
+    """Reads that lie outside every region leave the plan without pairs; the
+    keep flags must still be written (0 for every read), not left as the
+    caller's buffer held them."""
+    from biovoltron_amd import pairhmm
+
+    n = 12
+    seq = np.frombuffer(b"ACGTACGTAC" * 3, np.uint8)
+    pb = pairhmm.PackedBatch([0, 0], [0, 1], [0, 10, 20, 30], seq, np.full(30, 30, np.uint8),
+                             np.full(30, 45, np.uint8), np.full(30, 45, np.uint8), np.full(30, 10, np.uint8),
+                             [0, n], np.frombuffer(b"ACGTACGTACGT", np.uint8))
+    assert pb.n_pairs == 0 and pb.n_reads == 3
+    plan = pairhmm.Plan(pb)
+    plan.run()
+    keep = np.full(pb.n_reads, 7, np.uint8)
+    pairhmm._check(plan._lib.bv_plan_postprocess(plan._h, 1, None, None, keep.ctypes.data, None, None))
+    np.testing.assert_array_equal(keep, np.zeros(3, np.uint8))
+    plan.close()
