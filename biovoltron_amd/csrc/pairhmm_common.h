@@ -94,7 +94,21 @@ struct KArgs {
   double verify_above;
   int verify_pass;  // FP32 pass over those pairs: keep the FP64 result when S < 1e-28
   unsigned long long* fp64_cells;  // FP64 passes: sum of R*H over the pairs they compute
+  // Longest haplotype the carry rows hold; a pair with a longer (or empty)
+  // haplotype or an empty read gets NaN instead of a sweep past its row.
+  int hcap;
+  // Allocation-free per-pair form: when rbases is set, read and haplotype
+  // bytes (AVX alphabet) are turned into classes and match masks as they are
+  // loaded (rclass and hmask are unused), and task i is pair i.
+  const uint8_t* rbases;
+  const uint8_t* hbases;
+  int identity_tasks;
 };
+
+// Scratch of the allocation-free entry point: a 256-byte header (the task
+// queue counter) followed by the per-wavefront carry rows.
+constexpr int64_t kNoallocScratchBytes = int64_t(32) << 20;
+constexpr int64_t kNoallocHeader = 256;
 
 // Post-processing of one run's results (post_kernel): per read, the
 // normalised row, the filter decision and the per-allele maxima.

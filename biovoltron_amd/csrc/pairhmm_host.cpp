@@ -46,6 +46,7 @@ hipError_t launch_forward(const KArgs& a, int grid, hipStream_t st);
 template <typename T>
 int max_blocks_per_cu();
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
+hipError_t noalloc_scratch_address(void** ptr);
 hipError_t launch_post(const PostArgs& a, hipStream_t st);
 hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, const uint8_t* triplets,
                                     uint8_t* ins, uint8_t* del, uint8_t* gcp, hipStream_t st);
@@ -142,6 +143,9 @@ struct DeviceCtx {
   float* d_mm_f = nullptr;
   double* d_mm_d = nullptr;
   double* d_err_scalar = nullptr;
+  // device address of the allocation-free entry point's static scratch on
+  // this device (looked up once at init, with the device current)
+  void* d_noalloc_scratch = nullptr;
 };
 
 std::mutex g_mu;
@@ -173,6 +177,8 @@ int ensure_init_locked(int device) {
   HIP_TRY(hipMemcpy(g_ctx.d_mm_f, h.mm_f, sizeof(h.mm_f), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g_ctx.d_mm_d, h.mm_d, sizeof(h.mm_d), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g_ctx.d_err_scalar, h.err_scalar, sizeof(h.err_scalar), hipMemcpyHostToDevice));
+  HIP_TRY(noalloc_scratch_address(&g_ctx.d_noalloc_scratch));
+  if (!g_ctx.d_noalloc_scratch) return fail(BV_EHIP, "no device address for the allocation-free scratch");
   g_ctx.bpc_f32 = max_blocks_per_cu<float>();
   g_ctx.bpc_f64 = max_blocks_per_cu<double>();
   g_ctx.bpc_long = band_long_blocks_per_cu();
@@ -434,6 +440,7 @@ KArgs base_args(const bv_plan* p) {
   a.rescue_cells = p->d_rescue_cells;
   a.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
   a.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
+  a.hcap = p->hmax;
   return a;
 }
 
@@ -1514,6 +1521,80 @@ int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* o
   return rc;
 }
 
+static_assert(BV_NOALLOC_SCRATCH_BYTES == kNoallocScratchBytes, "header and kernel scratch sizes differ");
+
+int bv_compute_full_prob_noalloc(const bv_device_pairs* pr, int precision, int max_hap_len,
+                                 double* out_scaled, void* stream) {
+  if (!pr || !out_scaled) return fail(BV_EINVAL, "NULL argument");
+  if (precision != 32 && precision != 64) return fail(BV_EINVAL, "precision must be 32 or 64");
+  if (pr->n_pairs < 0 || pr->n_pairs >= (int64_t(1) << 31)) return fail(BV_EINVAL, "pair count out of range");
+  if (pr->n_pairs == 0) return BV_OK;
+  if (!pr->read_off || !pr->read_bases || !pr->read_qual || !pr->read_ins || !pr->read_del ||
+      !pr->read_gcp || !pr->hap_off || !pr->hap_bases)
+    return fail(BV_EINVAL, "device arrays are NULL");
+  if (max_hap_len < 1) return fail(BV_EINVAL, "max_hap_len must be at least 1");
+  struct {
+    int device, cus, bpc_f32, bpc_f64;
+    void* scratch;
+    Tables tab;
+  } ctx{};
+  {
+    // the set-up (tables, the scratch address) allocates, so it is not done here
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx.ready) return fail(BV_ESTATE, "call bv_pairhmm_init before bv_compute_full_prob_noalloc");
+    ctx.device = g_ctx.device;
+    ctx.cus = g_ctx.cus;
+    ctx.bpc_f32 = g_ctx.bpc_f32;
+    ctx.bpc_f64 = g_ctx.bpc_f64;
+    ctx.scratch = g_ctx.d_noalloc_scratch;
+    ctx.tab.ph2pr_f = g_ctx.d_ph2pr_f;
+    ctx.tab.ph2pr_d = g_ctx.d_ph2pr_d;
+    ctx.tab.mm_f = g_ctx.d_mm_f;
+    ctx.tab.mm_d = g_ctx.d_mm_d;
+    ctx.tab.err_scalar = g_ctx.d_err_scalar;
+  }
+  const int64_t entry = precision == 32 ? 2 * sizeof(float) : 2 * sizeof(double);
+  const int64_t stride = static_cast<int64_t>(max_hap_len) + kCarryPad;
+  const int64_t waves = (kNoallocScratchBytes - kNoallocHeader) / (stride * entry);
+  if (waves < 1)
+    return fail(BV_EINVAL, "max_hap_len " + std::to_string(max_hap_len) + " needs a carry row beyond the " +
+                               std::to_string(kNoallocScratchBytes >> 20) + " MiB scratch");
+  const int bpc = precision == 32 ? ctx.bpc_f32 : ctx.bpc_f64;
+  const int grid = static_cast<int>(
+      std::max<int64_t>(1, std::min<int64_t>({pr->n_pairs, static_cast<int64_t>(ctx.cus) * bpc, waves})));
+  HIP_TRY(hipSetDevice(ctx.device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(ctx.scratch);
+  int* counter = reinterpret_cast<int*>(base);
+  // the queue counter keeps the last call's value: cleared by a kernel (no memset node under capture)
+  HIP_TRY(launch_zero_ints(counter, 1, st));
+  KArgs a{};
+  a.q = pr->read_qual;
+  a.ins = pr->read_ins;
+  a.del = pr->read_del;
+  a.gcp = pr->read_gcp;
+  a.read_off = pr->read_off;
+  a.hap_off = pr->hap_off;
+  a.rbases = pr->read_bases;
+  a.hbases = pr->hap_bases;
+  a.identity_tasks = 1;
+  a.n_tasks = static_cast<int>(pr->n_pairs);
+  a.counter = counter;
+  a.out = out_scaled;
+  a.carry = base + kNoallocHeader;
+  a.carry_stride = stride;
+  a.hcap = max_hap_len;
+  a.tab = ctx.tab;
+  a.mode = BV_MODE_AVX;
+  a.renorm = 0;  // the reference's fixed 2^120 / 2^1020 scaling, as bv_compute_full_prob
+  a.out_raw = 1;
+  if (precision == 32) {
+    HIP_TRY(launch_forward<float>(a, grid, st));
+  } else {
+    HIP_TRY(launch_forward<double>(a, grid, st));
+  }
+  return BV_OK;
+}
 
 // ---- band model (MLP) ---------------------------------------------------------
 

@@ -36,6 +36,22 @@
 namespace bvp {
 
 
+// AVX alphabet (pairhmm_common.h:26-44): A0 C1 T2 G3, N a wildcard on either
+// side, every other byte acts as A -- the class byte and haplotype mask the
+// host otherwise builds once per plan.
+__device__ __forceinline__ uint8_t avx_read_class(uint8_t b) {
+  return b == 'C' ? 1 : b == 'T' ? 2 : b == 'G' ? 3 : b == 'N' ? kWildcard : 0;
+}
+__device__ __forceinline__ uint8_t avx_hap_mask(uint8_t b) {
+  return b == 'C' ? 2 : b == 'T' ? 4 : b == 'G' ? 8 : b == 'N' ? 0xFF : 1;
+}
+__device__ __forceinline__ uint8_t read_class(const KArgs& a, int64_t p) {
+  return a.rbases ? avx_read_class(a.rbases[p]) : a.rclass[p];
+}
+__device__ __forceinline__ uint8_t hap_mask(const KArgs& a, int64_t p) {
+  return a.rbases ? avx_hap_mask(a.hbases[p]) : a.hmask[p];
+}
+
 template <typename T>
 struct alignas(2 * sizeof(T)) Pair2 {
   T z, j;
@@ -105,7 +121,7 @@ __device__ __forceinline__ void row_params(const KArgs& a, int64_t ro, int R, in
   mm_n = T(0); im_n = T(0); dm_n = T(0); mx_n = T(0); xx_n = T(0);
   if (lane >= 1 && r >= 1 && r <= R) {
     const int64_t p = ro + r - 1;
-    const uint8_t rc = a.rclass[p];
+    const uint8_t rc = read_class(a, p);
     T e;
     if (a.mode == kModeAvx) {
       e = N::ph2pr(a.tab, a.q[p] & 127);
@@ -220,7 +236,7 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
         pre_z = T(0);
         pre_j = T(0);
       }
-      pre_m = (c >= 1 && c <= H) ? a.hmask[ho + c - 1] : uint8_t(0);
+      pre_m = (c >= 1 && c <= H) ? hap_mask(a, ho + c - 1) : uint8_t(0);
     }
     T Za = T(0), Zb = T(0), J1 = T(0), M1 = T(0), D1 = T(0);
     T maxabs = T(0);
@@ -252,7 +268,7 @@ __device__ double pair_forward(const KArgs& a, Lds<T>& s, Pair2<T>* carry, int l
           pre_z = T(0);
           pre_j = T(0);
         }
-        pre_m = (c >= 1 && c <= H) ? a.hmask[ho + c - 1] : uint8_t(0);
+        pre_m = (c >= 1 && c <= H) ? hap_mask(a, ho + c - 1) : uint8_t(0);
       }
       __syncthreads();
       const int left = steps - kBlock * b;
@@ -315,17 +331,29 @@ __global__ __launch_bounds__(64) void pairhmm_forward_kernel(KArgs a) {
     if (lane == 0) ti = atomicAdd(a.counter, 1);
     ti = __builtin_amdgcn_readfirstlane(__shfl(ti, 0));
     if (ti >= n_tasks) break;
-    const Task tk = a.tasks[ti];
+    const Task tk = a.identity_tasks ? Task{ti, ti, 1, 0, ti} : a.tasks[ti];
     for (int h = 0; h < tk.nhap; ++h) {
       int shift = 0;
       const int64_t o = tk.out0 + h;
+      {
+        const int64_t R = a.read_off[tk.read + 1] - a.read_off[tk.read];
+        const int64_t H = a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h];
+        if (R < 1 || H < 1 || H > a.hcap) {
+          // no sweep past the carry row: the pair gets NaN
+          if (lane == 0) {
+            a.out[o] = __builtin_nan("");
+            if (a.form) a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
+          }
+          continue;
+        }
+      }
       const double S = pair_forward<T>(a, s, carry, lane, tk.read, tk.hap0 + h, shift);
       if (lane == 0) {
         cells_w += static_cast<unsigned long long>(a.read_off[tk.read + 1] - a.read_off[tk.read]) *
                    static_cast<unsigned long long>(a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h]);
         if (a.out_raw) {
           a.out[o] = ldexp(S, -shift);
-          a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
+          if (a.form) a.form[o] = sizeof(T) == 4 ? kFormFP32 : kFormFP64;
         } else if constexpr (sizeof(T) == 4) {
           if (a.verify_pass) {
             // the reference rule for a long read run in FP64 first: its FP32
@@ -392,9 +420,22 @@ __global__ __launch_bounds__(64) void expand_read_bytes_kernel(const int64_t* __
 
 }  // namespace bvp
 
+// Device scratch of bv_compute_full_prob_noalloc, for callers that cannot
+// allocate: the task queue counter and the per-wavefront carry rows
+// (kNoallocScratchBytes, held by each device for the life of the process).
+// Host code never passes this name to a kernel: it is a host-side stand-in.
+// Its device address comes from noalloc_scratch_address, once per device.
+static __device__ __align__(256) unsigned char scratch[bvp::kNoallocScratchBytes];
+
 // ---- launchers (host) ----------------------------------------------------
 
 namespace bvp {
+
+// The device address of the scratch array on the current device; the
+// library's init calls it once per device with that device current.
+hipError_t noalloc_scratch_address(void** ptr) {
+  return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
 
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st) {
   hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, st, p, n);

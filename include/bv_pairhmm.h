@@ -87,6 +87,37 @@ typedef struct bv_testcase {
  * reference kernels return (not a log). */
 int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* out_scaled);
 
+/* Allocation-free form of bv_compute_full_prob.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Pair k is read bytes [read_off[k], read_off[k+1]) of the five read arrays
+ * against haplotype bytes [hap_off[k], hap_off[k+1]); every pointer,
+ * offsets included, is DEVICE memory, and every haplotype must be at most
+ * max_hap_len bases (a pair with a longer or empty haplotype, or an empty
+ * read, gets NaN).  Writes the scaled sum S per pair to out_scaled (device,
+ * n_pairs doubles), as bv_compute_full_prob does, enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) and asynchronous: synchronise the
+ * stream before reading.  Needs bv_pairhmm_init first (BV_ESTATE otherwise).
+ * Calls no device allocator: its scratch (queue counter and carry rows) is a
+ * static device array of BV_NOALLOC_SCRATCH_BYTES per device, held until the
+ * process ends; max_hap_len is rejected (BV_EINVAL) when one carry row does
+ * not fit it.  Every call on a device shares that array, so calls that
+ * overlap on one device race: serialise them (one stream, or events). */
+#define BV_NOALLOC_SCRATCH_BYTES (32u << 20)
+typedef struct bv_device_pairs {
+  int64_t n_pairs;
+  const int64_t* read_off; /* n_pairs + 1 */
+  const uint8_t* read_bases;
+  const uint8_t* read_qual;
+  const uint8_t* read_ins;
+  const uint8_t* read_del;
+  const uint8_t* read_gcp;
+  const int64_t* hap_off; /* n_pairs + 1 */
+  const uint8_t* hap_bases;
+} bv_device_pairs;
+int bv_compute_full_prob_noalloc(const bv_device_pairs* pairs, int precision, int max_hap_len,
+                                 double* out_scaled, void* stream);
+
 /* Regions of reads x haplotypes in packed structure-of-arrays form.  Region g
  * owns reads [region_read_off[g], region_read_off[g+1]) and haplotypes
  * [region_hap_off[g], region_hap_off[g+1]); read r's bytes are
