@@ -109,17 +109,20 @@ int gpu_run() {
 
   // HaplotypeCaller batching: windows of different shapes in one device
   // run equal one compute_likelihoods call per window, reads filtered alike
-  // (both models; rows compared bit for bit).
+  // (both models; rows compared bit for bit).  Each window's inputs and the
+  // batch's kept rows are printed for the oracle check in the test driver.
   std::vector<std::vector<Hap>> whaps;
   std::vector<std::vector<Read>> wreads;
-  for (int w = 0; w < 5; ++w) {
+  for (int w = 0; w < 6; ++w) {
     const std::string b = base.substr(static_cast<std::size_t>(w), 60);
     std::vector<Hap> hs{{b}, {b.substr(0, 25) + "A" + b.substr(26)}, {b.substr(0, 40) + b.substr(43)}};
     hs.resize(static_cast<std::size_t>(1 + w % 3));
     std::vector<Read> rs;
-    for (int i = 0; i < 2 + w; ++i)
-      rs.push_back(make_read(b.substr(static_cast<std::size_t>(i * 3), 30 + 4 * i), static_cast<char>(15 + 5 * i)));
-    rs.push_back(make_read(std::string(20, 'T'), 40));  // poorly modelled: its row is kept (filter as written)
+    if (w != 4) {  // window 4 has haplotypes but no reads
+      for (int i = 0; i < 2 + w; ++i)
+        rs.push_back(make_read(b.substr(static_cast<std::size_t>(i * 3), 30 + 4 * i), static_cast<char>(15 + 5 * i)));
+      rs.push_back(make_read(std::string(20, 'T'), 40));  // poorly modelled: its row is kept (filter as written)
+    }
     whaps.push_back(hs);
     wreads.push_back(rs);
   }
@@ -140,6 +143,16 @@ int gpu_run() {
       rows += one.size();
     }
     std::printf("%s batch %s (%zu rows)\n", model == 0 ? "avx" : "scalar", same ? "equal" : "DIFFERENT", rows);
+    for (std::size_t w = 0; w < whaps.size(); ++w) {
+      std::printf("window %d %zu %zu %zu\n", model, w, whaps[w].size(), wreads[w].size());
+      for (const auto& h : whaps[w]) std::printf("hap %s\n", h.seq.c_str());
+      for (const auto& r : wreads[w]) std::printf("read %s %d\n", r.seq.c_str(), static_cast<int>(r.qual[0]));
+      for (std::size_t i = 0; i < all[w].size(); ++i) {
+        std::printf("row %s", batch_reads[w][i].seq.c_str());
+        for (double v : all[w][i]) std::printf(" %.17g", v);
+        std::printf("\n");
+      }
+    }
   }
   return 0;
 }
