@@ -18,7 +18,7 @@ INCLUDE = ROOT / "include"
 HIP_LIB = PKG / "libbvpairhmm.so"
 SYNTH_LIB = PKG / "libbv_synth.so"
 
-HIP_SOURCES = ["pairhmm_kernels.hip", "pairhmm_band.hip", "band_kernels.hip", "post_kernels.hip",
+HIP_SOURCES = ["pairhmm_kernels.hip", "pairhmm_stack.hip", "pairhmm_band.hip", "band_kernels.hip", "post_kernels.hip",
                "sw_kernels.hip", "pairhmm_host.cpp"]
 
 # Packed FP32 ops are disabled for all device code ("platform

@@ -27,8 +27,28 @@ enum Form : uint8_t {
 };
 
 // Read base classes.  Bits 0-2: match class (slot in the haplotype match
-// window); bit 7: wildcard (the read base matches everything).
+// window); bit 6: the last base of its read; bit 7: wildcard (the read base
+// matches everything).
 constexpr uint8_t kWildcard = 0x80;
+constexpr uint8_t kReadEnd = 0x40;
+
+// One unit of the stacked full-matrix kernel: reads [read0, read0 + nreads)
+// of one region, consecutive in the read arrays, stacked row after row,
+// against haplotypes [hap0, hap0 + nhap).  The result of read r and
+// haplotype h goes to out[out0 + (r - read0) * stride + (h - hap0)].
+struct alignas(16) StackTask {
+  int32_t read0;
+  int32_t nreads;
+  int32_t hap0;
+  int32_t nhap;
+  int64_t out0;
+  int32_t stride;
+  int32_t rows;  // sum of the reads' lengths
+};
+constexpr int kStackEnds = 4;     // read ends a stripe holds besides its last lane
+constexpr int kStackHmax = 1024;  // longest haplotype of the stacked kernel (LDS carry row)
+constexpr int kStackPad = 64;     // carry entries before column 0
+constexpr int kStackRows = 1008;  // a stacked task holds at most this many rows (16 stripes)
 
 constexpr int kLanes = 64;        // wavefront width
 constexpr int kRowsPerStripe = 63;// lane 0 carries the previous stripe's last row
@@ -103,6 +123,10 @@ struct KArgs {
   const uint8_t* rbases;
   const uint8_t* hbases;
   int identity_tasks;
+  // Stacked full-matrix kernel (FP32): its tasks and the length of its LDS
+  // carry row (columns, at least the longest haplotype + 65)
+  const StackTask* stasks;
+  int carry_cols;
 };
 
 // Scratch of the allocation-free entry point: a 256-byte header (the task

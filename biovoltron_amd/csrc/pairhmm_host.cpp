@@ -47,6 +47,8 @@ template <typename T>
 int max_blocks_per_cu();
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
 hipError_t noalloc_scratch_address(void** ptr);
+hipError_t launch_stack(const KArgs& a, int grid, hipStream_t st);
+int stack_blocks_per_cu(int carry_cols);
 hipError_t launch_post(const PostArgs& a, hipStream_t st);
 hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, const uint8_t* triplets,
                                     uint8_t* ins, uint8_t* del, uint8_t* gcp, hipStream_t st);
@@ -274,6 +276,16 @@ struct bv_plan {
   int64_t* d_hap_off = nullptr;
   Task* d_tasks = nullptr;         // longest first (forward kernels)
   Task* d_tasks_region = nullptr;  // region order (seed features: haplotype tables stay in L2)
+  // FP32 full-matrix pass: stacked tasks (runs of reads of a region against
+  // its haplotypes, pairhmm_stack_kernel), longest first, and the tasks of
+  // the reads they leave out (a haplotype longer than kStackHmax), for the
+  // striped kernel
+  StackTask* d_stasks = nullptr;
+  int n_stasks = 0;
+  Task* d_tasks_rest = nullptr;
+  int n_tasks_rest = 0;
+  int grid_stack = 1, grid_stack_max = 1;
+  int stack_cols = 0;  // LDS carry row length of the stacked kernel
   Task* d_tasks64 = nullptr;       // long reads, FP64 first (FP32 mode), one pair each
   Task* d_verify = nullptr;        // their pairs that may not underflow FP32
   int n_tasks64 = 0;
@@ -299,7 +311,7 @@ struct bv_plan {
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
   // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
   // cells (u64), [8..9] swept cells (u64), [10] wide banded item count,
-  // [11] unused, [12] narrow banded item count, [13] unused, [14]
+  // [11] stacked FP32 queue, [12] narrow banded item count, [13] unused, [14]
   // verify count, [15] its queue, [16] FP64-first queue, [17] moving-band
   // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
   // [21] accepted moving-band pairs, [22..23] FP64 cells (u64), [24..31]
@@ -345,6 +357,7 @@ void free_plan(bv_plan* p) {
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_gcp_const, p->d_read_off,
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
+                  p->d_stasks, p->d_tasks_rest,
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items32, p->d_items_long, p->d_long_full,
@@ -643,16 +656,26 @@ int enqueue_run(bv_plan* p, bool timed) {
     HIP_TRY(launch_band_forward(b, p->grid_band, p->stream));
     if (timed) HIP_TRY(hipEventRecord(p->ev[6], p->stream));
   } else if (fp32) {
-    // Pass 1: every pair in FP32 on the full matrix.
+    // Pass 1: every pair in FP32 on the full matrix: stacked reads, and the
+    // reads of regions with haplotypes too long for them on the striped kernel.
     KArgs a = base_args(p);
-    a.tasks = p->d_tasks;
-    a.n_tasks = p->n_tasks;
-    a.counter = p->d_counters + 0;
     a.rescue = p->d_rescue;
     a.rescue_count = p->d_counters + 2;
-    a.carry = p->d_carry32;
-    a.carry_stride = p->stride32;
-    HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+    if (p->n_stasks > 0) {
+      a.stasks = p->d_stasks;
+      a.n_tasks = p->n_stasks;
+      a.counter = p->d_counters + 11;
+      a.carry_cols = p->stack_cols;
+      HIP_TRY(launch_stack(a, p->grid_stack, p->stream));
+    }
+    if (p->n_tasks_rest > 0) {
+      a.tasks = p->d_tasks_rest;
+      a.n_tasks = p->n_tasks_rest;
+      a.counter = p->d_counters + 0;
+      a.carry = p->d_carry32;
+      a.carry_stride = p->stride32;
+      HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+    }
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
   if (fp32 && p->has_band) {
@@ -698,7 +721,8 @@ int enqueue_run(bv_plan* p, bool timed) {
 // separate threads and merged pairwise, each round's merges in parallel: a
 // single-threaded sort of the 2.4e7 tasks of a 100,000-region plan took
 // seconds of the plan's set-up.
-void longest_first(std::vector<Task>& t, const std::vector<int64_t>& c) {
+template <typename TaskT>
+void longest_first(std::vector<TaskT>& t, const std::vector<int64_t>& c) {
   using Key = std::pair<int64_t, int64_t>;  // (-cost, position): all distinct
   const int64_t n = static_cast<int64_t>(t.size());
   std::vector<Key> k(static_cast<size_t>(n));
@@ -723,7 +747,7 @@ void longest_first(std::vector<Task>& t, const std::vector<int64_t>& c) {
     }
     run(th);
   }
-  std::vector<Task> sorted(static_cast<size_t>(n));
+  std::vector<TaskT> sorted(static_cast<size_t>(n));
   parallel_for(n, [&](int64_t a, int64_t b) {
     for (int64_t i = a; i < b; ++i) sorted[i] = t[k[i].second];
   });
@@ -852,6 +876,9 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   parallel_for(rbytes, [&](int64_t a0, int64_t a1) {
     for (int64_t i = a0; i < a1; ++i) rclass[i] = al.read_class[b->read_bases[rb0 + i]];
   });
+  // the last base of every read carries kReadEnd (the stacked kernel's read boundaries)
+  for (int64_t r = 0; r < b->n_reads; ++r)
+    if (roff[r + 1] > roff[r]) rclass[roff[r + 1] - 1] |= kReadEnd;
   parallel_for(hbytes, [&](int64_t a0, int64_t a1) {
     for (int64_t i = a0; i < a1; ++i) hmask[i] = al.hap_mask[b->hap_bases[hb0 + i]];
   });
@@ -909,10 +936,17 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   // the FP64 list instead: they underflow FP32's range at these lengths, and
   // the few that would not are re-run in FP32 afterwards (verify pass).
   const bool fp32_mode = opt.mode == BV_MODE_AVX && !opt.force_fp64;
-  std::vector<Task> tasks, tasks64;
+  std::vector<Task> tasks, tasks64, tasks_rest;
   tasks.reserve(static_cast<size_t>(b->n_reads));
-  std::vector<int64_t> cost, cost64;
+  std::vector<int64_t> cost, cost64, cost_rest, cost_stack;
   cost.reserve(static_cast<size_t>(b->n_reads));
+  // FP32 mode: runs of consecutive reads (shorter than kFp64FirstRows) of a
+  // region whose haplotypes all fit the stacked kernel's LDS carry row become
+  // stacked tasks of at most kStackRows rows.  A run is cut early once it
+  // fills its last 63-row stripe to within 6 rows (lanes left empty at the
+  // task's end are the kernel's only partial stripe).
+  std::vector<StackTask> stasks;
+  int stack_hmax = 0;
   int64_t pair_base = 0, cells = 0;
   p->n_regions = b->n_regions;
   p->region_read_off.assign(b->region_read_off, b->region_read_off + (b->n_regions ? b->n_regions + 1 : 0));
@@ -923,8 +957,39 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     const int64_t r0 = b->region_read_off[g], r1 = b->region_read_off[g + 1];
     const int64_t h0 = b->region_hap_off[g], h1 = b->region_hap_off[g + 1];
     const int64_t nh = h1 - h0;
-    int64_t hsum = 0;
-    for (int64_t h = h0; h < h1; ++h) hsum += hoff[h + 1] - hoff[h];
+    int64_t hsum = 0, hmax_g = 0;
+    for (int64_t h = h0; h < h1; ++h) {
+      hsum += hoff[h + 1] - hoff[h];
+      hmax_g = std::max<int64_t>(hmax_g, hoff[h + 1] - hoff[h]);
+    }
+    const bool stackable = fp32_mode && nh > 0 && hmax_g <= kStackHmax;
+    if (stackable) {
+      stack_hmax = std::max<int>(stack_hmax, static_cast<int>(hmax_g));
+      int64_t run0 = -1, rows = 0;
+      auto flush = [&](int64_t r_end) {
+        if (run0 < 0) return;
+        stasks.push_back(StackTask{static_cast<int32_t>(run0), static_cast<int32_t>(r_end - run0),
+                                   static_cast<int32_t>(h0), static_cast<int32_t>(nh),
+                                   pair_base + (run0 - r0) * nh, static_cast<int32_t>(nh),
+                                   static_cast<int32_t>(rows)});
+        cost_stack.push_back(rows * hsum);
+        run0 = -1;
+        rows = 0;
+      };
+      for (int64_t r = r0; r < r1; ++r) {
+        const int64_t R = roff[r + 1] - roff[r];
+        if (R >= kFp64FirstRows) {
+          flush(r);
+          continue;
+        }
+        if (run0 >= 0 && rows + R > kStackRows) flush(r);
+        if (run0 < 0) run0 = r;
+        rows += R;
+        if (rows >= 8 * kRowsPerStripe && (kRowsPerStripe - rows % kRowsPerStripe) % kRowsPerStripe <= 6)
+          flush(r + 1);
+      }
+      flush(r1);
+    }
     for (int64_t r = r0; r < r1; ++r) {
       if (nh == 0) continue;
       const int64_t R = roff[r + 1] - roff[r];
@@ -939,11 +1004,19 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
         for (int64_t h = h0; h < h1; ++h) {
           tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
           cost.push_back(R * (hoff[h + 1] - hoff[h]));
+          if (fp32_mode && !stackable) {
+            tasks_rest.push_back(tasks.back());
+            cost_rest.push_back(cost.back());
+          }
         }
       } else {
         tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
                              static_cast<int32_t>(nh), 0, out0});
         cost.push_back(R * hsum);
+        if (fp32_mode && !stackable) {
+          tasks_rest.push_back(tasks.back());
+          cost_rest.push_back(cost.back());
+        }
       }
     }
     pair_base += (r1 - r0) * nh;
@@ -956,6 +1029,11 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   std::vector<Task> tasks_region = tasks;
   longest_first(tasks, cost);
   longest_first(tasks64, cost64);
+  longest_first(tasks_rest, cost_rest);
+  longest_first(stasks, cost_stack);
+  p->n_stasks = static_cast<int>(stasks.size());
+  p->n_tasks_rest = static_cast<int>(tasks_rest.size());
+  p->stack_cols = stack_hmax + 128;  // lane 0 reads up to column H + 126
   p->n_tasks = static_cast<int>(tasks.size());
   p->n_tasks64 = static_cast<int>(tasks64.size());
 
@@ -974,6 +1052,10 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid_long = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_long, p->n_tasks64)));
   p->grid_long_max = p->grid_long;
+  p->grid_stack = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * stack_blocks_per_cu(p->stack_cols),
+                           std::max(1, p->n_stasks))));
+  p->grid_stack_max = p->grid_stack;
   p->stride32 = hmax + kCarryPad;
   p->stride64 = hmax + kCarryPad;
 
@@ -992,6 +1074,8 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st)) ||
       (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)) ||
       (rc = dev_upload(&p->d_tasks_region, tasks_region.data(), static_cast<int64_t>(tasks_region.size()), st)) ||
+      (rc = dev_upload(&p->d_stasks, stasks.data(), static_cast<int64_t>(stasks.size()), st)) ||
+      (rc = dev_upload(&p->d_tasks_rest, tasks_rest.data(), static_cast<int64_t>(tasks_rest.size()), st)) ||
       (rc = dev_upload(&p->d_tasks64, tasks64.data(), static_cast<int64_t>(tasks64.size()), st)) ||
       (rc = dev_upload(&p->d_verify, tasks64.data(), static_cast<int64_t>(tasks64.size()), st)))
     return cleanup(rc);
