@@ -45,7 +45,7 @@ struct alignas(16) StackTask {
   int32_t stride;
   int32_t rows;  // sum of the reads' lengths
 };
-constexpr int kStackEnds = 4;     // read ends a stripe holds besides its last lane
+constexpr int kStackEnds = 2;     // read ends a stripe holds besides its last lane
 constexpr int kStackHmax = 1024;  // longest haplotype of the stacked kernel (LDS carry row)
 constexpr int kStackPad = 64;     // carry entries before column 0
 constexpr int kStackRows = 1008;  // a stacked task holds at most this many rows (16 stripes)

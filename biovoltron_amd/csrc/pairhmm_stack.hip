@@ -56,18 +56,28 @@ namespace {
 
 constexpr float kK32 = 0x1p120f;                                   // Context.h:109 (FP32)
 constexpr double kLog10K32 = 120.0 * 0.30102999566398119521;       // log10(2^120)
-constexpr float kSelOutside = -1e30f;  // selector outside [1, H]: the clamped prior is 0
-constexpr int kStackSlots = 4;         // AVX read classes A C T G (N is a wildcard row)
+constexpr unsigned short kSelOutside = 0xFBFF;  // f16 -65504 outside [1, H]: the clamped prior is 0
+constexpr unsigned short kSelMatch = 0x3C00;    // f16 1.0
+constexpr int kStackSlots = 4;                  // AVX read classes A C T G (N is a wildcard row)
+// Two copies of the f16 selector window, the second shifted by one column,
+// so that every lane reads the selectors of two consecutive steps as one
+// aligned dword (even lanes from copy 0, odd lanes from copy 1).  Copy 1
+// starts 17 dwords later modulo the 32 banks, so the even and odd lanes of a
+// half wave read 32 different banks; class rows are 64 dwords apart (the
+// same banks), so lanes of different classes do not conflict either.
+constexpr int kWinRow = 2 * kBlock;             // f16 entries per class row
+constexpr int kWinCopy1 = kStackSlots * kWinRow / 2 + 17;  // dword offset of copy 1
+constexpr int kWinDwords = kWinCopy1 + kStackSlots * kWinRow / 2;
 
 // LDS pointers with their address space spelled out, so that a base kept in
 // a VGPR (see the sweep) still compiles to DS instructions.
 typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) v2f lds_f2;
 
 struct StackLds {
-  float win[kStackSlots][2 * kBlock];      // selectors, columns [64b - 64, 64b + 64): class-major,
-                                           // so the lanes of a step read consecutive banks
+  uint32_t win[kWinDwords];                // f16 selectors, columns [64b - 64, 64b + 64), two copies
   float2 ring[kStackEnds][kBlock];         // per-column J (= M + I, in .y) of the rows that end a read
   float2 carry[1];                         // (Z, J) per column, kStackPad + carry_cols (dynamic)
 };
@@ -85,16 +95,14 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // one step earlier; this step loads the next step's.  Step U + 1 reads
 // column 64b + U + 1 while step U's stores go to column 64b + U - 63 and the
 // ring, so the early loads never see a value this block still writes.
-#define STK_STEP(U, ZS)                                                                   \
+#define STK_STEP(U, ZS, HALF)                                                             \
   {                                                                                       \
-    const float sel = sel_n;                                                              \
     float M = c_n.x, w = c_n.y, prior, t, zn;                                             \
-    if ((U) + 1 < kBlock) {                                                               \
-      sel_n = selp[(U) + 1];                                                              \
-      c_n = crd[(U) + 1];                                                                 \
-    }                                                                                     \
+    if ((U) + 1 < kBlock) c_n = crd[(U) + 1];                                             \
+    if ((HALF) == 0 && (U) + 2 < kBlock) selw_n = selp[((U) + 2) / 2];                     \
     asm(                                                                                  \
-        "v_fma_f32 %[prior], %[sel], %[dp], %[pmis] clamp\n\t"                            \
+        "v_fma_mix_f32 %[prior], %[sel], %[dp], %[pmis] op_sel:[" #HALF                   \
+        ",0,0] op_sel_hi:[1,0,0] clamp\n\t"                                              \
         "v_mul_f32 %[t], %[E], %[yy]\n\t"                                                 \
         "v_fmac_f32_dpp %[E], %[J1], %[im] wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"     \
         "v_mul_f32_dpp %[w], %[J1], %[xx] wave_shr:1 row_mask:0xf bank_mask:0xf\n\t"      \
@@ -104,7 +112,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
         "v_fma_f32 %[zn], %[M], %[mm], %[E]"                                              \
         : [prior] "=&v"(prior), [t] "=&v"(t), [zn] "=v"(zn), [M] "+v"(M), [E] "+v"(E),    \
           [w] "+v"(w)                                                                     \
-        : [sel] "v"(sel), [dp] "v"(dp), [pmis] "v"(pmis), [yy] "v"(yy), [J1] "v"(J1),      \
+        : [sel] "v"(selw), [dp] "v"(dp), [pmis] "v"(pmis), [yy] "v"(yy), [J1] "v"(J1),     \
           [im] "v"(im), [xx] "v"(xx), [Z2] "v"(ZS), [mm] "v"(mm), [mx] "v"(mx),            \
           [aa] "v"(aa));                                                                  \
     if (store) {                                                                          \
@@ -114,9 +122,11 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     E = t;                                                                                \
     ZS = zn;                                                                              \
     J1 = w;                                                                               \
+    if ((HALF) == 1) selw = selw_n;                                                       \
   }
 
-#define STK_STEP2(U) STK_STEP((U), Za) STK_STEP((U) + 1, Zb)
+// an even step takes the selector word's low half, the odd step after it the high half
+#define STK_STEP2(U) STK_STEP((U), Za, 0) STK_STEP((U) + 1, Zb, 1)
 #define STK_STEP16(U) \
   STK_STEP2((U)) STK_STEP2((U) + 2) STK_STEP2((U) + 4) STK_STEP2((U) + 6) \
   STK_STEP2((U) + 8) STK_STEP2((U) + 10) STK_STEP2((U) + 12) STK_STEP2((U) + 14)
@@ -227,36 +237,46 @@ __global__ __launch_bounds__(64) void pairhmm_stack_kernel(KArgs a) {
         const int nb = (steps + kBlock - 1) / kBlock;
         uint8_t pre_m = (lane >= 1 && lane <= H) ? a.hmask[ho + lane - 1] : uint8_t(0);
         float Za = 0.f, Zb = 0.f, J1 = 0.f;
-        double acc_k[kStackEnds] = {0.0, 0.0, 0.0, 0.0};  // this lane's columns of ring slot k's sum
+        double acc_k[kStackEnds] = {};  // this lane's columns of ring slot k's sum
         double acc63 = 0.0;
-        const float* selp0 = &s.win[cls][kBlock - lane];
+        // this lane's selector dword for steps 0 and 1 (window index 64 - lane and the next)
+        const uint32_t* selp0 = (lane & 1) ? &s.win[kWinCopy1 + (cls * kWinRow + kBlock - 1 - lane) / 2]
+                                           : &s.win[(cls * kWinRow + kBlock - lane) / 2];
+        unsigned short* win16 = reinterpret_cast<unsigned short*>(s.win);
+        unsigned short lo_[kStackSlots];  // selectors of window index lane (the previous block's columns)
+#pragma unroll
+        for (int k = 0; k < kStackSlots; ++k) lo_[k] = kSelOutside;
         for (int b = 0; b < nb; ++b) {
           __syncthreads();
           {
-            // selector window: previous block's columns move down, this block's come in
-            float lo_[kStackSlots];
-#pragma unroll
-            for (int k = 0; k < kStackSlots; ++k) lo_[k] = (b == 0) ? kSelOutside : s.win[k][kBlock + lane];
-#pragma unroll
-            for (int k = 0; k < kStackSlots; ++k) s.win[k][lane] = lo_[k];
+            // selector window: the previous block's columns at index lane, this
+            // block's at 64 + lane (copy 1 holds index i + 1 at entry i)
             const int c = kBlock * b + lane;
             const bool inside = c >= 1 && c <= H;
 #pragma unroll
-            for (int k = 0; k < kStackSlots; ++k)
-              s.win[k][kBlock + lane] = !inside ? kSelOutside : (((pre_m >> k) & 1) ? 1.f : 0.f);
+            for (int k = 0; k < kStackSlots; ++k) {
+              const unsigned short v = !inside ? kSelOutside : (((pre_m >> k) & 1) ? kSelMatch : uint16_t(0));
+              unsigned short* c0 = win16 + k * kWinRow;
+              unsigned short* c1 = win16 + 2 * kWinCopy1 + k * kWinRow;
+              c0[lane] = lo_[k];
+              c0[kBlock + lane] = v;
+              if (lane >= 1) c1[lane - 1] = lo_[k];
+              c1[kBlock - 1 + lane] = v;
+              lo_[k] = v;
+            }
             const int cn = c + kBlock;
             pre_m = (cn >= 1 && cn <= H) ? a.hmask[ho + cn - 1] : uint8_t(0);
           }
           __syncthreads();
           // the carry loads are the same address on every lane, but a DS
           // instruction takes its address from a VGPR: keep the bases there
-          lds_f32* selp = (lds_f32*)(selp0);
+          lds_u32* selp = (lds_u32*)(selp0);
           lds_f2* crd = (lds_f2*)(&carry[kStackPad + kBlock * b]);
           // lane 63 stores (Z, J) to the carry row, an ending row to the ring
           lds_f32* stp = (lane == 63) ? (lds_f32*)(&carry[kStackPad + kBlock * b - 63].x)
                                       : (lds_f32*)(&s.ring[slot < kStackEnds ? slot : 0][0].x);
           asm("" : "+v"(crd), "+v"(stp), "+v"(selp));
-          float sel_n = selp[0];
+          uint32_t selw = selp[0], selw_n = 0;
           v2f c_n = crd[0];
           const int left = steps - kBlock * b;
           STK_STEP16(0)
