@@ -54,9 +54,12 @@ struct BArgs {
   int* item_count;
   BandItem* items32;
   int* item32_count;
-  // pairs for the full-matrix FP32 pass (ineligible or rejected)
+  // pairs for the full-matrix FP32 pass (ineligible or rejected): haplotypes
+  // of at most kStackHmax bases (stacked kernel), and longer ones (striped kernel)
   Task* full;
   int* full_count;
+  Task* full_long;
+  int* full_long_count;
   // tables
   const float* ph2pr;  // [128]
   const float* mm;     // [8256]

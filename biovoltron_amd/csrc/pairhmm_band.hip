@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   const int lane = threadIdx.x & 63;
   Task tk{};
   bool gcp_const = false;
-  int n64 = 0, n32 = 0, nf = 0;
+  int n64 = 0, n32 = 0, nf = 0, nfl = 0;
   if (t < a.n_tasks) {
     tk = a.tasks[t];
     gcp_const = a.gcp_const[tk.read] != 0;
@@ -240,23 +240,29 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
       const int c = band_class(a, tk, h, gcp_const, b);
       n64 += c == kBandLanes;
       n32 += c == kBandNarrow;
-      nf += c == 0;
+      if (c == 0) {
+        const int hap = tk.hap0 + h;
+        if (a.hap_off[hap + 1] - a.hap_off[hap] <= kStackHmax) ++nf; else ++nfl;
+      }
     }
   }
   const int n32p = (n32 + 1) & ~1;
-  int t64 = 0, t32 = 0, tf = 0;
+  int t64 = 0, t32 = 0, tf = 0, tfl = 0;
   const int o64 = wave_excl_scan(n64, lane, t64);
   const int o32 = wave_excl_scan(n32p, lane, t32);
   const int of = wave_excl_scan(nf, lane, tf);
-  int b64 = 0, b32 = 0, bf = 0;
+  const int ofl = wave_excl_scan(nfl, lane, tfl);
+  int b64 = 0, b32 = 0, bf = 0, bfl = 0;
   if (lane == 0) {
     if (t64) b64 = atomicAdd(a.item_count, t64);
     if (t32) b32 = atomicAdd(a.item32_count, t32);
     if (tf) bf = atomicAdd(a.full_count, tf);
+    if (tfl) bfl = atomicAdd(a.full_long_count, tfl);
   }
   b64 = __shfl(b64, 0) + o64;
   b32 = __shfl(b32, 0) + o32;
   bf = __shfl(bf, 0) + of;
+  bfl = __shfl(bfl, 0) + ofl;
   if (t < a.n_tasks) {
     for (int h = 0; h < tk.nhap; ++h) {
       int2 b;
@@ -266,8 +272,10 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
         const int w = b.y - b.x + 1;
         const BandItem it{tk.read, tk.hap0 + h, b.x - ((c - w) >> 1), 0, o};
         if (c == kBandLanes) a.items[b64++] = it; else a.items32[b32++] = it;
-      } else {
+      } else if (a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h] <= kStackHmax) {
         a.full[bf++] = Task{tk.read, tk.hap0 + h, 1, -1, o};
+      } else {
+        a.full_long[bfl++] = Task{tk.read, tk.hap0 + h, 1, -1, o};
       }
     }
     if (n32 & 1) a.items32[b32] = BandItem{tk.read, -1, 0, 0, -1};

@@ -54,6 +54,9 @@ hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, co
                                     uint8_t* ins, uint8_t* del, uint8_t* gcp, hipStream_t st);
 }  // namespace bvp
 
+static_assert(bvp::kBandHapMax <= bvp::kStackHmax,
+              "rejected banded pairs go to the stacked kernel's list");
+
 namespace {
 
 using namespace bvp;
@@ -78,7 +81,7 @@ int fail(int code, const std::string& msg) {
 // log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
-constexpr int kCounters = 48;  // device counter block of a plan (see bv_plan)
+constexpr int kCounters = 56;  // device counter block of a plan (see bv_plan)
 constexpr int kFeatCounters = 24;  // [24..31]: the features kernel's per-XCD task claims
 constexpr int kBandClaims = 32;    // [32..47]: the banded kernels' per-XCD chunk claims
 constexpr int kJac = 80001;
@@ -285,12 +288,15 @@ struct bv_plan {
   Task* d_tasks_rest = nullptr;
   int n_tasks_rest = 0;
   int grid_stack = 1, grid_stack_max = 1;
+  int grid_fbstack = 1;  // stacked kernel over the learned-band run's full-matrix list
   int stack_cols = 0;  // LDS carry row length of the stacked kernel
   Task* d_tasks64 = nullptr;       // long reads, FP64 first (FP32 mode), one pair each
   Task* d_verify = nullptr;        // their pairs that may not underflow FP32
   int n_tasks64 = 0;
   Task* d_rescue = nullptr;
-  Task* d_fallback = nullptr;
+  Task* d_fallback = nullptr;       // full-matrix list of the learned-band run (stacked kernel)
+  Task* d_fallback_long = nullptr;  // its pairs with haplotypes over kStackHmax (striped kernel)
+  int64_t n_longhap_pairs = 0;      // short-read pairs with such haplotypes (that list's capacity)
   int2* d_band = nullptr;
   BandItem* d_items = nullptr;    // banded work lists, built on the device each run
   BandItem* d_items32 = nullptr;
@@ -316,7 +322,8 @@ struct bv_plan {
   // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
   // [21] accepted moving-band pairs, [22..23] FP64 cells (u64), [24..31]
   // features task claims (one per XCD), [32..47] banded chunk claims (per
-  // XCD, wide then narrow kernel)
+  // XCD, wide then narrow kernel), [48] full-matrix list with haplotypes
+  // longer than kStackHmax, [49] its queue
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -357,7 +364,7 @@ void free_plan(bv_plan* p) {
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_gcp_const, p->d_read_off,
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
-                  p->d_stasks, p->d_tasks_rest,
+                  p->d_stasks, p->d_tasks_rest, p->d_fallback_long,
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items32, p->d_items_long, p->d_long_full,
@@ -485,6 +492,9 @@ int ensure_band_lists(bv_plan* p) {
   // narrow items are padded to an even count per read
   if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_tasks)) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
+  if (!p->d_fallback_long &&
+      hipMalloc(&p->d_fallback_long, sizeof(Task) * std::max<int64_t>(1, p->n_longhap_pairs)) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
   const int64_t n64 = std::max(1, p->n_tasks64);
   if (!p->d_items_long && hipMalloc(&p->d_items_long, sizeof(BandItem) * n64) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
@@ -552,6 +562,8 @@ BArgs band_args(const bv_plan* p) {
   b.item32_count = p->d_counters + 12;
   b.full = p->d_fallback;
   b.full_count = p->d_counters + 3;
+  b.full_long = p->d_fallback_long;
+  b.full_long_count = p->d_counters + 48;
   b.ph2pr = g_ctx.d_ph2pr_f;
   b.mm = g_ctx.d_mm_f;
   b.out = p->d_out;
@@ -679,7 +691,9 @@ int enqueue_run(bv_plan* p, bool timed) {
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[1], p->stream));
   if (fp32 && p->has_band) {
-    // Pass 2: the full-matrix list (FP32, same rescue rule).
+    // Pass 2: the full-matrix list (FP32, same rescue rule): pairs on the
+    // stacked kernel (one read each), those with haplotypes too long for its
+    // LDS carry row on the striped kernel.
     KArgs a = base_args(p);
     a.tasks = p->d_fallback;
     a.n_tasks = 0;
@@ -687,9 +701,16 @@ int enqueue_run(bv_plan* p, bool timed) {
     a.counter = p->d_counters + 4;
     a.rescue = p->d_rescue;
     a.rescue_count = p->d_counters + 2;
-    a.carry = p->d_carry32;
-    a.carry_stride = p->stride32;
-    HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+    a.carry_cols = kStackHmax + 128;
+    HIP_TRY(launch_stack(a, p->grid_fbstack, p->stream));
+    if (p->n_longhap_pairs > 0) {
+      a.tasks = p->d_fallback_long;
+      a.n_tasks_dev = p->d_counters + 48;
+      a.counter = p->d_counters + 49;
+      a.carry = p->d_carry32;
+      a.carry_stride = p->stride32;
+      HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
+    }
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[4], p->stream));
   {
@@ -1002,6 +1023,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
         }
       } else if (R >= kSplitRows) {
         for (int64_t h = h0; h < h1; ++h) {
+          if (hoff[h + 1] - hoff[h] > kStackHmax) ++p->n_longhap_pairs;
           tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
           cost.push_back(R * (hoff[h + 1] - hoff[h]));
           if (fp32_mode && !stackable) {
@@ -1010,6 +1032,8 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
           }
         }
       } else {
+        for (int64_t h = h0; h < h1; ++h)
+          if (hoff[h + 1] - hoff[h] > kStackHmax) ++p->n_longhap_pairs;
         tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
                              static_cast<int32_t>(nh), 0, out0});
         cost.push_back(R * hsum);
@@ -1056,6 +1080,9 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * stack_blocks_per_cu(p->stack_cols),
                            std::max(1, p->n_stasks))));
   p->grid_stack_max = p->grid_stack;
+  p->grid_fbstack = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * stack_blocks_per_cu(kStackHmax + 128),
+                           std::max<int64_t>(1, p->n_pairs))));
   p->stride32 = hmax + kCarryPad;
   p->stride64 = hmax + kCarryPad;
 
@@ -1245,7 +1272,7 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   if (!p || !s) return fail(BV_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(g_ctx.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  int cnt[24] = {0};
+  int cnt[kCounters] = {0};
   HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
   unsigned long long c64 = 0, swept = 0;
   std::memcpy(&c64, cnt + 22, sizeof(c64));
@@ -1263,7 +1290,7 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->last_ms_f32 = p->last_ms32;
   s->last_ms_f64 = p->last_ms64;
   s->band_pairs = p->has_band ? cnt[5] + cnt[21] : 0;
-  s->fallback_pairs = p->has_band ? cnt[3] : 0;
+  s->fallback_pairs = p->has_band ? cnt[3] + cnt[48] : 0;
   s->cells_swept = static_cast<int64_t>(swept);
   s->last_ms_fallback = p->last_msfb;
   s->last_ms_band = p->last_msband;

@@ -42,7 +42,7 @@
 //   it receives (im' = xx' = 0).  Lane 0 publishes the row-0 Z the same way
 //   when the stripe starts a read.  A first row below a last row sees that Z
 //   at every column, also left of column 1: the selector of columns outside
-//   [1, H] is -1e30, which the clamp turns into prior 0 (M = 0) for rows with
+//   [1, H] is -65504, which the clamp turns into prior 0 (M = 0) for rows with
 //   pm > pmis; a read whose first row is not such a row starts a stripe.
 #include <hip/hip_runtime.h>
 
@@ -150,12 +150,33 @@ __global__ __launch_bounds__(64) void pairhmm_stack_kernel(KArgs a) {
     if (lane == 0) ti = atomicAdd(a.counter, 1);
     ti = __builtin_amdgcn_readfirstlane(__shfl(ti, 0));
     if (ti >= n_tasks) break;
-    const StackTask tk = a.stasks[ti];
+    // a stacked task, or one read against a run of haplotypes from a pair list
+    // (the learned-band run's full-matrix list; band -2: a band it rejected)
+    StackTask tk;
+    uint8_t form_ok = kFormFP32;
+    if (a.stasks) {
+      tk = a.stasks[ti];
+    } else {
+      const Task t1 = a.tasks[ti];
+      tk = StackTask{t1.read, 1, t1.hap0, t1.nhap, t1.out0, 0, 0};
+      if (t1.band == -2) form_ok = kFormBandFallback;
+    }
     const int64_t p0 = a.read_off[tk.read0], p1 = a.read_off[tk.read0 + tk.nreads];
     for (int h = 0; h < tk.nhap; ++h) {
       const int hap = tk.hap0 + h;
       const int64_t ho = a.hap_off[hap];
       const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+      if (H + 128 > a.carry_cols) {
+        // longer than the LDS carry row (lists are split so that this does not
+        // happen): NaN rather than a sweep past the row
+        if (lane == 0)
+          for (int r = 0; r < tk.nreads; ++r) {
+            const int64_t o = tk.out0 + static_cast<int64_t>(r) * tk.stride + h;
+            a.out[o] = __builtin_nan("");
+            a.form[o] = form_ok;
+          }
+        continue;
+      }
       if (lane == 0) cells_w += static_cast<unsigned long long>(p1 - p0) * static_cast<unsigned long long>(H);
       const float kh = kK32 / static_cast<float>(H);  // D[0][j] = K/H (avx-pairhmm-template.h:86-92)
       int64_t ps = p0;         // first base of this stripe (lane 1)
@@ -334,7 +355,7 @@ __global__ __launch_bounds__(64) void pairhmm_stack_kernel(KArgs a) {
                 atomicAdd(a.rescue_cells, cells);
               } else {
                 a.out[o] = log10(S) - kLog10K32;
-                a.form[o] = kFormFP32;
+                a.form[o] = form_ok;
               }
             }
           }

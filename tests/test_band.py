@@ -142,7 +142,9 @@ def test_misplaced_bands_fall_back_to_full():
     say("rejected", int((form == 3).sum()), "of", len(sel))
     assert (form == 3).sum() >= 0.9 * len(sel)
     fb = form == 3
-    assert np.array_equal(out[fb], full[fb])
+    # rejected pairs run on the full matrix: the same values up to FP32
+    # rounding (the fallback list and the plain pass use different kernels)
+    assert np.abs(out[fb] - full[fb]).max() <= 2e-5
     assert np.abs(out - full).max() <= 1e-3
 
 
@@ -161,7 +163,8 @@ def test_ineligible_bands_run_full():
     bands[2] = (1, 0)        # no band
     out, form, st, _ = run_plan(pb, bands=bands)
     assert form[0] == 0 and form[1] == 0 and form[2] == 0
-    assert np.array_equal(out[:3], full[:3])
+    # on the full matrix: the plain pass's values up to FP32 rounding
+    assert np.abs(out[:3] - full[:3]).max() <= 2e-5
     assert np.all(form[3:] == 2)
 
 
