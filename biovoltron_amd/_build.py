@@ -19,10 +19,11 @@ HIP_LIB = PKG / "libbvpairhmm.so"
 SYNTH_LIB = PKG / "libbv_synth.so"
 
 HIP_SOURCES = ["pairhmm_kernels.hip", "pairhmm_stack.hip", "pairhmm_band.hip", "band_kernels.hip", "post_kernels.hip",
-               "sw_kernels.hip", "pairhmm_host.cpp"]
+               "sw_kernels.hip", "pairhmm_host.cpp", "model_io.cpp"]
 
-# Packed FP32 ops are disabled for all device code ("platform
-# constraints"); hipcc's host pass warns that it does not know the feature.
+# Packed FP32 ops (v_pk_*_f32) are disabled for all device code: some of
+# their forms have given wrong results on this platform without an error.
+# hipcc's host pass warns that it does not know the feature.
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

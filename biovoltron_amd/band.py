@@ -29,6 +29,10 @@ import numpy as np
 from . import pairhmm, synth
 
 DEFAULT_MODEL = Path(__file__).resolve().parent / "models" / "band_mlp.npz"
+# the same parameters in the library's own file format (bv_band_model_read),
+# which the C++ drop-in loads without Python
+DEFAULT_MODEL_BIN = DEFAULT_MODEL.with_suffix(".bin")
+MODEL_MAGIC = b"BVBANDM\0"
 N_FEAT = 36
 N_PARAMS = 6658
 FORMAT_VERSION = 1
@@ -58,6 +62,12 @@ def _lib():
         lib.bv_plan_bands.argtypes = [vp, vp]
         lib.bv_plan_set_bands.argtypes = [vp, vp]
         lib.bv_band_labels.argtypes = [C.c_void_p, C.c_double, C.c_int, vp]
+        lib.bv_band_model_get_state.argtypes = [vp, vp, vp, vp]
+        lib.bv_band_model_set_state.argtypes = [vp, vp, vp, i64]
+        lib.bv_band_model_read.argtypes = [C.c_char_p, vp]
+        lib.bv_band_model_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        lib.bv_band_model_write.argtypes = [vp, C.c_char_p]
+        lib.bv_band_model_default_path.argtypes = [C.c_char_p, i64]
         _bound = True
     return lib
 
@@ -120,13 +130,31 @@ class BandModel:
                                              losses.ctypes.data))
         return losses
 
+    def state(self):
+        """Adam's moments and step count (training resumes from them)."""
+        m1 = np.zeros(N_PARAMS, np.float32)
+        m2 = np.zeros(N_PARAMS, np.float32)
+        step = C.c_int64()
+        _check(self._lib.bv_band_model_get_state(self._h, m1.ctypes.data, m2.ctypes.data, C.byref(step)))
+        return m1, m2, int(step.value)
+
+    def set_state(self, m1, m2, step: int) -> None:
+        m1 = np.ascontiguousarray(m1, np.float32)
+        m2 = np.ascontiguousarray(m2, np.float32)
+        _check(self._lib.bv_band_model_set_state(self._h, m1.ctypes.data, m2.ctypes.data, int(step)))
+
     def save(self, path, **meta) -> str:
+        """An .npz with the parameters, their SHA-256, Adam's state and the
+        training metadata; the parameters also go to a .bin beside it in the
+        library's format (the file the C++ drop-in loads)."""
         p = self.params()
+        m1, m2, step = self.state()
         digest = hashlib.sha256(p.tobytes()).hexdigest()
         m = dict(self.meta)
         m.update(meta)
         np.savez(path, params=p, version=np.int64(FORMAT_VERSION), sha256=np.array(digest),
-                 meta=np.array(repr(sorted(m.items()))))
+                 meta=np.array(repr(sorted(m.items()))), adam_m1=m1, adam_m2=m2, adam_step=np.int64(step))
+        write_model_file(Path(path).with_suffix(".bin"), p)
         return digest
 
     @classmethod
@@ -138,7 +166,20 @@ class BandModel:
         if hashlib.sha256(p.tobytes()).hexdigest() != str(d["sha256"]):
             raise ValueError("band model parameters do not match their SHA-256")
         m = cls(params=p)
+        if "adam_step" in d.files:
+            m.set_state(d["adam_m1"], d["adam_m2"], int(d["adam_step"]))
         m.meta = {"source": str(path), "meta": str(d["meta"])}
+        return m
+
+    @classmethod
+    def load_bin(cls, path=None) -> "BandModel":
+        """A model from a file in the library's format (default: the one the
+        library ships, next to it), read and checked by the library itself."""
+        lib = _lib()
+        h = C.c_void_p()
+        _check(lib.bv_band_model_load(None if path is None else str(path).encode(), C.byref(h)))
+        m = cls.__new__(cls)
+        m._h, m._lib, m.meta = h, lib, {"source": str(path)}
         return m
 
     def close(self):
@@ -151,6 +192,25 @@ class BandModel:
             self.close()
         except Exception:
             pass
+
+
+def write_model_file(path, params) -> None:
+    """The library's band model format (bv_band_model_read): magic, uint32
+    version 1, uint32 parameter count, little-endian float32 parameters, and
+    the SHA-256 of the parameter bytes."""
+    p = np.ascontiguousarray(params, "<f4")
+    assert p.shape == (N_PARAMS,)
+    body = p.tobytes()
+    Path(path).write_bytes(MODEL_MAGIC + np.array([1, N_PARAMS], "<u4").tobytes() + body
+                           + hashlib.sha256(body).digest())
+
+
+def read_model_file(path) -> np.ndarray:
+    """Parameters of a file in the library's format, checked by the library
+    (bv_band_model_read, host code only)."""
+    out = np.zeros(N_PARAMS, np.float32)
+    _check(_lib().bv_band_model_read(str(path).encode(), out.ctypes.data))
+    return out
 
 
 def plan_features(plan: pairhmm.Plan, with_ends: bool = False):

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -48,6 +49,9 @@ int max_blocks_per_cu();
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
 hipError_t noalloc_scratch_address(void** ptr);
 hipError_t launch_stack(const KArgs& a, int grid, hipStream_t st);
+int read_model_file(const char* path, float* params, int count, std::string& err);
+int write_model_file(const char* path, const float* params, int count, std::string& err);
+std::string default_model_path();
 int stack_blocks_per_cu(int carry_cols);
 hipError_t launch_post(const PostArgs& a, hipStream_t st);
 hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, const uint8_t* triplets,
@@ -1799,6 +1803,63 @@ int bv_band_model_set_params(bv_band_model* m, const float* in) {
   std::vector<float> zero(kParams, 0.f);
   HIP_TRY(hipMemcpy(m->d_m1, zero.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(m->d_m2, zero.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
+  return BV_OK;
+}
+
+int bv_band_model_get_state(const bv_band_model* m, float* m1, float* m2, int64_t* step) {
+  if (!m) return fail(BV_EINVAL, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  if (m1) HIP_TRY(hipMemcpy(m1, m->d_m1, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+  if (m2) HIP_TRY(hipMemcpy(m2, m->d_m2, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+  if (step) *step = m->step;
+  return BV_OK;
+}
+
+int bv_band_model_set_state(bv_band_model* m, const float* m1, const float* m2, int64_t step) {
+  if (!m || !m1 || !m2 || step < 0 || step > INT32_MAX) return fail(BV_EINVAL, "bad argument");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(m->d_m1, m1, sizeof(float) * kParams, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m->d_m2, m2, sizeof(float) * kParams, hipMemcpyHostToDevice));
+  m->step = static_cast<int>(step);
+  return BV_OK;
+}
+
+int bv_band_model_read(const char* path, float* params) {
+  if (!params) return fail(BV_EINVAL, "params is NULL");
+  std::string err;
+  const std::string def = path ? std::string() : default_model_path();
+  if (int rc = read_model_file(path ? path : def.c_str(), params, kParams, err)) return fail(rc, err);
+  return BV_OK;
+}
+
+int bv_band_model_load(const char* path, bv_band_model** out) {
+  if (!out) return fail(BV_EINVAL, "model output pointer is NULL");
+  *out = nullptr;
+  std::vector<float> p(kParams);
+  if (int rc = bv_band_model_read(path, p.data())) return rc;
+  bv_band_model* m = nullptr;
+  if (int rc = bv_band_model_create(&m, 0)) return rc;
+  if (int rc = bv_band_model_set_params(m, p.data())) {
+    bv_band_model_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return BV_OK;
+}
+
+int bv_band_model_write(const bv_band_model* m, const char* path) {
+  std::vector<float> p(kParams);
+  if (int rc = bv_band_model_get_params(m, p.data())) return rc;
+  std::string err;
+  if (int rc = write_model_file(path, p.data(), kParams, err)) return fail(rc, err);
+  return BV_OK;
+}
+
+int bv_band_model_default_path(char* buf, int64_t size) {
+  const std::string p = default_model_path();
+  if (p.empty()) return fail(BV_ESTATE, "cannot locate the library to find its model directory");
+  if (!buf || size < static_cast<int64_t>(p.size()) + 1) return fail(BV_EINVAL, "buffer too small");
+  std::memcpy(buf, p.c_str(), p.size() + 1);
   return BV_OK;
 }
 

@@ -23,12 +23,20 @@
 // and const-callable, so it can sit in HaplotypeCaller's `const PairHMM
 // pairhmm` member (applications/haplotypecaller/haplotypecaller.hpp:29,90).
 // Unlike AvxPairHMM it keeps no state between calls.
+//
+// Learned bands (no reference counterpart): GpuPairHMM(LearnedBand{}) runs
+// the AVX model through the band network and the banded kernel, with the
+// full matrix as fallback for every pair whose band is not accepted (results
+// within 1e-3 in log10 of the full matrix).  The model is the trained one
+// shipped next to libbvpairhmm.so unless a path is given; copies of the
+// object share it.
 #pragma once
 
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -38,6 +46,15 @@
 #include "bv_pairhmm.h"
 
 namespace biovoltron_amd {
+
+// Options of the learned band (bv_plan_set_band_model): the model file
+// (empty: models/band_mlp.bin beside the library), the diagonals added per
+// side of the predicted span, and the widest span given a band.
+struct LearnedBand {
+  std::string model_path;
+  int margin = 10;
+  int max_width = 68;
+};
 
 struct PairHMMError : std::runtime_error {
   int status;
@@ -160,10 +177,20 @@ struct PlanGuard {
 // Every window as one region of one plan: one run, then normalise and
 // filter on the device (bv_plan_postprocess); each window's reads are
 // filtered in place and its kept rows returned, in input order.
+// A loaded band model and how a plan uses it.
+struct BandSetup {
+  std::shared_ptr<bv_band_model> model;
+  int margin = 10, max_width = 68;
+};
+
+inline void set_band(bv_plan* plan, const BandSetup* band) {
+  if (band && band->model) check(bv_plan_set_band_model(plan, band->model.get(), band->margin, band->max_width));
+}
+
 template <class Hap, class Read>
 std::vector<std::vector<std::vector<double>>> run_windows(
     const std::vector<std::pair<const std::vector<Hap>*, std::vector<Read>*>>& windows,
-    const bv_options& opt) {
+    const bv_options& opt, const BandSetup* band = nullptr) {
   Packed p;
   for (const auto& [haps, reads] : windows) {
     if (!haps || !reads) throw PairHMMError(BV_EINVAL, "a window has a null haplotype or read list");
@@ -176,6 +203,7 @@ std::vector<std::vector<std::vector<double>>> run_windows(
   if (batch.n_reads == 0) return out;
   PlanGuard g;
   check(bv_plan_create(&batch, &opt, &g.plan));
+  set_band(g.plan, band);
   check(bv_plan_run(g.plan));
   bv_plan_stats st{};
   check(bv_plan_stats_get(g.plan, &st));
@@ -207,13 +235,22 @@ std::vector<std::vector<std::vector<double>>> run_windows(
 template <class Hap, class Read>
 std::vector<std::vector<double>> raw_likelihoods(const std::vector<Hap>& haps,
                                                  const std::vector<Read>& reads,
-                                                 const bv_options& opt) {
+                                                 const bv_options& opt,
+                                                 const detail::BandSetup* band = nullptr) {
   std::vector<std::vector<double>> out(reads.size(), std::vector<double>(haps.size(), 0.0));
   if (reads.empty() || haps.empty()) return out;
   const auto packed = detail::pack_region(haps, reads, opt.mode == BV_MODE_AVX);
   const auto batch = packed.batch();
   std::vector<double> flat(reads.size() * haps.size());
-  check(bv_pairhmm_batch(&batch, &opt, flat.data(), nullptr));
+  if (band && band->model) {
+    detail::PlanGuard g;
+    check(bv_plan_create(&batch, &opt, &g.plan));
+    detail::set_band(g.plan, band);
+    check(bv_plan_run(g.plan));
+    check(bv_plan_results(g.plan, flat.data(), nullptr));
+  } else {
+    check(bv_pairhmm_batch(&batch, &opt, flat.data(), nullptr));
+  }
   for (std::size_t i = 0; i < reads.size(); ++i)
     std::copy_n(flat.begin() + static_cast<std::ptrdiff_t>(i * haps.size()), haps.size(), out[i].begin());
   return out;
@@ -223,6 +260,16 @@ class GpuPairHMM {
  public:
   GpuPairHMM() = default;
   explicit GpuPairHMM(bool fp64_renorm) : renorm_(fp64_renorm) {}
+  // The AVX model with learned bands (loads the model now; throws
+  // PairHMMError when the file is missing or fails its checks).
+  explicit GpuPairHMM(const LearnedBand& band, bool fp64_renorm = true) : renorm_(fp64_renorm) {
+    bv_band_model* m = nullptr;
+    check(bv_band_model_load(band.model_path.empty() ? nullptr : band.model_path.c_str(), &m));
+    band_.model = std::shared_ptr<bv_band_model>(m, bv_band_model_destroy);
+    band_.margin = band.margin;
+    band_.max_width = band.max_width;
+  }
+  bool learned_band() const { return static_cast<bool>(band_.model); }
 
   // AvxPairHMM::compute_likelihoods: per-base GOP/GCP, FP32 + FP64 re-run.
   template <class Hap, class Read>
@@ -232,7 +279,7 @@ class GpuPairHMM {
     bv_options_default(&opt);
     opt.mode = BV_MODE_AVX;
     opt.fp64_renorm = renorm_ ? 1 : 0;
-    auto ll = raw_likelihoods(haps, reads, opt);
+    auto ll = raw_likelihoods(haps, reads, opt, &band_);
     normalize_likelihoods(ll);
     filter_poorly_modeled_reads(reads, ll);
     return ll;
@@ -264,7 +311,7 @@ class GpuPairHMM {
     bv_options_default(&opt);
     opt.mode = BV_MODE_AVX;
     opt.fp64_renorm = renorm_ ? 1 : 0;
-    return detail::run_windows(windows, opt);
+    return detail::run_windows(windows, opt, &band_);
   }
 
   // The same for the scalar model with a transition matrix.
@@ -282,6 +329,7 @@ class GpuPairHMM {
 
  private:
   bool renorm_ = true;
+  detail::BandSetup band_;  // learned bands for the AVX model (empty: full matrix)
 };
 
 }  // namespace biovoltron_amd

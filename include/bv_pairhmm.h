@@ -278,6 +278,23 @@ int bv_band_model_get_params(const bv_band_model* model, float* params /* BV_BAN
 int bv_band_model_set_params(bv_band_model* model, const float* params);
 /* X [n][36] -> Y [n][2] (host buffers), device forward kernel. */
 int bv_band_model_predict(bv_band_model* model, const float* X, int64_t n, float* Y);
+/* Adam state (first and second moments, BV_BAND_PARAMS each, and the step
+ * count), so that training can resume where it stopped.  Any pointer of
+ * get_state may be NULL. */
+int bv_band_model_get_state(const bv_band_model* model, float* m1, float* m2, int64_t* step);
+int bv_band_model_set_state(bv_band_model* model, const float* m1, const float* m2, int64_t step);
+/* Band model files: "BVBANDM\0", uint32 version 1, uint32 BV_BAND_PARAMS,
+ * the parameters as little-endian float32, then the SHA-256 of those
+ * parameter bytes (the digest band.py records).  read checks the file and
+ * fills params (host memory, no device use); load makes a device model from
+ * it; write saves a model.  A NULL path means the trained model shipped with
+ * the library, models/band_mlp.bin in the library's own directory
+ * (bv_band_model_default_path).  Wrong magic, version, size or digest is
+ * BV_EINVAL. */
+int bv_band_model_read(const char* path, float* params /* BV_BAND_PARAMS */);
+int bv_band_model_load(const char* path, bv_band_model** model);
+int bv_band_model_write(const bv_band_model* model, const char* path);
+int bv_band_model_default_path(char* buf, int64_t size);
 /* Loss 0.5*mean(w0*(Y0-T0)^2 + w1*(Y1-T1)^2) and its gradient (device kernels). */
 int bv_band_model_loss_grad(bv_band_model* model, const float* X, const float* T, int64_t n,
                             float w0, float w1, float* loss, float* grad /* BV_BAND_PARAMS */);

@@ -2,10 +2,12 @@
 // uses PairHMM: a const member called per region with SamRecord-like reads.
 //   test_adapter cpu   normalisation/filtering known answers, error paths
 //   test_adapter gpu   prints the likelihood matrix of a small region
+//   test_adapter band F  windows from file F with and without the learned band
 #include <biovoltron_amd/gpu_pairhmm.hpp>
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -157,7 +159,81 @@ int gpu_run() {
   return 0;
 }
 
+// Windows from a text file ("W n_haps n_reads", then "H seq" per haplotype
+// and "R seq q0,q1,... gop gcp" per read), run with the learned band and
+// without; prints each window's kept rows for both, and the plan statistics
+// of a learned-band run through the C ABI on the same windows.
+int band_run(const char* path) {
+  using namespace biovoltron_amd;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return 2;
+  std::vector<std::vector<Hap>> whaps;
+  std::vector<std::vector<Read>> wreads;
+  char tag[4];
+  while (std::fscanf(f, "%3s", tag) == 1) {
+    if (tag[0] == 'W') {
+      int nh = 0, nr = 0;
+      if (std::fscanf(f, "%d %d", &nh, &nr) != 2) return 3;
+      whaps.emplace_back();
+      wreads.emplace_back();
+    } else if (tag[0] == 'H') {
+      char buf[8192];
+      if (std::fscanf(f, "%8191s", buf) != 1) return 3;
+      whaps.back().push_back(Hap{buf});
+    } else if (tag[0] == 'R') {
+      char seq[8192], quals[65536];
+      int gop = 0, gcp = 0;
+      if (std::fscanf(f, "%8191s %65535s %d %d", seq, quals, &gop, &gcp) != 4) return 3;
+      Read r = make_read(seq, 0, static_cast<char>(gop), static_cast<char>(gcp));
+      std::size_t k = 0;
+      for (char* t = std::strtok(quals, ","); t && k < r.qual.size(); t = std::strtok(nullptr, ","))
+        r.qual[k++] = static_cast<char>(std::atoi(t));
+      wreads.back().push_back(r);
+    }
+  }
+  std::fclose(f);
+  const GpuPairHMM learned(LearnedBand{}), plain;
+  std::printf("learned %d\n", learned.learned_band() ? 1 : 0);
+  for (int model = 0; model < 2; ++model) {
+    auto reads = wreads;
+    std::vector<std::pair<const std::vector<Hap>*, std::vector<Read>*>> windows;
+    for (std::size_t w = 0; w < whaps.size(); ++w) windows.emplace_back(&whaps[w], &reads[w]);
+    const auto all = (model == 0 ? learned : plain).compute_likelihoods_batch(windows);
+    for (std::size_t w = 0; w < all.size(); ++w) {
+      std::printf("bwindow %d %zu %zu\n", model, w, all[w].size());
+      for (std::size_t i = 0; i < all[w].size(); ++i) {
+        std::printf("brow %s", reads[w][i].seq.c_str());
+        for (double v : all[w][i]) std::printf(" %.17g", v);
+        std::printf("\n");
+      }
+    }
+  }
+  // one window through compute_likelihoods with the learned band
+  auto one = wreads[0];
+  const auto ll = learned.compute_likelihoods(whaps[0], one);
+  std::printf("single %zu\n", ll.size());
+  // the same windows through the C ABI: how many pairs took a band
+  detail::Packed pk;
+  for (std::size_t w = 0; w < whaps.size(); ++w) detail::append_region(pk, whaps[w], wreads[w], true);
+  const auto batch = pk.batch();
+  bv_options o;
+  bv_options_default(&o);
+  detail::PlanGuard g;
+  check(bv_plan_create(&batch, &o, &g.plan));
+  bv_band_model* m = nullptr;
+  check(bv_band_model_load(nullptr, &m));
+  check(bv_plan_set_band_model(g.plan, m, 10, 68));
+  bv_band_model_destroy(m);
+  check(bv_plan_run(g.plan));
+  bv_plan_stats st{};
+  check(bv_plan_stats_get(g.plan, &st));
+  std::printf("bstats %lld %lld %lld\n", static_cast<long long>(st.n_pairs), static_cast<long long>(st.band_pairs),
+              static_cast<long long>(st.fallback_pairs));
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc > 1 && std::strcmp(argv[1], "gpu") == 0) return gpu_run();
+  if (argc > 2 && std::strcmp(argv[1], "band") == 0) return band_run(argv[2]);
   return cpu_tests();
 }

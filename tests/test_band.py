@@ -341,6 +341,29 @@ def test_learned_band_against_heuristic_band(tmp_path):
     assert res["learned"][3] < res["rules_narrow"][3]
     loaded = band.BandModel.load(tmp_path / "m.npz")
     assert np.array_equal(loaded.params(), model.params())
+    # Adam's moments and step count are saved, so training can resume
+    m1, m2, step = model.state()
+    l1, l2, lstep = loaded.state()
+    assert step == lstep > 0
+    assert np.array_equal(m1, l1) and np.array_equal(m2, l2)
+    # and the library-format file beside the .npz holds the same parameters
+    assert np.array_equal(band.read_model_file(tmp_path / "m.bin"), model.params())
+
+
+@pytest.mark.gpu
+def test_shipped_bin_model_runs_like_npz_model():
+    """The model file the C++ drop-in loads (models/band_mlp.bin, read and
+    checked by the library) gives the same bands and bit-identical results
+    as the trained .npz on a plan of short-read regions."""
+    ev = synth.generate(1, seed=synth.SEED_BASE + 998, count=3)
+    pb = pairhmm.PackedBatch.from_synth(ev)
+    a_out, a_form, a_st, a_used = run_plan(pb, model=band.BandModel.load(band.DEFAULT_MODEL))
+    b_out, b_form, b_st, b_used = run_plan(pb, model=band.BandModel.load_bin())
+    say("pairs", pb.n_pairs, "band", a_st.band_pairs, "full pass", a_st.fallback_pairs)
+    assert a_st.band_pairs > 0
+    assert np.array_equal(a_used, b_used)
+    assert np.array_equal(a_form, b_form)
+    assert a_out.tobytes() == b_out.tobytes()
 
 
 SLIDE = 8  # the wide band's start slide (diagonals)

@@ -10,10 +10,10 @@ from conftest import ROOT
 BIN = ROOT / "tests" / "cpp" / "test_adapter"
 
 
-def run(mode):
+def run(mode, *args):
     if not BIN.exists():
         pytest.fail(f"{BIN} is not built (run __graft_entry__.build())")
-    res = subprocess.run([str(BIN), mode], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([str(BIN), mode, *args], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     return res.stdout
 
@@ -95,3 +95,65 @@ def check_windows_against_oracle(oracle, lines):
         assert [o.seq for o in objs] == [seq for seq, _ in rows], (model, w)
         for want, (_, got) in zip(raw, rows):
             np.testing.assert_allclose(got, want, rtol=0, atol=1e-3 if model == 0 else 1e-9)
+
+
+@pytest.mark.gpu
+def test_adapter_learned_band_vs_oracle(oracle, tmp_path):
+    """GpuPairHMM(LearnedBand{}) with the model shipped beside the library:
+    HaplotypeCaller-shaped windows (short-read regions of the synthetic
+    generator, up to 24 reads and 6 haplotypes each) in one batched call.
+    The kept rows match the oracle's reference rule after the reference's
+    normalise and filter, match the full-matrix adapter within 1e-3, and the
+    same windows through the C ABI put most pairs on the banded kernel."""
+    from biovoltron_amd import synth
+    from biovoltron_amd.pairhmm import PairHMM
+
+    b = synth.generate(1, seed=synth.SEED_BASE + 11, count=6)
+    lines, wins = [], []
+    for g in range(b.n_regions):
+        reads, haps = b.region(g)
+        reads, haps = reads[:24], haps[:6]
+        wins.append((reads, haps))
+        lines.append(f"W {len(haps)} {len(reads)}")
+        lines += [f"H {h.decode()}" for h in haps]
+        for r in reads:
+            assert len(set(r["i"])) == 1 and len(set(r["c"])) == 1
+            lines.append(f"R {r['seq'].decode()} {','.join(str(x) for x in r['q'])} {r['i'][0]} {r['c'][0]}")
+    path = tmp_path / "windows.txt"
+    path.write_text("\n".join(lines) + "\n")
+    out = run("band", str(path)).strip().splitlines()
+    assert out[0] == "learned 1"
+    rows = {}
+    cur = None
+    for ln in out[1:]:
+        f = ln.split()
+        if f[0] == "bwindow":
+            cur = (int(f[1]), int(f[2]))
+            rows[cur] = []
+        elif f[0] == "brow":
+            rows[cur].append((f[1].encode(), np.array([float(x) for x in f[2:]])))
+    stats = [ln for ln in out if ln.startswith("bstats")][0].split()
+    n_pairs, band_pairs = int(stats[1]), int(stats[2])
+    assert band_pairs >= 0.5 * n_pairs, stats
+    assert [ln for ln in out if ln.startswith("single")][0].split()[1] == str(len(rows[(0, 0)]))
+
+    class Rd:
+        def __init__(self, seq):
+            self.seq = seq
+
+    differ = 0
+    for w, (reads, haps) in enumerate(wins):
+        raw = [[oracle.log10_reference(rd, h)[0] for h in haps] for rd in reads]
+        objs = [Rd(rd["seq"]) for rd in reads]
+        PairHMM.normalize_likelihoods(raw)
+        PairHMM.filter_poorly_modeled_reads(objs, raw)
+        for model in (0, 1):
+            got = rows[(model, w)]
+            assert [s for s, _ in got] == [o.seq for o in objs], (model, w)
+            for want, (_, v) in zip(raw, got):
+                np.testing.assert_allclose(v, want, rtol=0, atol=1e-3)
+        for (_, a), (_, c) in zip(rows[(0, w)], rows[(1, w)]):
+            assert np.abs(a - c).max() <= 1e-3
+            differ += int((a != c).any())
+    # banded FP32 sums round differently from the full matrix: the band ran
+    assert differ > 0
