@@ -488,7 +488,7 @@ def main():
         "flops_per_cell": FLOPS_PER_CELL, "kernel_ms": round(st.last_ms_f32, 3), "traffic": None,
     }
     tr = profiled_traffic(f"pairhmm_workload{args.workload}.md", "full",
-                          ["full-matrix FP32 pass (short reads)"])
+                          ["full-matrix FP32 pass (short reads, stacked)"])
     if tr is not None:
         full_roof["traffic"] = tr[0]
         full_roof["traffic_source"] = f"profiles/{tr[1]} (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"

@@ -3,6 +3,7 @@
 //   test_adapter cpu   normalisation/filtering known answers, error paths
 //   test_adapter gpu   prints the likelihood matrix of a small region
 //   test_adapter band F  windows from file F with and without the learned band
+//   test_adapter shacc F windows from file F through shacc_pairhmm::calculate
 #include <biovoltron_amd/gpu_pairhmm.hpp>
 
 #include <cmath>
@@ -160,38 +161,49 @@ int gpu_run() {
 }
 
 // Windows from a text file ("W n_haps n_reads", then "H seq" per haplotype
-// and "R seq q0,q1,... gop gcp" per read), run with the learned band and
-// without; prints each window's kept rows for both, and the plan statistics
-// of a learned-band run through the C ABI on the same windows.
-int band_run(const char* path) {
-  using namespace biovoltron_amd;
+// and "R seq q0,q1,... gop gcp" per read); false on a malformed file.
+static bool read_windows(const char* path, std::vector<std::vector<Hap>>& whaps,
+                         std::vector<std::vector<Read>>& wreads) {
   FILE* f = std::fopen(path, "r");
-  if (!f) return 2;
-  std::vector<std::vector<Hap>> whaps;
-  std::vector<std::vector<Read>> wreads;
+  if (!f) return false;
+  bool ok = true;
   char tag[4];
-  while (std::fscanf(f, "%3s", tag) == 1) {
+  while (ok && std::fscanf(f, "%3s", tag) == 1) {
     if (tag[0] == 'W') {
       int nh = 0, nr = 0;
-      if (std::fscanf(f, "%d %d", &nh, &nr) != 2) return 3;
+      ok = std::fscanf(f, "%d %d", &nh, &nr) == 2;
       whaps.emplace_back();
       wreads.emplace_back();
-    } else if (tag[0] == 'H') {
-      char buf[8192];
-      if (std::fscanf(f, "%8191s", buf) != 1) return 3;
-      whaps.back().push_back(Hap{buf});
-    } else if (tag[0] == 'R') {
-      char seq[8192], quals[65536];
+    } else if (tag[0] == 'H' && !whaps.empty()) {
+      static char buf[1 << 16];
+      ok = std::fscanf(f, "%65535s", buf) == 1;
+      if (ok) whaps.back().push_back(Hap{buf});
+    } else if (tag[0] == 'R' && !wreads.empty()) {
+      static char seq[1 << 16], quals[1 << 19];
       int gop = 0, gcp = 0;
-      if (std::fscanf(f, "%8191s %65535s %d %d", seq, quals, &gop, &gcp) != 4) return 3;
+      ok = std::fscanf(f, "%65535s %524287s %d %d", seq, quals, &gop, &gcp) == 4;
+      if (!ok) break;
       Read r = make_read(seq, 0, static_cast<char>(gop), static_cast<char>(gcp));
       std::size_t k = 0;
       for (char* t = std::strtok(quals, ","); t && k < r.qual.size(); t = std::strtok(nullptr, ","))
         r.qual[k++] = static_cast<char>(std::atoi(t));
       wreads.back().push_back(r);
+    } else {
+      ok = false;
     }
   }
   std::fclose(f);
+  return ok;
+}
+
+// The windows of a file run with the learned band and without; prints each
+// window's kept rows for both, and the plan statistics of a learned-band run
+// through the C ABI on the same windows.
+int band_run(const char* path) {
+  using namespace biovoltron_amd;
+  std::vector<std::vector<Hap>> whaps;
+  std::vector<std::vector<Read>> wreads;
+  if (!read_windows(path, whaps, wreads) || whaps.empty()) return 3;
   const GpuPairHMM learned(LearnedBand{}), plain;
   std::printf("learned %d\n", learned.learned_band() ? 1 : 0);
   for (int model = 0; model < 2; ++model) {
@@ -232,8 +244,76 @@ int band_run(const char* path) {
   return 0;
 }
 
+// The batch accelerator hook as the reference declares it
+// (lib/intel_native/pairhmm/shacc_pairhmm.h:10-36): the same layout, bound
+// at link time to the library's strong definition.
+namespace shacc_pairhmm {
+struct Read {
+  int length;
+  const char* bases;
+  const char* q;
+  const char* i;
+  const char* d;
+  const char* c;
+};
+struct Haplotype {
+  int length;
+  const char* bases;
+};
+struct Batch {
+  int num_reads;
+  int num_haps;
+  long num_cells;
+  Read* reads;
+  Haplotype* haps;
+  float* results;
+};
+bool calculate(Batch& batch);
+}  // namespace shacc_pairhmm
+
+// Each window of a file as one shacc_pairhmm::Batch; prints the results
+// (read-major log10 likelihoods) and the hook's answers on an empty batch
+// and on a batch holding an empty read.
+int shacc_run(const char* path) {
+  std::vector<std::vector<Hap>> whaps;
+  std::vector<std::vector<Read>> wreads;
+  if (!read_windows(path, whaps, wreads) || whaps.empty()) return 3;
+  for (std::size_t w = 0; w < whaps.size(); ++w) {
+    std::vector<shacc_pairhmm::Read> rs;
+    std::vector<shacc_pairhmm::Haplotype> hs;
+    long cells = 0;
+    for (const auto& r : wreads[w])
+      rs.push_back({static_cast<int>(r.seq.size()), r.seq.data(), r.qual.data(), r.gop.data(), r.gop.data(),
+                    r.gcp.data()});
+    for (const auto& h : whaps[w]) hs.push_back({static_cast<int>(h.seq.size()), h.seq.data()});
+    for (const auto& r : rs)
+      for (const auto& h : hs) cells += static_cast<long>(r.length) * h.length;
+    std::vector<float> res(rs.size() * hs.size(), 0.f);
+    shacc_pairhmm::Batch b{static_cast<int>(rs.size()), static_cast<int>(hs.size()), cells, rs.data(), hs.data(),
+                           res.data()};
+    const bool ok = shacc_pairhmm::calculate(b);
+    std::printf("swindow %zu %d %zu %zu\n", w, ok ? 1 : 0, rs.size(), hs.size());
+    for (std::size_t i = 0; i < rs.size(); ++i) {
+      std::printf("srow");
+      for (std::size_t h = 0; h < hs.size(); ++h) std::printf(" %.9g", res[i * hs.size() + h]);
+      std::printf("\n");
+    }
+  }
+  shacc_pairhmm::Batch empty{0, 0, 0, nullptr, nullptr, nullptr};
+  std::printf("sempty %d\n", shacc_pairhmm::calculate(empty) ? 1 : 0);
+  const char* hb = "ACGT";
+  shacc_pairhmm::Read zero{0, "", "", "", "", ""};
+  shacc_pairhmm::Haplotype one{4, hb};
+  float out = 0.f;
+  shacc_pairhmm::Batch bad{1, 1, 0, &zero, &one, &out};
+  const bool r = shacc_pairhmm::calculate(bad);
+  std::printf("sbad %d %s\n", r ? 1 : 0, bv_pairhmm_last_error());
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc > 1 && std::strcmp(argv[1], "gpu") == 0) return gpu_run();
   if (argc > 2 && std::strcmp(argv[1], "band") == 0) return band_run(argv[2]);
+  if (argc > 2 && std::strcmp(argv[1], "shacc") == 0) return shacc_run(argv[2]);
   return cpu_tests();
 }

@@ -97,6 +97,72 @@ def check_windows_against_oracle(oracle, lines):
             np.testing.assert_allclose(got, want, rtol=0, atol=1e-3 if model == 0 else 1e-9)
 
 
+def write_windows(path, wins):
+    """Windows [(reads, haps)] in the adapter test's text format."""
+    lines = []
+    for reads, haps in wins:
+        lines.append(f"W {len(haps)} {len(reads)}")
+        lines += [f"H {h.decode()}" for h in haps]
+        for r in reads:
+            assert r["i"] == r["d"] and len(set(r["i"])) == 1 and len(set(r["c"])) == 1
+            lines.append(f"R {r['seq'].decode()} {','.join(str(x) for x in r['q'])} {r['i'][0]} {r['c'][0]}")
+    path.write_text("\n".join(lines) + "\n")
+    return path
+
+
+@pytest.mark.gpu
+def test_shacc_hook_vs_oracle(oracle, tmp_path):
+    """shacc_pairhmm::calculate (the weak hook of
+    lib/intel_native/pairhmm/shacc_pairhmm.h:10-36, defined by the library)
+    called through the reference's declaration: every result is the pair's
+    log10 likelihood under the reference's FP32-then-FP64 rule, read-major
+    [num_reads][num_haps], as float.  Windows: short-read regions of the
+    synthetic generator, and one with N bases, low qualities and reads longer
+    than the haplotypes.  An empty batch succeeds; an empty read fails with a
+    message."""
+    from biovoltron_amd import synth
+
+    b = synth.generate(1, seed=synth.SEED_BASE + 12, count=3)
+    wins = []
+    for g in range(b.n_regions):
+        reads, haps = b.region(g)
+        wins.append((reads[:16], haps[:5]))
+    rng = np.random.default_rng(5)
+    haps = [bytes(rng.choice(list(b"ACGT"), n).astype(np.uint8)) for n in (30, 7, 90)]
+    reads = []
+    for n in (1, 12, 45, 100):
+        s = bytearray(rng.choice(list(b"ACGTN"), n, p=[.23, .23, .23, .23, .08]).astype(np.uint8))
+        q = rng.integers(0, 41, n).astype(np.uint8)
+        reads.append(dict(seq=bytes(s), q=q.tobytes(), i=bytes([40]) * n, d=bytes([40]) * n, c=bytes([12]) * n))
+    wins.append((reads, haps))
+    path = write_windows(tmp_path / "windows.txt", wins)
+    out = run("shacc", str(path)).strip().splitlines()
+    k = 0
+    got = {}
+    for ln in out:
+        f = ln.split()
+        if f[0] == "swindow":
+            k = int(f[1])
+            assert f[2] == "1", ln
+            got[k] = []
+        elif f[0] == "srow":
+            got[k].append([float(x) for x in f[1:]])
+    assert len(got) == len(wins)
+    worst = 0.0
+    for w, (reads, haps) in enumerate(wins):
+        assert len(got[w]) == len(reads)
+        for rd, row in zip(reads, got[w]):
+            assert len(row) == len(haps)
+            for hap, v in zip(haps, row):
+                want = oracle.log10_reference(rd, hap)[0]
+                worst = max(worst, abs(v - want))
+                assert abs(v - want) <= 1e-3, (w, len(rd["seq"]), len(hap), v, want)
+    print("[shacc] max |dlog10| vs oracle", worst, flush=True)
+    assert [ln for ln in out if ln.startswith("sempty")] == ["sempty 1"]
+    bad = [ln for ln in out if ln.startswith("sbad")][0]
+    assert bad.startswith("sbad 0") and "empty read" in bad
+
+
 @pytest.mark.gpu
 def test_adapter_learned_band_vs_oracle(oracle, tmp_path):
     """GpuPairHMM(LearnedBand{}) with the model shipped beside the library:
@@ -109,18 +175,11 @@ def test_adapter_learned_band_vs_oracle(oracle, tmp_path):
     from biovoltron_amd.pairhmm import PairHMM
 
     b = synth.generate(1, seed=synth.SEED_BASE + 11, count=6)
-    lines, wins = [], []
+    wins = []
     for g in range(b.n_regions):
         reads, haps = b.region(g)
-        reads, haps = reads[:24], haps[:6]
-        wins.append((reads, haps))
-        lines.append(f"W {len(haps)} {len(reads)}")
-        lines += [f"H {h.decode()}" for h in haps]
-        for r in reads:
-            assert len(set(r["i"])) == 1 and len(set(r["c"])) == 1
-            lines.append(f"R {r['seq'].decode()} {','.join(str(x) for x in r['q'])} {r['i'][0]} {r['c'][0]}")
-    path = tmp_path / "windows.txt"
-    path.write_text("\n".join(lines) + "\n")
+        wins.append((reads[:24], haps[:6]))
+    path = write_windows(tmp_path / "windows.txt", wins)
     out = run("band", str(path)).strip().splitlines()
     assert out[0] == "learned 1"
     rows = {}
