@@ -78,6 +78,33 @@ def build_cpp_tests() -> Path:
     return CPP_TEST
 
 
+SANITIZE = ROOT / "tests" / "cpp" / "sanitize"
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+             "-fopenmp"]
+
+
+def build_sanitize() -> Path:
+    """Host code under ASan + UBSan (g++/gcc only, no device code): the
+    oracle's C restatement, the synthetic generator and the model file I/O,
+    linked with the driver in tests/cpp/sanitize_main.cpp."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        objs = []
+        for src, cc, std in ((ROOT / "oracle" / "pairhmm_oracle.c", "gcc", "-std=c11"),
+                             (ROOT / "oracle" / "sw_oracle.c", "gcc", "-std=c11"),
+                             (CSRC / "synth.cpp", "g++", "-std=c++20"),
+                             (CSRC / "model_io.cpp", "g++", "-std=c++20"),
+                             (ROOT / "tests" / "cpp" / "sanitize_main.cpp", "g++", "-std=c++20")):
+            obj = Path(td) / (src.stem + ".o")
+            extra = ["-ffp-contract=off"] if cc == "gcc" else []
+            _run([cc, std, *SAN_FLAGS, *extra, f"-I{INCLUDE}", "-c", "-o", str(obj), str(src)])
+            objs.append(str(obj))
+        tmp = SANITIZE.with_suffix(".tmp")
+        _run(["g++", *SAN_FLAGS, "-o", str(tmp), *objs, "-ldl", "-lm"])
+        os.replace(tmp, SANITIZE)
+    return SANITIZE
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """The CPU checker (test infrastructure); the reference build only where
     the reference tree exists."""
@@ -91,6 +118,7 @@ def build_all() -> None:
     build_hip()
     build_cpp_tests()
     build_oracle()
+    build_sanitize()
 
 
 if __name__ == "__main__":

@@ -15,11 +15,12 @@ def test_profiled_traffic_reads_the_committed_summary():
 
     band = bench.profiled_traffic("pairhmm_workload4.md", "learned",
                                   ["band_forward_kernel<64>", "band_forward_kernel<32>"])
-    full = bench.profiled_traffic("pairhmm_workload4.md", "full", ["full-matrix FP32 pass (short reads)"])
+    full = bench.profiled_traffic("pairhmm_workload4.md", "full", ["full-matrix FP32 pass (short reads, stacked)"])
     assert band is not None and full is not None
     # both banded kernels together move some hundreds of GB per launch; the
-    # full-matrix FP32 pass moves more (its carry rows)
-    assert 1e11 < band[0] < 1e12 and full[0] > band[0]
+    # stacked full-matrix FP32 pass keeps its carry row in LDS and moves
+    # about a hundred GB (read bytes and per-read sums)
+    assert 1e11 < band[0] < 1e12 and 1e10 < full[0] < 1e12
     assert bench.profiled_traffic("pairhmm_workload4.md", "learned", ["no_such_kernel"]) is None
 
 
