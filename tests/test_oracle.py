@@ -219,6 +219,83 @@ def test_band_posterior_rows_sum_to_one(oracle, golden):
         assert ll == pytest.approx(oracle.log10_exact(rd, hap, 0), abs=1e-9)
 
 
+def brute_force_visits(rows, hap, match):
+    """Expected visits of the alignment path per diagonal o = j - r (index
+    o + R - 1), by enumerating every M/I/D path as brute_force does and
+    counting the cells (r >= 1) it enters, weighted by the path's share of
+    the likelihood.  No DP is involved."""
+    R, H = len(rows), len(hap)
+    mass = np.zeros(R + H - 1)
+    total = 0.0
+
+    def prior(r, j):
+        p = rows[r - 1]
+        return p["pm"] if match(p["code"], hap[j - 1]) else p["pmis"]
+
+    def walk(state, r, j, w, cells):
+        nonlocal total
+        if state in "MI" and r == R:
+            total += w
+            for rr, jj in cells:
+                mass[jj - rr + R - 1] += w
+            return
+        nxt = r + 1
+        if state == "D" and r == 0:
+            if j + 1 <= H:
+                walk("M", 1, j + 1, w * rows[0]["dm"] * prior(1, j + 1), cells + [(1, j + 1)])
+            return
+        if state == "M":
+            if nxt <= R and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["mm"] * prior(nxt, j + 1), cells + [(nxt, j + 1)])
+            if nxt <= R:
+                walk("I", nxt, j, w * rows[nxt - 1]["mi"], cells + [(nxt, j)])
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["md"], cells + [(r, j + 1)])
+        elif state == "I":
+            if nxt <= R and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["im"] * prior(nxt, j + 1), cells + [(nxt, j + 1)])
+            if nxt <= R:
+                walk("I", nxt, j, w * rows[nxt - 1]["ii"], cells + [(nxt, j)])
+        elif state == "D":
+            if nxt <= R and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["dm"] * prior(nxt, j + 1), cells + [(nxt, j + 1)])
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["dd"], cells + [(r, j + 1)])
+
+    for j0 in range(H + 1):
+        walk("D", 0, j0, 1.0 / H, [])
+    return math.log10(total), mass / total
+
+
+def test_band_posterior_vs_brute_force(oracle):
+    """The per-diagonal expected visits that label the training pairs
+    (bvo_band_posterior, restated by the host's bv_band_labels) against path
+    enumeration: every diagonal's value, not only the row sums, so a shifted
+    diagonal index or a dropped state fails."""
+    t = np.load(GOLDEN / "tables.npz")
+    n = 0
+    for rd, hap in small_cases(11, 80):
+        ll_bf, mass_bf = brute_force_visits(row_params_avx(rd, t), hap, match_avx)
+        ll, mass, rows = oracle.band_posterior(rd, hap)
+        assert ll == pytest.approx(ll_bf, abs=1e-12)
+        np.testing.assert_allclose(mass, mass_bf, rtol=0, atol=1e-12)
+        n += 1
+    assert n == 80
+
+
+def test_band_posterior_brute_force_catches_a_shifted_diagonal(oracle):
+    """A posterior written one diagonal off (the mistake the row-sum and
+    total checks cannot see) differs from the enumeration."""
+    t = np.load(GOLDEN / "tables.npz")
+    hit = 0
+    for rd, hap in small_cases(12, 30):
+        _, mass_bf = brute_force_visits(row_params_avx(rd, t), hap, match_avx)
+        _, mass, _ = oracle.band_posterior(rd, hap)
+        if len(mass) > 1 and np.abs(np.roll(mass, 1) - mass_bf).max() > 1e-6:
+            hit += 1
+    assert hit >= 20  # the rest are 1x1 matrices and symmetric cases
+
+
 # ---- planted mistakes (in a Python DP on the CPU) ---------------------------
 
 def python_dp(rows, hap, match, mistake=None):
