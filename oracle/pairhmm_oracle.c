@@ -493,6 +493,17 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
                                int irun, const int* sched, double* touch_ratio, double* lost,
                                int* offs) {
   bvo_init();
+  /* edge < 0: zones of -edge diagonals that are active on every row (the
+   * anti-diagonal banded kernel's rule), not only where the band cuts */
+  const int always = edge < 0;
+  if (always) edge = -edge;
+  /* irun bit 2: the anti-diagonal kernel's rule -- T follows the recurrence
+   * on zone cells too, except that the innermost zone cells take the
+   * forward value of their input from the band's middle (I on diagonal
+   * edge - 1, D on width - edge); with T = F on the zones at row 0 that is
+   * T = F on every zone cell, and the entry runs added at the top slot then
+   * propagate through the zone instead of being overwritten (with edge < 0) */
+  const int kernel_rule = (irun & 4) != 0;
   const size_t W = (size_t)width;
   double* buf = (double*)calloc(12 * W, sizeof(double));
   double *fM = buf, *fI = buf + W, *fD = buf + 2 * W, *tM = buf + 3 * W, *tI = buf + 4 * W,
@@ -505,7 +516,7 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
     const int j = o + k;
     if (j < 0 || j > H) continue;
     fD[k] = 1.0 / (double)H;
-    const int zone = (k < edge && o - 1 >= 0) || (k >= width - edge && o + width <= H);
+    const int zone = (k < edge && (always || o - 1 >= 0)) || (k >= width - edge && (always || o + width <= H));
     if (zone) tD[k] = fD[k];
   }
   if (offs) offs[0] = o;
@@ -557,14 +568,14 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
     if (offs) offs[r] = o;
     row_par p;
     row_params(0, r - 1, rs, q, ins, del, gcp, &p);
-    const int lact = r + o - 1 >= 1, ract = r + o + width <= H;
+    const int lact = always || r + o - 1 >= 1, ract = always || r + o + width <= H;
     double top_in = 0.0;
     if (irun) {
       /* counted on rows 2..48 (entries) and from the last 48 rows (exits):
        * beyond them a run's factor ii^rows is below 1e-46 */
       if (r >= 3) run_prod *= p.ii;
       const int c = r + o + width - 1;
-      if ((irun & 1) && r >= 2 && r <= 48 && c >= 1 && c <= H) {
+      if ((irun & 5) && r >= 2 && r <= 48 && c >= 1 && c <= H) {
         const double pr1 = is_match(0, p1.code, hap[c - 1]) ? p1.pm : p1.pmis;
         top_in = ldexp(p1.dm * pr1 * p2.mi * run_prod / (double)H, -(int)log2scale);
       }
@@ -588,7 +599,7 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
       nI[k] = upM * p.mi + upI * p.ii;
       nD[k] = lfM * p.md + lfD * p.dd;
       const int zone = (k < edge && lact) || (k >= width - edge && ract);
-      if (zone) {
+      if (zone && !kernel_rule) {
         uM[k] = nM[k];
         uI[k] = nI[k];
         uD[k] = nD[k];
@@ -598,6 +609,11 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
         uM[k] = (tM[k] * p.mm + tI[k] * p.im + tD[k] * p.dm) * prior;
         uI[k] = tuM * p.mi + tuI * p.ii;
         uD[k] = tlM * p.md + tlD * p.dd;
+      }
+      if (kernel_rule) {
+        /* the zones' innermost inputs from the band's middle: T = F there */
+        if (k == edge - 1) uI[k] = nI[k];
+        if (k == width - edge) uD[k] = nD[k];
       }
       mx = fmax(mx, fmax(nM[k], fmax(nI[k], nD[k])));
     }

@@ -126,7 +126,12 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
  * touch mass the pure insertion runs that enter through the top edge from a
  * read start above the band (bit 0, rows 2..48) and those that leave through
  * the bottom edge to end the read below it (bit 1, the last 48 rows; the
- * kernels count these).  Returns log10 of
+ * kernels count these).  edge < 0 gives zones of -edge diagonals active on
+ * every row (not only where the band cuts the matrix).  irun bit 2 (with
+ * edge < 0): the anti-diagonal banded kernel's rule -- T = F on the zones is
+ * kept by taking the forward value only for the innermost zone cells' inputs
+ * from the band's middle, and the entry runs (bit 0 implied) then propagate
+ * through the zone into T.  Returns log10 of
  * the banded likelihood, the touch ratio T/S (the accept test), the mass share
  * the moves dropped (*lost) and, if offs is not NULL, the band offset used on
  * each row 0..R. */

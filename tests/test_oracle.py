@@ -409,7 +409,7 @@ def test_band_forward_covering_band_is_the_full_matrix(oracle, golden):
 
 # ---- the band that follows the alignment (long reads) -----------------------
 
-def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
+def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False, always=False, kernel_rule=False):
     """(likelihood, touch) over the paths that stay in each row's band
     [offs[r], offs[r] + width - 1] and whose row-to-row steps start on a
     diagonal the next row's band still holds (a move drops the others).
@@ -427,8 +427,9 @@ def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
 
     def zone(r, j):
         d, o = j - r, offs[r]
-        lo = o - 1 >= 0 if r == 0 else r + o - 1 >= 1
-        return (d < o + edge and lo) or (d > o + width - 1 - edge and r + o + width <= H)
+        lo = always or (o - 1 >= 0 if r == 0 else r + o - 1 >= 1)
+        hi = always or r + o + width <= H
+        return (d < o + edge and lo) or (d > o + width - 1 - edge and hi)
 
     def prior(r, j):
         p = rows[r - 1]
@@ -444,7 +445,14 @@ def brute_force_moving_band(rows, hap, match, offs, width, edge, irun=False):
         if not inb(r, j - r):
             return
         z = zone(r, j)
-        if ghost and z and not entry:
+        if kernel_rule:
+            # the innermost zone cells take F for their input from the
+            # middle: a touch-only path dies on stepping in that way
+            d = j - r
+            if ghost and not entry and ((state == "I" and d == offs[r] + edge - 1)
+                                        or (state == "D" and d == offs[r] + width - edge)):
+                return
+        elif ghost and z and not entry:
             return  # T = F on zone cells
         touched = touched or z
         if state in "MI" and r == R:
@@ -619,6 +627,124 @@ def test_band_follow_insertion_runs_vs_brute_force(oracle):
         checked += 1
         changed += ratio > ratio0 * (1 + 1e-9)
     assert checked > 150 and changed > 50
+
+
+def test_band_follow_zones_on_every_row_vs_brute_force(oracle):
+    """Edge zones active on every row (edge < 0, the anti-diagonal banded
+    kernel's rule) on fixed bands, with the read-end insertion runs: the
+    enumeration with zone cells wherever the band's edge diagonals lie, also
+    where the band reaches past the matrix; and never less touch mass than
+    zones active only where the band cuts (the rule only adds zone cells)."""
+    rng = np.random.default_rng(61)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = more = 0
+    for rd, hap in small_cases(67, 500):
+        R, H = len(rd["seq"]), len(hap)
+        width = int(rng.integers(2, 6))
+        edge = int(rng.integers(1, 3))
+        sched = np.full(R + 1, int(rng.integers(-R - 1, H + 1)), np.int32)
+        v, ratio, _, _ = oracle.band_follow(rd, hap, 0, width, -edge, max_move=0, irun=2, sched=sched)
+        _, ratio_cut, _, _ = oracle.band_follow(rd, hap, 0, width, edge, max_move=0, irun=2, sched=sched)
+        bf, bt = brute_force_moving_band(row_params_avx(rd, t), hap, match_avx, sched, width, edge,
+                                         irun=False, always=True)
+        if bf == 0.0:
+            assert v == -np.inf
+            continue
+        # irun bit 1 only: the exit runs, enumerated by the brute force's
+        # bottom-slot term (its entry-run term is bit 0)
+        bt_exit = brute_force_exit_runs(row_params_avx(rd, t), hap, match_avx, sched, width)
+        assert v == pytest.approx(math.log10(bf), abs=1e-12)
+        assert ratio == pytest.approx((bt + bt_exit) / bf, rel=1e-9, abs=1e-15)
+        assert ratio >= ratio_cut * (1 - 1e-12)
+        checked += 1
+        more += ratio > ratio_cut * (1 + 1e-9)
+    assert checked > 150 and more > 20
+
+
+def test_band_follow_kernel_rule_entry_runs_vs_brute_force(oracle):
+    """The anti-diagonal kernel's rule (irun bit 2, zones on every row): the
+    pure insertion runs entering the top slot are added to T and follow the
+    recurrence through the zones, dying only where the innermost zone cells
+    take the forward value; in-band paths count as touched exactly as with
+    zones overridden everywhere.  Enumerated on fixed bands."""
+    rng = np.random.default_rng(71)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = entered = 0
+    for rd, hap in small_cases(73, 600):
+        R, H = len(rd["seq"]), len(hap)
+        width = int(rng.integers(3, 6))
+        edge = 1
+        sched = np.full(R + 1, int(rng.integers(-R - 1, H + 1)), np.int32)
+        v, ratio, _, _ = oracle.band_follow(rd, hap, 0, width, -edge, max_move=0, irun=6, sched=sched)
+        _, ratio_z, _, _ = oracle.band_follow(rd, hap, 0, width, -edge, max_move=0, irun=2, sched=sched)
+        bf, bt = brute_force_moving_band(row_params_avx(rd, t), hap, match_avx, sched, width, edge,
+                                         irun=True, always=True, kernel_rule=True)
+        if bf == 0.0:
+            assert v == -np.inf
+            continue
+        assert v == pytest.approx(math.log10(bf), abs=1e-12)
+        assert ratio == pytest.approx(bt / bf, rel=1e-9, abs=1e-15)
+        assert ratio >= ratio_z * (1 - 1e-12)
+        checked += 1
+        entered += ratio > ratio_z * (1 + 1e-9)
+    assert checked > 200 and entered > 30
+
+
+def brute_force_exit_runs(rows, hap, match, offs, width):
+    """Mass of the in-band paths that step from the bottom slot into an I
+    below the band on one of the last 48 rows and stay in I to the last row
+    (the bottom-slot term of brute_force_moving_band alone)."""
+    R, H = len(rows), len(hap)
+    tot = [0.0]
+
+    def inb(r, d):
+        return offs[r] <= d <= offs[r] + width - 1
+
+    def prior(r, j):
+        p = rows[r - 1]
+        return p["pm"] if match(p["code"], hap[j - 1]) else p["pmis"]
+
+    def ii_prod(a, b):
+        out = 1.0
+        for k in range(a, b + 1):
+            out *= rows[k - 1]["ii"]
+        return out
+
+    def walk(state, r, j, w):
+        if not inb(r, j - r):
+            return
+        if state in "MI" and r == R:
+            return
+        nxt, d = r + 1, j - r
+        down = nxt <= R and inb(nxt, d)
+        if down and state in "MI" and d == offs[nxt] and 1 <= j <= H and nxt > R - 48:
+            step = rows[nxt - 1]["mi"] if state == "M" else rows[nxt - 1]["ii"]
+            tot[0] += w * step * ii_prod(nxt + 1, R)
+        if state == "D" and r == 0:
+            if j + 1 <= H and down:
+                walk("M", 1, j + 1, w * rows[0]["dm"] * prior(1, j + 1))
+            return
+        if state == "M":
+            if down and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["mm"] * prior(nxt, j + 1))
+            if down:
+                walk("I", nxt, j, w * rows[nxt - 1]["mi"])
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["md"])
+        elif state == "I":
+            if down and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["im"] * prior(nxt, j + 1))
+            if down:
+                walk("I", nxt, j, w * rows[nxt - 1]["ii"])
+        elif state == "D":
+            if down and j + 1 <= H:
+                walk("M", nxt, j + 1, w * rows[nxt - 1]["dm"] * prior(nxt, j + 1))
+            if j + 1 <= H:
+                walk("D", r, j + 1, w * rows[r - 1]["dd"])
+
+    for j0 in range(H + 1):
+        walk("D", 0, j0, 1.0 / H)
+    return tot[0]
 
 
 def test_band_follow_hold_rule(oracle, golden):
