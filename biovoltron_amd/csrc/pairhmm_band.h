@@ -21,9 +21,15 @@ constexpr int kXcdCount = 8;   // MI355X accelerator complex dies (workgroups go
 constexpr int kBandChunk = 8;  // consecutive units per wavefront assignment (short-read kernel)
 constexpr int kLongStartLift = 16;// long-read start band: the anchor sits this far above lane 0
                                   // less than the centre (room above the alignment)
+constexpr int kQuadPairs = 4;     // wide band, quad shape: pairs of one read per wavefront
+constexpr int kQuadLanes = 16;    // lanes per pair (one DPP row)
+constexpr int kQuadCells = 4;     // diagonals per lane
+static_assert(kQuadLanes * kQuadCells == kBandLanes, "a quad pair covers the wide band");
+static_assert(kQuadCells == kBandEdge, "an edge zone is one lane of the quad shape");
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
-// list items 2u and 2u+1 share a read; hap < 0 marks padding.
+// list items 2u and 2u+1 share a read, in the wide list of the quad shape
+// items 4u..4u+3; hap < 0 marks padding.
 struct alignas(8) BandItem {
   int32_t read;
   int32_t hap;
@@ -88,6 +94,10 @@ struct BArgs {
   // short-read kernel: selector window columns per read class (dynamic LDS),
   // at least the longest banded read + kBandLanes + 2 * kBandSlide
   int sel_cols;
+  // wide band shape: 1 = one pair per wavefront (one diagonal per lane),
+  // kQuadPairs = four pairs of one read per wavefront (four diagonals per
+  // lane); the dispatch writes the wide list in groups of this many per task
+  int wide_pairs;
 };
 
 // Dynamic LDS bytes of the short-read banded kernel for `pairs` pairs per

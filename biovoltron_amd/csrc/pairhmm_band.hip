@@ -225,7 +225,8 @@ __device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h,
 
 // Splits every task's pairs into the two banded lists and full-matrix tasks.
 // One thread per task; appends are reserved once per wavefront.  A task's
-// narrow items are written in pairs (padded with an item of hap -1).
+// narrow items are written in pairs, and with the quad shape its wide items
+// in fours (padded with items of hap -1).
 __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -247,8 +248,10 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
     }
   }
   const int n32p = (n32 + 1) & ~1;
+  // the quad shape takes a task's wide pairs four at a time (padded)
+  const int n64p = a.wide_pairs == kQuadPairs ? (n64 + kQuadPairs - 1) & ~(kQuadPairs - 1) : n64;
   int t64 = 0, t32 = 0, tf = 0, tfl = 0;
-  const int o64 = wave_excl_scan(n64, lane, t64);
+  const int o64 = wave_excl_scan(n64p, lane, t64);
   const int o32 = wave_excl_scan(n32p, lane, t32);
   const int of = wave_excl_scan(nf, lane, tf);
   const int ofl = wave_excl_scan(nfl, lane, tfl);
@@ -279,6 +282,7 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
       }
     }
     if (n32 & 1) a.items32[b32] = BandItem{tk.read, -1, 0, 0, -1};
+    for (int q = n64; q < n64p; ++q) a.items[b64++] = BandItem{tk.read, -1, 0, 0, -1};
   }
 }
 
@@ -573,6 +577,401 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   if (lane == 0) {
     if (acc_w) atomicAdd(a.accepted, acc_w);
     if (cells_w && a.cells_done) atomicAdd(a.cells_done, cells_w);
+  }
+}
+
+// ---- the wide band, quad shape: four pairs of one read per wavefront --------
+//
+// Pair slot p (0..3) owns the 16-lane DPP row p of the wavefront; lane k of
+// that row holds the four diagonals o + 4k + c (c = 0..3) of its pair's
+// 64-diagonal band, one read row per step as above.  What changes is the D
+// recurrence along a row, D_d = my*M_{d-1} + yy*D_{d-1}: each lane first runs
+// it along its own four cells (a chain of three FMAs from the carry-in of the
+// lane below), the lanes' top values are then combined by a decayed scan over
+// the 16 lanes of the row (four row_shr levels with factors yy^4, yy^8, yy^16,
+// yy^32; no row_bcast: the pairs of a wavefront never exchange data), and one
+// FMA per cell adds the carry from the lanes below.  That is 3.5 VALU per
+// cell for D against 6, and I crosses a lane only for a lane's top cell.  A
+// lane is exactly one edge zone (kQuadCells == kBandEdge): lane 0 of a row
+// holds the band's 4 lowest diagonals, lane 15 its 4 highest.
+//
+// Touch mass with fewer overrides.  T equals F on active edge-zone cells.
+// When a zone was active on the row before, the cells of its lane compute
+// T = F by themselves (the same operations on the same inputs), except for
+// the inputs that come from a lane outside the zone: the top cell's I of
+// lane 0 (from lane 1) and the D values of lane 15 (carried up from below).
+// A row after the start slide therefore overrides those five values only.
+// The left zone is active on a suffix of the rows (from where the band's
+// left edge column enters the matrix) and the right zone on a prefix, so
+// the first row, the slide rows and the row where a left zone turns on take
+// every override, exactly as the one-pair kernel does; T is then the same,
+// bit for bit, as with every override on every row.
+namespace {
+
+constexpr int kRowShr1 = kRowShr + 1;
+constexpr int kRowShl1 = 0x101;  // row_shl:1: lane k <- lane k+1 inside a 16-lane row, lane 15 <- 0
+constexpr int kQuadChunk = 2;    // quad units per claim
+
+// The two lane-crossing FMAs of a step that read the row before or the
+// products just formed, for F and T: the top cell's I, r += dpp_shl1(m0)*mx
+// (r arrives as dpp_shl1(i0)*xx; lane 15 gets 0, the diagonal above the band
+// being outside it), and the first link of the D chain, y0 += dpp_shr1(y3)*yy
+// (lane 0 has no lane below).  s_nop 1 first: a DPP operand may have been
+// written by the instruction before.
+__device__ __forceinline__ void quad_cross(float& r, float& tr, float& y0, float& ty0, float m0, float tm0,
+                                           float y3, float ty3, float mx, float yy) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %4, %8 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %1, %5, %8 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %2, %6, %9 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_fmac_f32_dpp %3, %7, %9 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(r), "+v"(tr), "+v"(y0), "+v"(ty0)
+      : "v"(m0), "v"(tm0), "v"(y3), "v"(ty3), "v"(mx), "v"(yy));
+}
+
+// The decayed scan of the lanes' top D values over each 16-lane row, for F
+// (x) and T (tx): x_k += yy^4 x_{k-1}, += yy^8 x_{k-2}, += yy^16 x_{k-4},
+// += yy^32 x_{k-8}; then the carry from the lanes below into cells 1 and 2,
+// d1 += yy^2 x_{k-1}, d2 += yy^3 x_{k-1} (lane 0 has none).  F and T
+// alternate inside each level, with one s_nop 0 between levels, so every DPP
+// read of a level's result comes two wait states after its write.
+__device__ __forceinline__ void quad_scan(float& x, float& tx, float& d1, float& td1, float& d2, float& td2,
+                                          float y4, float y8, float y16, float y32, float y2, float y3) {
+#define BVQ_LVL(CTRL, Y)                                                             \
+  "v_fmac_f32_dpp %0, %0, " Y " " CTRL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+  "v_fmac_f32_dpp %1, %1, " Y " " CTRL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+  asm volatile("s_nop 1\n\t"
+               BVQ_LVL("row_shr:1", "%6") "s_nop 0\n\t"
+               BVQ_LVL("row_shr:2", "%7") "s_nop 0\n\t"
+               BVQ_LVL("row_shr:4", "%8") "s_nop 0\n\t"
+               BVQ_LVL("row_shr:8", "%9") "s_nop 1\n\t"
+               "v_fmac_f32_dpp %2, %0, %10 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+               "v_fmac_f32_dpp %3, %1, %10 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+               "v_fmac_f32_dpp %4, %0, %11 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+               "v_fmac_f32_dpp %5, %1, %11 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+               : "+v"(x), "+v"(tx), "+v"(d1), "+v"(td1), "+v"(d2), "+v"(td2)
+               : "v"(y4), "v"(y8), "v"(y16), "v"(y32), "v"(y2), "v"(y3));
+#undef BVQ_LVL
+}
+
+// Sum over each 16-lane row; lane 15 of the row holds the total.
+__device__ __forceinline__ float row_total(float v) {
+  v += dpp<kRowShr + 1>(v);
+  v += dpp<kRowShr + 2>(v);
+  v += dpp<kRowShr + 4>(v);
+  v += dpp<kRowShr + 8>(v);
+  return v;
+}
+
+// Selector f16 units per pair slot: four class rows of sel_cols, plus two
+// (4 bytes) so that the slots' windows start on different LDS banks.
+__device__ __host__ constexpr int quad_sel_stride(int sel_cols) { return 4 * sel_cols + 2; }
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
+  RowRec* tab = reinterpret_cast<RowRec*>(band_lds);
+  const int C = a.sel_cols;
+  const int lane = threadIdx.x;
+  const int p = lane >> 4;  // pair slot: the DPP row
+  const int k = lane & 15;  // lane in the row: diagonals o + 4k .. o + 4k + 3
+  const bool left_lane = k == 0;
+  const bool right_lane = k == kQuadLanes - 1;
+  const int n_units = *a.item_count / kQuadPairs;
+  const int n_chunks = (n_units + kQuadChunk - 1) / kQuadChunk;
+  int accepted = 0;
+  unsigned long long cells = 0;
+  // chunks claimed from a counter per XCD, as in band_forward_kernel
+  const bool by_xcd = gridDim.x % kXcdCount == 0;
+  const int xcd = by_xcd ? static_cast<int>(blockIdx.x % kXcdCount) : 0;
+  int* const claim = a.claims + xcd;
+  _Float16* const selL = reinterpret_cast<_Float16*>(band_lds + kTabBytes) + p * quad_sel_stride(C);  // [4][C]
+  for (;;) {
+    int cl = 0;
+    if (lane == 0) cl = atomicAdd(claim, 1);
+    cl = __builtin_amdgcn_readfirstlane(__shfl(cl, 0));
+    const int chunk = by_xcd ? xcd + kXcdCount * cl : cl;
+    if (chunk >= n_chunks) break;
+    const int u_end = min(n_units, (chunk + 1) * kQuadChunk);
+    for (int u = chunk * kQuadChunk; u < u_end; ++u) {
+      const BandItem w = a.items[kQuadPairs * u + p];
+      const BandItem w0 = a.items[kQuadPairs * u];  // slot 0 of a unit is never padding
+      const bool live = w.hap >= 0;
+      const int read = __builtin_amdgcn_readfirstlane(w0.read);
+      const int hap = live ? w.hap : w0.hap;  // padding mirrors slot 0
+      const int o_lo = live ? w.o_lo : w0.o_lo;
+      const int64_t ro = a.read_off[read];
+      const int R = static_cast<int>(a.read_off[read + 1] - ro);
+      const int64_t ho = a.hap_off[hap];
+      const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+      // start slide, as in band_forward_kernel<64>: off(r) is the offset the
+      // schedule asks for on row r
+      const int Pt = min(kBandSlide, max(0, (H - 1) - (o_lo + kBandLanes - 1)));
+      auto off = [&](int r) { return o_lo + max(0, Pt + 1 - r); };
+      const int wb = o_lo;  // window slot s is haplotype column wb + 1 + s
+
+      __syncthreads();  // the previous unit's LDS reads are done
+      for (int s = k; s < R + kBandLanes + Pt; s += kQuadLanes) {
+        const int j = wb + 1 + s;
+        const uint32_t m = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) selL[q * C + s] = ((m >> q) & 1u) ? _Float16(1) : _Float16(0);
+      }
+
+      const int cq0 = a.gcp[ro] & 127;
+      const float yy = a.ph2pr[cq0];
+      const float gx = 1.f - a.ph2pr[cq0];
+      const double igx = 1.0 / static_cast<double>(gx);
+      const float yy2 = yy * yy, yy3 = yy2 * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4, yy16 = yy8 * yy8,
+                  yy32 = yy16 * yy16;
+
+      // Row 0: M = I = 0, D = K/H for columns 0..H (times g); T = F on the
+      // edge zones active there.
+      int o_cur = off(1);
+      const _Float16* sl = selL + (4 * k + o_cur - wb);  // this lane's cell 0 slot, less the row's part
+      float M[4], I[4], D[4], TM[4], TI[4], TD[4];
+      {
+        const float d0 = gx * (kK / static_cast<float>(H));
+        const bool ez0 = (left_lane && o_cur - 1 >= 0) || (right_lane && o_cur + kBandLanes <= H);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int d = o_cur + 4 * k + c;
+          M[c] = I[c] = TM[c] = TI[c] = 0.f;
+          D[c] = (d >= 0 && d <= H) ? d0 : 0.f;
+          TD[c] = ez0 ? D[c] : 0.f;
+        }
+      }
+      float xg = 0.f;  // exit runs through the bottom lane, as in band_forward_kernel
+
+      for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
+        __syncthreads();
+        {
+          const int r = i0 + lane;
+          if (r <= R) {
+            const int64_t q = ro + r - 1;
+            const uint8_t rc = a.rclass[q];
+            const int iq = a.ins[q] & 127, dq = a.del[q] & 127;
+            const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
+            const float e = a.ph2pr[a.q[q] & 127];
+            RowRec rr;
+            const float pm = 1.f - e;
+            rr.pe = (rc & kWildcard) ? pm : e / 3.f;
+            rr.dp = pm - rr.pe;
+            rr.mm = a.mm[((hi * (hi + 1)) >> 1) + lo];
+            rr.mx = gx * a.ph2pr[iq];
+            rr.xx = yy;
+            rr.my = gx * a.ph2pr[dq];
+            rr.selo = static_cast<uint32_t>(2 * ((rc & 3) * C + r - 1));
+            rr.pad = 0.f;
+            tab[lane] = rr;
+          }
+        }
+        __syncthreads();
+        const int n = __builtin_amdgcn_readfirstlane(min(kBandLanes, R - i0 + 1));
+
+        // Row v of the block.  kFull: every edge-zone override (else the
+        // five of a steady row, given ez_from / ez_to); kExit: 0 no exit-run
+        // term, 1 the row is in the exit zone, 2 check.
+        int ez_from = INT_MAX, ez_to = INT_MIN;
+        auto row = [&](auto kFull, auto kExit, int v) {
+          constexpr bool full = decltype(kFull)::value;
+          constexpr int kX = decltype(kExit)::value;
+          const int i = i0 + v;
+          const RowRec rr = tab[v];
+          const char* sb = reinterpret_cast<const char*>(sl) + rr.selo;
+          float pr[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            pr[c] = __builtin_fmaf(static_cast<float>(*reinterpret_cast<const _Float16*>(sb + 2 * c)), rr.dp, rr.pe);
+          if (kX == 1 || (kX == 2 && i > R - kEdgeRunRows)) {
+            const int c0 = i - 1 + o_cur;  // the bottom cell's column on row i - 1
+            const float xe = __builtin_fmaf(M[0], rr.mx, I[0] * rr.xx);
+            xg = __builtin_fmaf(xg, yy, (c0 >= 1 && c0 <= H) ? xe : 0.f);
+          }
+          bool ez = false;
+          if constexpr (full) {
+            const bool lon = i + o_cur >= 2, ron = i + o_cur + kBandLanes <= H;
+            ez = (left_lane && lon) || (right_lane && ron);
+          }
+          float Mn[4], In[4], TMn[4], TIn[4], Y[4], TY[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            Mn[c] = __builtin_fmaf(M[c], rr.mm, I[c] + D[c]) * pr[c];
+            TMn[c] = __builtin_fmaf(TM[c], rr.mm, TI[c] + TD[c]) * pr[c];
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            In[c] = __builtin_fmaf(M[c + 1], rr.mx, I[c + 1] * rr.xx);
+            TIn[c] = __builtin_fmaf(TM[c + 1], rr.mx, TI[c + 1] * rr.xx);
+          }
+          In[3] = dpp<kRowShl1>(I[0]) * rr.xx;
+          TIn[3] = dpp<kRowShl1>(TI[0]) * rr.xx;
+          if constexpr (full) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              if (ez) TMn[c] = Mn[c];
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            Y[c] = Mn[c] * rr.my;
+            TY[c] = TMn[c] * rr.my;
+          }
+          quad_cross(In[3], TIn[3], Y[0], TY[0], M[0], TM[0], Y[3], TY[3], rr.mx, yy);
+          // D chain along the lane: d1 = Y0 + yy*carry (done above), d2, d3
+          float d1 = Y[0], td1 = TY[0];
+          float d2 = __builtin_fmaf(d1, yy, Y[1]), td2 = __builtin_fmaf(td1, yy, TY[1]);
+          float x = __builtin_fmaf(d2, yy, Y[2]), tx = __builtin_fmaf(td2, yy, TY[2]);
+          quad_scan(x, tx, d1, td1, d2, td2, yy4, yy8, yy16, yy32, yy2, yy3);
+          // cell 0 of the lane above: Y3 + yy*D3 of this lane
+          const float z = __builtin_fmaf(x, yy, Y[3]), tz = __builtin_fmaf(tx, yy, TY[3]);
+          const float Dn0 = dpp<kRowShr1>(z);
+          float TDn0 = dpp<kRowShr1>(tz);
+          float TDn1 = td1, TDn2 = td2, TDn3 = tx;
+          if constexpr (full) {
+            if (ez) {
+#pragma unroll
+              for (int c = 0; c < 4; ++c) TIn[c] = In[c];
+              TDn0 = Dn0;
+              TDn1 = d1;
+              TDn2 = d2;
+              TDn3 = x;
+            }
+          } else {
+            if (i >= ez_from) TIn[3] = In[3];  // left lanes whose zone is on
+            if (i <= ez_to) {                  // right lanes whose zone is on
+              TDn0 = Dn0;
+              TDn1 = d1;
+              TDn2 = d2;
+              TDn3 = x;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            M[c] = Mn[c];
+            I[c] = In[c];
+            TM[c] = TMn[c];
+            TI[c] = TIn[c];
+          }
+          D[0] = Dn0;
+          D[1] = d1;
+          D[2] = d2;
+          D[3] = x;
+          TD[0] = TDn0;
+          TD[1] = TDn1;
+          TD[2] = TDn2;
+          TD[3] = TDn3;
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        using X0 = std::integral_constant<int, 0>;
+        using X1 = std::integral_constant<int, 1>;
+        using X2 = std::integral_constant<int, 2>;
+
+        // The first row and the start slide: every override, and a pair
+        // whose schedule asks for it moves down one diagonal (lane k takes
+        // the cells one diagonal below, the top cell is dropped) unless the
+        // top cell holds more than kBandHold of the pair's row mass.
+        int v = 0;
+        while (v < n && (i0 + v == 1 || __builtin_amdgcn_ballot_w64(o_cur > o_lo) != 0)) {
+          const int i = i0 + v;
+          // straight-line code (no branch around the test or the move):
+          // conditional state changes make the compiler copy the whole
+          // state at the loop's end
+          const bool want = off(i) < o_cur;
+          const float ig = static_cast<float>(igx);
+          float wl = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) wl += __builtin_fmaf(I[c] + D[c], ig, M[c]);
+          const float wtop = __builtin_fmaf(I[3] + D[3], ig, M[3]);
+          const float tot = __shfl(row_total(wl), lane | 15);
+          const float top = __shfl(wtop, lane | 15);
+          const bool go = want && top <= kBandHold * tot;
+          auto shift = [&](float* s) {
+            const float below = dpp<kRowShr1>(s[3]);
+            s[3] = go ? s[2] : s[3];
+            s[2] = go ? s[1] : s[2];
+            s[1] = go ? s[0] : s[1];
+            s[0] = go ? below : s[0];
+          };
+          shift(M);
+          shift(I);
+          shift(D);
+          shift(TM);
+          shift(TI);
+          shift(TD);
+          o_cur -= go ? 1 : 0;
+          sl -= go ? 1 : 0;
+          row(T_{}, X2{}, v);
+          ++v;
+        }
+        // Steady rows: the overrides of a zone that was on the row before;
+        // full rows where a left zone turns on (row 2 - o of its pair).
+        ez_from = left_lane ? 2 - o_cur : INT_MAX;
+        ez_to = right_lane ? H - o_cur - kBandLanes : INT_MIN;
+        const int act0 = __builtin_amdgcn_readlane(2 - o_cur, 0);
+        const int act1 = __builtin_amdgcn_readlane(2 - o_cur, 16);
+        const int act2 = __builtin_amdgcn_readlane(2 - o_cur, 32);
+        const int act3 = __builtin_amdgcn_readlane(2 - o_cur, 48);
+        auto steady = [&](auto kExit, int vb, int ve) {
+          while (vb < ve) {
+            const int i = i0 + vb;
+            int na = INT_MAX;  // the next row at or after i where a left zone turns on
+            if (act0 >= i) na = min(na, act0);
+            if (act1 >= i) na = min(na, act1);
+            if (act2 >= i) na = min(na, act2);
+            if (act3 >= i) na = min(na, act3);
+            const int stop = static_cast<int>(min(static_cast<int64_t>(ve), static_cast<int64_t>(na) - i0));
+            // two rows per iteration: the state returns to the same registers
+            for (; vb + 1 < stop; vb += 2) {
+              row(F_{}, kExit, vb);
+              row(F_{}, kExit, vb + 1);
+            }
+            if (vb < stop) {
+              row(F_{}, kExit, vb);
+              ++vb;
+            }
+            if (vb < ve) {
+              row(T_{}, X2{}, vb);
+              ++vb;
+            }
+          }
+        };
+        const int vx = min(n, max(v, R - kEdgeRunRows + 1 - i0));  // first row of the exit zone
+        steady(X0{}, v, vx);
+        steady(X1{}, vx, n);
+      }
+      // S = sum over the last row of M + I, columns 1..H (I carried times g);
+      // lane 15 of the row gets its pair's totals.
+      const float ig = static_cast<float>(igx);
+      float sS = 0.f, sT = 0.f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = R + o_cur + 4 * k + c;
+        if (j >= 1 && j <= H) {
+          sS += __builtin_fmaf(I[c], ig, M[c]);
+          sT += __builtin_fmaf(TI[c], ig, TM[c]);
+        }
+      }
+      if (left_lane) sT += xg * ig;
+      const float S = row_total(sS);
+      const float Tm = row_total(sT);
+      if (right_lane && live) {
+        cells += static_cast<unsigned long long>(R) * kBandLanes;
+        if (S >= kMinAccepted && Tm <= static_cast<float>(a.tol) * S) {
+          a.out[w.out] = log10_pos(S) - kLog10K;
+          a.form[w.out] = kFormBandFP32;
+          ++accepted;
+        } else {
+          const int q = atomicAdd(a.full_count, 1);
+          a.full[q] = Task{read, hap, 1, -2, w.out};
+        }
+      }
+    }
+  }
+  if (right_lane) {
+    if (accepted) atomicAdd(a.accepted, accepted);
+    if (cells && a.cells_done) atomicAdd(a.cells_done, cells);
   }
 }
 
@@ -892,20 +1291,32 @@ static size_t band_lds_bytes(int pairs, int sel_cols) {
   return static_cast<size_t>(kTabBytes) + static_cast<size_t>(pairs) * 4 * sel_cols * sizeof(_Float16);
 }
 
+static size_t quad_lds_bytes(int sel_cols) {
+  return static_cast<size_t>(kTabBytes) + static_cast<size_t>(kQuadPairs) * quad_sel_stride(sel_cols) * sizeof(_Float16);
+}
+
 hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st) {
   if (a.sel_cols < band_sel_cols(0) || a.sel_cols > band_sel_cols(kBandReadMax)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(grid), dim3(kBandLanes),
-                     band_lds_bytes(1, a.sel_cols), st, a);
+  if (a.wide_pairs == kQuadPairs)
+    hipLaunchKernelGGL(band_quad_kernel, dim3(grid), dim3(kBandLanes), quad_lds_bytes(a.sel_cols), st, a);
+  else if (a.wide_pairs == 1)
+    hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(grid), dim3(kBandLanes),
+                       band_lds_bytes(1, a.sel_cols), st, a);
+  else
+    return hipErrorInvalidValue;
   hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid), dim3(kBandLanes),
                      band_lds_bytes(2, a.sel_cols), st, a);
   return hipGetLastError();
 }
 
-int band_blocks_per_cu(int sel_cols) {
+int band_blocks_per_cu(int sel_cols, int wide_pairs) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel<kBandLanes>, kBandLanes,
-                                                   band_lds_bytes(1, sel_cols)) != hipSuccess)
-    return 8;
+  const hipError_t e =
+      wide_pairs == kQuadPairs
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_quad_kernel, kBandLanes, quad_lds_bytes(sel_cols))
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel<kBandLanes>, kBandLanes,
+                                                         band_lds_bytes(1, sel_cols));
+  if (e != hipSuccess) return 8;
   return n > 0 ? n : 1;
 }
 

@@ -22,7 +22,7 @@
 namespace bvp {
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
 hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st);
-int band_blocks_per_cu(int sel_cols);
+int band_blocks_per_cu(int sel_cols, int wide_pairs);
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
 int band_long_blocks_per_cu();
 hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, const int64_t* htab_off,
@@ -271,6 +271,7 @@ struct bv_plan {
   int grid32 = 0, grid64 = 0, grid_band = 0, grid_long = 1;
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
   int sel_cols = 0;  // short-read banded kernel: selector columns per class
+  int wide_pairs = kQuadPairs;  // wide-band shape: pairs per wavefront (1 or kQuadPairs)
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -489,11 +490,20 @@ hipError_t launch_features(bv_plan* p) {
   return launch_band_features(f, grid, p->stream);
 }
 
+// The banded kernels' persistent grid: the wide kernel's occupancy (for the
+// plan's shape and selector window) on every CU, at most one wavefront per pair.
+int band_grid_cap(const bv_plan* p) {
+  return static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * band_blocks_per_cu(p->sel_cols, p->wide_pairs),
+                           std::max<int64_t>(1, p->n_pairs))));
+}
+
 int ensure_band_lists(bv_plan* p) {
   const int64_t n = std::max<int64_t>(1, p->n_pairs);
-  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
+  // wide items are padded to a multiple of kQuadPairs per task (quad
+  // shape), narrow items to an even count per task
+  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * (n + (kQuadPairs - 1) * int64_t{p->n_tasks})) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
-  // narrow items are padded to an even count per read
   if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_tasks)) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_fallback_long &&
@@ -590,6 +600,7 @@ BArgs band_args(const bv_plan* p) {
   b.verify_above = kVerifyAbove;
   b.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
   b.sel_cols = p->sel_cols;
+  b.wide_pairs = p->wide_pairs;
   return b;
 }
 
@@ -1070,8 +1081,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid64 = opt.grid_f64 > 0 ? opt.grid_f64 : g_ctx.cus * bpc64;
   // the FP32 passes also run the full-matrix and verify lists (up to one task per pair)
   p->grid32 = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p->grid32, std::max<int64_t>(1, p->n_pairs))));
-  p->grid_band = static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * band_blocks_per_cu(p->sel_cols), p->n_pairs)));
+  p->grid_band = band_grid_cap(p);
   const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
   p->grid64 = std::max(1, static_cast<int>(std::min<int64_t>(p->grid64, std::max<int64_t>(1, max_fp64_tasks))));
   p->grid32_max = p->grid32;
@@ -1148,10 +1158,29 @@ int bv_plan_autotune(bv_plan* p, int iters, int32_t* grids_out) {
   if (!p || iters <= 0) return fail(BV_EINVAL, "bad plan or iteration count");
   HIP_TRY(hipSetDevice(g_ctx.device));
   int* knobs[4] = {&p->grid_band, &p->grid32, &p->grid64, &p->grid_long};
-  const int caps[4] = {p->grid_band_max, p->grid32_max, p->grid64_max, p->grid_long_max};
   float best_ms = 0.f;
   int rc = bv_plan_time(p, iters, &best_ms);
   if (rc) return rc;
+  // The wide band's shape (pairs per wavefront) first, each at its own
+  // occupancy-derived grid; then the grids of the chosen shape.
+  if (p->has_band) {
+    const int cur = p->wide_pairs;
+    for (int wp : {1, kQuadPairs}) {
+      if (wp == p->wide_pairs) continue;
+      const int keep = p->wide_pairs, keep_grid = p->grid_band;
+      if ((rc = bv_plan_set_band_shape(p, wp))) return rc;
+      float ms = 0.f;
+      if ((rc = bv_plan_time(p, iters, &ms))) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+      } else {
+        if ((rc = bv_plan_set_band_shape(p, keep))) return rc;
+        p->grid_band = keep_grid;
+      }
+    }
+    (void)cur;
+  }
+  const int caps[4] = {p->grid_band_max, p->grid32_max, p->grid64_max, p->grid_long_max};
   // Coordinate search: each persistent grid over fractions of its
   // occupancy-derived maximum, timing whole runs on the plan's own data.
   for (int k = 0; k < 4; ++k) {
@@ -1192,6 +1221,26 @@ int bv_plan_set_grids(bv_plan* p, const int32_t* grids) {
     if (grids[k] > 0) *knobs[k] = std::min(grids[k], std::max(1, caps[k]));
   }
   drop_graph(p);
+  return BV_OK;
+}
+
+int bv_plan_set_band_shape(bv_plan* p, int32_t wide_pairs) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  if (wide_pairs == 0) wide_pairs = kQuadPairs;
+  if (wide_pairs != 1 && wide_pairs != kQuadPairs)
+    return fail(BV_EINVAL, "the wide band takes 1 or 4 pairs per wavefront");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (wide_pairs != p->wide_pairs) {
+    p->wide_pairs = wide_pairs;
+    p->grid_band = p->grid_band_max = band_grid_cap(p);
+    drop_graph(p);
+  }
+  return BV_OK;
+}
+
+int bv_plan_band_shape(bv_plan* p, int32_t* wide_pairs) {
+  if (!p || !wide_pairs) return fail(BV_EINVAL, "NULL argument");
+  *wide_pairs = p->wide_pairs;
   return BV_OK;
 }
 

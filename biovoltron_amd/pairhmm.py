@@ -84,7 +84,8 @@ EXPORTS = (
     "bv_pairhmm_init", "bv_pairhmm_shutdown", "bv_pairhmm_last_error", "bv_pairhmm_device_info",
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
-    "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_postprocess", "bv_sw_align",
+    "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_set_band_shape", "bv_plan_band_shape",
+    "bv_plan_postprocess", "bv_sw_align",
     "bv_pairhmm_batch",
 )
 
@@ -115,6 +116,8 @@ def library() -> C.CDLL:
         lib.bv_plan_stats_get.argtypes = [vp, C.POINTER(PlanStats)]
         lib.bv_plan_autotune.argtypes = [vp, i32, vp]
         lib.bv_plan_set_grids.argtypes = [vp, vp]
+        lib.bv_plan_set_band_shape.argtypes = [vp, i32]
+        lib.bv_plan_band_shape.argtypes = [vp, vp]
         lib.bv_plan_postprocess.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
@@ -255,11 +258,22 @@ class Plan:
         return out, form
 
     def autotune(self, iters: int = 1) -> dict:
-        """Picks the persistent grids (banded, FP32, FP64, long-read band) by
-        timing runs of this plan; returns them."""
+        """Picks the wide band's shape (pairs per wavefront) and the
+        persistent grids (banded, FP32, FP64, long-read band) by timing runs
+        of this plan; returns them."""
         g = (C.c_int32 * 4)()
         _check(self._lib.bv_plan_autotune(self._h, int(iters), g))
-        return dict(band=g[0], fp32=g[1], fp64=g[2], long=g[3])
+        return dict(band=g[0], fp32=g[1], fp64=g[2], long=g[3], wide_pairs=self.band_shape())
+
+    def set_band_shape(self, wide_pairs: int) -> None:
+        """Pairs per wavefront of the 64-diagonal banded kernel: 4 (default)
+        or 1 (bv_plan_set_band_shape)."""
+        _check(self._lib.bv_plan_set_band_shape(self._h, int(wide_pairs)))
+
+    def band_shape(self) -> int:
+        v = C.c_int32()
+        _check(self._lib.bv_plan_band_shape(self._h, C.byref(v)))
+        return int(v.value)
 
     def set_grids(self, grids: dict) -> None:
         """Sets persistent grids (keys band, fp32, fp64, long; missing or 0

@@ -327,6 +327,14 @@ int bv_plan_autotune(bv_plan* plan, int iters, int32_t* grids_out);
  * is clamped to the plan's occupancy-derived maximum), e.g. to reuse grids
  * tuned on a smaller plan of the same workload. */
 int bv_plan_set_grids(bv_plan* plan, const int32_t* grids);
+/* Shape of the 64-diagonal banded kernel: pairs per wavefront.  4 (the
+ * default, also for 0): four pairs of one read share a wavefront, 16 lanes
+ * and four diagonals per lane each; 1: one pair, one diagonal per lane.
+ * The two compute the same recurrence and edge-zone test; results agree to
+ * FP32 rounding.  Resets the banded grid to the shape's occupancy-derived
+ * maximum; bv_plan_autotune also searches it.  No reference counterpart. */
+int bv_plan_set_band_shape(bv_plan* plan, int32_t wide_pairs);
+int bv_plan_band_shape(bv_plan* plan, int32_t* wide_pairs);
 /* The band table of the last run (n_pairs x 2 int32). */
 int bv_plan_bands(bv_plan* plan, int32_t* bands_out);
 /* Training labels (FP64 forward-backward on the host), lo_hi [n_pairs][2]:

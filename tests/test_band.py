@@ -64,8 +64,9 @@ def test_targets_roundtrip():
 
 # ---- GPU ----------------------------------------------------------------------
 
-def run_plan(pb, bands=None, model=None, **kw):
+def run_plan(pb, bands=None, model=None, shape=0, **kw):
     with pairhmm.Plan(pb) as plan:
+        plan.set_band_shape(shape)
         if bands is not None:
             band.set_plan_bands(plan, bands)
         if model is not None:
@@ -399,17 +400,18 @@ def test_slide_schedule_shape():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("width", [64, 32])
-def test_band_kernel_matches_oracle_band_forward(oracle, width):
-    """The banded kernel against the oracle's FP64 restatement of the same
+@pytest.mark.parametrize("width,shape", [(64, 4), (64, 1), (32, 4)])
+def test_band_kernel_matches_oracle_band_forward(oracle, width, shape):
+    """The banded kernels against the oracle's FP64 restatement of the same
     band (likelihood and the touch-mass decision), on bands placed around and
-    across the alignment; the wide band with its end slides."""
+    across the alignment; the wide band with its start slide, in both shapes
+    (four pairs of one read per wavefront, or one)."""
     sel, pb = fixture_batch(step=2)
     lab = band.labels(pb)
     rng = np.random.default_rng(width)
     centre = (lab[:, 0] + lab[:, 1]) // 2 + rng.integers(-width // 2, width // 2 + 1, len(sel))
     bands = np.stack([centre - width // 2, centre - width // 2 + width - 1], 1).astype(np.int32)
-    out, form, st, _ = run_plan(pb, bands=bands)
+    out, form, st, _ = run_plan(pb, bands=bands, shape=shape)
     tol = pairhmm.default_options().band_tol
     n_acc = n_rej = 0
     worst = 0.0
@@ -422,7 +424,8 @@ def test_band_kernel_matches_oracle_band_forward(oracle, width):
         elif form[k] == 3:
             n_rej += 1
             assert ratio > 0.1 * tol, k
-    say(f"width {width}: accepted {n_acc}, rejected {n_rej}, max |dlog10| vs oracle band {worst:.3g}")
+    say(f"width {width}, shape {shape}: accepted {n_acc}, rejected {n_rej}, "
+        f"max |dlog10| vs oracle band {worst:.3g}")
     assert n_acc > 50 and n_rej > 10
     assert worst <= 2e-5
 
@@ -491,15 +494,37 @@ def test_graph_with_long_reads_on_second_stream():
 
 @pytest.mark.gpu
 def test_autotune_changes_speed_not_results():
-    ev = synth.generate(1, seed=synth.SEED_BASE + 78, count=10)
+    """Grids change only the speed (bit-identical results); the wide band's
+    two shapes give the same forms and agree to FP32 rounding (the D scan
+    sums in another order), and so does whatever shape the autotuner picks."""
+    ev = synth.generate(1, seed=synth.SEED_BASE + 78, count=40)
     pb = pairhmm.PackedBatch.from_synth(ev)
     with pairhmm.Plan(pb) as plan:
         band.set_plan_model(plan, band.BandModel.load(band.DEFAULT_MODEL))
-        plan.run()
-        ref, ref_form = plan.results()
+        res = {}
+        for shape in (4, 1):
+            plan.set_band_shape(shape)
+            assert plan.band_shape() == shape
+            plan.run()
+            res[shape] = plan.results()
+            # another grid for the same shape (not a multiple of the XCD
+            # count: one shared claim counter): bit-identical
+            plan.set_grids(dict(band=257))
+            plan.run()
+            o2, f2 = plan.results()
+            assert np.array_equal(o2, res[shape][0]) and np.array_equal(f2, res[shape][1])
+        (o4, f4), (o1, f1) = res[4], res[1]
+        assert np.array_equal(f4, f1)
+        banded = f4 == 2
+        assert banded.sum() > 1000
+        d = np.abs(o4 - o1)
+        assert np.array_equal(o4[~banded], o1[~banded])
+        say(f"shapes 4 vs 1: {banded.sum()} banded pairs, max |dlog10| {d.max():.3g}")
+        assert d.max() <= 1e-5
         grids = plan.autotune(1)
         plan.run()
         out, form = plan.results()
-    say("autotuned grids", grids)
-    assert all(v >= 1 for v in grids.values())
-    assert np.array_equal(out, ref) and np.array_equal(form, ref_form)
+    say("autotuned", grids)
+    assert all(v >= 1 for v in grids.values()) and grids["wide_pairs"] in (1, 4)
+    ref = res[grids["wide_pairs"]]
+    assert np.array_equal(form, ref[1]) and np.abs(out - ref[0]).max() <= 1e-5
