@@ -28,8 +28,8 @@ static_assert(kQuadLanes * kQuadCells == kBandLanes, "a quad pair covers the wid
 static_assert(kQuadCells == kBandEdge, "an edge zone is one lane of the quad shape");
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
-// list items 2u and 2u+1 share a read, in the wide list of the quad shape
-// items 4u..4u+3; hap < 0 marks padding.
+// list items 2u and 2u+1 share a read (hap < 0 marks padding), in the wide
+// list of the quad shape items 4u..4u+3.
 struct alignas(8) BandItem {
   int32_t read;
   int32_t hap;
@@ -54,10 +54,14 @@ struct BArgs {
   const Task* tasks;
   int n_tasks;
   const int2* band;
-  // banded work lists (dispatch output, band kernel input): 64-wide bands,
-  // and 32-wide bands in read-sharing pairs
+  // banded work lists (dispatch output, band kernel input): 64-wide bands
+  // (with the quad shape: groups of four pairs of one read, and in items1
+  // the rest of each read's wide pairs, fewer than four, for the one-pair
+  // kernel), and 32-wide bands in read-sharing pairs
   BandItem* items;
   int* item_count;
+  BandItem* items1;
+  int* item1_count;
   BandItem* items32;
   int* item32_count;
   // pairs for the full-matrix FP32 pass (ineligible or rejected): haplotypes
@@ -74,7 +78,8 @@ struct BArgs {
   uint8_t* form;
   float tol;                       // accept when touch mass <= tol * S
   int* accepted;                   // banded pairs accepted
-  int* claims;                     // [16] chunk claim counters: per XCD, wide then narrow kernel
+  int* claims;                     // [16] chunk claim counters: per XCD, one-pair wide then narrow kernel
+  int* quad_claims;                // [8] the same for the quad kernel
   unsigned long long* cells_done;  // lane-cells computed
   // long reads (FP64 first, one task per pair): the moving-band list and the
   // pairs left to the full-matrix FP64 pass (no band, or band rejected)
@@ -96,7 +101,7 @@ struct BArgs {
   int sel_cols;
   // wide band shape: 1 = one pair per wavefront (one diagonal per lane),
   // kQuadPairs = four pairs of one read per wavefront (four diagonals per
-  // lane); the dispatch writes the wide list in groups of this many per task
+  // lane) for each read's whole groups of four
   int wide_pairs;
 };
 

@@ -223,10 +223,11 @@ __device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h,
 
 }  // namespace
 
-// Splits every task's pairs into the two banded lists and full-matrix tasks.
+// Splits every task's pairs into the banded lists and full-matrix tasks.
 // One thread per task; appends are reserved once per wavefront.  A task's
-// narrow items are written in pairs, and with the quad shape its wide items
-// in fours (padded with items of hap -1).
+// narrow items are written in pairs (padded with an item of hap -1), and
+// with the quad shape its wide items in fours, the last fewer than four to
+// the one-pair kernel's list.
 __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -248,25 +249,32 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
     }
   }
   const int n32p = (n32 + 1) & ~1;
-  // the quad shape takes a task's wide pairs four at a time (padded)
-  const int n64p = a.wide_pairs == kQuadPairs ? (n64 + kQuadPairs - 1) & ~(kQuadPairs - 1) : n64;
-  int t64 = 0, t32 = 0, tf = 0, tfl = 0;
-  const int o64 = wave_excl_scan(n64p, lane, t64);
+  // the quad shape takes a task's wide pairs four at a time; the rest (fewer
+  // than four) go to the one-pair kernel's list
+  const bool quad = a.wide_pairs == kQuadPairs;
+  const int n64q = quad ? n64 & ~(kQuadPairs - 1) : n64;
+  const int n64r = n64 - n64q;
+  int t64 = 0, t64r = 0, t32 = 0, tf = 0, tfl = 0;
+  const int o64 = wave_excl_scan(n64q, lane, t64);
+  const int o64r = wave_excl_scan(n64r, lane, t64r);
   const int o32 = wave_excl_scan(n32p, lane, t32);
   const int of = wave_excl_scan(nf, lane, tf);
   const int ofl = wave_excl_scan(nfl, lane, tfl);
-  int b64 = 0, b32 = 0, bf = 0, bfl = 0;
+  int b64 = 0, b64r = 0, b32 = 0, bf = 0, bfl = 0;
   if (lane == 0) {
     if (t64) b64 = atomicAdd(a.item_count, t64);
+    if (t64r) b64r = atomicAdd(a.item1_count, t64r);
     if (t32) b32 = atomicAdd(a.item32_count, t32);
     if (tf) bf = atomicAdd(a.full_count, tf);
     if (tfl) bfl = atomicAdd(a.full_long_count, tfl);
   }
   b64 = __shfl(b64, 0) + o64;
+  b64r = __shfl(b64r, 0) + o64r;
   b32 = __shfl(b32, 0) + o32;
   bf = __shfl(bf, 0) + of;
   bfl = __shfl(bfl, 0) + ofl;
   if (t < a.n_tasks) {
+    int k64 = 0;  // wide pairs of this task so far
     for (int h = 0; h < tk.nhap; ++h) {
       int2 b;
       const int64_t o = tk.out0 + h;
@@ -274,7 +282,11 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
       if (c) {
         const int w = b.y - b.x + 1;
         const BandItem it{tk.read, tk.hap0 + h, b.x - ((c - w) >> 1), 0, o};
-        if (c == kBandLanes) a.items[b64++] = it; else a.items32[b32++] = it;
+        if (c == kBandLanes) {
+          if (k64++ < n64q) a.items[b64++] = it; else a.items1[b64r++] = it;
+        } else {
+          a.items32[b32++] = it;
+        }
       } else if (a.hap_off[tk.hap0 + h + 1] - a.hap_off[tk.hap0 + h] <= kStackHmax) {
         a.full[bf++] = Task{tk.read, tk.hap0 + h, 1, -1, o};
       } else {
@@ -282,7 +294,6 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
       }
     }
     if (n32 & 1) a.items32[b32] = BandItem{tk.read, -1, 0, 0, -1};
-    for (int q = n64; q < n64p; ++q) a.items[b64++] = BandItem{tk.read, -1, 0, 0, -1};
   }
 }
 
@@ -296,8 +307,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   const int lane = threadIdx.x;
   const int grp = lane / G;     // pair slot of this lane
   const int k = lane % G;       // lane within the slot: diagonal o_lo + k
-  const BandItem* items = G == kBandLanes ? a.items : a.items32;
-  const int n_units = G == kBandLanes ? *a.item_count : (*a.item32_count) / 2;
+  // the wide kernel takes the quad shape's leftovers when that shape is on
+  const bool rest = G == kBandLanes && a.wide_pairs == kQuadPairs;
+  const BandItem* items = G == kBandLanes ? (rest ? a.items1 : a.items) : a.items32;
+  const int n_units = G == kBandLanes ? *(rest ? a.item1_count : a.item_count) : (*a.item32_count) / 2;
   const bool left_lane = k < kBandEdge;
   const bool right_lane = k >= G - kBandEdge;
   const bool bottom_lane = k == 0;
@@ -685,7 +698,7 @@ __global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
   // chunks claimed from a counter per XCD, as in band_forward_kernel
   const bool by_xcd = gridDim.x % kXcdCount == 0;
   const int xcd = by_xcd ? static_cast<int>(blockIdx.x % kXcdCount) : 0;
-  int* const claim = a.claims + xcd;
+  int* const claim = a.quad_claims + xcd;
   _Float16* const selL = reinterpret_cast<_Float16*>(band_lds + kTabBytes) + p * quad_sel_stride(C);  // [4][C]
   for (;;) {
     int cl = 0;
@@ -1295,16 +1308,17 @@ static size_t quad_lds_bytes(int sel_cols) {
   return static_cast<size_t>(kTabBytes) + static_cast<size_t>(kQuadPairs) * quad_sel_stride(sel_cols) * sizeof(_Float16);
 }
 
-hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st) {
+// grid: the wide kernel of the plan's shape; grid1: the one-pair-per-wave
+// kernels that follow it (the quad shape's leftovers, the narrow band).
+hipError_t launch_band_forward(const BArgs& a, int grid, int grid1, hipStream_t st) {
   if (a.sel_cols < band_sel_cols(0) || a.sel_cols > band_sel_cols(kBandReadMax)) return hipErrorInvalidValue;
-  if (a.wide_pairs == kQuadPairs)
-    hipLaunchKernelGGL(band_quad_kernel, dim3(grid), dim3(kBandLanes), quad_lds_bytes(a.sel_cols), st, a);
-  else if (a.wide_pairs == 1)
-    hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(grid), dim3(kBandLanes),
-                       band_lds_bytes(1, a.sel_cols), st, a);
-  else
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid), dim3(kBandLanes),
+  if (a.wide_pairs != 1 && a.wide_pairs != kQuadPairs) return hipErrorInvalidValue;
+  const bool quad = a.wide_pairs == kQuadPairs;
+  if (quad) hipLaunchKernelGGL(band_quad_kernel, dim3(grid), dim3(kBandLanes), quad_lds_bytes(a.sel_cols), st, a);
+  // one pair per wavefront: every wide pair, or the quad shape's leftovers
+  hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(quad ? grid1 : grid), dim3(kBandLanes),
+                     band_lds_bytes(1, a.sel_cols), st, a);
+  hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid1), dim3(kBandLanes),
                      band_lds_bytes(2, a.sel_cols), st, a);
   return hipGetLastError();
 }

@@ -21,7 +21,7 @@
 
 namespace bvp {
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
-hipError_t launch_band_forward(const BArgs& a, int grid, hipStream_t st);
+hipError_t launch_band_forward(const BArgs& a, int grid, int grid1, hipStream_t st);
 int band_blocks_per_cu(int sel_cols, int wide_pairs);
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
 int band_long_blocks_per_cu();
@@ -85,9 +85,11 @@ int fail(int code, const std::string& msg) {
 // log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
-constexpr int kCounters = 56;  // device counter block of a plan (see bv_plan)
+constexpr int kCounters = 72;  // device counter block of a plan (see bv_plan)
 constexpr int kFeatCounters = 24;  // [24..31]: the features kernel's per-XCD task claims
 constexpr int kBandClaims = 32;    // [32..47]: the banded kernels' per-XCD chunk claims
+constexpr int kQuadClaims = 56;    // [56..63]: the quad banded kernel's per-XCD chunk claims
+constexpr int kItem1Count = 64;    // [64]: the quad shape's leftover wide pairs
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -269,6 +271,7 @@ struct bv_plan {
   int hmax = 0;
   int n_tasks = 0;
   int grid32 = 0, grid64 = 0, grid_band = 0, grid_long = 1;
+  int grid_band1 = 1;  // one-pair banded kernels after the quad one (leftovers, narrow band)
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
   int sel_cols = 0;  // short-read banded kernel: selector columns per class
   int wide_pairs = kQuadPairs;  // wide-band shape: pairs per wavefront (1 or kQuadPairs)
@@ -304,6 +307,7 @@ struct bv_plan {
   int64_t n_longhap_pairs = 0;      // short-read pairs with such haplotypes (that list's capacity)
   int2* d_band = nullptr;
   BandItem* d_items = nullptr;    // banded work lists, built on the device each run
+  BandItem* d_items1 = nullptr;   // the quad shape's leftover wide pairs
   BandItem* d_items32 = nullptr;
   BandItem* d_items_long = nullptr;  // long reads on the moving band
   Task* d_long_full = nullptr;       // long reads left to the full-matrix FP64 pass
@@ -372,7 +376,7 @@ void free_plan(bv_plan* p) {
                   p->d_stasks, p->d_tasks_rest, p->d_fallback_long,
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
-                  p->d_items32, p->d_items_long, p->d_long_full,
+                  p->d_items1, p->d_items32, p->d_items_long, p->d_long_full,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off,
                   p->d_region_tab, p->d_hap_allele, p->d_n_alleles, p->d_allele_off, p->d_lnorm,
                   p->d_keep, p->d_allele, p->d_carry32_long, p->d_carry64_long};
@@ -492,17 +496,21 @@ hipError_t launch_features(bv_plan* p) {
 
 // The banded kernels' persistent grid: the wide kernel's occupancy (for the
 // plan's shape and selector window) on every CU, at most one wavefront per pair.
-int band_grid_cap(const bv_plan* p) {
+int band_grid_cap(const bv_plan* p, int wide_pairs) {
   return static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * band_blocks_per_cu(p->sel_cols, p->wide_pairs),
+      1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * band_blocks_per_cu(p->sel_cols, wide_pairs),
                            std::max<int64_t>(1, p->n_pairs))));
 }
 
 int ensure_band_lists(bv_plan* p) {
   const int64_t n = std::max<int64_t>(1, p->n_pairs);
-  // wide items are padded to a multiple of kQuadPairs per task (quad
-  // shape), narrow items to an even count per task
-  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * (n + (kQuadPairs - 1) * int64_t{p->n_tasks})) != hipSuccess)
+  // the quad shape leaves at most kQuadPairs - 1 wide pairs per task to the
+  // one-pair kernel; narrow items are padded to an even count per task
+  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
+  if (!p->d_items1 &&
+      hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, (kQuadPairs - 1) * int64_t{p->n_tasks})) !=
+          hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_tasks)) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
@@ -572,6 +580,8 @@ BArgs band_args(const bv_plan* p) {
   b.band = p->d_band;
   b.items = p->d_items;
   b.item_count = p->d_counters + 10;
+  b.items1 = p->d_items1;
+  b.item1_count = p->d_counters + kItem1Count;
   b.items32 = p->d_items32;
   b.item32_count = p->d_counters + 12;
   b.full = p->d_fallback;
@@ -585,6 +595,7 @@ BArgs band_args(const bv_plan* p) {
   b.tol = p->opt.band_tol;
   b.accepted = p->d_counters + 5;
   b.claims = p->d_counters + kBandClaims;
+  b.quad_claims = p->d_counters + kQuadClaims;
   b.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
   b.tasks64 = p->d_tasks64;
   b.n_tasks64 = p->n_tasks64;
@@ -680,7 +691,7 @@ int enqueue_run(bv_plan* p, bool timed) {
     const BArgs b = band_args(p);
     HIP_TRY(launch_band_dispatch(b, p->stream));
     if (timed) HIP_TRY(hipEventRecord(p->ev[5], p->stream));
-    HIP_TRY(launch_band_forward(b, p->grid_band, p->stream));
+    HIP_TRY(launch_band_forward(b, p->grid_band, p->grid_band1, p->stream));
     if (timed) HIP_TRY(hipEventRecord(p->ev[6], p->stream));
   } else if (fp32) {
     // Pass 1: every pair in FP32 on the full matrix: stacked reads, and the
@@ -1081,7 +1092,8 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid64 = opt.grid_f64 > 0 ? opt.grid_f64 : g_ctx.cus * bpc64;
   // the FP32 passes also run the full-matrix and verify lists (up to one task per pair)
   p->grid32 = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p->grid32, std::max<int64_t>(1, p->n_pairs))));
-  p->grid_band = band_grid_cap(p);
+  p->grid_band = band_grid_cap(p, p->wide_pairs);
+  p->grid_band1 = band_grid_cap(p, 1);
   const int64_t max_fp64_tasks = (opt.mode == BV_MODE_AVX && !opt.force_fp64) ? p->n_pairs : p->n_tasks;
   p->grid64 = std::max(1, static_cast<int>(std::min<int64_t>(p->grid64, std::max<int64_t>(1, max_fp64_tasks))));
   p->grid32_max = p->grid32;
@@ -1232,7 +1244,7 @@ int bv_plan_set_band_shape(bv_plan* p, int32_t wide_pairs) {
   HIP_TRY(hipSetDevice(g_ctx.device));
   if (wide_pairs != p->wide_pairs) {
     p->wide_pairs = wide_pairs;
-    p->grid_band = p->grid_band_max = band_grid_cap(p);
+    p->grid_band = p->grid_band_max = band_grid_cap(p, wide_pairs);
     drop_graph(p);
   }
   return BV_OK;
