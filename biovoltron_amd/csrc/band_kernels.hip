@@ -106,177 +106,245 @@ __global__ __launch_bounds__(256) void hap_table_kernel(const uint8_t* __restric
   }
 }
 
-// One wavefront per task (a read and its region's haplotypes).  Lane l holds
-// one seed (a 12-mer, <= 64 spread over the read) and looks it up in each
-// haplotype's table: the first occurrence gives the seed's hit offset (the
-// anchor is their median), and up to kMaxOcc occurrences feed the offset
-// histogram, so repeat copies show as extra offsets.
+// ---- seed features: helpers shared by the features kernel and the fused
+// band prologue ---------------------------------------------------------------
+
+// The read side of a task: quality summary and this lane's seed (a 12-mer,
+// <= 64 spread over the read).
+struct ReadSeeds {
+  int R;
+  int nvalid;       // seeds without a wildcard base
+  int my_p;         // this lane's seed start, -1 for none
+  uint32_t my_key;  // its 2-bit code
+  float fq_mean, fq_min;
+};
+
+__device__ __forceinline__ ReadSeeds read_seeds(const uint8_t* __restrict__ rclass, const uint8_t* __restrict__ qual,
+                                                int64_t ro, int R, int lane) {
+  ReadSeeds s;
+  s.R = R;
+  double qsum = 0.0;
+  int qmin = 255;
+  for (int p = lane; p < R; p += 64) {
+    const int q = qual[ro + p] & 127;
+    qsum += q;
+    qmin = min(qmin, q);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    qsum += __shfl_xor(qsum, o);
+    qmin = min(qmin, __shfl_xor(qmin, o));
+  }
+  s.fq_mean = static_cast<float>(qsum / R) / 40.f;
+  s.fq_min = static_cast<float>(qmin) / 40.f;
+  const int nseeds = R >= kK ? min(64, R - kK + 1) : 0;
+  const int stride = nseeds > 0 ? max(1, (R - kK + 1 + 63) / 64) : 1;
+  s.my_p = -1;
+  s.my_key = 0;
+  if (lane < nseeds) {
+    s.my_p = lane * stride;
+    if (s.my_p + kK > R) s.my_p = -1;
+    if (s.my_p >= 0) {
+      uint8_t c[kK];
+#pragma unroll
+      for (int k = 0; k < kK; ++k) c[k] = rclass[ro + s.my_p + k];
+      uint8_t wild = 0;
+#pragma unroll
+      for (int k = 0; k < kK; ++k) {
+        wild |= c[k];
+        s.my_key = (s.my_key << 2) | (c[k] & 3);
+      }
+      if (wild & kWildcard) s.my_p = -1;
+    }
+  }
+  s.nvalid = __popcll(__ballot(s.my_p >= 0));
+  return s;
+}
+
+// This lane's seed looked up in the tables of haplotypes hap0 .. hap0 + n - 1
+// (n <= kHapChunk).  Per haplotype: the first occurrence (smallest position)
+// and the offsets q - p of the first kMaxOcc occurrences met along the probe
+// sequence, in occ[c][0 .. nocc[c] - 1], newest first (the histogram counts
+// them as a set, so a shift register serves: an insert is kMaxOcc moves,
+// where indexing by the count took a compare and select per register).
+struct ChunkHits {
+  int first[kHapChunk];
+  int occ[kHapChunk][kMaxOcc];
+  int nocc[kHapChunk];
+};
+
+__device__ __forceinline__ void chunk_lookups(const int64_t* __restrict__ hap_off, const int64_t* __restrict__ htab_off,
+                                              const unsigned long long* __restrict__ htab, int hap0, int n,
+                                              int my_p, uint32_t my_key, ChunkHits& ch) {
+#pragma unroll
+  for (int c = 0; c < kHapChunk; ++c) {
+    ch.first[c] = INT_MAX;
+    ch.nocc[c] = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxOcc; ++k) ch.occ[c][k] = 0;
+    const int hap = hap0 + c;
+    if (c >= n || my_p < 0) continue;
+    if (hap_off[hap + 1] - hap_off[hap] > kBandMaxHap) continue;  // beyond the banded kernel
+    const int64_t t0 = htab_off[hap], t1 = htab_off[hap + 1];
+    if (t1 - t0 < 2) continue;  // no table
+    const unsigned long long* t = htab + t0;
+    const uint32_t mask = static_cast<uint32_t>(t1 - t0 - 1);
+    uint32_t sl = slot_of(my_key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+      const unsigned long long e = t[sl];
+      if (e == kEmpty) break;
+      if (static_cast<uint32_t>(e) == my_key) {
+        const int q = static_cast<int>(e >> 32);
+        ch.first[c] = min(ch.first[c], q);
+        if (ch.nocc[c] < kMaxOcc) {
+#pragma unroll
+          for (int k = kMaxOcc - 1; k > 0; --k) ch.occ[c][k] = ch.occ[c][k - 1];
+          ch.occ[c][0] = q - my_p;
+          ++ch.nocc[c];
+        }
+      }
+      sl = (sl + 1) & mask;
+    }
+  }
+}
+
+// The features of one pair from its lookups: the anchor (median of the
+// seeds' first-occurrence offsets; (H - R) / 2 without hits), this lane's
+// histogram bin count (lane b < kBins counts bin b: 4 bp over
+// [anchor - 64, anchor + 64), every kept occurrence), and the read span
+// covered by seeds hitting within kEndAnchorTol of the anchor ((-1, -1)
+// without such seeds).
+struct PairFeat {
+  int anchor;
+  int cnt;
+  int pf, pl;
+};
+
+__device__ __forceinline__ PairFeat pair_features(int first, const int* occ, int nocc, int my_p, int R, int H,
+                                                  int lane) {
+  PairFeat f;
+  f.anchor = (H - R) / 2;
+  const int found_off = first != INT_MAX ? first - my_p : INT_MIN;
+  // Median of the hit offsets as the anchor: walk the distinct offsets in
+  // ascending order (wave minimum, then the lanes holding it by ballot)
+  // until more than half of the hits are behind -- one or two steps as a
+  // rule, at most one per hit.
+  const bool hit = found_off != INT_MIN;
+  const int nhit = __popcll(__ballot(hit));
+  if (nhit > 0) {
+    const int half = (nhit - 1) / 2;
+    int rest = hit ? found_off : INT_MAX;
+    int behind = 0;
+    for (;;) {
+      const int v = wave_min(rest);
+      behind += __popcll(__ballot(rest == v));
+      if (behind > half) {
+        f.anchor = v;
+        break;
+      }
+      if (rest == v) rest = INT_MAX;
+    }
+  }
+  // Bins by ballot: each round counts the lanes whose occurrence (slot k)
+  // falls in one bin -- as many rounds as distinct bins.
+  int my_cnt = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxOcc; ++k) {
+    int bin = -1;
+    if (k < nocc) {
+      const int b = (occ[k] - f.anchor + 64) >> 2;
+      if (b >= 0 && b < kBins) bin = b;
+    }
+    for (;;) {
+      const unsigned long long live = __ballot(bin >= 0);
+      if (live == 0) break;
+      const int bb = __builtin_amdgcn_readlane(bin, __ffsll(static_cast<long long>(live)) - 1);
+      const unsigned long long same = __ballot(bin == bb);
+      if (lane == bb) my_cnt += __popcll(same);
+      if (bin == bb) bin = -1;
+    }
+  }
+  f.cnt = my_cnt;
+  // read ends anchored by seeds that hit next to the anchor
+  const bool near = hit && abs(found_off - f.anchor) <= kEndAnchorTol;
+  f.pf = wave_min(near ? my_p : INT_MAX);
+  f.pl = wave_max(near ? my_p + kK : -1);
+  if (f.pl < 0) f.pf = f.pl = -1;
+  return f;
+}
+
+// Workgroups go round-robin to the 8 XCDs, each with its own L2.  Tasks are
+// in region order; each XCD takes a contiguous eighth of them, and its
+// wavefronts claim the next kFeatClaim tasks of that eighth from a counter
+// of their own XCD, so that the tasks in flight on an XCD stay a few
+// regions wide and those regions' haplotype tables stay in its L2.  (A fixed
+// stride over the eighth lets fast wavefronts run ahead of slow ones: the
+// tasks in flight then span dozens of regions and nearly every table probe
+// missed the L2.)
+struct TaskClaims {
+  int begin, end, ctr;
+};
+
+__device__ __forceinline__ TaskClaims xcd_task_range(int n_tasks) {
+  TaskClaims c{0, n_tasks, 0};
+  if (gridDim.x % kXcds == 0 && gridDim.x >= kXcds) {
+    c.ctr = blockIdx.x % kXcds;
+    const int chunk = (n_tasks + kXcds - 1) / kXcds;
+    c.begin = min(n_tasks, c.ctr * chunk);
+    c.end = min(n_tasks, c.begin + chunk);
+  }
+  return c;
+}
+
+// The first task of the next claim of this wavefront (wave-uniform), or -1.
+__device__ __forceinline__ int claim_tasks(int* next, const TaskClaims& c, int lane) {
+  int c0 = 0;
+  if (lane == 0) c0 = atomicAdd(next + c.ctr, kFeatClaim);
+  c0 = __builtin_amdgcn_readfirstlane(__shfl(c0, 0));
+  return c0 >= c.end - c.begin ? -1 : c.begin + c0;
+}
+
+// One wavefront per task (a read and its region's haplotypes): features of
+// every pair into X, anchor and ends (the band API's features, and the
+// trainer's inputs).
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
   const int lane = threadIdx.x;
-  // Workgroups go round-robin to the 8 XCDs, each with its own L2.  Tasks are
-  // in region order; each XCD takes a contiguous eighth of them, and its
-  // wavefronts claim the next kFeatClaim tasks of that eighth from a counter
-  // of their own XCD, so that the tasks in flight on an XCD stay a few
-  // regions wide and those regions' haplotype tables stay in its L2.  (A
-  // fixed stride over the eighth lets fast wavefronts run ahead of slow
-  // ones: the tasks in flight then span dozens of regions and nearly every
-  // table probe missed the L2.)
-  int t_begin = 0, t_end = a.n_tasks, ctr = 0;
-  if (gridDim.x % kXcds == 0 && gridDim.x >= kXcds) {
-    ctr = blockIdx.x % kXcds;
-    const int chunk = (a.n_tasks + kXcds - 1) / kXcds;
-    t_begin = min(a.n_tasks, ctr * chunk);
-    t_end = min(a.n_tasks, t_begin + chunk);
-  }
+  const TaskClaims cl = xcd_task_range(a.n_tasks);
   for (;;) {
-    int c0 = 0;
-    if (lane == 0) c0 = atomicAdd(a.next + ctr, kFeatClaim);
-    c0 = __builtin_amdgcn_readfirstlane(__shfl(c0, 0));
-    const int tc0 = t_begin + c0;
-    if (c0 >= t_end - t_begin) break;
-    const int tc1 = min(t_end, tc0 + kFeatClaim);
-  for (int ti = tc0; ti < tc1; ++ti) {
-    const Task tk = a.tasks[ti];
-    const int64_t ro = a.read_off[tk.read];
-    const int R = static_cast<int>(a.read_off[tk.read + 1] - ro);
-    // read quality summary
-    double qsum = 0.0;
-    int qmin = 255;
-    for (int p = lane; p < R; p += 64) {
-      const int q = a.q[ro + p] & 127;
-      qsum += q;
-      qmin = min(qmin, q);
-    }
+    const int tc0 = claim_tasks(a.next, cl, lane);
+    if (tc0 < 0) break;
+    const int tc1 = min(cl.end, tc0 + kFeatClaim);
+    for (int ti = tc0; ti < tc1; ++ti) {
+      const Task tk = a.tasks[ti];
+      const int64_t ro = a.read_off[tk.read];
+      const int R = static_cast<int>(a.read_off[tk.read + 1] - ro);
+      const ReadSeeds rs = read_seeds(a.rclass, a.q, ro, R, lane);
+      const float inv = rs.nvalid > 0 ? 1.f / static_cast<float>(rs.nvalid) : 0.f;
+      for (int h0 = 0; h0 < tk.nhap; h0 += kHapChunk) {
+        ChunkHits ch;
+        chunk_lookups(a.hap_off, a.htab_off, a.htab, tk.hap0 + h0, min(kHapChunk, tk.nhap - h0), rs.my_p,
+                      rs.my_key, ch);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      qsum += __shfl_xor(qsum, o);
-      qmin = min(qmin, __shfl_xor(qmin, o));
-    }
-    const float fq_mean = static_cast<float>(qsum / R) / 40.f;
-    const float fq_min = static_cast<float>(qmin) / 40.f;
-    // this lane's read seed: positions spread over the read, <= 64 seeds
-    const int nseeds = R >= kK ? min(64, R - kK + 1) : 0;
-    const int stride = nseeds > 0 ? max(1, (R - kK + 1 + 63) / 64) : 1;
-    int my_p = -1;
-    uint32_t my_key = 0;
-    if (lane < nseeds) {
-      my_p = lane * stride;
-      if (my_p + kK > R) my_p = -1;
-      if (my_p >= 0) {
-        uint8_t c[kK];
-#pragma unroll
-        for (int k = 0; k < kK; ++k) c[k] = a.rclass[ro + my_p + k];
-        uint8_t wild = 0;
-#pragma unroll
-        for (int k = 0; k < kK; ++k) {
-          wild |= c[k];
-          my_key = (my_key << 2) | (c[k] & 3);
-        }
-        if (wild & kWildcard) my_p = -1;
-      }
-    }
-    const int nvalid = __popcll(__ballot(my_p >= 0));
-    for (int h0 = 0; h0 < tk.nhap; h0 += kHapChunk) {
-      // Table lookups for a chunk of haplotypes first (independent loads in
-      // flight together), then the per-haplotype reductions.
-      int first_c[kHapChunk];
-      int occ_c[kHapChunk][kMaxOcc];
-      int nocc_c[kHapChunk];
-#pragma unroll
-      for (int c = 0; c < kHapChunk; ++c) {
-        first_c[c] = INT_MAX;
-        nocc_c[c] = 0;
-        const int hap = tk.hap0 + h0 + c;
-        if (h0 + c >= tk.nhap || my_p < 0) continue;
-        if (a.hap_off[hap + 1] - a.hap_off[hap] > kBandMaxHap) continue;  // beyond the banded kernel
-        const int64_t t0 = a.htab_off[hap], t1 = a.htab_off[hap + 1];
-        if (t1 - t0 < 2) continue;  // no table
-        const unsigned long long* t = a.htab + t0;
-        const uint32_t mask = static_cast<uint32_t>(t1 - t0 - 1);
-        uint32_t sl = slot_of(my_key, mask);
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-          const unsigned long long e = t[sl];
-          if (e == kEmpty) break;
-          if (static_cast<uint32_t>(e) == my_key) {
-            const int q = static_cast<int>(e >> 32);
-            first_c[c] = min(first_c[c], q);
-            if (nocc_c[c] < kMaxOcc) occ_c[c][nocc_c[c]++] = q - my_p;
+        for (int c = 0; c < kHapChunk; ++c) {
+          const int h = h0 + c;
+          if (h >= tk.nhap) break;
+          const int hap = tk.hap0 + h;
+          const int64_t o = tk.out0 + h;
+          const int H = static_cast<int>(a.hap_off[hap + 1] - a.hap_off[hap]);
+          const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane);
+          float* X = a.X + o * kFeat;
+          if (lane == 0) {
+            X[0] = static_cast<float>(R) / 256.f;
+            X[1] = static_cast<float>(H) / 512.f;
+            X[2] = rs.fq_mean;
+            X[3] = rs.fq_min;
+            a.anchor[o] = f.anchor;
+            a.ends[o] = int2{f.pf, f.pl};
           }
-          sl = (sl + 1) & mask;
+          if (lane < kBins) X[4 + lane] = static_cast<float>(f.cnt) * inv;
         }
-      }
-#pragma unroll
-      for (int c = 0; c < kHapChunk; ++c) {
-      const int h = h0 + c;
-      if (h >= tk.nhap) break;
-      const int hap = tk.hap0 + h;
-      const int64_t o = tk.out0 + h;
-      const int64_t ho = a.hap_off[hap];
-      const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
-      float* X = a.X + o * kFeat;
-      // lengths and quality
-      if (lane == 0) {
-        X[0] = static_cast<float>(R) / 256.f;
-        X[1] = static_cast<float>(H) / 512.f;
-        X[2] = fq_mean;
-        X[3] = fq_min;
-      }
-      int anchor = (H - R) / 2;
-      const int found_off = first_c[c] != INT_MAX ? first_c[c] - my_p : INT_MIN;
-      const int nocc = nocc_c[c];
-      const int* occ = occ_c[c];
-      // Median of the hit offsets as the anchor: walk the distinct offsets in
-      // ascending order (wave minimum, then the lanes holding it by ballot)
-      // until more than half of the hits are behind -- one or two steps as a
-      // rule, at most one per hit.
-      const bool hit = found_off != INT_MIN;
-      const int nhit = __popcll(__ballot(hit));
-      if (nhit > 0) {
-        const int half = (nhit - 1) / 2;
-        int rest = hit ? found_off : INT_MAX;
-        int behind = 0;
-        for (;;) {
-          const int v = wave_min(rest);
-          behind += __popcll(__ballot(rest == v));
-          if (behind > half) {
-            anchor = v;
-            break;
-          }
-          if (rest == v) rest = INT_MAX;
-        }
-      }
-      // 32 bins of 4 bp over [anchor - 64, anchor + 64): lane b < 32 counts
-      // bin b.  Each round counts the lanes whose occurrence (slot k) falls in
-      // one bin, by ballot -- as many rounds as distinct bins.
-      const float inv = nvalid > 0 ? 1.f / static_cast<float>(nvalid) : 0.f;
-      int my_cnt = 0;
-#pragma unroll
-      for (int k = 0; k < kMaxOcc; ++k) {
-        int bin = -1;
-        if (k < nocc) {
-          const int b = (occ[k] - anchor + 64) >> 2;
-          if (b >= 0 && b < kBins) bin = b;
-        }
-        for (;;) {
-          const unsigned long long live = __ballot(bin >= 0);
-          if (live == 0) break;
-          const int bb = __builtin_amdgcn_readlane(bin, __ffsll(static_cast<long long>(live)) - 1);
-          const unsigned long long same = __ballot(bin == bb);
-          if (lane == bb) my_cnt += __popcll(same);
-          if (bin == bb) bin = -1;
-        }
-      }
-      if (lane < kBins) X[4 + lane] = static_cast<float>(my_cnt) * inv;
-      // read ends anchored by seeds that hit next to the anchor
-      const bool near = hit && abs(found_off - anchor) <= kEndAnchorTol;
-      const int pf = wave_min(near ? my_p : INT_MAX), pl = wave_max(near ? my_p + kK : -1);
-      if (lane == 0) {
-        a.anchor[o] = anchor;
-        a.ends[o] = pl >= 0 ? int2{pf, pl} : int2{-1, -1};
-      }
       }
     }
-  }
   }
 }
 
@@ -512,6 +580,275 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
   }
 }
 
+// ---- fused band prologue: features, MLP, band rules and dispatch ------------
+//
+// One pass over the tasks (region order, claimed per XCD as above) that
+// writes the band table and the banded and full-matrix work lists directly:
+// the 36 features, the prediction and the rule inputs of a pair live only in
+// LDS.  Each wavefront buffers the pairs of whole tasks (up to 64 pairs; a
+// task with more haplotypes is taken 64 at a time), then runs the MLP with one
+// pair per lane -- the same FMA sequence per output as mlp_forward_kernel,
+// skipping only histogram inputs that are zero for every buffered pair
+// (adding w * 0 leaves a sum unchanged, and ReLU maps either zero to +0), so
+// predictions and bands are bit for bit those of the separate kernels -- and
+// applies the rules of band_from_prediction_kernel and the list split of
+// band_dispatch_kernel (a task's wide pairs in groups of four for the quad
+// shape, the rest to the one-pair list; narrow pairs padded to an even count
+// per task) with one reservation per list per buffer.
+constexpr int kProWaves = 8;  // wavefronts per workgroup, sharing the weights in LDS
+
+struct ProlArgs {
+  FeatArgs f;
+  BArgs b;           // lists and counters written; b.band unused
+  int2* band;        // [pairs]: the band table (short-read pairs)
+  const float* params;
+  int margin, max_width, allow_narrow;
+};
+
+// Per-wavefront buffer of up to 64 pairs (LDS).  A bin count fits a byte: a
+// seed adds at most kMaxOcc occurrences to one 4-bp bin only if its 12-mer
+// occurs 3 or 4 times within 4 bases (a homopolymer: 4 such keys, and the
+// seeds sharing one share its kept occurrences, so with distinct seed
+// positions each key adds at most 16), and at most 2 otherwise (period 2 or
+// 3), so a bin holds at most 4 * 16 + 2 * 64 = 192.
+struct ProBuf {
+  uint32_t cnt[64][kBins / 4];  // histogram counts, one byte per bin
+  int hap[64], H[64], anchor[64], pf[64], pl[64], read[64], R[64];
+  float inv[64], fqm[64], fqn[64];
+  long long out[64];
+  int gc[64];  // the read's gap continuation is constant
+};
+
+__device__ __forceinline__ unsigned long long below(int n) {
+  return n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+}
+
+// MLP, rules and dispatch for the `fill` buffered pairs; seg_first marks the
+// first pair of each task (or 64-haplotype piece of one).
+__device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* __restrict__ w, ProBuf& B, int fill,
+                                               unsigned long long seg_first, int lane) {
+  const BArgs& a = A.b;
+  const bool valid = lane < fill;
+  const int s = valid ? lane : 0;
+  uint32_t cw[kBins / 4];
+#pragma unroll
+  for (int k = 0; k < kBins / 4; ++k) cw[k] = B.cnt[s][k];
+  const int R = B.R[s], H = B.H[s];
+  const float inv = B.inv[s];
+  // layer 1 (weights transposed in LDS: w1t[k][u]), input-major so that a
+  // histogram bin that is zero for every pair is skipped
+  const float* w1t = w;
+  const float* bias1 = w1t + kFeat * kHid;
+  const float* W2 = bias1 + kHid;
+  const float* b2 = W2 + kHid * kHid;
+  const float* W3 = b2 + kHid;
+  const float* b3 = W3 + 2 * kHid;
+  float h1[kHid];
+#pragma unroll
+  for (int u = 0; u < kHid; ++u) h1[u] = bias1[u];
+  const float x0[4] = {static_cast<float>(R) / 256.f, static_cast<float>(H) / 512.f, B.fqm[s], B.fqn[s]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+#pragma unroll
+    for (int u = 0; u < kHid; ++u) h1[u] = __builtin_fmaf(w1t[k * kHid + u], x0[k], h1[u]);
+  }
+#pragma unroll
+  for (int b = 0; b < kBins; ++b) {
+    const uint32_t cb = (cw[b >> 2] >> (8 * (b & 3))) & 0xffu;
+    if (__ballot(cb != 0) == 0) continue;
+    const float x = static_cast<float>(cb) * inv;
+#pragma unroll
+    for (int u = 0; u < kHid; ++u) h1[u] = __builtin_fmaf(w1t[(4 + b) * kHid + u], x, h1[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < kHid; ++u) h1[u] = h1[u] > 0.f ? h1[u] : 0.f;
+  // layers 2 and 3: output o sums W3[o][u] * h2[u] over u in order, so each
+  // h2[u] is folded in as it is formed
+  float y0 = b3[0], y1 = b3[1];
+  for (int u = 0; u < kHid; ++u) {
+    float acc = b2[u];
+#pragma unroll
+    for (int k = 0; k < kHid; ++k) acc = __builtin_fmaf(W2[u * kHid + k], h1[k], acc);
+    const float h2 = acc > 0.f ? acc : 0.f;
+    y0 = __builtin_fmaf(W3[u], h2, y0);
+    y1 = __builtin_fmaf(W3[kHid + u], h2, y1);
+  }
+  // rules (band_from_prediction_kernel)
+  bool hits = false;
+  int2 band{1, 0};
+  {
+    const int anchor = B.anchor[s];
+    const int pf = B.pf[s], pl = B.pl[s];
+    auto inside = [&](int lo, int hi) {
+      bool ok = true;
+#pragma unroll
+      for (int b = 0; b < kBins; ++b) {
+        const uint32_t cb = (cw[b >> 2] >> (8 * (b & 3))) & 0xffu;
+        const int blo = -64 + 4 * b;
+        if (cb != 0 && (blo < lo || blo + 3 > hi)) ok = false;
+      }
+      return ok;
+    };
+#pragma unroll
+    for (int k = 0; k < kBins / 4; ++k) hits |= cw[k] != 0;
+    hits = hits && inv > 0.f;
+    const float lw = fminf(fmaxf(y1, 0.f), 12.f);
+    const int width = static_cast<int>(ceilf(exp2f(lw))) + 2 * A.margin;
+    if (!(H > kBandHapMax || !hits || width > A.max_width)) {
+      const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(y0, -8.f), 8.f)));
+      constexpr int kIn = kBandNarrow / 2 - kBandEdge;
+      const bool narrow = A.allow_narrow && width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
+                          inside(-kIn, kIn - 1) && pf >= 0 && pf <= kEndSlack && pl >= R - kEndSlack;
+      constexpr int kInW = kBandLanes / 2 - kBandEdge;
+      const bool wide = inside(dc - kInW, dc + kInW - 1) && pf >= 0 && pf <= kEndSlackWide &&
+                        pl >= R - kEndSlackWide && 2 * (pl - pf) >= R;
+      if (narrow) {
+        band = int2{anchor - kBandNarrow / 2, anchor + kBandNarrow / 2 - 1};
+      } else if (wide) {
+        const int c = anchor + dc;
+        band = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
+      }
+    }
+  }
+  const long long out = B.out[s];
+  const int read = B.read[s], hap = B.hap[s];
+  if (valid) A.band[out] = band;
+  // band class (band_dispatch_kernel)
+  int cls = 0;
+  if (valid && B.gc[s] && band.x <= band.y && H <= kBandHapMax && R <= kBandReadMax && band.x >= -(1 << 26) &&
+      band.y <= (1 << 26)) {
+    const int wd = band.y - band.x + 1;
+    cls = wd <= kBandNarrow ? kBandNarrow : (wd <= kBandLanes ? kBandLanes : 0);
+  }
+  // this pair's task piece: lanes [start, end)
+  const unsigned long long lt = below(lane);
+  const unsigned long long upto = seg_first & below(lane + 1);
+  const int start = 63 - __clzll(upto);
+  const unsigned long long above = seg_first & ~below(lane + 1);
+  const int end = above ? __ffsll(static_cast<long long>(above)) - 1 : fill;
+  const unsigned long long seg = below(end) & ~below(start);
+  const bool quad = a.wide_pairs == kQuadPairs;
+  const unsigned long long w64 = __ballot(cls == kBandLanes);
+  const int n64 = __popcll(w64 & seg), r64 = __popcll(w64 & seg & lt);
+  const bool to_q = cls == kBandLanes && (!quad || r64 < (n64 & ~(kQuadPairs - 1)));
+  const bool to_1 = cls == kBandLanes && !to_q;
+  const unsigned long long w32 = __ballot(cls == kBandNarrow);
+  const bool first = valid && ((seg_first >> lane) & 1ull);
+  const bool odd = first && (__popcll(w32 & seg) & 1);
+  const bool to_f = valid && cls == 0 && H <= kStackHmax;
+  const bool to_fl = valid && cls == 0 && H > kStackHmax;
+  const unsigned long long mq = __ballot(to_q), m1 = __ballot(to_1), mo = __ballot(odd), mf = __ballot(to_f),
+                           mfl = __ballot(to_fl);
+  int bq = 0, b1 = 0, b32 = 0, bf = 0, bfl = 0;
+  if (lane == 0) {
+    if (mq) bq = atomicAdd(a.item_count, __popcll(mq));
+    if (m1) b1 = atomicAdd(a.item1_count, __popcll(m1));
+    if (w32) b32 = atomicAdd(a.item32_count, __popcll(w32) + __popcll(mo));
+    if (mf) bf = atomicAdd(a.full_count, __popcll(mf));
+    if (mfl) bfl = atomicAdd(a.full_long_count, __popcll(mfl));
+  }
+  bq = __shfl(bq, 0);
+  b1 = __shfl(b1, 0);
+  b32 = __shfl(b32, 0);
+  bf = __shfl(bf, 0);
+  bfl = __shfl(bfl, 0);
+  const int pads_before = __popcll(mo & below(start));
+  if (cls) {
+    const int wd = band.y - band.x + 1;
+    const BandItem it{read, hap, band.x - ((cls - wd) >> 1), 0, out};
+    if (to_q) a.items[bq + __popcll(mq & lt)] = it;
+    else if (to_1) a.items1[b1 + __popcll(m1 & lt)] = it;
+    else a.items32[b32 + __popcll(w32 & lt) + pads_before] = it;
+  } else if (to_f) {
+    a.full[bf + __popcll(mf & lt)] = Task{read, hap, 1, -1, out};
+  } else if (to_fl) {
+    a.full_long[bfl + __popcll(mfl & lt)] = Task{read, hap, 1, -1, out};
+  }
+  if (odd) a.items32[b32 + __popcll(w32 & below(end)) + pads_before] = BandItem{read, -1, 0, 0, -1};
+}
+
+__global__ __launch_bounds__(64 * kProWaves) __attribute__((amdgpu_waves_per_eu(4))) void band_prologue_kernel(
+    ProlArgs A) {
+  __shared__ float w[kParams];
+  __shared__ ProBuf bufs[kProWaves];
+  const FeatArgs& a = A.f;
+  // weights: W1 transposed (input-major), the rest as stored
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
+    float v;
+    if (i < kHid * kFeat) {
+      const int k = i / kHid, u = i % kHid;
+      v = A.params[u * kFeat + k];
+    } else {
+      v = A.params[i];
+    }
+    w[i] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  ProBuf& B = bufs[threadIdx.x >> 6];
+  const TaskClaims cl = xcd_task_range(a.n_tasks);
+  int fill = 0;
+  unsigned long long seg_first = 0;
+  for (;;) {
+    const int tc0 = claim_tasks(a.next, cl, lane);
+    if (tc0 < 0) break;
+    const int tc1 = min(cl.end, tc0 + kFeatClaim);
+    for (int ti = tc0; ti < tc1; ++ti) {
+      const Task tk = a.tasks[ti];
+      const int64_t ro = a.read_off[tk.read];
+      const int R = static_cast<int>(a.read_off[tk.read + 1] - ro);
+      const ReadSeeds rs = read_seeds(a.rclass, a.q, ro, R, lane);
+      const float inv = rs.nvalid > 0 ? 1.f / static_cast<float>(rs.nvalid) : 0.f;
+      const int gc = A.b.gcp_const[tk.read] != 0;
+      for (int hs = 0; hs < tk.nhap; hs += 64) {
+        const int nsub = min(64, tk.nhap - hs);
+        if (fill + nsub > 64) {
+          __builtin_amdgcn_wave_barrier();
+          prologue_flush(A, w, B, fill, seg_first, lane);
+          __builtin_amdgcn_wave_barrier();
+          fill = 0;
+          seg_first = 0;
+        }
+        seg_first |= 1ull << fill;
+        for (int h0 = 0; h0 < nsub; h0 += kHapChunk) {
+          ChunkHits ch;
+          chunk_lookups(a.hap_off, a.htab_off, a.htab, tk.hap0 + hs + h0, min(kHapChunk, nsub - h0), rs.my_p,
+                        rs.my_key, ch);
+#pragma unroll
+          for (int c = 0; c < kHapChunk; ++c) {
+            const int h = hs + h0 + c;
+            if (h0 + c >= nsub) break;
+            const int hap = tk.hap0 + h;
+            const int H = static_cast<int>(a.hap_off[hap + 1] - a.hap_off[hap]);
+            const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane);
+            const int slot = fill + h0 + c;
+            if (lane < kBins) reinterpret_cast<uint8_t*>(B.cnt[slot])[lane] = static_cast<uint8_t>(f.cnt);
+            if (lane == 0) {
+              B.hap[slot] = hap;
+              B.H[slot] = H;
+              B.anchor[slot] = f.anchor;
+              B.pf[slot] = f.pf;
+              B.pl[slot] = f.pl;
+              B.read[slot] = tk.read;
+              B.R[slot] = R;
+              B.inv[slot] = inv;
+              B.fqm[slot] = rs.fq_mean;
+              B.fqn[slot] = rs.fq_min;
+              B.out[slot] = tk.out0 + h;
+              B.gc[slot] = gc;
+            }
+          }
+        }
+        fill += nsub;
+      }
+    }
+  }
+  if (fill > 0) {
+    __builtin_amdgcn_wave_barrier();
+    prologue_flush(A, w, B, fill, seg_first, lane);
+  }
+}
+
 // Start bands of long reads (one task per pair, run in FP64 on the band that
 // follows the alignment).  One wavefront per pair: lane l looks up the read's
 // 12-mer at position 4l (the first 256 bases) in the haplotype's table; seeds
@@ -620,6 +957,27 @@ hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, c
 
 hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st) {
   hipLaunchKernelGGL(band_features_kernel, dim3(grid), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+// The fused prologue's persistent grid: its occupancy on every CU, a
+// multiple of the XCD count (per-XCD task claims).
+int band_prologue_grid(int cus, int n_tasks) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_prologue_kernel, 64 * kProWaves, 0) != hipSuccess ||
+      n < 1)
+    n = 1;
+  const int want = (n_tasks + kProWaves - 1) / kProWaves;
+  int g = std::max(1, std::min(cus * n, want));
+  if (g >= kXcds) g -= g % kXcds;
+  return g;
+}
+
+hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, const float* params, int margin,
+                                int max_width, int allow_narrow, int grid, hipStream_t st) {
+  if (f.n_tasks <= 0) return hipSuccess;
+  ProlArgs A{f, b, band, params, margin, max_width, allow_narrow};
+  hipLaunchKernelGGL(band_prologue_kernel, dim3(grid), dim3(64 * kProWaves), 0, st, A);
   return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, c
                                const unsigned long long* htab, const Task* tasks, int n_tasks, int2* band,
                                int grid, hipStream_t st);
 hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st);
+int band_prologue_grid(int cus, int n_tasks);
+hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, const float* params, int margin,
+                                int max_width, int allow_narrow, int grid, hipStream_t st);
 hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
                               float* H2, hipStream_t st);
 hipError_t launch_mlp_backward(const float* params, const float* X, const float* Y, const float* T,
@@ -270,6 +273,7 @@ struct bv_plan {
   int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
   int hmax = 0;
   int n_tasks = 0;
+  int64_t n_subtasks = 0;  // tasks counted per 64 haplotypes (the fused prologue's pieces)
   int grid32 = 0, grid64 = 0, grid_band = 0, grid_long = 1;
   int grid_band1 = 1;  // one-pair banded kernels after the quad one (leftovers, narrow band)
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
@@ -321,6 +325,8 @@ struct bv_plan {
   float* d_Y = nullptr;
   float* d_model = nullptr;  // MLP parameters (copy)
   bool has_model = false;
+  bool fused_prologue = true;  // learned bands: one kernel from features to work lists
+  int grid_prologue = 1;
   int band_margin = 8, band_max_width = 256;
   bool has_band = false;
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
@@ -473,7 +479,7 @@ KArgs base_args(const bv_plan* p) {
   return a;
 }
 
-hipError_t launch_features(bv_plan* p) {
+FeatArgs feat_args(const bv_plan* p) {
   FeatArgs f{};
   f.rclass = p->d_rclass;
   f.q = p->d_q;
@@ -488,6 +494,11 @@ hipError_t launch_features(bv_plan* p) {
   f.htab = p->d_htab;
   f.htab_off = p->d_htab_off;
   f.next = p->d_counters + kFeatCounters;
+  return f;
+}
+
+hipError_t launch_features(bv_plan* p) {
+  const FeatArgs f = feat_args(p);
   // the claim counters are cleared by a kernel (graph-safe, no memset node)
   if (hipError_t e = launch_zero_ints(f.next, 8, p->stream); e != hipSuccess) return e;
   const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 32));
@@ -504,15 +515,16 @@ int band_grid_cap(const bv_plan* p, int wide_pairs) {
 
 int ensure_band_lists(bv_plan* p) {
   const int64_t n = std::max<int64_t>(1, p->n_pairs);
-  // the quad shape leaves at most kQuadPairs - 1 wide pairs per task to the
-  // one-pair kernel; narrow items are padded to an even count per task
+  // the quad shape leaves at most kQuadPairs - 1 wide pairs per task (per
+  // 64 haplotypes of a task in the fused prologue) to the one-pair kernel;
+  // narrow items are padded to an even count per task (or piece)
   if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_items1 &&
-      hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, (kQuadPairs - 1) * int64_t{p->n_tasks})) !=
+      hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, (kQuadPairs - 1) * p->n_subtasks)) !=
           hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
-  if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_tasks)) != hipSuccess)
+  if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_subtasks)) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_fallback_long &&
       hipMalloc(&p->d_fallback_long, sizeof(Task) * std::max<int64_t>(1, p->n_longhap_pairs)) != hipSuccess)
@@ -661,12 +673,21 @@ int enqueue_run(bv_plan* p, bool timed) {
   if (p->n_tasks == 0 && p->n_tasks64 == 0) return BV_OK;
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   if (timed) HIP_TRY(hipEventRecord(p->ev[0], p->stream));
-  if (fp32 && p->has_model) {
+  const bool fused = fp32 && p->has_model && p->has_band && p->fused_prologue;
+  if (fused) {
+    // Learned bands in one kernel: seed features, MLP, band rules, band
+    // table and the banded / full-matrix work lists.
+    FeatArgs f = feat_args(p);
+    HIP_TRY(launch_band_prologue(f, band_args(p), p->d_band, p->d_model, p->band_margin, p->band_max_width,
+                                 p->opt.band_narrow, p->grid_prologue, p->stream));
+  } else if (fp32 && p->has_model) {
     // Learned bands: seed features, MLP, band table -- all on the device.
     HIP_TRY(launch_features(p));
     HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
     HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_ends, p->d_X, p->n_pairs, p->band_margin,
                                         p->band_max_width, p->opt.band_narrow, p->d_band, p->stream));
+  }
+  if (fp32 && p->has_model) {
     // Long reads (outside the network's training range): start bands from
     // the seeds of their first bases; the moving band follows from there.
     const int grid_anchor = std::max(1, std::min(p->n_tasks64, g_ctx.cus * 32));
@@ -689,7 +710,7 @@ int enqueue_run(bv_plan* p, bool timed) {
     // Pass 1: pairs with an eligible band on the banded kernel; the rest,
     // and every band it rejects, go to the full-matrix list.
     const BArgs b = band_args(p);
-    HIP_TRY(launch_band_dispatch(b, p->stream));
+    if (!fused) HIP_TRY(launch_band_dispatch(b, p->stream));  // (the fused prologue wrote the lists)
     if (timed) HIP_TRY(hipEventRecord(p->ev[5], p->stream));
     HIP_TRY(launch_band_forward(b, p->grid_band, p->grid_band1, p->stream));
     if (timed) HIP_TRY(hipEventRecord(p->ev[6], p->stream));
@@ -1085,6 +1106,9 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->n_tasks_rest = static_cast<int>(tasks_rest.size());
   p->stack_cols = stack_hmax + 128;  // lane 0 reads up to column H + 126
   p->n_tasks = static_cast<int>(tasks.size());
+  p->n_subtasks = 0;
+  for (const Task& t : tasks) p->n_subtasks += (t.nhap + 63) / 64;
+  p->grid_prologue = band_prologue_grid(g_ctx.cus, p->n_tasks);
   p->n_tasks64 = static_cast<int>(tasks64.size());
 
   const int bpc32 = g_ctx.bpc_f32, bpc64 = g_ctx.bpc_f64;
@@ -1245,6 +1269,16 @@ int bv_plan_set_band_shape(bv_plan* p, int32_t wide_pairs) {
   if (wide_pairs != p->wide_pairs) {
     p->wide_pairs = wide_pairs;
     p->grid_band = p->grid_band_max = band_grid_cap(p, wide_pairs);
+    drop_graph(p);
+  }
+  return BV_OK;
+}
+
+int bv_plan_set_band_prologue(bv_plan* p, int32_t fused) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  if (fused != 0 && fused != 1) return fail(BV_EINVAL, "fused is 0 or 1");
+  if ((fused != 0) != p->fused_prologue) {
+    p->fused_prologue = fused != 0;
     drop_graph(p);
   }
   return BV_OK;

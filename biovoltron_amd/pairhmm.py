@@ -85,6 +85,7 @@ EXPORTS = (
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
     "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_set_band_shape", "bv_plan_band_shape",
+    "bv_plan_set_band_prologue",
     "bv_plan_postprocess", "bv_sw_align",
     "bv_pairhmm_batch",
 )
@@ -118,6 +119,7 @@ def library() -> C.CDLL:
         lib.bv_plan_set_grids.argtypes = [vp, vp]
         lib.bv_plan_set_band_shape.argtypes = [vp, i32]
         lib.bv_plan_band_shape.argtypes = [vp, vp]
+        lib.bv_plan_set_band_prologue.argtypes = [vp, i32]
         lib.bv_plan_postprocess.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
@@ -269,6 +271,12 @@ class Plan:
         """Pairs per wavefront of the 64-diagonal banded kernel: 4 (default)
         or 1 (bv_plan_set_band_shape)."""
         _check(self._lib.bv_plan_set_band_shape(self._h, int(wide_pairs)))
+
+    def set_band_prologue(self, fused: bool) -> None:
+        """With a band model: one fused kernel from seed features to work
+        lists (default), or the separate features, MLP, band-rule and
+        dispatch kernels (bv_plan_set_band_prologue)."""
+        _check(self._lib.bv_plan_set_band_prologue(self._h, 1 if fused else 0))
 
     def band_shape(self) -> int:
         v = C.c_int32()

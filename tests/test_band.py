@@ -528,3 +528,77 @@ def test_autotune_changes_speed_not_results():
     assert all(v >= 1 for v in grids.values()) and grids["wide_pairs"] in (1, 4)
     ref = res[grids["wide_pairs"]]
     assert np.array_equal(form, ref[1]) and np.abs(out - ref[0]).max() <= 1e-5
+
+
+def edge_regions(seed):
+    """Regions at the fused prologue's edges: a read against 80 haplotypes
+    (more than one 64-pair piece), reads shorter than a seed, a haplotype
+    over 1024 bases (striped full-matrix list), a read over 448 bases (no
+    band), and a read whose gap continuation varies (no band)."""
+    rng = np.random.default_rng(seed)
+
+    def hap(n):
+        return rng.choice(list(b"ACGT"), n).astype(np.uint8)
+
+    def mutate(h, rate=0.01):
+        h = h.copy()
+        k = rng.random(h.size) < rate
+        h[k] = rng.choice(list(b"ACGT"), int(k.sum()))
+        return h
+
+    def read(h, R, c=10):
+        o = int(rng.integers(0, max(1, h.size - R + 1)))
+        s = mutate(h[o:o + R])
+        return dict(seq=s.tobytes(), q=rng.integers(20, 41, s.size).astype(np.uint8).tobytes(),
+                    i=bytes([45]) * s.size, d=bytes([45]) * s.size,
+                    c=(bytes([c]) * s.size if np.isscalar(c) else bytes(c[:s.size])))
+
+    base = hap(400)
+    haps80 = [mutate(base).tobytes() for _ in range(80)]
+    many = ([read(np.frombuffer(haps80[k % 80], np.uint8), int(rng.integers(60, 200))) for k in range(6)], haps80)
+    b2 = hap(300)
+    short = ([read(b2, 8), read(b2, 11), read(b2, 12), read(b2, 150)], [mutate(b2).tobytes() for _ in range(3)])
+    b3 = hap(1100)
+    longhap = ([read(b3, 150), read(b3, 200)], [b3.tobytes(), mutate(b3[:900]).tobytes()])
+    b4 = hap(700)
+    vary = rng.integers(5, 30, 460).astype(np.uint8)
+    tall = ([read(b4, 460), read(b4, 449), read(b4, 300, c=vary)], [mutate(b4).tobytes() for _ in range(5)])
+    return [many, short, longhap, tall]
+
+
+@pytest.mark.gpu
+def test_fused_prologue_equals_separate_kernels():
+    """The fused prologue (features, MLP, band rules and list split in one
+    kernel) gives the band table, forms and results of the separate
+    kernels bit for bit.  The exception: a task of more than 64 haplotypes
+    is split into 64-pair pieces, which may group its wide pairs four at a
+    time differently, and the quad and one-pair kernels agree to FP32
+    rounding there."""
+    b = synth.generate(1, seed=synth.workload_seed(1) + 977, count=150)
+    regions = [b.region(g) for g in range(b.n_regions)] + edge_regions(5)
+    pb = pairhmm.PackedBatch.from_regions(regions)
+    model = band.BandModel.load(band.DEFAULT_MODEL)
+    res = {}
+    for fused in (True, False):
+        with pairhmm.Plan(pb) as plan:
+            band.set_plan_model(plan, model)
+            plan.set_band_prologue(fused)
+            say("prologue", "fused" if fused else "separate")
+            plan.run()
+            out, form = plan.results()
+            st = plan.stats()
+            res[fused] = (out, form, band.plan_bands(plan), int(st.band_pairs), int(st.fallback_pairs))
+    (o1, f1, b1, n1, m1), (o0, f0, b0, n0, m0) = res[True], res[False]
+    assert np.array_equal(b1, b0)
+    assert np.array_equal(f1, f0)
+    assert (n1, m1) == (n0, m0)
+    # pairs of the 80-haplotype region (the first edge region)
+    sizes = [len(r) * len(h) for r, h in regions]
+    p0 = int(np.sum(sizes[:b.n_regions]))
+    many = np.zeros(o1.size, bool)
+    many[p0:p0 + sizes[b.n_regions]] = True
+    assert np.array_equal(o1[~many], o0[~many])
+    assert np.abs(o1[many] - o0[many]).max() <= 1e-6
+    # the edge cases took the lists they should
+    assert (f1 == 2).sum() > 0.9 * b.n_pairs
+    say("pairs", o1.size, "banded", int((f1 == 2).sum()), "full", int(np.isin(f1, (0, 1, 3)).sum()))
