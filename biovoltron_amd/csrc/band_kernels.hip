@@ -221,7 +221,7 @@ struct PairFeat {
 };
 
 __device__ __forceinline__ PairFeat pair_features(int first, const int* occ, int nocc, int my_p, int R, int H,
-                                                  int lane) {
+                                                  int lane, uint32_t* hist) {
   PairFeat f;
   f.anchor = (H - R) / 2;
   const int found_off = first != INT_MAX ? first - my_p : INT_MIN;
@@ -245,25 +245,19 @@ __device__ __forceinline__ PairFeat pair_features(int first, const int* occ, int
       if (rest == v) rest = INT_MAX;
     }
   }
-  // Bins by ballot: each round counts the lanes whose occurrence (slot k)
-  // falls in one bin -- as many rounds as distinct bins.
-  int my_cnt = 0;
+  // Bin counts by LDS atomics on the wavefront's own 32 counters (its LDS
+  // operations run in order: clear, adds, read).
+  __builtin_amdgcn_wave_barrier();
+  if (lane < kBins) hist[lane] = 0u;
+  __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int k = 0; k < kMaxOcc; ++k) {
-    int bin = -1;
-    if (k < nocc) {
-      const int b = (occ[k] - f.anchor + 64) >> 2;
-      if (b >= 0 && b < kBins) bin = b;
-    }
-    for (;;) {
-      const unsigned long long live = __ballot(bin >= 0);
-      if (live == 0) break;
-      const int bb = __builtin_amdgcn_readlane(bin, __ffsll(static_cast<long long>(live)) - 1);
-      const unsigned long long same = __ballot(bin == bb);
-      if (lane == bb) my_cnt += __popcll(same);
-      if (bin == bb) bin = -1;
-    }
+    const int b = (occ[k] - f.anchor + 64) >> 2;
+    if (k < nocc && b >= 0 && b < kBins)
+      __hip_atomic_fetch_add(hist + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
+  __builtin_amdgcn_wave_barrier();
+  const int my_cnt = lane < kBins ? static_cast<int>(hist[lane]) : 0;
   f.cnt = my_cnt;
   // read ends anchored by seeds that hit next to the anchor
   const bool near = hit && abs(found_off - f.anchor) <= kEndAnchorTol;
@@ -308,6 +302,7 @@ __device__ __forceinline__ int claim_tasks(int* next, const TaskClaims& c, int l
 // every pair into X, anchor and ends (the band API's features, and the
 // trainer's inputs).
 __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
+  __shared__ uint32_t hist[kBins];
   const int lane = threadIdx.x;
   const TaskClaims cl = xcd_task_range(a.n_tasks);
   for (;;) {
@@ -331,7 +326,7 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
           const int hap = tk.hap0 + h;
           const int64_t o = tk.out0 + h;
           const int H = static_cast<int>(a.hap_off[hap + 1] - a.hap_off[hap]);
-          const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane);
+          const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane, hist);
           float* X = a.X + o * kFeat;
           if (lane == 0) {
             X[0] = static_cast<float>(R) / 256.f;
@@ -617,6 +612,7 @@ struct ProBuf {
   float inv[64], fqm[64], fqn[64];
   long long out[64];
   int gc[64];  // the read's gap continuation is constant
+  uint32_t hist[kBins];  // bin counters of the pair being featurised
 };
 
 __device__ __forceinline__ unsigned long long below(int n) {
@@ -820,7 +816,7 @@ __global__ __launch_bounds__(64 * kProWaves) __attribute__((amdgpu_waves_per_eu(
             if (h0 + c >= nsub) break;
             const int hap = tk.hap0 + h;
             const int H = static_cast<int>(a.hap_off[hap + 1] - a.hap_off[hap]);
-            const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane);
+            const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane, B.hist);
             const int slot = fill + h0 + c;
             if (lane < kBins) reinterpret_cast<uint8_t*>(B.cnt[slot])[lane] = static_cast<uint8_t>(f.cnt);
             if (lane == 0) {

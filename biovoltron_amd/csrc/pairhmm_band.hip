@@ -603,8 +603,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
 // lane below), the lanes' top values are then combined by a decayed scan over
 // the 16 lanes of the row (four row_shr levels with factors yy^4, yy^8, yy^16,
 // yy^32; no row_bcast: the pairs of a wavefront never exchange data), and one
-// FMA per cell adds the carry from the lanes below.  That is 3.5 VALU per
-// cell for D against 6, and I crosses a lane only for a lane's top cell.  A
+// FMA per cell adds the carry from the lanes below.  D is carried divided by
+// its row's my (constant along a row), so the chain takes M itself and no
+// multiply per cell: about 2.25 VALU per cell for D against 6 in the one-pair
+// kernel, and I crosses a lane only for a lane's top cell.  A
 // lane is exactly one edge zone (kQuadCells == kBandEdge): lane 0 of a row
 // holds the band's 4 lowest diagonals, lane 15 its 4 highest.
 //
@@ -683,7 +685,7 @@ __device__ __host__ constexpr int quad_sel_stride(int sel_cols) { return 4 * sel
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void band_quad_kernel(BArgs a) {
   RowRec* tab = reinterpret_cast<RowRec*>(band_lds);
   const int C = a.sel_cols;
   const int lane = threadIdx.x;
@@ -743,7 +745,12 @@ __global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
       // edge zones active there.
       int o_cur = off(1);
       const _Float16* sl = selL + (4 * k + o_cur - wb);  // this lane's cell 0 slot, less the row's part
+      // D is carried divided by its row's my (D' = D / my, constant along a
+      // row): the chain along a row is then D'_d = M_{d-1} + yy * D'_{d-1},
+      // with no multiply per cell, and the next row takes my * D' in one FMA.
+      // Row 0's D is kept as is (myp = 1).
       float M[4], I[4], D[4], TM[4], TI[4], TD[4];
+      float myp = 1.f;
       {
         const float d0 = gx * (kK / static_cast<float>(H));
         const bool ez0 = (left_lane && o_cur - 1 >= 0) || (right_lane && o_cur + kBandLanes <= H);
@@ -810,8 +817,8 @@ __global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
           float Mn[4], In[4], TMn[4], TIn[4], Y[4], TY[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            Mn[c] = __builtin_fmaf(M[c], rr.mm, I[c] + D[c]) * pr[c];
-            TMn[c] = __builtin_fmaf(TM[c], rr.mm, TI[c] + TD[c]) * pr[c];
+            Mn[c] = __builtin_fmaf(M[c], rr.mm, __builtin_fmaf(D[c], myp, I[c])) * pr[c];
+            TMn[c] = __builtin_fmaf(TM[c], rr.mm, __builtin_fmaf(TD[c], myp, TI[c])) * pr[c];
           }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -827,9 +834,10 @@ __global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
           }
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            Y[c] = Mn[c] * rr.my;
-            TY[c] = TMn[c] * rr.my;
+            Y[c] = Mn[c];
+            TY[c] = TMn[c];
           }
+          myp = rr.my;
           quad_cross(In[3], TIn[3], Y[0], TY[0], M[0], TM[0], Y[3], TY[3], rr.mx, yy);
           // D chain along the lane: d1 = Y0 + yy*carry (done above), d2, d3
           float d1 = Y[0], td1 = TY[0];
@@ -895,8 +903,8 @@ __global__ __launch_bounds__(64) void band_quad_kernel(BArgs a) {
           const float ig = static_cast<float>(igx);
           float wl = 0.f;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) wl += __builtin_fmaf(I[c] + D[c], ig, M[c]);
-          const float wtop = __builtin_fmaf(I[3] + D[3], ig, M[3]);
+          for (int c = 0; c < 4; ++c) wl += __builtin_fmaf(__builtin_fmaf(D[c], myp, I[c]), ig, M[c]);
+          const float wtop = __builtin_fmaf(__builtin_fmaf(D[3], myp, I[3]), ig, M[3]);
           const float tot = __shfl(row_total(wl), lane | 15);
           const float top = __shfl(wtop, lane | 15);
           const bool go = want && top <= kBandHold * tot;

@@ -486,6 +486,9 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
  * hold_eps of row r - 1's mass is not made.  The edge zones and the touch DP
  * are those of bvo_band_forward at the band's current position.  offs
  * (optional, R + 1 entries) receives the offset used on each row. */
+static double leak_rho = 2.5e-7;
+void bvo_set_leak_rho(double rho) { leak_rho = rho; }
+
 double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
                                const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
                                const uint8_t* hap, int o_init, int width, int edge, int every,
@@ -504,6 +507,46 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
    * T = F on every zone cell, and the entry runs added at the top slot then
    * propagate through the zone instead of being overwritten (with edge < 0) */
   const int kernel_rule = (irun & 4) != 0;
+  /* irun bit 3: *touch_ratio receives the leak bound E / S instead of T / S.
+   * On an active zone cell T = F, so the touched mass can leave the zones'
+   * diagonals only by: D from the left zone into diagonal o + edge (rows
+   * where the left zone is active), I from the right zone into diagonal
+   * o + width - edge - 1 (the row after one where the right zone is active),
+   * or a zone cell that the next row does not keep in an active zone of its
+   * side (the zone turns off, or a move carries the cell elsewhere).  E adds those F flows, the
+   * zones' M + I on the last row (where active) and the exit runs (bit 1).
+   * Every continuation is a probability, at most 1 -- times phi, the
+   * product over rows of max(1, the out-weight sums of M and D), which is 1
+   * when the gap penalties are constant along the read -- so E * phi >= T,
+   * and a band that passes with E passes with T. */
+  const int leak = (irun & 8) != 0;
+  /* irun bits 4-7 drop one term of E each (left leak, right leak, turn-off
+   * and move leaks, last-row zone mass): planted mistakes for the tests */
+  const int drop = (irun >> 4) & 15;
+  /* irun >> 8 = r0: with bit 3, T is carried exactly for rows 1..r0 and its
+   * mass on row r0 (every cell, continuation at most 1) starts E; the leak
+   * terms count from row r0 + 1 on (r0 = 0: E alone) */
+  const int r0 = irun >> 8;
+  double phi = 1.0;
+  if (leak) {
+    for (int r = 1; r < R; ++r) {
+      row_par a, b;
+      row_params(0, r - 1, rs, q, ins, del, gcp, &a);
+      row_params(0, r, rs, q, ins, del, gcp, &b);
+      double f;
+      if (a.dm > 0.0) {
+        const double fd = b.dm / a.dm;
+        const double fm = b.mm + b.mi + a.md * fd;
+        f = fm > fd ? fm : fd;
+      } else {
+        f = 1.0 + (double)H; /* D runs along the whole row */
+      }
+      if (f > 1.0) phi *= f;
+    }
+  }
+  double E = 0.0;
+#define LACT_AT(rr, oo) (always || ((rr) == 0 ? (oo) - 1 >= 0 : (rr) + (oo) - 1 >= 1))
+#define RACT_AT(rr, oo) (always || (rr) + (oo) + width <= H)
   const size_t W = (size_t)width;
   double* buf = (double*)calloc(12 * W, sizeof(double));
   double *fM = buf, *fI = buf + W, *fD = buf + 2 * W, *tM = buf + 3 * W, *tI = buf + 4 * W,
@@ -552,6 +595,22 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
         if (!(k - s >= 0 && k - s < width)) dropped += fM[k] + fI[k] + fD[k];  /* slot k lands on k - s */
       if (hold_eps > 0.0 && dropped > hold_eps * Wt) s = 0;
       else if (Wt > 0.0) lost_sum += dropped / Wt;
+    }
+    const int lprev = LACT_AT(r - 1, o), rprev = RACT_AT(r - 1, o);
+    if (leak && r - 1 >= r0) {
+      /* A zone cell of row r - 1 stays tracked only where row r puts it in
+       * the same side's zone, active on row r; otherwise (the zone turns
+       * off, or a move carries the cell into the interior or the other
+       * side's zone) its mass leaves the tracked diagonals.  Cells a move
+       * drops from the band are not counted (nor are they in T). */
+      const int lnow = LACT_AT(r, o + s), rnow = RACT_AT(r, o + s);
+      for (int k = 0; k < width; ++k) {
+        const int lz = k < edge && lprev, rz = k >= width - edge && rprev;
+        const int kn = k - s;
+        if ((!lz && !rz) || kn < 0 || kn >= width) continue;
+        const int kept = (lz && kn < edge && lnow) || (rz && kn >= width - edge && rnow);
+        if (!kept && !(drop & 4)) E += fM[k] + fI[k] + fD[k];
+      }
     }
     if (s != 0) {
       double* arrs[6] = {fM, fI, fD, tM, tI, tD};
@@ -618,14 +677,29 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
       mx = fmax(mx, fmax(nM[k], fmax(nI[k], nD[k])));
     }
     uI[width - 1] += top_in;
+    if (leak && r > r0) {
+      if (lact && edge < width && !(drop & 1)) E += nD[edge];
+      if (rprev && width - edge - 1 >= 0 && !(drop & 2)) E += nI[width - edge - 1];
+    }
     int e = 0;
     if (mx > 0.0) frexp(mx, &e);
     x_tail = ldexp(x_tail, -e);
+    E = ldexp(E, -e);
     for (int k = 0; k < width; ++k) {
       fM[k] = ldexp(nM[k], -e); fI[k] = ldexp(nI[k], -e); fD[k] = ldexp(nD[k], -e);
       tM[k] = ldexp(uM[k], -e); tI[k] = ldexp(uI[k], -e); tD[k] = ldexp(uD[k], -e);
     }
     log2scale += e;
+    if (leak && r == r0 && r < R) {
+      /* Every path crosses from row r0 to row r0 + 1 once, from one cell and
+       * state, and T and F continue alike from there: the cells where
+       * T <= rho * F carry at most rho * S of T, the others at most their T. */
+      for (int k = 0; k < width; ++k) {
+        if (tM[k] > leak_rho * fM[k]) E += tM[k];
+        if (tI[k] > leak_rho * fI[k]) E += tI[k];
+        if (tD[k] > leak_rho * fD[k]) E += tD[k];
+      }
+    }
   }
   double S = 0.0, T = 0.0;
   for (int k = 0; k < width; ++k) {
@@ -635,8 +709,22 @@ double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q
     T += tM[k] + tI[k];
   }
   T += x_tail;
+  if (leak && r0 >= R) {
+    E = T - x_tail; /* T carried to the last row: E is T */
+  } else if (leak) {
+    for (int k = 0; k < width; ++k) {
+      const int j = R + o + k;
+      if (j < 1 || j > H) continue;
+      if (((k < edge && LACT_AT(R, o)) || (k >= width - edge && RACT_AT(R, o))) && !(drop & 8))
+        E += fM[k] + fI[k];
+    }
+  }
+  if (leak) E = (E + x_tail) * phi;
+#undef LACT_AT
+#undef RACT_AT
   free(buf);
-  if (touch_ratio) *touch_ratio = S > 0.0 ? T / S : 1.0;
+  if (touch_ratio)
+    *touch_ratio = S > 0.0 ? (leak ? E / S + (r0 > 0 && r0 < R ? leak_rho : 0.0) : T / S) : 1.0;
   if (lost) *lost = lost_sum;
   if (!(S > 0.0)) return -INFINITY;
   return (log2(S) + log2scale) * log10(2.0);

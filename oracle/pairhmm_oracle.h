@@ -131,10 +131,15 @@ double bvo_band_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
  * edge < 0): the anti-diagonal banded kernel's rule -- T = F on the zones is
  * kept by taking the forward value only for the innermost zone cells' inputs
  * from the band's middle, and the entry runs (bit 0 implied) then propagate
- * through the zone into T.  Returns log10 of
+ * through the zone into T.  irun bit 3: the touch ratio is replaced by
+ * the leak bound E / S, an upper bound on T / S computed from forward values
+ * only (the banded kernels' accept test; see the definition); bits 4-7 then
+ * drop one of its terms each (test use: planted mistakes).  Returns log10 of
  * the banded likelihood, the touch ratio T/S (the accept test), the mass share
  * the moves dropped (*lost) and, if offs is not NULL, the band offset used on
  * each row 0..R. */
+/* The split level rho of the leak bound with r0 > 0 (default 2.5e-7). */
+void bvo_set_leak_rho(double rho);
 double bvo_band_follow_forward(int R, int H, const uint8_t* rs, const uint8_t* q,
                                const uint8_t* ins, const uint8_t* del, const uint8_t* gcp,
                                const uint8_t* hap, int o_init, int width, int edge, int every,

@@ -137,10 +137,10 @@ def brute_force(rows, hap, match):
     return math.log10(total)
 
 
-def small_cases(seed, n, alphabet=b"ACGTN"):
+def small_cases(seed, n, alphabet=b"ACGTN", rmax=4, hmax=4):
     rng = np.random.default_rng(seed)
     for _ in range(n):
-        R = int(rng.integers(1, 5)); H = int(rng.integers(1, 5))
+        R = int(rng.integers(1, rmax + 1)); H = int(rng.integers(1, hmax + 1))
         rd = dict(seq=bytes(rng.choice(list(alphabet), R).astype(np.uint8)),
                   q=rng.integers(3, 30, R).astype(np.uint8).tobytes(),
                   i=rng.integers(1, 30, R).astype(np.uint8).tobytes(),
@@ -860,3 +860,39 @@ def test_region_driver_matches_pair_rule(oracle, golden):
     oracle.lib.bvo_region_log10(64, ro.ctypes.data, *[a.ctypes.data for a in arrs], 8,
                                 ho.ctypes.data, hp.ctypes.data, 4, out.ctypes.data)
     assert same_bits(out, d["log10"])
+
+
+def test_leak_bound_covers_touch_mass(oracle):
+    """The leak bound E (irun bit 3, the banded kernels' accept test) is an
+    upper bound on the touch mass T, which the enumeration of in-band paths
+    pins, on static, sliding and moving bands with the exit runs; dropping
+    any one of E's terms (irun bits 4-7, planted in the oracle) breaks the
+    bound on some of the same cases, so every term is needed."""
+    rng = np.random.default_rng(83)
+    t = np.load(GOLDEN / "tables.npz")
+    checked = looser = 0
+    broken = {b: 0 for b in (16, 32, 64, 128)}
+    for rd, hap in list(small_cases(89, 500)) + list(small_cases(97, 300, rmax=6, hmax=7)):
+        R, H = len(rd["seq"]), len(hap)
+        width = int(rng.integers(2, 7))
+        edge = int(rng.integers(1, width // 2 + 1))
+        kind = rng.integers(0, 3)
+        steps = (np.zeros(R, int) if kind == 0 else rng.integers(-1, 1, R) if kind == 1
+                 else rng.integers(-1, 2, R))
+        sched = np.concatenate([[0, 0], np.cumsum(steps[1:])]).astype(np.int32) + int(rng.integers(-3, H + 1))
+        _, T, _, _ = oracle.band_follow(rd, hap, 0, width, edge, max_move=1, irun=2, sched=sched)
+        bf, bt = brute_force_moving_band(row_params_avx(rd, t), hap, match_avx, sched, width, edge)
+        if bf == 0.0:
+            continue
+        # T without the exit runs against the enumeration; E against T with them
+        _, T0, _, _ = oracle.band_follow(rd, hap, 0, width, edge, max_move=1, irun=0, sched=sched)
+        assert T0 == pytest.approx(bt / bf, rel=1e-9, abs=1e-15)
+        _, E, _, _ = oracle.band_follow(rd, hap, 0, width, edge, max_move=1, irun=2 | 8, sched=sched)
+        assert E >= T * (1 - 1e-12), (R, H, width, edge, kind)
+        checked += 1
+        looser += E > T * (1 + 1e-9)
+        for b in broken:
+            _, Eb, _, _ = oracle.band_follow(rd, hap, 0, width, edge, max_move=1, irun=2 | 8 | b, sched=sched)
+            broken[b] += Eb < T * (1 - 1e-9)
+    assert checked > 300 and looser > 50
+    assert all(n > 0 for n in broken.values()), broken
