@@ -305,7 +305,13 @@ def main():
     t_plan0 = time.perf_counter()
     plan = pairhmm.Plan(pairhmm.PackedBatch.from_synth(b))
     t_plan = time.perf_counter() - t_plan0
-    print(f"[bench] plan ready ({t_plan:.1f} s)", file=sys.stderr, flush=True)
+    st0 = plan.stats()
+    setup = dict(host_ms=round(st0.setup_ms_host, 1), upload_ms=round(st0.setup_ms_upload, 1),
+                 upload_bytes=int(st0.setup_bytes),
+                 upload_gbps=round(st0.setup_bytes / (st0.setup_ms_upload / 1e3) / 1e9, 2)
+                 if st0.setup_ms_upload > 0 else None)
+    print(f"[bench] plan ready ({t_plan:.1f} s: host {setup['host_ms']} ms, upload {setup['upload_ms']} ms, "
+          f"{setup['upload_bytes'] / 1e9:.2f} GB)", file=sys.stderr, flush=True)
     use_band = not args.no_band and Path(args.band_model).exists()
     t_model = 0.0
     tuned_full = tuned = None
@@ -405,7 +411,7 @@ def main():
         # tables), one run, copy of the results and form bytes back.
         e2e_s = t_plan + t_model + ms_band / 1e3 + t_copy
         band_info["end_to_end"] = dict(
-            plan_create_s=round(t_plan, 3), band_model_setup_s=round(t_model, 3),
+            plan_create_s=round(t_plan, 3), plan_create_parts=setup, band_model_setup_s=round(t_model, 3),
             run_s=round(ms_band / 1e3, 3), result_copy_s=round(t_copy, 3),
             gcups=round(b.cells / e2e_s / 1e9, 1))
         print(f"[bench] learned band: {ms_band:.2f} ms/step ({band_info['speedup_vs_full']}x), "

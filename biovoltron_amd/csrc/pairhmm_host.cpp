@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <atomic>
 #include <cmath>
@@ -29,6 +30,12 @@ hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, c
                                const unsigned long long* htab, const Task* tasks, int n_tasks, int2* band,
                                int grid, hipStream_t st);
 hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st);
+template <typename T>
+hipError_t sort_longest_first(const T* d_in, const uint64_t* d_cost, int64_t n, int end_bit, T* d_out,
+                              hipStream_t st);
+hipError_t launch_expand_bases(const int64_t* read_off, int64_t n_reads, const uint8_t* rpack, const int64_t* rexc,
+                               int64_t n_rexc, uint8_t* rclass, int64_t hbytes, const uint8_t* hpack,
+                               const int64_t* hexc, int64_t n_hexc, uint8_t* hmask, hipStream_t st);
 int band_prologue_grid(int cus, int n_tasks);
 hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, const float* params, int margin,
                                 int max_width, int allow_narrow, int grid, hipStream_t st);
@@ -246,9 +253,9 @@ int alphabet_scalar(Alphabet& a, const uint8_t* bases, int64_t n) {
 }
 
 template <typename F>
-void parallel_for(int64_t n, F&& f) {
+void parallel_for(int64_t n, F&& f, int64_t serial_below = 1 << 16) {
   const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < (1 << 16) || nt == 1) {
+  if (n < serial_below || nt == 1) {
     f(0, n);
     return;
   }
@@ -360,6 +367,8 @@ struct bv_plan {
   int64_t allele_cap = 0;
   hipEvent_t ev_post[2] = {nullptr, nullptr};
   float last_mspost = 0.f;
+  float setup_ms_host = 0.f, setup_ms_upload = 0.f;  // bv_plan_create: host work, uploads (+ expansion)
+  int64_t setup_bytes = 0;                           // bytes it uploaded
   // long reads on the band run on a second stream beside the short-read
   // passes that follow the banded kernels, with their own carry scratch
   hipStream_t side = nullptr;
@@ -403,11 +412,64 @@ void free_plan(bv_plan* p) {
 
 // Allocates max(n, capacity, 1) elements and copies n of them from src.
 template <typename T>
-int dev_upload(T** dst, const T* src, int64_t n, hipStream_t st, int64_t capacity = 0) {
+int dev_upload(T** dst, const T* src, int64_t n, hipStream_t st, int64_t capacity = 0, int64_t* counted = nullptr) {
   const size_t bytes = sizeof(T) * static_cast<size_t>(std::max<int64_t>(std::max<int64_t>(n, capacity), 1));
   HIP_TRY(hipMalloc(dst, bytes));
   if (n > 0) HIP_TRY(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
+  if (counted && n > 0) *counted += static_cast<int64_t>(sizeof(T)) * n;
   return BV_OK;
+}
+
+// A task list written longest first into *dst: uploaded with its costs (or
+// taken from src_dev, the same list already on the device in input order)
+// and ordered there by a stable sort on descending cost.
+template <typename T>
+int upload_longest_first(T** dst, const T* src_dev, const std::vector<T>& t, const std::vector<uint64_t>& cost,
+                         hipStream_t st, int64_t* up) {
+  const int64_t n = static_cast<int64_t>(t.size());
+  HIP_TRY(hipMalloc(dst, sizeof(T) * static_cast<size_t>(std::max<int64_t>(n, 1))));
+  if (n == 0) return BV_OK;
+  T* tmp = nullptr;
+  uint64_t* dcost = nullptr;
+  int rc = BV_OK;
+  if (!src_dev) {
+    rc = dev_upload(&tmp, t.data(), n, st, 0, up);
+    src_dev = tmp;
+  }
+  if (!rc) rc = dev_upload(&dcost, cost.data(), n, st, 0, up);
+  uint64_t mx = 0;
+  for (uint64_t c : cost) mx = std::max(mx, c);
+  int bits = 1;
+  while (bits < 64 && (mx >> bits) != 0) ++bits;
+  if (!rc && sort_longest_first(src_dev, dcost, n, bits, *dst, st) != hipSuccess)
+    rc = fail(BV_EHIP, "task sort failed");
+  if (tmp) (void)hipFree(tmp);
+  if (dcost) (void)hipFree(dcost);
+  return rc;
+}
+
+// Bases at two bits each (AVX mode: A C T G are classes 0..3, any other byte
+// acts as A), four per byte, base i in bits 2*(i%4); the positions whose
+// byte is 'N' (the wildcard) listed apart.  The device expands them to the
+// per-base class bytes and haplotype match masks the kernels read.
+void pack_bases(const uint8_t* bases, int64_t n, const uint8_t* code_of, uint8_t nbyte,
+                std::vector<uint8_t>& packed, std::vector<int64_t>& exc) {
+  packed.assign(static_cast<size_t>((n + 3) / 4), 0);
+  exc.clear();
+  std::mutex mu;
+  parallel_for(static_cast<int64_t>(packed.size()), [&](int64_t a0, int64_t a1) {
+    std::vector<int64_t> mine;
+    for (int64_t k = a0; k < a1; ++k) {
+      uint8_t v = 0;
+      for (int64_t i = 4 * k, e = std::min<int64_t>(n, 4 * k + 4); i < e; ++i) {
+        v |= static_cast<uint8_t>((code_of[bases[i]] & 3u) << (2 * (i & 3)));
+        if (bases[i] == nbyte) mine.push_back(i);
+      }
+      packed[k] = v;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    exc.insert(exc.end(), mine.begin(), mine.end());
+  });
 }
 
 int validate(const bv_region_batch* b) {
@@ -784,43 +846,6 @@ int enqueue_run(bv_plan* p, bool timed) {
   if (timed) HIP_TRY(hipEventRecord(p->ev[2], p->stream));
   return BV_OK;
 }
-// Orders tasks by descending cost, ties in input order (the order a stable
-// sort gives, so the work list is deterministic).  Chunks are sorted on
-// separate threads and merged pairwise, each round's merges in parallel: a
-// single-threaded sort of the 2.4e7 tasks of a 100,000-region plan took
-// seconds of the plan's set-up.
-template <typename TaskT>
-void longest_first(std::vector<TaskT>& t, const std::vector<int64_t>& c) {
-  using Key = std::pair<int64_t, int64_t>;  // (-cost, position): all distinct
-  const int64_t n = static_cast<int64_t>(t.size());
-  std::vector<Key> k(static_cast<size_t>(n));
-  parallel_for(n, [&](int64_t a, int64_t b) {
-    for (int64_t i = a; i < b; ++i) k[i] = Key{-c[i], i};
-  });
-  const int nt = n < (1 << 16) ? 1 : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<int64_t> bound(static_cast<size_t>(nt) + 1);
-  for (int i = 0; i <= nt; ++i) bound[i] = n * i / nt;
-  auto run = [](std::vector<std::thread>& th) {
-    for (auto& x : th) x.join();
-    th.clear();
-  };
-  std::vector<std::thread> th;
-  for (int i = 0; i < nt; ++i)
-    th.emplace_back([&, i] { std::sort(k.begin() + bound[i], k.begin() + bound[i + 1]); });
-  run(th);
-  for (int w = 1; w < nt; w *= 2) {
-    for (int i = 0; i + w < nt; i += 2 * w) {
-      const int64_t lo = bound[i], mid = bound[i + w], hi = bound[std::min(nt, i + 2 * w)];
-      th.emplace_back([&k, lo, mid, hi] { std::inplace_merge(k.begin() + lo, k.begin() + mid, k.begin() + hi); });
-    }
-    run(th);
-  }
-  std::vector<TaskT> sorted(static_cast<size_t>(n));
-  parallel_for(n, [&](int64_t a, int64_t b) {
-    for (int64_t i = a; i < b; ++i) sorted[i] = t[k[i].second];
-  });
-  t.swap(sorted);
-}
 
 }  // namespace
 
@@ -940,42 +965,57 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   std::vector<int64_t> roff(b->n_reads + 1), hoff(b->n_haps + 1);
   for (int64_t r = 0; r <= b->n_reads; ++r) roff[r] = b->n_reads ? b->read_off[r] - rb0 : 0;
   for (int64_t h = 0; h <= b->n_haps; ++h) hoff[h] = b->n_haps ? b->hap_off[h] - hb0 : 0;
-  std::vector<uint8_t> rclass(static_cast<size_t>(rbytes)), hmask(static_cast<size_t>(hbytes));
-  parallel_for(rbytes, [&](int64_t a0, int64_t a1) {
-    for (int64_t i = a0; i < a1; ++i) rclass[i] = al.read_class[b->read_bases[rb0 + i]];
-  });
-  // the last base of every read carries kReadEnd (the stacked kernel's read boundaries)
-  for (int64_t r = 0; r < b->n_reads; ++r)
-    if (roff[r + 1] > roff[r]) rclass[roff[r + 1] - 1] |= kReadEnd;
-  parallel_for(hbytes, [&](int64_t a0, int64_t a1) {
-    for (int64_t i = a0; i < a1; ++i) hmask[i] = al.hap_mask[b->hap_bases[hb0 + i]];
-  });
-  // Reads whose gap-continuation quality is the same at every base and not 0
-  // (the banded kernels' deletion scan needs a constant decay along the read,
-  // and the short-read kernel carries I and D times the gap-to-match factor
-  // 1 - ph2pr[q], which is 0 for q = 0).
+  const auto t_host0 = std::chrono::steady_clock::now();
+  // AVX mode: bases travel at two bits each (plus the 'N' positions) and are
+  // expanded on the device; the scalar mode's classes (up to 8 raw bytes)
+  // travel as class bytes.
+  const bool packed_bases = opt.mode == BV_MODE_AVX;
+  std::vector<uint8_t> rclass, hmask, rpack, hpack;
+  std::vector<int64_t> rexc, hexc;
+  if (packed_bases) {
+    uint8_t code_of[256];
+    for (int k = 0; k < 256; ++k) code_of[k] = al.read_class[k] & 3u;
+    pack_bases(b->read_bases + rb0, rbytes, code_of, 'N', rpack, rexc);
+    pack_bases(b->hap_bases + hb0, hbytes, code_of, 'N', hpack, hexc);
+  } else {
+    rclass.resize(static_cast<size_t>(rbytes));
+    hmask.resize(static_cast<size_t>(hbytes));
+    parallel_for(rbytes, [&](int64_t a0, int64_t a1) {
+      for (int64_t i = a0; i < a1; ++i) rclass[i] = al.read_class[b->read_bases[rb0 + i]];
+    });
+    // the last base of every read carries kReadEnd (the stacked kernel's read boundaries)
+    for (int64_t r = 0; r < b->n_reads; ++r)
+      if (roff[r + 1] > roff[r]) rclass[roff[r + 1] - 1] |= kReadEnd;
+    parallel_for(hbytes, [&](int64_t a0, int64_t a1) {
+      for (int64_t i = a0; i < a1; ++i) hmask[i] = al.hap_mask[b->hap_bases[hb0 + i]];
+    });
+  }
+  // One pass over the gap bytes.  Reads whose gap-continuation quality is the
+  // same at every base and not 0 (the banded kernels' deletion scan needs a
+  // constant decay along the read, and the short-read kernel carries I and D
+  // times the gap-to-match factor 1 - ph2pr[q], which is 0 for q = 0).  And
+  // whether the insertion, deletion and gap-continuation bytes are each
+  // constant along every read (every SamRecord's GOP/GCP strings, sam.hpp:
+  // 451-475): then they travel as one triplet per read and a kernel writes
+  // the per-base bytes on the device.
   std::vector<uint8_t> gcp_const(static_cast<size_t>(std::max<int64_t>(1, b->n_reads)), 0);
-  parallel_for(b->n_reads, [&](int64_t a0, int64_t a1) {
-    for (int64_t r = a0; r < a1; ++r) {
-      const uint8_t* c = b->read_gcp + b->read_off[r];
-      const int64_t R = b->read_off[r + 1] - b->read_off[r];
-      bool same = R == 0 || (c[0] & 127) != 0;
-      for (int64_t k = 1; k < R && same; ++k) same = (c[k] & 127) == (c[0] & 127);
-      gcp_const[r] = same ? 1 : 0;
-    }
-  });
-  // Reads whose insertion, deletion and gap-continuation bytes are each
-  // constant along the read (every SamRecord's GOP/GCP strings, sam.hpp:
-  // 451-475): when all reads are, they travel as one triplet per read and a
-  // kernel writes the per-base bytes on the device.
   std::vector<uint8_t> triplets(static_cast<size_t>(3 * std::max<int64_t>(1, b->n_reads)), 0);
   std::atomic<bool> all_uniform{true};
   parallel_for(b->n_reads, [&](int64_t a0, int64_t a1) {
     bool ok = true;
-    for (int64_t r = a0; r < a1 && ok; ++r) {
+    for (int64_t r = a0; r < a1; ++r) {
       const int64_t o = b->read_off[r], R = b->read_off[r + 1] - o;
       const uint8_t *i = b->read_ins + o, *d = b->read_del + o, *c = b->read_gcp + o;
-      for (int64_t k = 1; k < R && ok; ++k) ok = i[k] == i[0] && d[k] == d[0] && c[k] == c[0];
+      bool same_c = true, same_id = true;
+      for (int64_t k = 1; k < R; ++k) {
+        same_c = same_c && c[k] == c[0];
+        same_id = same_id && i[k] == i[0] && d[k] == d[0];
+      }
+      // the banded kernels compare the 7-bit quality (the byte & 127)
+      bool same_q = R == 0 || (c[0] & 127) != 0;
+      for (int64_t k = 1; k < R && same_q && !same_c; ++k) same_q = (c[k] & 127) == (c[0] & 127);
+      gcp_const[r] = same_q ? 1 : 0;
+      ok = ok && same_c && same_id;
       triplets[3 * r] = i[0];
       triplets[3 * r + 1] = d[0];
       triplets[3 * r + 2] = c[0];
@@ -1004,24 +1044,29 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   // the FP64 list instead: they underflow FP32's range at these lengths, and
   // the few that would not are re-run in FP32 afterwards (verify pass).
   const bool fp32_mode = opt.mode == BV_MODE_AVX && !opt.force_fp64;
-  std::vector<Task> tasks, tasks64, tasks_rest;
-  tasks.reserve(static_cast<size_t>(b->n_reads));
-  std::vector<int64_t> cost, cost64, cost_rest, cost_stack;
-  cost.reserve(static_cast<size_t>(b->n_reads));
+  // Built per chunk of consecutive regions on its own thread, then joined
+  // in region order (the same lists a single pass would build).
+  struct Built {
+    std::vector<Task> tasks, tasks64, rest;
+    std::vector<uint64_t> cost, cost64, cost_rest, cost_stack;
+    std::vector<StackTask> stasks;
+    int64_t cells = 0, longhap = 0;
+    int stack_hmax = 0;
+  };
+  p->n_regions = b->n_regions;
+  p->region_read_off.assign(b->region_read_off, b->region_read_off + (b->n_regions ? b->n_regions + 1 : 0));
+  p->region_hap_off.assign(b->region_hap_off, b->region_hap_off + (b->n_regions ? b->n_regions + 1 : 0));
+  p->region_pair_off.assign(static_cast<size_t>(b->n_regions ? b->n_regions + 1 : 0), 0);
+  for (int64_t g = 0; g < b->n_regions; ++g)
+    p->region_pair_off[g + 1] = p->region_pair_off[g] + (b->region_read_off[g + 1] - b->region_read_off[g]) *
+                                                            (b->region_hap_off[g + 1] - b->region_hap_off[g]);
   // FP32 mode: runs of consecutive reads (shorter than kFp64FirstRows) of a
   // region whose haplotypes all fit the stacked kernel's LDS carry row become
   // stacked tasks of at most kStackRows rows.  A run is cut early once it
   // fills its last 63-row stripe to within 6 rows (lanes left empty at the
   // task's end are the kernel's only partial stripe).
-  std::vector<StackTask> stasks;
-  int stack_hmax = 0;
-  int64_t pair_base = 0, cells = 0;
-  p->n_regions = b->n_regions;
-  p->region_read_off.assign(b->region_read_off, b->region_read_off + (b->n_regions ? b->n_regions + 1 : 0));
-  p->region_hap_off.assign(b->region_hap_off, b->region_hap_off + (b->n_regions ? b->n_regions + 1 : 0));
-  p->region_pair_off.assign(static_cast<size_t>(b->n_regions ? b->n_regions + 1 : 0), 0);
-  for (int64_t g = 0; g < b->n_regions; ++g) {
-    p->region_pair_off[g] = pair_base;
+  auto build_region = [&](int64_t g, Built& o) {
+    const int64_t pair_base = p->region_pair_off[g];
     const int64_t r0 = b->region_read_off[g], r1 = b->region_read_off[g + 1];
     const int64_t h0 = b->region_hap_off[g], h1 = b->region_hap_off[g + 1];
     const int64_t nh = h1 - h0;
@@ -1032,15 +1077,15 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     }
     const bool stackable = fp32_mode && nh > 0 && hmax_g <= kStackHmax;
     if (stackable) {
-      stack_hmax = std::max<int>(stack_hmax, static_cast<int>(hmax_g));
+      o.stack_hmax = std::max<int>(o.stack_hmax, static_cast<int>(hmax_g));
       int64_t run0 = -1, rows = 0;
       auto flush = [&](int64_t r_end) {
         if (run0 < 0) return;
-        stasks.push_back(StackTask{static_cast<int32_t>(run0), static_cast<int32_t>(r_end - run0),
-                                   static_cast<int32_t>(h0), static_cast<int32_t>(nh),
-                                   pair_base + (run0 - r0) * nh, static_cast<int32_t>(nh),
-                                   static_cast<int32_t>(rows)});
-        cost_stack.push_back(rows * hsum);
+        o.stasks.push_back(StackTask{static_cast<int32_t>(run0), static_cast<int32_t>(r_end - run0),
+                                     static_cast<int32_t>(h0), static_cast<int32_t>(nh),
+                                     pair_base + (run0 - r0) * nh, static_cast<int32_t>(nh),
+                                     static_cast<int32_t>(rows)});
+        o.cost_stack.push_back(static_cast<uint64_t>(rows * hsum));
         run0 = -1;
         rows = 0;
       };
@@ -1058,50 +1103,86 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       }
       flush(r1);
     }
+    int64_t longhaps = 0;
+    for (int64_t h = h0; h < h1; ++h) longhaps += hoff[h + 1] - hoff[h] > kStackHmax;
     for (int64_t r = r0; r < r1; ++r) {
       if (nh == 0) continue;
       const int64_t R = roff[r + 1] - roff[r];
       const int64_t out0 = pair_base + (r - r0) * nh;
-      cells += R * hsum;
+      o.cells += R * hsum;
       if (fp32_mode && R >= kFp64FirstRows) {
         for (int64_t h = h0; h < h1; ++h) {
-          tasks64.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
-          cost64.push_back(R * (hoff[h + 1] - hoff[h]));
+          o.tasks64.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
+          o.cost64.push_back(static_cast<uint64_t>(R * (hoff[h + 1] - hoff[h])));
         }
       } else if (R >= kSplitRows) {
+        o.longhap += longhaps;
         for (int64_t h = h0; h < h1; ++h) {
-          if (hoff[h + 1] - hoff[h] > kStackHmax) ++p->n_longhap_pairs;
-          tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
-          cost.push_back(R * (hoff[h + 1] - hoff[h]));
+          o.tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
+          o.cost.push_back(static_cast<uint64_t>(R * (hoff[h + 1] - hoff[h])));
           if (fp32_mode && !stackable) {
-            tasks_rest.push_back(tasks.back());
-            cost_rest.push_back(cost.back());
+            o.rest.push_back(o.tasks.back());
+            o.cost_rest.push_back(o.cost.back());
           }
         }
       } else {
-        for (int64_t h = h0; h < h1; ++h)
-          if (hoff[h + 1] - hoff[h] > kStackHmax) ++p->n_longhap_pairs;
-        tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0),
-                             static_cast<int32_t>(nh), 0, out0});
-        cost.push_back(R * hsum);
+        o.longhap += longhaps;
+        o.tasks.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0), static_cast<int32_t>(nh), 0, out0});
+        o.cost.push_back(static_cast<uint64_t>(R * hsum));
         if (fp32_mode && !stackable) {
-          tasks_rest.push_back(tasks.back());
-          cost_rest.push_back(cost.back());
+          o.rest.push_back(o.tasks.back());
+          o.cost_rest.push_back(o.cost.back());
         }
       }
     }
-    pair_base += (r1 - r0) * nh;
+  };
+  // chunks of regions with about equal read counts
+  const int64_t n_chunks = b->n_regions < 256 ? 1 : 64;
+  std::vector<int64_t> cut(static_cast<size_t>(n_chunks) + 1, b->n_regions);
+  cut[0] = 0;
+  for (int64_t c = 1, g = 0; c < n_chunks; ++c) {
+    const int64_t want = b->n_reads * c / n_chunks;
+    while (g < b->n_regions && b->region_read_off[g] < want) ++g;
+    cut[c] = std::max(cut[c - 1], g);
   }
-  if (b->n_regions) p->region_pair_off[b->n_regions] = pair_base;
+  std::vector<Built> parts(static_cast<size_t>(n_chunks));
+  parallel_for(n_chunks, [&](int64_t c0, int64_t c1) {
+    for (int64_t c = c0; c < c1; ++c)
+      for (int64_t g = cut[c]; g < cut[c + 1]; ++g) build_region(g, parts[c]);
+  }, 1);
+  // join in chunk order, each list's parts copied in parallel
+  auto join = [&](auto member, auto& dst) {
+    std::vector<int64_t> at(static_cast<size_t>(n_chunks) + 1, 0);
+    for (int64_t c = 0; c < n_chunks; ++c) at[c + 1] = at[c] + static_cast<int64_t>((parts[c].*member).size());
+    dst.resize(static_cast<size_t>(at[n_chunks]));
+    parallel_for(n_chunks, [&](int64_t c0, int64_t c1) {
+      for (int64_t c = c0; c < c1; ++c)
+        std::copy((parts[c].*member).begin(), (parts[c].*member).end(), dst.begin() + at[c]);
+    }, 1);
+  };
+  std::vector<Task> tasks, tasks64, tasks_rest;
+  std::vector<uint64_t> cost, cost64, cost_rest, cost_stack;
+  std::vector<StackTask> stasks;
+  join(&Built::tasks, tasks);
+  join(&Built::cost, cost);
+  join(&Built::tasks64, tasks64);
+  join(&Built::cost64, cost64);
+  join(&Built::rest, tasks_rest);
+  join(&Built::cost_rest, cost_rest);
+  join(&Built::stasks, stasks);
+  join(&Built::cost_stack, cost_stack);
+  int stack_hmax = 0;
+  int64_t pair_base = b->n_regions ? p->region_pair_off[b->n_regions] : 0, cells = 0;
+  for (const Built& o : parts) {
+    cells += o.cells;
+    p->n_longhap_pairs += o.longhap;
+    stack_hmax = std::max(stack_hmax, o.stack_hmax);
+  }
+  parts.clear();
   p->n_pairs = pair_base;
   p->cells = cells;
   if (tasks.size() >= (1u << 31) || tasks64.size() >= (1u << 31))
     return cleanup(fail(BV_EINVAL, "too many tasks"));
-  std::vector<Task> tasks_region = tasks;
-  longest_first(tasks, cost);
-  longest_first(tasks64, cost64);
-  longest_first(tasks_rest, cost_rest);
-  longest_first(stasks, cost_stack);
   p->n_stasks = static_cast<int>(stasks.size());
   p->n_tasks_rest = static_cast<int>(tasks_rest.size());
   p->stack_cols = stack_hmax + 128;  // lane 0 reads up to column H + 126
@@ -1137,24 +1218,30 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->stride64 = hmax + kCarryPad;
 
   hipStream_t st = p->stream;
-  if ((rc = dev_upload(&p->d_rclass, rclass.data(), rbytes, st)) ||
-      (rc = dev_upload(&p->d_q, b->read_qual + rb0, rbytes, st)) ||
+  const auto t_up0 = std::chrono::steady_clock::now();
+  int64_t* up = &p->setup_bytes;
+  if ((rc = dev_upload(&p->d_rclass, packed_bases ? nullptr : rclass.data(), packed_bases ? 0 : rbytes, st, rbytes,
+                       up)) ||
+      (rc = dev_upload(&p->d_q, b->read_qual + rb0, rbytes, st, 0, up)) ||
       (rc = dev_upload(&p->d_ins, gaps_per_read ? nullptr : b->read_ins + rb0, gaps_per_read ? 0 : rbytes, st,
-                       rbytes)) ||
+                       rbytes, up)) ||
       (rc = dev_upload(&p->d_del, gaps_per_read ? nullptr : b->read_del + rb0, gaps_per_read ? 0 : rbytes, st,
-                       rbytes)) ||
+                       rbytes, up)) ||
       (rc = dev_upload(&p->d_gcp, gaps_per_read ? nullptr : b->read_gcp + rb0, gaps_per_read ? 0 : rbytes, st,
-                       rbytes)) ||
-      (rc = dev_upload(&p->d_gcp_const, gcp_const.data(), b->n_reads, st)) ||
-      (rc = dev_upload(&p->d_read_off, roff.data(), b->n_reads + 1, st)) ||
-      (rc = dev_upload(&p->d_hmask, hmask.data(), hbytes, st)) ||
-      (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st)) ||
-      (rc = dev_upload(&p->d_tasks, tasks.data(), static_cast<int64_t>(tasks.size()), st)) ||
-      (rc = dev_upload(&p->d_tasks_region, tasks_region.data(), static_cast<int64_t>(tasks_region.size()), st)) ||
-      (rc = dev_upload(&p->d_stasks, stasks.data(), static_cast<int64_t>(stasks.size()), st)) ||
-      (rc = dev_upload(&p->d_tasks_rest, tasks_rest.data(), static_cast<int64_t>(tasks_rest.size()), st)) ||
-      (rc = dev_upload(&p->d_tasks64, tasks64.data(), static_cast<int64_t>(tasks64.size()), st)) ||
-      (rc = dev_upload(&p->d_verify, tasks64.data(), static_cast<int64_t>(tasks64.size()), st)))
+                       rbytes, up)) ||
+      (rc = dev_upload(&p->d_gcp_const, gcp_const.data(), b->n_reads, st, 0, up)) ||
+      (rc = dev_upload(&p->d_read_off, roff.data(), b->n_reads + 1, st, 0, up)) ||
+      (rc = dev_upload(&p->d_hmask, packed_bases ? nullptr : hmask.data(), packed_bases ? 0 : hbytes, st, hbytes,
+                       up)) ||
+      (rc = dev_upload(&p->d_hap_off, hoff.data(), b->n_haps + 1, st, 0, up)) ||
+      (rc = dev_upload(&p->d_tasks_region, tasks.data(), static_cast<int64_t>(tasks.size()), st, 0, up)) ||
+      (rc = upload_longest_first(&p->d_tasks, p->d_tasks_region, tasks, cost, st, up)) ||
+      (rc = upload_longest_first(&p->d_stasks, static_cast<const StackTask*>(nullptr), stasks, cost_stack, st, up)) ||
+      (rc = upload_longest_first(&p->d_tasks_rest, static_cast<const Task*>(nullptr), tasks_rest, cost_rest, st,
+                                 up)) ||
+      (rc = upload_longest_first(&p->d_tasks64, static_cast<const Task*>(nullptr), tasks64, cost64, st, up)) ||
+      (rc = dev_upload(&p->d_verify, static_cast<const Task*>(nullptr), 0, st,
+                       std::max<int64_t>(1, static_cast<int64_t>(tasks64.size())))))
     return cleanup(rc);
   if (hipMalloc(&p->d_rescue, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_counters, sizeof(int) * kCounters) != hipSuccess ||
@@ -1164,9 +1251,33 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       hipMalloc(&p->d_carry32, sizeof(float) * 2 * p->stride32 * p->grid32) != hipSuccess ||
       hipMalloc(&p->d_carry64, sizeof(double) * 2 * p->stride64 * p->grid64) != hipSuccess)
     return cleanup(fail(BV_ENOMEM, "device allocation failed"));
+  if (packed_bases) {
+    // two-bit bases and the 'N' positions, expanded to class bytes (with the
+    // read-end flag) and haplotype match masks on the device
+    uint8_t *d_rpack = nullptr, *d_hpack = nullptr;
+    int64_t *d_rexc = nullptr, *d_hexc = nullptr;
+    auto drop = [&] {
+      for (void* q : {static_cast<void*>(d_rpack), static_cast<void*>(d_hpack), static_cast<void*>(d_rexc),
+                      static_cast<void*>(d_hexc)})
+        if (q) (void)hipFree(q);
+    };
+    if ((rc = dev_upload(&d_rpack, rpack.data(), static_cast<int64_t>(rpack.size()), st, 0, up)) ||
+        (rc = dev_upload(&d_hpack, hpack.data(), static_cast<int64_t>(hpack.size()), st, 0, up)) ||
+        (rc = dev_upload(&d_rexc, rexc.data(), static_cast<int64_t>(rexc.size()), st, 0, up)) ||
+        (rc = dev_upload(&d_hexc, hexc.data(), static_cast<int64_t>(hexc.size()), st, 0, up))) {
+      drop();
+      return cleanup(rc);
+    }
+    const hipError_t e = launch_expand_bases(p->d_read_off, b->n_reads, d_rpack, d_rexc,
+                                             static_cast<int64_t>(rexc.size()), p->d_rclass, hbytes, d_hpack, d_hexc,
+                                             static_cast<int64_t>(hexc.size()), p->d_hmask, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    drop();
+    if (e != hipSuccess || es != hipSuccess) return cleanup(fail(BV_EHIP, "base expansion failed"));
+  }
   if (gaps_per_read) {
     uint8_t* d_trip = nullptr;
-    if ((rc = dev_upload(&d_trip, triplets.data(), 3 * b->n_reads, st))) {
+    if ((rc = dev_upload(&d_trip, triplets.data(), 3 * b->n_reads, st, 0, up))) {
       if (d_trip) (void)hipFree(d_trip);
       return cleanup(rc);
     }
@@ -1179,6 +1290,9 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   // The 64-bit rescue cell counter shares the zeroed counter block.
   p->d_rescue_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 6);
   if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(BV_EHIP, "upload failed"));
+  const auto t_end = std::chrono::steady_clock::now();
+  p->setup_ms_host = std::chrono::duration<float, std::milli>(t_up0 - t_host0).count();
+  p->setup_ms_upload = std::chrono::duration<float, std::milli>(t_end - t_up0).count();
   *out = p;
   return BV_OK;
 }
@@ -1396,6 +1510,9 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->last_ms_long = p->last_mslong;
   s->long_band_pairs = long_banded ? cnt[21] : 0;
   s->last_ms_post = p->last_mspost;
+  s->setup_ms_host = p->setup_ms_host;
+  s->setup_ms_upload = p->setup_ms_upload;
+  s->setup_bytes = p->setup_bytes;
   return BV_OK;
 }
 

@@ -28,6 +28,7 @@
 // classes, expanded into a sliding LDS window of f16 selectors; the prior is
 // one fused multiply-add: prior = sel * (pm - pmis) + pmis.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -418,6 +419,37 @@ __global__ __launch_bounds__(64) void expand_read_bytes_kernel(const int64_t* __
   }
 }
 
+// Two-bit bases to the kernels' per-base bytes.  Reads: class = the code
+// (A C T G -> 0..3), kReadEnd on a read's last base; haplotypes: match mask
+// 1 << code.  Then the 'N' positions: the wildcard class bit for reads, the
+// all-classes mask for haplotypes.  One workgroup per read at a time; the
+// haplotype bytes grid-stride.
+__global__ __launch_bounds__(256) void expand_bases_kernel(const int64_t* __restrict__ read_off, int64_t n_reads,
+                                                           const uint8_t* __restrict__ rpack,
+                                                           uint8_t* __restrict__ rclass, int64_t hbytes,
+                                                           const uint8_t* __restrict__ hpack,
+                                                           uint8_t* __restrict__ hmask) {
+  for (int64_t r = blockIdx.x; r < n_reads; r += gridDim.x) {
+    const int64_t a = read_off[r], b = read_off[r + 1];
+    for (int64_t p = a + threadIdx.x; p < b; p += blockDim.x) {
+      const uint8_t code = (rpack[p >> 2] >> (2 * (p & 3))) & 3u;
+      rclass[p] = static_cast<uint8_t>(code | (p == b - 1 ? kReadEnd : 0));
+    }
+  }
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < hbytes;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    hmask[i] = static_cast<uint8_t>(1u << ((hpack[i >> 2] >> (2 * (i & 3))) & 3u));
+}
+
+__global__ __launch_bounds__(256) void mark_wildcards_kernel(const int64_t* __restrict__ rexc, int64_t n_rexc,
+                                                             uint8_t* __restrict__ rclass,
+                                                             const int64_t* __restrict__ hexc, int64_t n_hexc,
+                                                             uint8_t* __restrict__ hmask) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n_rexc) rclass[rexc[i]] |= kWildcard;
+  if (i < n_hexc) hmask[hexc[i]] = 0xFF;
+}
+
 }  // namespace bvp
 
 // Device scratch of bv_compute_full_prob_noalloc, for callers that cannot
@@ -448,6 +480,73 @@ hipError_t launch_expand_read_bytes(const int64_t* read_off, int64_t n_reads, co
   const int64_t grid = std::min<int64_t>(n_reads, 65536);
   hipLaunchKernelGGL(expand_read_bytes_kernel, dim3(static_cast<unsigned>(grid)), dim3(64), 0, st, read_off,
                      n_reads, triplets, ins, del, gcp);
+  return hipGetLastError();
+}
+
+// Longest first on the device: a stable radix sort of the task indices by
+// descending cost (ties keep the input order, as the host's stable sort
+// did), then a gather of the tasks.  Costs are below 2^end_bit.
+__global__ void iota_kernel(int32_t* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[i] = static_cast<int32_t>(i);
+}
+
+template <typename T>
+__global__ void gather_kernel(const T* __restrict__ in, const int32_t* __restrict__ idx, int64_t n,
+                              T* __restrict__ out) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[i] = in[idx[i]];
+}
+
+template <typename T>
+hipError_t sort_longest_first(const T* d_in, const uint64_t* d_cost, int64_t n, int end_bit, T* d_out,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int num = static_cast<int>(n);
+  uint64_t* keys_out = nullptr;
+  int32_t *idx = nullptr, *idx_out = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, d_cost, keys_out, idx, idx_out,
+                                                              num, 0, end_bit, st);
+  if (e == hipSuccess) e = hipMalloc(&keys_out, sizeof(uint64_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&idx, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&idx_out, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 65536));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(iota_kernel, dim3(grid), dim3(256), 0, st, idx, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, d_cost, keys_out, idx, idx_out, num, 0, end_bit,
+                                                     st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(gather_kernel<T>, dim3(grid), dim3(256), 0, st, d_in, idx_out, n, d_out);
+    e = hipGetLastError();
+  }
+  const hipError_t es = hipStreamSynchronize(st);
+  for (void* q : {static_cast<void*>(keys_out), static_cast<void*>(idx), static_cast<void*>(idx_out), tmp})
+    if (q) (void)hipFree(q);
+  return e != hipSuccess ? e : es;
+}
+
+template hipError_t sort_longest_first<Task>(const Task*, const uint64_t*, int64_t, int, Task*, hipStream_t);
+template hipError_t sort_longest_first<StackTask>(const StackTask*, const uint64_t*, int64_t, int, StackTask*,
+                                                  hipStream_t);
+
+hipError_t launch_expand_bases(const int64_t* read_off, int64_t n_reads, const uint8_t* rpack, const int64_t* rexc,
+                               int64_t n_rexc, uint8_t* rclass, int64_t hbytes, const uint8_t* hpack,
+                               const int64_t* hexc, int64_t n_hexc, uint8_t* hmask, hipStream_t st) {
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(std::max(n_reads, (hbytes + 255) / 256), 65536));
+  hipLaunchKernelGGL(expand_bases_kernel, dim3(static_cast<unsigned>(grid)), dim3(256), 0, st, read_off, n_reads,
+                     rpack, rclass, hbytes, hpack, hmask);
+  const int64_t n = std::max(n_rexc, n_hexc);
+  if (n > 0)
+    hipLaunchKernelGGL(mark_wildcards_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, rexc,
+                       n_rexc, rclass, hexc, n_hexc, hmask);
   return hipGetLastError();
 }
 

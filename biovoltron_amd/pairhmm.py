@@ -65,6 +65,7 @@ class PlanStats(C.Structure):
         ("band_pairs", C.c_int64), ("fallback_pairs", C.c_int64), ("cells_swept", C.c_int64),
         ("last_ms_fallback", C.c_float), ("last_ms_band", C.c_float),
         ("long_band_pairs", C.c_int64), ("last_ms_long", C.c_float), ("last_ms_post", C.c_float),
+        ("setup_ms_host", C.c_float), ("setup_ms_upload", C.c_float), ("setup_bytes", C.c_int64),
     ]
 
 

@@ -187,6 +187,9 @@ typedef struct bv_plan_stats {
   int64_t long_band_pairs; /* long reads accepted on the moving band (also in band_pairs) */
   float last_ms_long;      /* moving-band kernel (long reads), last timed run */
   float last_ms_post;      /* post-processing kernel, last bv_plan_postprocess */
+  float setup_ms_host;     /* bv_plan_create: host work (classes, work lists) */
+  float setup_ms_upload;   /* bv_plan_create: uploads and their expansion on the device */
+  int64_t setup_bytes;     /* bytes bv_plan_create uploaded */
 } bv_plan_stats;
 
 /* Uploads a batch to device memory and builds the length-ordered work list. */
