@@ -441,16 +441,25 @@ def main():
     swp = sw_pairs(b, args.sw_regions)
     sw_info = None
     if swp:
-        timing = {}
-        t_sw = time.perf_counter()
-        sw_res = sw.align_batch(swp, timing=timing)
-        t_sw = time.perf_counter() - t_sw
+        # two calls: the first allocates the library's Smith-Waterman
+        # workspace, the second finds it in place (a caller's steady state)
+        calls = []
+        for _ in range(2):
+            timing = {}
+            t_sw = time.perf_counter()
+            sw_res = sw.align_batch(swp, timing=timing)
+            calls.append((time.perf_counter() - t_sw, timing["kernel_ms"], timing["call_ms"]))
         sw_cells = sum(len(r) * len(a) for r, a in swp)
-        sw_info = dict(pairs=len(swp), cells=sw_cells, kernel_ms=round(timing["kernel_ms"], 3),
-                       gcups=round(sw_cells / (timing["kernel_ms"] / 1e3) / 1e9, 2), call_s=round(t_sw, 3),
+        (t_first, _, c_first), (t_sw, k_ms, c_ms) = calls
+        sw_info = dict(pairs=len(swp), cells=sw_cells, kernel_ms=round(k_ms, 3),
+                       gcups=round(sw_cells / (k_ms / 1e3) / 1e9, 2),
+                       library_call_ms=round(c_ms, 1), library_call_gcups=round(sw_cells / (c_ms / 1e3) / 1e9, 2),
+                       first_library_call_ms=round(c_first, 1),
+                       python_call_s=round(t_sw, 3), first_python_call_s=round(t_first, 3),
                        params="NEW_SW_PARAMETERS (200, -150, -260, -11)")
-        print(f"[bench] smith-waterman: {len(swp)} pairs, {timing['kernel_ms']:.2f} ms kernel, "
-              f"{sw_info['gcups']} GCUPS", file=sys.stderr, flush=True)
+        print(f"[bench] smith-waterman: {len(swp)} pairs, {k_ms:.2f} ms kernel, {sw_info['gcups']} GCUPS; "
+              f"library call {c_ms:.1f} ms ({sw_info['library_call_gcups']} GCUPS; first {c_first:.1f} ms); "
+              f"Python wrapper {t_sw:.3f} s", file=sys.stderr, flush=True)
     print(f"[bench] post-processing: {post_ms:.3f} ms, {post_info['hbm_gbps']} GB/s, "
           f"copy-back {post_info['d2h_bytes'] / 1e9:.2f} GB vs {post_info['d2h_bytes_full_matrix'] / 1e9:.2f} GB",
           file=sys.stderr, flush=True)

@@ -167,9 +167,15 @@ struct DeviceCtx {
   // device address of the allocation-free entry point's static scratch on
   // this device (looked up once at init, with the device current)
   void* d_noalloc_scratch = nullptr;
+  // Smith-Waterman workspace kept across bv_sw_align calls (grown, never
+  // shrunk, until bv_pairhmm_shutdown): per-wavefront scratch and the queues
+  char* d_sw_scratch = nullptr;
+  int64_t sw_scratch_bytes = 0;
+  int* d_sw_queue = nullptr;
 };
 
 std::mutex g_mu;
+std::mutex g_sw_mu;  // one bv_sw_align at a time owns the workspace
 DeviceCtx g_ctx;
 
 int ensure_init_locked(int device) {
@@ -866,6 +872,11 @@ void bv_pairhmm_shutdown(void) {
   (void)hipFree(g_ctx.d_mm_f);
   (void)hipFree(g_ctx.d_mm_d);
   (void)hipFree(g_ctx.d_err_scalar);
+  {
+    std::lock_guard<std::mutex> sk(g_sw_mu);
+    if (g_ctx.d_sw_scratch) (void)hipFree(g_ctx.d_sw_scratch);
+    if (g_ctx.d_sw_queue) (void)hipFree(g_ctx.d_sw_queue);
+  }
   g_ctx = DeviceCtx{};
 }
 
@@ -1719,16 +1730,34 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
         1, std::min<int64_t>({nc, static_cast<int64_t>(g_ctx.cus) * 32, budget / cls_stride[c]})));
     scratch_bytes = std::max(scratch_bytes, cls_stride[c] * cls_grid[c]);
   }
+  // The per-wavefront scratch and the queues live in the library's
+  // workspace, grown to this call's need; calls take turns on it.
+  std::unique_lock<std::mutex> sw_lock(g_sw_mu);
+  if (g_ctx.sw_scratch_bytes < scratch_bytes) {
+    if (g_ctx.d_sw_scratch) (void)hipFree(g_ctx.d_sw_scratch);
+    g_ctx.d_sw_scratch = nullptr;
+    g_ctx.sw_scratch_bytes = 0;
+    if (hipMalloc(&g_ctx.d_sw_scratch, static_cast<size_t>(scratch_bytes)) != hipSuccess) {
+      g_ctx.d_sw_scratch = nullptr;
+      return fail(BV_ENOMEM, "Smith-Waterman scratch allocation failed");
+    }
+    g_ctx.sw_scratch_bytes = scratch_bytes;
+  }
+  if (!g_ctx.d_sw_queue && hipMalloc(&g_ctx.d_sw_queue, sizeof(int) * kClasses) != hipSuccess) {
+    g_ctx.d_sw_queue = nullptr;
+    return fail(BV_ENOMEM, "Smith-Waterman queue allocation failed");
+  }
+  char* const d_scr = g_ctx.d_sw_scratch;
+  int* const d_queue = g_ctx.d_sw_queue;
   hipStream_t st = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   int64_t *d_roff = nullptr, *d_aoff = nullptr, *d_out = nullptr;
   uint8_t *d_ref = nullptr, *d_alt = nullptr;
   int32_t *d_offset = nullptr, *d_n = nullptr, *d_lens = nullptr, *d_order = nullptr;
-  char *d_ops = nullptr, *d_scr = nullptr;
-  int* d_queue = nullptr;
+  char* d_ops = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&](int code) {
-    void* bufs[] = {d_roff, d_aoff, d_out, d_ref, d_alt, d_offset, d_n, d_lens, d_ops, d_scr, d_queue, d_order};
+    void* bufs[] = {d_roff, d_aoff, d_out, d_ref, d_alt, d_offset, d_n, d_lens, d_ops, d_order};
     for (void* x : bufs)
       if (x) (void)hipFree(x);
     if (e0) (void)hipEventDestroy(e0);
@@ -1744,9 +1773,7 @@ int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out
       (rc = dev_upload(&d_order, order.data(), n, st)))
     return cleanup(rc);
   if (hipMalloc(&d_offset, sizeof(int32_t) * n) != hipSuccess || hipMalloc(&d_n, sizeof(int32_t) * n) != hipSuccess ||
-      hipMalloc(&d_lens, sizeof(int32_t) * slots) != hipSuccess || hipMalloc(&d_ops, slots) != hipSuccess ||
-      hipMalloc(&d_scr, static_cast<size_t>(scratch_bytes)) != hipSuccess ||
-      hipMalloc(&d_queue, sizeof(int) * kClasses) != hipSuccess)
+      hipMalloc(&d_lens, sizeof(int32_t) * slots) != hipSuccess || hipMalloc(&d_ops, slots) != hipSuccess)
     return cleanup(fail(BV_ENOMEM, "Smith-Waterman device allocation failed"));
   if (hipMemsetAsync(d_queue, 0, sizeof(int) * kClasses, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess)

@@ -250,7 +250,10 @@ void bv_plan_destroy(bv_plan* plan);
  * Outputs: offset_out[k], n_elems_out[k], and the CIGAR elements of pair k
  * in ops_out / lens_out from slot sum_{m<k} (R_m + H_m + 3) on.  Sequences
  * of 1..4096 bases (the reference asserts non-empty ones).  kernel_ms
- * (optional) receives the kernel time.  Synchronous. */
+ * (optional) receives the kernel time.  Synchronous.  The per-wavefront
+ * scratch (sized by the call's longest pairs, at most 16 GiB) stays
+ * allocated for the next call, grown when a call needs more, and is freed
+ * by bv_pairhmm_shutdown; concurrent calls take turns on it. */
 typedef struct bv_sw_batch {
   int64_t n_pairs;
   const int64_t* ref_off;

@@ -233,3 +233,38 @@ def test_device_sw_rejects_bad_input():
         sw.align_batch([(b"A" * 5000, b"ACGT")])
     with pytest.raises(pairhmm.PairHMMError, match="65536"):
         sw.align_batch([(b"ACGT", b"ACGA")], (1 << 20, -1, -4, -3))
+
+
+@pytest.mark.gpu
+def test_device_sw_size_class_edges_and_longest_pairs():
+    """Pairs on both sides of every size-class edge (longer side 512/513,
+    1024/1025, 2048/2049 bases), pairs of 2,049 to 4,096 bases (the largest
+    class, with the largest per-wavefront scratch) and one 4,096 x 4,096
+    pair, in one call and again in a second call (the workspace kept from
+    the first), element for element against the oracle."""
+    from biovoltron_amd import sw
+
+    rng = np.random.default_rng(47)
+    pairs = []
+    for edge in (512, 1024, 2048):
+        for m in (edge, edge + 1):
+            r, a = random_pair(rng, m, mut=0.02)
+            pairs.append((r[:m], a))              # reference side exactly m bases
+            pairs.append((a[: max(1, m // 3)], r[:m]))  # alternate side m bases
+    pairs += [random_pair(rng, int(rng.integers(2049, 4000)), mut=0.03) for _ in range(4)]
+    r4 = bytes(rng.choice(list(b"ACGT"), 4096).astype(np.uint8))
+    a4 = bytearray(r4)
+    for k in rng.integers(0, 4096, 60):
+        a4[int(k)] = int(rng.choice(list(b"ACGT")))
+    pairs.append((r4, bytes(a4)))
+    pairs = [(r[:4096], a[:4096]) for r, a in pairs]
+    print(f"sw: {len(pairs)} long and edge pairs, longest {max(max(len(r), len(a)) for r, a in pairs)}", flush=True)
+    for call in range(2):
+        got = sw.align_batch(pairs)
+        for k, (r, a) in enumerate(pairs):
+            assert got[k] == oracle_align(r, a), (call, k, len(r), len(a))
+    # a small call after the large ones reuses the workspace
+    small = _device_pairs(rng, 50, hi=120)
+    got = sw.align_batch(small)
+    for k in range(len(small)):
+        assert got[k] == oracle_align(*small[k]), k
