@@ -14,10 +14,10 @@ def test_profiled_traffic_reads_the_committed_summary():
     import bench
 
     band = bench.profiled_traffic("pairhmm_workload4.md", "learned",
-                                  ["band_forward_kernel<64>", "band_forward_kernel<32>"])
+                                  ["band_quad_kernel", "band_forward_kernel<64>", "band_forward_kernel<32>"])
     full = bench.profiled_traffic("pairhmm_workload4.md", "full", ["full-matrix FP32 pass (short reads, stacked)"])
     assert band is not None and full is not None
-    # both banded kernels together move some hundreds of GB per launch; the
+    # the three banded kernels together move over a hundred GB per run; the
     # stacked full-matrix FP32 pass keeps its carry row in LDS and moves
     # about a hundred GB (read bytes and per-read sums)
     assert 1e11 < band[0] < 1e12 and 1e10 < full[0] < 1e12
