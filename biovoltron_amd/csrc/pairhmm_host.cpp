@@ -1661,6 +1661,15 @@ int bv_pairhmm_batch(const bv_region_batch* b, const bv_options* opt, double* lo
   return rc;
 }
 
+void bv_sw_release(void) {
+  std::lock_guard<std::mutex> sk(g_sw_mu);
+  if (g_ctx.d_sw_scratch) (void)hipFree(g_ctx.d_sw_scratch);
+  if (g_ctx.d_sw_queue) (void)hipFree(g_ctx.d_sw_queue);
+  g_ctx.d_sw_scratch = nullptr;
+  g_ctx.d_sw_queue = nullptr;
+  g_ctx.sw_scratch_bytes = 0;
+}
+
 int bv_sw_align(const bv_sw_batch* b, const int32_t* params, int32_t* offset_out, int32_t* n_elems_out,
                 char* ops_out, int32_t* lens_out, float* kernel_ms) {
   if (!b || !params || !offset_out || !n_elems_out || !ops_out || !lens_out)

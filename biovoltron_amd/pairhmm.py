@@ -87,7 +87,7 @@ EXPORTS = (
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
     "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_set_band_shape", "bv_plan_band_shape",
     "bv_plan_set_band_prologue",
-    "bv_plan_postprocess", "bv_sw_align",
+    "bv_plan_postprocess", "bv_sw_align", "bv_sw_release",
     "bv_pairhmm_batch",
 )
 

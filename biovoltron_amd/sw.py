@@ -73,6 +73,14 @@ def align_batch(pairs, params=None, timing: dict | None = None):
     return out
 
 
+def release() -> None:
+    """Frees the aligner's scratch that the library keeps between calls
+    (bv_sw_release)."""
+    lib = _lib()
+    lib.bv_sw_release.restype = None
+    lib.bv_sw_release()
+
+
 def cigar_string(elements) -> str:
     """Cigar's string form (cigar.hpp:340-344): every element, zero-length
     ones included."""

@@ -253,7 +253,8 @@ void bv_plan_destroy(bv_plan* plan);
  * (optional) receives the kernel time.  Synchronous.  The per-wavefront
  * scratch (sized by the call's longest pairs, at most 16 GiB) stays
  * allocated for the next call, grown when a call needs more, and is freed
- * by bv_pairhmm_shutdown; concurrent calls take turns on it. */
+ * by bv_sw_release or bv_pairhmm_shutdown; concurrent calls take turns on
+ * it. */
 typedef struct bv_sw_batch {
   int64_t n_pairs;
   const int64_t* ref_off;
@@ -263,6 +264,10 @@ typedef struct bv_sw_batch {
 } bv_sw_batch;
 int bv_sw_align(const bv_sw_batch* batch, const int32_t* params, int32_t* offset_out, int32_t* n_elems_out,
                 char* ops_out, int32_t* lens_out, float* kernel_ms);
+/* Frees bv_sw_align's kept scratch (the next call allocates it again).  No
+ * reference counterpart: for callers that align once and then need the
+ * device memory back, e.g. for a large resident PairHMM plan. */
+void bv_sw_release(void);
 
 #define BV_POST_NORMALIZE 1
 int bv_plan_postprocess(bv_plan* plan, int32_t flags, const int32_t* hap_allele,

@@ -263,8 +263,12 @@ def test_device_sw_size_class_edges_and_longest_pairs():
         got = sw.align_batch(pairs)
         for k, (r, a) in enumerate(pairs):
             assert got[k] == oracle_align(r, a), (call, k, len(r), len(a))
-    # a small call after the large ones reuses the workspace
+    # a small call after the large ones reuses the workspace; after a
+    # release the next call allocates it again
     small = _device_pairs(rng, 50, hi=120)
-    got = sw.align_batch(small)
-    for k in range(len(small)):
-        assert got[k] == oracle_align(*small[k]), k
+    for release in (False, True):
+        if release:
+            sw.release()
+        got = sw.align_batch(small)
+        for k in range(len(small)):
+            assert got[k] == oracle_align(*small[k]), (release, k)
