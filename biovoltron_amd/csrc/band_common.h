@@ -33,7 +33,15 @@ struct FeatArgs {
   int* anchor;   // [pairs]
   int2* ends;    // [pairs]: first seed start, last seed end (read positions) among
                  // hits within kEndAnchorTol of the anchor; (-1, -1) without hits
-  int* next;     // [8] task claim counters, one per XCD, zero at launch
+  int* next;     // [8] task claim counters, one per XCD (kCtrStride apart), zero at launch
+  // Compact records instead of X / anchor / ends when not null: per pair
+  // kRecBytes (the 32 bin counts as bytes, the anchor, the seed-end span as
+  // two int16), per task (region order) the 1 / seeds factor and the mean
+  // and minimum quality features.
+  uint8_t* rec;
+  float* trec;  // [tasks][3]
 };
+
+constexpr int kRecBytes = 48;  // 16-byte aligned: three 16-byte loads
 
 }  // namespace bvp

@@ -293,7 +293,7 @@ __device__ __forceinline__ TaskClaims xcd_task_range(int n_tasks) {
 // The first task of the next claim of this wavefront (wave-uniform), or -1.
 __device__ __forceinline__ int claim_tasks(int* next, const TaskClaims& c, int lane) {
   int c0 = 0;
-  if (lane == 0) c0 = atomicAdd(next + c.ctr, kFeatClaim);
+  if (lane == 0) c0 = atomicAdd(next + c.ctr * kCtrStride, kFeatClaim);
   c0 = __builtin_amdgcn_readfirstlane(__shfl(c0, 0));
   return c0 >= c.end - c.begin ? -1 : c.begin + c0;
 }
@@ -327,6 +327,15 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
           const int64_t o = tk.out0 + h;
           const int H = static_cast<int>(a.hap_off[hap + 1] - a.hap_off[hap]);
           const PairFeat f = pair_features(ch.first[c], ch.occ[c], ch.nocc[c], rs.my_p, R, H, lane, hist);
+          if (a.rec != nullptr) {
+            uint8_t* rp = a.rec + o * kRecBytes;
+            if (lane < kBins) rp[lane] = static_cast<uint8_t>(f.cnt);
+            if (lane == 0)
+              *reinterpret_cast<int2*>(rp + kBins) =
+                  int2{f.anchor, static_cast<int>((static_cast<uint32_t>(f.pf) & 0xffffu) |
+                                                  (static_cast<uint32_t>(f.pl) << 16))};
+            continue;
+          }
           float* X = a.X + o * kFeat;
           if (lane == 0) {
             X[0] = static_cast<float>(R) / 256.f;
@@ -338,6 +347,11 @@ __global__ __launch_bounds__(64) void band_features_kernel(FeatArgs a) {
           }
           if (lane < kBins) X[4 + lane] = static_cast<float>(f.cnt) * inv;
         }
+      }
+      if (a.trec != nullptr && lane == 0) {
+        a.trec[3 * ti] = inv;
+        a.trec[3 * ti + 1] = rs.fq_mean;
+        a.trec[3 * ti + 2] = rs.fq_min;
       }
     }
   }
@@ -763,6 +777,103 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
   if (odd) a.items32[b32 + __popcll(w32 & below(end)) + pads_before] = BandItem{read, -1, 0, 0, -1};
 }
 
+// The two-kernel prologue's buffer filling: the features come as compact
+// records from band_features_kernel.  A wavefront claims kRecClaim tasks at
+// once and lane j loads task j's fields, so the loads of a claim overlap; the
+// slots of the buffer (slot = lane) are then assigned to (task, haplotype)
+// in task order with the same pieces and flush points as the one-kernel
+// prologue, and every assigned slot loads its record in one pass before a
+// flush or the next claim.
+constexpr int kRecClaim = 64;
+
+__device__ __forceinline__ void prologue_records(const ProlArgs& A, const float* __restrict__ w, ProBuf& B,
+                                                 const TaskClaims& cl, int lane) {
+  const FeatArgs& a = A.f;
+  int fill = 0;
+  unsigned long long seg_first = 0;
+  for (;;) {
+    int c0 = 0;
+    if (lane == 0) c0 = atomicAdd(a.next + cl.ctr * kCtrStride, kRecClaim);
+    c0 = __builtin_amdgcn_readfirstlane(__shfl(c0, 0));
+    if (c0 >= cl.end - cl.begin) break;
+    const int tc0 = cl.begin + c0;
+    const int nt = min(cl.end, tc0 + kRecClaim) - tc0;
+    Task tk{0, 0, 0, 0, 0};
+    int R = 0, gc = 0;
+    float inv = 0.f, fqm = 0.f, fqn = 0.f;
+    if (lane < nt) {
+      const int ti = tc0 + lane;
+      tk = a.tasks[ti];
+      const int64_t ro = a.read_off[tk.read];
+      R = static_cast<int>(a.read_off[tk.read + 1] - ro);
+      inv = a.trec[3 * ti];
+      fqm = a.trec[3 * ti + 1];
+      fqn = a.trec[3 * ti + 2];
+      gc = A.b.gcp_const[tk.read] != 0;
+    }
+    int my_t = 0, my_h = 0, loaded = fill;
+    // slots [loaded, fill) take their task's fields and their record
+    auto load_slots = [&]() {
+      const bool mine = lane >= loaded && lane < fill;
+      const int read = __shfl(tk.read, my_t), hap0 = __shfl(tk.hap0, my_t);
+      const long long out0 = __shfl(static_cast<long long>(tk.out0), my_t);
+      const int Rs = __shfl(R, my_t), gcs = __shfl(gc, my_t);
+      const float invs = __shfl(inv, my_t), fqms = __shfl(fqm, my_t), fqns = __shfl(fqn, my_t);
+      if (mine) {
+        const int hap = hap0 + my_h;
+        const long long o = out0 + my_h;
+        const uint4* rp = reinterpret_cast<const uint4*>(a.rec + o * kRecBytes);
+        const uint4 q0 = rp[0], q1 = rp[1];
+        const int2 ap = *reinterpret_cast<const int2*>(a.rec + o * kRecBytes + kBins);
+        const int H = static_cast<int>(a.hap_off[hap + 1] - a.hap_off[hap]);
+        B.cnt[lane][0] = q0.x; B.cnt[lane][1] = q0.y; B.cnt[lane][2] = q0.z; B.cnt[lane][3] = q0.w;
+        B.cnt[lane][4] = q1.x; B.cnt[lane][5] = q1.y; B.cnt[lane][6] = q1.z; B.cnt[lane][7] = q1.w;
+        B.hap[lane] = hap;
+        B.H[lane] = H;
+        B.anchor[lane] = ap.x;
+        B.pf[lane] = static_cast<int16_t>(ap.y & 0xffff);
+        B.pl[lane] = static_cast<int16_t>(static_cast<uint32_t>(ap.y) >> 16);
+        B.read[lane] = read;
+        B.R[lane] = Rs;
+        B.inv[lane] = invs;
+        B.fqm[lane] = fqms;
+        B.fqn[lane] = fqns;
+        B.out[lane] = o;
+        B.gc[lane] = gcs;
+      }
+    };
+    for (int t = 0; t < nt; ++t) {
+      const int nhap = __builtin_amdgcn_readlane(tk.nhap, t);
+      for (int hs = 0; hs < nhap; hs += 64) {
+        const int nsub = min(64, nhap - hs);
+        if (fill + nsub > 64) {
+          load_slots();
+          __builtin_amdgcn_wave_barrier();
+          prologue_flush(A, w, B, fill, seg_first, lane);
+          __builtin_amdgcn_wave_barrier();
+          fill = 0;
+          loaded = 0;
+          seg_first = 0;
+        }
+        seg_first |= 1ull << fill;
+        if (lane >= fill && lane < fill + nsub) {
+          my_t = t;
+          my_h = hs + lane - fill;
+        }
+        fill += nsub;
+      }
+    }
+    load_slots();
+  }
+  if (fill > 0) {
+    __builtin_amdgcn_wave_barrier();
+    prologue_flush(A, w, B, fill, seg_first, lane);
+  }
+}
+
+// kRecords: the features come as compact records from band_features_kernel
+// (the two-kernel prologue) instead of being computed here.
+template <bool kRecords>
 __global__ __launch_bounds__(64 * kProWaves) __attribute__((amdgpu_waves_per_eu(4))) void band_prologue_kernel(
     ProlArgs A) {
   __shared__ float w[kParams];
@@ -783,6 +894,10 @@ __global__ __launch_bounds__(64 * kProWaves) __attribute__((amdgpu_waves_per_eu(
   const int lane = threadIdx.x & 63;
   ProBuf& B = bufs[threadIdx.x >> 6];
   const TaskClaims cl = xcd_task_range(a.n_tasks);
+  if constexpr (kRecords) {
+    prologue_records(A, w, B, cl, lane);
+    return;
+  }
   int fill = 0;
   unsigned long long seg_first = 0;
   for (;;) {
@@ -960,7 +1075,8 @@ hipError_t launch_band_features(const FeatArgs& a, int grid, hipStream_t st) {
 // multiple of the XCD count (per-XCD task claims).
 int band_prologue_grid(int cus, int n_tasks) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_prologue_kernel, 64 * kProWaves, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_prologue_kernel<false>, 64 * kProWaves, 0) !=
+          hipSuccess ||
       n < 1)
     n = 1;
   const int want = (n_tasks + kProWaves - 1) / kProWaves;
@@ -973,7 +1089,10 @@ hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, c
                                 int max_width, int allow_narrow, int grid, hipStream_t st) {
   if (f.n_tasks <= 0) return hipSuccess;
   ProlArgs A{f, b, band, params, margin, max_width, allow_narrow};
-  hipLaunchKernelGGL(band_prologue_kernel, dim3(grid), dim3(64 * kProWaves), 0, st, A);
+  if (f.rec != nullptr)
+    hipLaunchKernelGGL(band_prologue_kernel<true>, dim3(grid), dim3(64 * kProWaves), 0, st, A);
+  else
+    hipLaunchKernelGGL(band_prologue_kernel<false>, dim3(grid), dim3(64 * kProWaves), 0, st, A);
   return hipGetLastError();
 }
 

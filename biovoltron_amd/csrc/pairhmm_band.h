@@ -78,8 +78,8 @@ struct BArgs {
   uint8_t* form;
   float tol;                       // accept when touch mass <= tol * S
   int* accepted;                   // banded pairs accepted
-  int* claims;                     // [16] chunk claim counters: per XCD, one-pair wide then narrow kernel
-  int* quad_claims;                // [8] the same for the quad kernel
+  int* claims;                     // [16] chunk claim counters (kCtrStride apart): per XCD, one-pair wide then narrow kernel
+  int* quad_claims;                // [8] the same for the quad kernel (kCtrStride apart)
   unsigned long long* cells_done;  // lane-cells computed
   // long reads (FP64 first, one task per pair): the moving-band list and the
   // pairs left to the full-matrix FP64 pass (no band, or band rejected)

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   // serialised at the L2 at ~82 M units/s, slower than the units run.)
   const bool by_xcd = gridDim.x % kXcdCount == 0;
   const int xcd = by_xcd ? static_cast<int>(blockIdx.x % kXcdCount) : 0;
-  int* const claim = a.claims + (G == kBandLanes ? 0 : kXcdCount) + xcd;
+  int* const claim = a.claims + ((G == kBandLanes ? 0 : kXcdCount) + xcd) * kCtrStride;
   for (;;) {
     int c = 0;
     if (lane == 0) c = atomicAdd(claim, 1);
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
   // chunks claimed from a counter per XCD, as in band_forward_kernel
   const bool by_xcd = gridDim.x % kXcdCount == 0;
   const int xcd = by_xcd ? static_cast<int>(blockIdx.x % kXcdCount) : 0;
-  int* const claim = a.quad_claims + xcd;
+  int* const claim = a.quad_claims + xcd * kCtrStride;
   _Float16* const selL = reinterpret_cast<_Float16*>(band_lds + kTabBytes) + p * quad_sel_stride(C);  // [4][C]
   for (;;) {
     int cl = 0;

@@ -61,6 +61,13 @@ constexpr int kFp64FirstRows = 1024;  // reads at least this long go to FP64 fir
 // Rescue threshold of the reference driver (pairhmm_common.h:16).
 constexpr float kMinAccepted = 1e-28f;
 
+// A plan's device counters (list lengths, work-claim counters, tallies) are
+// kCtrStride ints apart, one 128-byte line each: atomics on one line from the
+// whole GPU serialise at about 88 M per second (measured on MI355X for one
+// line), while atomics on separate lines proceed in parallel.  Per-XCD claim
+// counters are arrays with this stride as well.
+constexpr int kCtrStride = 32;
+
 // Transition mode.
 enum Mode : int {
   kModeAvx = 0,     // per-base transitions from the quality bytes (AvxPairHMM)

@@ -95,8 +95,9 @@ int fail(int code, const std::string& msg) {
 // log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
-constexpr int kCounters = 72;  // device counter block of a plan (see bv_plan)
+constexpr int kCounters = 80;  // device counters of a plan, kCtrStride ints apart (see bv_plan)
 constexpr int kFeatCounters = 24;  // [24..31]: the features kernel's per-XCD task claims
+constexpr int kPredCounters = 72;  // [72..79]: the split prologue's second kernel's task claims
 constexpr int kBandClaims = 32;    // [32..47]: the banded kernels' per-XCD chunk claims
 constexpr int kQuadClaims = 56;    // [56..63]: the quad banded kernel's per-XCD chunk claims
 constexpr int kItem1Count = 64;    // [64]: the quad shape's leftover wide pairs
@@ -338,20 +339,28 @@ struct bv_plan {
   float* d_Y = nullptr;
   float* d_model = nullptr;  // MLP parameters (copy)
   bool has_model = false;
-  bool fused_prologue = true;  // learned bands: one kernel from features to work lists
+  // learned bands: 0 the separate features, MLP, band-rule and dispatch
+  // kernels; 1 one kernel from features to work lists; 2 the features
+  // kernel writing compact records, then one kernel from them to work lists
+  int prologue_mode = 2;
+  uint8_t* d_rec = nullptr;  // split prologue: per pair records (kRecBytes)
+  float* d_trec = nullptr;   // and per task records
   int grid_prologue = 1;
   int band_margin = 8, band_max_width = 256;
   bool has_band = false;
+  // Counter i lives at d_counters + i * kCtrStride, on a line of its own.
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
-  // list count, [4] its queue, [5] accepted banded pairs, [6..7] rescued
-  // cells (u64), [8..9] swept cells (u64), [10] wide banded item count,
-  // [11] stacked FP32 queue, [12] narrow banded item count, [13] unused, [14]
-  // verify count, [15] its queue, [16] FP64-first queue, [17] moving-band
-  // item count, [18] its queue, [19] long full-matrix count, [20] its queue,
-  // [21] accepted moving-band pairs, [22..23] FP64 cells (u64), [24..31]
-  // features task claims (one per XCD), [32..47] banded chunk claims (per
-  // XCD, wide then narrow kernel), [48] full-matrix list with haplotypes
-  // longer than kStackHmax, [49] its queue
+  // list count, [4] its queue, [5] accepted banded pairs, [6] rescued cells
+  // (u64), [8] swept cells (u64), [10] wide banded item count, [11] stacked
+  // FP32 queue, [12] narrow banded item count, [14] verify count, [15] its
+  // queue, [16] FP64-first queue, [17] moving-band item count, [18] its
+  // queue, [19] long full-matrix count, [20] its queue, [21] accepted
+  // moving-band pairs, [22] FP64 cells (u64), [24..31] features task claims
+  // (one per XCD), [32..47] banded chunk claims (per XCD, wide then narrow
+  // kernel), [48] full-matrix list with haplotypes longer than kStackHmax,
+  // [49] its queue, [56..63] quad banded chunk claims (per XCD), [64] the
+  // quad shape's leftover wide pairs, [72..79] the split prologue's second
+  // kernel's task claims (per XCD); the rest unused
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -389,6 +398,9 @@ struct bv_plan {
 
 namespace {
 
+// Counter i of a plan's counter block.
+int* ctr(const bv_plan* p, int i) { return p->d_counters + i * kCtrStride; }
+
 void free_plan(bv_plan* p) {
   if (!p) return;
   // d_rescue_cells points into d_counters and is not freed on its own.
@@ -398,7 +410,7 @@ void free_plan(bv_plan* p) {
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
                   p->d_items1, p->d_items32, p->d_items_long, p->d_long_full,
-                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off,
+                  p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off, p->d_rec, p->d_trec,
                   p->d_region_tab, p->d_hap_allele, p->d_n_alleles, p->d_allele_off, p->d_lnorm,
                   p->d_keep, p->d_allele, p->d_carry32_long, p->d_carry64_long};
   for (void* b : bufs)
@@ -541,8 +553,8 @@ KArgs base_args(const bv_plan* p) {
   a.mode = p->opt.mode;
   a.renorm = p->opt.fp64_renorm;
   a.rescue_cells = p->d_rescue_cells;
-  a.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
-  a.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
+  a.cells_done = reinterpret_cast<unsigned long long*>(ctr(p, 8));
+  a.fp64_cells = reinterpret_cast<unsigned long long*>(ctr(p, 22));
   a.hcap = p->hmax;
   return a;
 }
@@ -561,14 +573,16 @@ FeatArgs feat_args(const bv_plan* p) {
   f.ends = p->d_ends;
   f.htab = p->d_htab;
   f.htab_off = p->d_htab_off;
-  f.next = p->d_counters + kFeatCounters;
+  f.next = ctr(p, kFeatCounters);
+  f.rec = nullptr;
+  f.trec = nullptr;
   return f;
 }
 
 hipError_t launch_features(bv_plan* p) {
-  const FeatArgs f = feat_args(p);
+  const FeatArgs f = feat_args(p);  // writes X (allocated by ensure_x)
   // the claim counters are cleared by a kernel (graph-safe, no memset node)
-  if (hipError_t e = launch_zero_ints(f.next, 8, p->stream); e != hipSuccess) return e;
+  if (hipError_t e = launch_zero_ints(f.next, kXcdCount * kCtrStride, p->stream); e != hipSuccess) return e;
   const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 32));
   return launch_band_features(f, grid, p->stream);
 }
@@ -613,7 +627,9 @@ int ensure_band_lists(bv_plan* p) {
   return BV_OK;
 }
 
-int ensure_feature_buffers(bv_plan* p) {
+// The per-pair feature matrix (kFeat floats a pair), anchors, seed-end
+// spans and predictions: for bv_plan_features and the separate prologue only.
+int ensure_x(bv_plan* p) {
   const int64_t n = std::max<int64_t>(1, p->n_pairs);
   if (!p->d_X && hipMalloc(&p->d_X, sizeof(float) * kFeat * n) != hipSuccess)
     return fail(BV_ENOMEM, "feature buffer allocation failed");
@@ -621,6 +637,20 @@ int ensure_feature_buffers(bv_plan* p) {
     return fail(BV_ENOMEM, "anchor buffer allocation failed");
   if (!p->d_ends && hipMalloc(&p->d_ends, sizeof(int2) * n) != hipSuccess)
     return fail(BV_ENOMEM, "seed end buffer allocation failed");
+  if (!p->d_Y && hipMalloc(&p->d_Y, sizeof(float) * 2 * n) != hipSuccess)
+    return fail(BV_ENOMEM, "prediction buffer allocation failed");
+  return BV_OK;
+}
+
+int ensure_feature_buffers(bv_plan* p) {
+  const int64_t n = std::max<int64_t>(1, p->n_pairs);
+  int rc0 = BV_OK;
+  if (p->prologue_mode == 0 && (rc0 = ensure_x(p))) return rc0;
+  if (p->prologue_mode == 2 && !p->d_rec) {
+    if (hipMalloc(&p->d_rec, static_cast<size_t>(kRecBytes) * n) != hipSuccess ||
+        hipMalloc(&p->d_trec, sizeof(float) * 3 * std::max(1, p->n_tasks)) != hipSuccess)
+      return fail(BV_ENOMEM, "feature record allocation failed");
+  }
   if (!p->d_htab) {
     // haplotype 12-mer tables: fixed for the plan, built once
     std::vector<int64_t> off(static_cast<size_t>(p->n_haps) + 1, 0);
@@ -637,8 +667,6 @@ int ensure_feature_buffers(bv_plan* p) {
     HIP_TRY(launch_hap_tables(p->d_hmask, p->d_hap_off, p->d_htab_off, p->n_haps, p->d_htab, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
   }
-  if (!p->d_Y && hipMalloc(&p->d_Y, sizeof(float) * 2 * n) != hipSuccess)
-    return fail(BV_ENOMEM, "prediction buffer allocation failed");
   if (!p->d_band && hipMalloc(&p->d_band, sizeof(int2) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band buffer allocation failed");
   return ensure_band_lists(p);
@@ -659,37 +687,37 @@ BArgs band_args(const bv_plan* p) {
   b.n_tasks = p->n_tasks;
   b.band = p->d_band;
   b.items = p->d_items;
-  b.item_count = p->d_counters + 10;
+  b.item_count = ctr(p, 10);
   b.items1 = p->d_items1;
-  b.item1_count = p->d_counters + kItem1Count;
+  b.item1_count = ctr(p, kItem1Count);
   b.items32 = p->d_items32;
-  b.item32_count = p->d_counters + 12;
+  b.item32_count = ctr(p, 12);
   b.full = p->d_fallback;
-  b.full_count = p->d_counters + 3;
+  b.full_count = ctr(p, 3);
   b.full_long = p->d_fallback_long;
-  b.full_long_count = p->d_counters + 48;
+  b.full_long_count = ctr(p, 48);
   b.ph2pr = g_ctx.d_ph2pr_f;
   b.mm = g_ctx.d_mm_f;
   b.out = p->d_out;
   b.form = p->d_form;
   b.tol = p->opt.band_tol;
-  b.accepted = p->d_counters + 5;
-  b.claims = p->d_counters + kBandClaims;
-  b.quad_claims = p->d_counters + kQuadClaims;
-  b.cells_done = reinterpret_cast<unsigned long long*>(p->d_counters + 8);
+  b.accepted = ctr(p, 5);
+  b.claims = ctr(p, kBandClaims);
+  b.quad_claims = ctr(p, kQuadClaims);
+  b.cells_done = reinterpret_cast<unsigned long long*>(ctr(p, 8));
   b.tasks64 = p->d_tasks64;
   b.n_tasks64 = p->n_tasks64;
   b.long_on = p->opt.fp64_renorm != 0;
   b.long_items = p->d_items_long;
-  b.long_count = p->d_counters + 17;
-  b.long_queue = p->d_counters + 18;
+  b.long_count = ctr(p, 17);
+  b.long_queue = ctr(p, 18);
   b.long_full = p->d_long_full;
-  b.long_full_count = p->d_counters + 19;
-  b.long_accepted = p->d_counters + 21;
+  b.long_full_count = ctr(p, 19);
+  b.long_accepted = ctr(p, 21);
   b.ph2pr_d = g_ctx.d_ph2pr_d;
   b.mm_d = g_ctx.d_mm_d;
   b.verify_above = kVerifyAbove;
-  b.fp64_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 22);
+  b.fp64_cells = reinterpret_cast<unsigned long long*>(ctr(p, 22));
   b.sel_cols = p->sel_cols;
   b.wide_pairs = p->wide_pairs;
   return b;
@@ -711,24 +739,24 @@ int enqueue_long(bv_plan* p, bool timed, hipStream_t st, void* carry32, void* ca
     if (timed) HIP_TRY(hipEventRecord(p->ev[8], st));
     a.tasks = p->d_long_full;
     a.n_tasks = 0;
-    a.n_tasks_dev = p->d_counters + 19;
-    a.counter = p->d_counters + 20;
+    a.n_tasks_dev = ctr(p, 19);
+    a.counter = ctr(p, 20);
   } else {
     a.tasks = p->d_tasks64;
     a.n_tasks = p->n_tasks64;
-    a.counter = p->d_counters + 16;
+    a.counter = ctr(p, 16);
   }
   a.carry = carry64;
   a.carry_stride = p->stride64;
   a.verify = p->d_verify;
-  a.verify_count = p->d_counters + 14;
+  a.verify_count = ctr(p, 14);
   a.verify_above = kVerifyAbove;
   HIP_TRY(launch_forward<double>(a, p->grid64, st));
   KArgs v = base_args(p);
   v.tasks = p->d_verify;
   v.n_tasks = 0;
-  v.n_tasks_dev = p->d_counters + 14;
-  v.counter = p->d_counters + 15;
+  v.n_tasks_dev = ctr(p, 14);
+  v.counter = ctr(p, 15);
   v.carry = carry32;
   v.carry_stride = p->stride32;
   v.verify_pass = 1;
@@ -737,15 +765,23 @@ int enqueue_long(bv_plan* p, bool timed, hipStream_t st, void* carry32, void* ca
 }
 
 int enqueue_run(bv_plan* p, bool timed) {
-  HIP_TRY(launch_zero_ints(p->d_counters, kCounters, p->stream));
+  HIP_TRY(launch_zero_ints(p->d_counters, kCounters * kCtrStride, p->stream));
   if (p->n_tasks == 0 && p->n_tasks64 == 0) return BV_OK;
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   if (timed) HIP_TRY(hipEventRecord(p->ev[0], p->stream));
-  const bool fused = fp32 && p->has_model && p->has_band && p->fused_prologue;
+  const bool fused = fp32 && p->has_model && p->has_band && p->prologue_mode != 0;
   if (fused) {
-    // Learned bands in one kernel: seed features, MLP, band rules, band
-    // table and the banded / full-matrix work lists.
+    // Learned bands: seed features, MLP, band rules, band table and the
+    // banded / full-matrix work lists, in one kernel or (split) as the
+    // features kernel writing compact records and one kernel from them on.
     FeatArgs f = feat_args(p);
+    if (p->prologue_mode == 2) {
+      f.rec = p->d_rec;
+      f.trec = p->d_trec;
+      const int grid = std::max(1, std::min(p->n_tasks, g_ctx.cus * 32));
+      HIP_TRY(launch_band_features(f, grid, p->stream));
+      f.next = ctr(p, kPredCounters);
+    }
     HIP_TRY(launch_band_prologue(f, band_args(p), p->d_band, p->d_model, p->band_margin, p->band_max_width,
                                  p->opt.band_narrow, p->grid_prologue, p->stream));
   } else if (fp32 && p->has_model) {
@@ -787,18 +823,18 @@ int enqueue_run(bv_plan* p, bool timed) {
     // reads of regions with haplotypes too long for them on the striped kernel.
     KArgs a = base_args(p);
     a.rescue = p->d_rescue;
-    a.rescue_count = p->d_counters + 2;
+    a.rescue_count = ctr(p, 2);
     if (p->n_stasks > 0) {
       a.stasks = p->d_stasks;
       a.n_tasks = p->n_stasks;
-      a.counter = p->d_counters + 11;
+      a.counter = ctr(p, 11);
       a.carry_cols = p->stack_cols;
       HIP_TRY(launch_stack(a, p->grid_stack, p->stream));
     }
     if (p->n_tasks_rest > 0) {
       a.tasks = p->d_tasks_rest;
       a.n_tasks = p->n_tasks_rest;
-      a.counter = p->d_counters + 0;
+      a.counter = ctr(p, 0);
       a.carry = p->d_carry32;
       a.carry_stride = p->stride32;
       HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
@@ -812,16 +848,16 @@ int enqueue_run(bv_plan* p, bool timed) {
     KArgs a = base_args(p);
     a.tasks = p->d_fallback;
     a.n_tasks = 0;
-    a.n_tasks_dev = p->d_counters + 3;
-    a.counter = p->d_counters + 4;
+    a.n_tasks_dev = ctr(p, 3);
+    a.counter = ctr(p, 4);
     a.rescue = p->d_rescue;
-    a.rescue_count = p->d_counters + 2;
+    a.rescue_count = ctr(p, 2);
     a.carry_cols = kStackHmax + 128;
     HIP_TRY(launch_stack(a, p->grid_fbstack, p->stream));
     if (p->n_longhap_pairs > 0) {
       a.tasks = p->d_fallback_long;
-      a.n_tasks_dev = p->d_counters + 48;
-      a.counter = p->d_counters + 49;
+      a.n_tasks_dev = ctr(p, 48);
+      a.counter = ctr(p, 49);
       a.carry = p->d_carry32;
       a.carry_stride = p->stride32;
       HIP_TRY(launch_forward<float>(a, p->grid32, p->stream));
@@ -835,12 +871,12 @@ int enqueue_run(bv_plan* p, bool timed) {
     if (fp32) {
       a.tasks = p->d_rescue;
       a.n_tasks = 0;
-      a.n_tasks_dev = p->d_counters + 2;
+      a.n_tasks_dev = ctr(p, 2);
     } else {
       a.tasks = p->d_tasks;
       a.n_tasks = p->n_tasks;
     }
-    a.counter = p->d_counters + 1;
+    a.counter = ctr(p, 1);
     a.carry = p->d_carry64;
     a.carry_stride = p->stride64;
     HIP_TRY(launch_forward<double>(a, p->grid64, p->stream));
@@ -1255,7 +1291,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
                        std::max<int64_t>(1, static_cast<int64_t>(tasks64.size())))))
     return cleanup(rc);
   if (hipMalloc(&p->d_rescue, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
-      hipMalloc(&p->d_counters, sizeof(int) * kCounters) != hipSuccess ||
+      hipMalloc(&p->d_counters, sizeof(int) * kCounters * kCtrStride) != hipSuccess ||
       hipMalloc(&p->d_fallback, sizeof(Task) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_out, sizeof(double) * std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
       hipMalloc(&p->d_form, std::max<int64_t>(1, p->n_pairs)) != hipSuccess ||
@@ -1299,7 +1335,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
     if (e != hipSuccess || es != hipSuccess) return cleanup(fail(BV_EHIP, "read byte expansion failed"));
   }
   // The 64-bit rescue cell counter shares the zeroed counter block.
-  p->d_rescue_cells = reinterpret_cast<unsigned long long*>(p->d_counters + 6);
+  p->d_rescue_cells = reinterpret_cast<unsigned long long*>(ctr(p, 6));
   if (hipStreamSynchronize(st) != hipSuccess) return cleanup(fail(BV_EHIP, "upload failed"));
   const auto t_end = std::chrono::steady_clock::now();
   p->setup_ms_host = std::chrono::duration<float, std::milli>(t_up0 - t_host0).count();
@@ -1399,13 +1435,17 @@ int bv_plan_set_band_shape(bv_plan* p, int32_t wide_pairs) {
   return BV_OK;
 }
 
-int bv_plan_set_band_prologue(bv_plan* p, int32_t fused) {
+int bv_plan_set_band_prologue(bv_plan* p, int32_t mode) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
-  if (fused != 0 && fused != 1) return fail(BV_EINVAL, "fused is 0 or 1");
-  if ((fused != 0) != p->fused_prologue) {
-    p->fused_prologue = fused != 0;
+  if (mode < 0 || mode > 2) return fail(BV_EINVAL, "prologue mode is 0, 1 or 2");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (mode != p->prologue_mode) {
+    p->prologue_mode = mode;
     drop_graph(p);
   }
+  // the buffers the mode needs, once a band model is set (not under capture)
+  if (p->has_model)
+    if (int rc = ensure_feature_buffers(p)) return rc;
   return BV_OK;
 }
 
@@ -1496,11 +1536,13 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   if (!p || !s) return fail(BV_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(g_ctx.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  int cnt[kCounters] = {0};
-  HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  std::vector<int> line(kCounters * kCtrStride);
+  HIP_TRY(hipMemcpy(line.data(), p->d_counters, sizeof(int) * line.size(), hipMemcpyDeviceToHost));
+  int cnt[kCounters];
+  for (int i = 0; i < kCounters; ++i) cnt[i] = line[i * kCtrStride];
   unsigned long long c64 = 0, swept = 0;
-  std::memcpy(&c64, cnt + 22, sizeof(c64));
-  std::memcpy(&swept, cnt + 8, sizeof(swept));
+  std::memcpy(&c64, line.data() + 22 * kCtrStride, sizeof(c64));
+  std::memcpy(&swept, line.data() + 8 * kCtrStride, sizeof(swept));
   const bool fp32 = (p->opt.mode == BV_MODE_AVX) && !p->opt.force_fp64;
   s->n_pairs = p->n_pairs;
   s->cells = p->cells;
@@ -1861,11 +1903,11 @@ int bv_compute_full_prob(const bv_testcase* tcs, int n, int precision, double* o
   int rc = bv_plan_create(&b, &opt, &p);
   if (rc) return rc;
   auto run = [&]() -> int {
-    HIP_TRY(launch_zero_ints(p->d_counters, 12, p->stream));
+    HIP_TRY(launch_zero_ints(p->d_counters, 12 * kCtrStride, p->stream));
     KArgs a = base_args(p);
     a.tasks = p->d_tasks;
     a.n_tasks = p->n_tasks;
-    a.counter = p->d_counters;
+    a.counter = ctr(p, 0);
     a.out_raw = 1;
     if (precision == 32) {
       a.carry = p->d_carry32;
@@ -2222,6 +2264,7 @@ int bv_plan_features(bv_plan* p, float* X_out, int32_t* anchor_out, int32_t* end
   if (!p) return fail(BV_EINVAL, "plan is NULL");
   HIP_TRY(hipSetDevice(g_ctx.device));
   int rc = ensure_feature_buffers(p);
+  if (!rc) rc = ensure_x(p);
   if (rc) return rc;
   if (p->n_tasks > 0) HIP_TRY(launch_features(p));
   if (X_out && p->n_pairs)

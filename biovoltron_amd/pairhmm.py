@@ -273,11 +273,12 @@ class Plan:
         or 1 (bv_plan_set_band_shape)."""
         _check(self._lib.bv_plan_set_band_shape(self._h, int(wide_pairs)))
 
-    def set_band_prologue(self, fused: bool) -> None:
-        """With a band model: one fused kernel from seed features to work
-        lists (default), or the separate features, MLP, band-rule and
-        dispatch kernels (bv_plan_set_band_prologue)."""
-        _check(self._lib.bv_plan_set_band_prologue(self._h, 1 if fused else 0))
+    def set_band_prologue(self, mode: int) -> None:
+        """With a band model: 2 (default) the features kernel writing
+        compact records, then one kernel from them to the work lists; 1 one
+        kernel from seed features to work lists; 0 the separate features,
+        MLP, band-rule and dispatch kernels (bv_plan_set_band_prologue)."""
+        _check(self._lib.bv_plan_set_band_prologue(self._h, int(mode)))
 
     def band_shape(self) -> int:
         v = C.c_int32()

@@ -346,14 +346,15 @@ int bv_plan_set_grids(bv_plan* plan, const int32_t* grids);
  * maximum; bv_plan_autotune also searches it.  No reference counterpart. */
 int bv_plan_set_band_shape(bv_plan* plan, int32_t wide_pairs);
 int bv_plan_band_shape(bv_plan* plan, int32_t* wide_pairs);
-/* How a run with a band model builds its bands and work lists.  1 (the
- * default): one fused kernel computes every short-read pair's seed
- * features, the MLP, the band rules, the band table and the banded and
- * full-matrix lists, with the features kept in LDS; 0: the separate
- * features, MLP, band-rule and dispatch kernels.  Both give the same band
- * table and the same pairs in each list (list order may differ).  No
+/* How a run with a band model builds its bands and work lists.  2 (the
+ * default): the features kernel writes a compact record per pair (bin
+ * counts as bytes, anchor, seed-end span), then one kernel runs the MLP,
+ * the band rules, the band table and the banded and full-matrix lists from
+ * them; 1: one kernel does all of it, the features kept in LDS; 0: the
+ * separate features, MLP, band-rule and dispatch kernels.  All give the same
+ * band table and the same pairs in each list (list order may differ).  No
  * reference counterpart. */
-int bv_plan_set_band_prologue(bv_plan* plan, int32_t fused);
+int bv_plan_set_band_prologue(bv_plan* plan, int32_t mode);
 /* The band table of the last run (n_pairs x 2 int32). */
 int bv_plan_bands(bv_plan* plan, int32_t* bands_out);
 /* Training labels (FP64 forward-backward on the host), lo_hi [n_pairs][2]:

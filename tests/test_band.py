@@ -569,8 +569,9 @@ def edge_regions(seed):
 @pytest.mark.gpu
 def test_fused_prologue_equals_separate_kernels():
     """The fused prologue (features, MLP, band rules and list split in one
-    kernel) gives the band table, forms and results of the separate
-    kernels bit for bit.  The exception: a task of more than 64 haplotypes
+    kernel) and the split one (features as compact records, then one kernel
+    on) give the band table, forms and results of the separate kernels bit
+    for bit.  The exception: a task of more than 64 haplotypes
     is split into 64-pair pieces, which may group its wide pairs four at a
     time differently, and the quad and one-pair kernels agree to FP32
     rounding there."""
@@ -579,26 +580,30 @@ def test_fused_prologue_equals_separate_kernels():
     pb = pairhmm.PackedBatch.from_regions(regions)
     model = band.BandModel.load(band.DEFAULT_MODEL)
     res = {}
-    for fused in (True, False):
+    for mode in (2, 1, 0):  # split (records), fused, separate
         with pairhmm.Plan(pb) as plan:
             band.set_plan_model(plan, model)
-            plan.set_band_prologue(fused)
-            say("prologue", "fused" if fused else "separate")
+            plan.set_band_prologue(mode)
+            say("prologue mode", mode)
             plan.run()
             out, form = plan.results()
             st = plan.stats()
-            res[fused] = (out, form, band.plan_bands(plan), int(st.band_pairs), int(st.fallback_pairs))
-    (o1, f1, b1, n1, m1), (o0, f0, b0, n0, m0) = res[True], res[False]
-    assert np.array_equal(b1, b0)
-    assert np.array_equal(f1, f0)
-    assert (n1, m1) == (n0, m0)
+            res[mode] = (out, form, band.plan_bands(plan), int(st.band_pairs), int(st.fallback_pairs))
+    o0, f0, b0, n0, m0 = res[0]
     # pairs of the 80-haplotype region (the first edge region)
     sizes = [len(r) * len(h) for r, h in regions]
     p0 = int(np.sum(sizes[:b.n_regions]))
-    many = np.zeros(o1.size, bool)
+    many = np.zeros(o0.size, bool)
     many[p0:p0 + sizes[b.n_regions]] = True
-    assert np.array_equal(o1[~many], o0[~many])
-    assert np.abs(o1[many] - o0[many]).max() <= 1e-6
+    for mode in (2, 1):
+        o1, f1, b1, n1, m1 = res[mode]
+        assert np.array_equal(b1, b0), mode
+        assert np.array_equal(f1, f0), mode
+        assert (n1, m1) == (n0, m0), mode
+        assert np.array_equal(o1[~many], o0[~many]), mode
+        assert np.abs(o1[many] - o0[many]).max() <= 1e-6, mode
+    # the split and fused prologues group the long task's pieces alike
+    assert np.array_equal(res[2][0], res[1][0])
     # the edge cases took the lists they should
-    assert (f1 == 2).sum() > 0.9 * b.n_pairs
-    say("pairs", o1.size, "banded", int((f1 == 2).sum()), "full", int(np.isin(f1, (0, 1, 3)).sum()))
+    assert (f0 == 2).sum() > 0.9 * b.n_pairs
+    say("pairs", o0.size, "banded", int((f0 == 2).sum()), "full", int(np.isin(f0, (0, 1, 3)).sum()))
