@@ -26,6 +26,14 @@ constexpr int kQuadLanes = 16;    // lanes per pair (one DPP row)
 constexpr int kQuadCells = 4;     // diagonals per lane
 static_assert(kQuadLanes * kQuadCells == kBandLanes, "a quad pair covers the wide band");
 static_assert(kQuadCells == kBandEdge, "an edge zone is one lane of the quad shape");
+// The mid band: 48 diagonals, the quad shape with three diagonals per lane;
+// its edge zones are one lane (three diagonals) per side.
+constexpr int kMidCells = 3;
+constexpr int kBandMid = kQuadLanes * kMidCells;
+constexpr int kMidEdge = kMidCells;
+// Band classes a learned band may take besides the wide one (rule flags).
+constexpr int kAllowNarrow = 1;
+constexpr int kAllowMid = 2;
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
 // list items 2u and 2u+1 share a read (hap < 0 marks padding), in the wide
@@ -64,6 +72,10 @@ struct BArgs {
   int* item1_count;
   BandItem* items32;
   int* item32_count;
+  // 48-wide bands (quad shape only), groups of four pairs of one read; the
+  // rest of each read's mid pairs go to items1 as 64-wide bands
+  BandItem* items48;
+  int* item48_count;
   // pairs for the full-matrix FP32 pass (ineligible or rejected): haplotypes
   // of at most kStackHmax bases (stacked kernel), and longer ones (striped kernel)
   Task* full;
@@ -80,6 +92,7 @@ struct BArgs {
   int* accepted;                   // banded pairs accepted
   int* claims;                     // [16] chunk claim counters (kCtrStride apart): per XCD, one-pair wide then narrow kernel
   int* quad_claims;                // [8] the same for the quad kernel (kCtrStride apart)
+  int* quad48_claims;              // [8] the same for the mid-band quad kernel
   unsigned long long* cells_done;  // lane-cells computed
   // long reads (FP64 first, one task per pair): the moving-band list and the
   // pairs left to the full-matrix FP64 pass (no band, or band rejected)
@@ -103,6 +116,9 @@ struct BArgs {
   // kQuadPairs = four pairs of one read per wavefront (four diagonals per
   // lane) for each read's whole groups of four
   int wide_pairs;
+  // the quad shape also runs 48-wide bands (three diagonals per lane); else
+  // a band of 33-48 diagonals takes the wide lists
+  int mid_on;
 };
 
 // Dynamic LDS bytes of the short-read banded kernel for `pairs` pairs per

@@ -207,7 +207,8 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& tot) {
   return x - v;
 }
 
-// 0: full matrix, 32 or 64: the banded list the pair goes to.
+// 0: full matrix, 32, 48 or 64: the banded list the pair goes to (48 only
+// with the quad shape and the mid band on).
 __device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h, bool gcp_const,
                                           int2& b) {
   b = a.band[tk.out0 + h];
@@ -218,7 +219,9 @@ __device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h,
       b.y > (1 << 26))
     return 0;
   const int64_t w = static_cast<int64_t>(b.y) - b.x + 1;
-  return w <= kBandNarrow ? kBandNarrow : (w <= kBandLanes ? kBandLanes : 0);
+  if (w <= kBandNarrow) return kBandNarrow;
+  if (w <= kBandMid && a.mid_on && a.wide_pairs == kQuadPairs) return kBandMid;
+  return w <= kBandLanes ? kBandLanes : 0;
 }
 
 }  // namespace
@@ -226,14 +229,15 @@ __device__ __forceinline__ int band_class(const BArgs& a, const Task& tk, int h,
 // Splits every task's pairs into the banded lists and full-matrix tasks.
 // One thread per task; appends are reserved once per wavefront.  A task's
 // narrow items are written in pairs (padded with an item of hap -1), and
-// with the quad shape its wide items in fours, the last fewer than four to
-// the one-pair kernel's list.
+// with the quad shape its wide items and its mid items in fours, the last
+// fewer than four of each to the one-pair kernel's list (mid ones as 64-wide
+// bands with the same centre).
 __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   Task tk{};
   bool gcp_const = false;
-  int n64 = 0, n32 = 0, nf = 0, nfl = 0;
+  int n64 = 0, n48 = 0, n32 = 0, nf = 0, nfl = 0;
   if (t < a.n_tasks) {
     tk = a.tasks[t];
     gcp_const = a.gcp_const[tk.read] != 0;
@@ -241,6 +245,7 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
       int2 b;
       const int c = band_class(a, tk, h, gcp_const, b);
       n64 += c == kBandLanes;
+      n48 += c == kBandMid;
       n32 += c == kBandNarrow;
       if (c == 0) {
         const int hap = tk.hap0 + h;
@@ -253,28 +258,32 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   // than four) go to the one-pair kernel's list
   const bool quad = a.wide_pairs == kQuadPairs;
   const int n64q = quad ? n64 & ~(kQuadPairs - 1) : n64;
-  const int n64r = n64 - n64q;
-  int t64 = 0, t64r = 0, t32 = 0, tf = 0, tfl = 0;
+  const int n48q = n48 & ~(kQuadPairs - 1);  // mid pairs exist only with the quad shape
+  const int n64r = n64 - n64q + n48 - n48q;
+  int t64 = 0, t48 = 0, t64r = 0, t32 = 0, tf = 0, tfl = 0;
   const int o64 = wave_excl_scan(n64q, lane, t64);
+  const int o48 = wave_excl_scan(n48q, lane, t48);
   const int o64r = wave_excl_scan(n64r, lane, t64r);
   const int o32 = wave_excl_scan(n32p, lane, t32);
   const int of = wave_excl_scan(nf, lane, tf);
   const int ofl = wave_excl_scan(nfl, lane, tfl);
-  int b64 = 0, b64r = 0, b32 = 0, bf = 0, bfl = 0;
+  int b64 = 0, b48 = 0, b64r = 0, b32 = 0, bf = 0, bfl = 0;
   if (lane == 0) {
     if (t64) b64 = atomicAdd(a.item_count, t64);
+    if (t48) b48 = atomicAdd(a.item48_count, t48);
     if (t64r) b64r = atomicAdd(a.item1_count, t64r);
     if (t32) b32 = atomicAdd(a.item32_count, t32);
     if (tf) bf = atomicAdd(a.full_count, tf);
     if (tfl) bfl = atomicAdd(a.full_long_count, tfl);
   }
   b64 = __shfl(b64, 0) + o64;
+  b48 = __shfl(b48, 0) + o48;
   b64r = __shfl(b64r, 0) + o64r;
   b32 = __shfl(b32, 0) + o32;
   bf = __shfl(bf, 0) + of;
   bfl = __shfl(bfl, 0) + ofl;
   if (t < a.n_tasks) {
-    int k64 = 0;  // wide pairs of this task so far
+    int k64 = 0, k48 = 0;  // wide and mid pairs of this task so far
     for (int h = 0; h < tk.nhap; ++h) {
       int2 b;
       const int64_t o = tk.out0 + h;
@@ -284,6 +293,9 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
         const BandItem it{tk.read, tk.hap0 + h, b.x - ((c - w) >> 1), 0, o};
         if (c == kBandLanes) {
           if (k64++ < n64q) a.items[b64++] = it; else a.items1[b64r++] = it;
+        } else if (c == kBandMid) {
+          if (k48++ < n48q) a.items48[b48++] = it;
+          else a.items1[b64r++] = BandItem{tk.read, tk.hap0 + h, b.x - ((kBandLanes - w) >> 1), 0, o};
         } else {
           a.items32[b32++] = it;
         }
@@ -610,6 +622,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
 // lane is exactly one edge zone (kQuadCells == kBandEdge): lane 0 of a row
 // holds the band's 4 lowest diagonals, lane 15 its 4 highest.
 //
+// The template argument kC is the number of cells per lane: 4 for the wide
+// band, 3 for the 48-diagonal mid band (its own list), whose edge zones are
+// then one lane of three diagonals per side; the D scan's factors become
+// yy^3 .. yy^24 and the carry from below enters cell 1 only.
+//
 // Touch mass with fewer overrides.  T equals F on active edge-zone cells.
 // When a zone was active on the row before, the cells of its lane compute
 // T = F by themselves (the same operations on the same inputs), except for
@@ -651,11 +668,11 @@ __device__ __forceinline__ void quad_cross(float& r, float& tr, float& y0, float
 // d1 += yy^2 x_{k-1}, d2 += yy^3 x_{k-1} (lane 0 has none).  F and T
 // alternate inside each level, with one s_nop 0 between levels, so every DPP
 // read of a level's result comes two wait states after its write.
-__device__ __forceinline__ void quad_scan(float& x, float& tx, float& d1, float& td1, float& d2, float& td2,
-                                          float y4, float y8, float y16, float y32, float y2, float y3) {
 #define BVQ_LVL(CTRL, Y)                                                             \
   "v_fmac_f32_dpp %0, %0, " Y " " CTRL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
   "v_fmac_f32_dpp %1, %1, " Y " " CTRL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+__device__ __forceinline__ void quad_scan(float& x, float& tx, float& d1, float& td1, float& d2, float& td2,
+                                          float y4, float y8, float y16, float y32, float y2, float y3) {
   asm volatile("s_nop 1\n\t"
                BVQ_LVL("row_shr:1", "%6") "s_nop 0\n\t"
                BVQ_LVL("row_shr:2", "%7") "s_nop 0\n\t"
@@ -667,8 +684,24 @@ __device__ __forceinline__ void quad_scan(float& x, float& tx, float& d1, float&
                "v_fmac_f32_dpp %5, %1, %11 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
                : "+v"(x), "+v"(tx), "+v"(d1), "+v"(td1), "+v"(d2), "+v"(td2)
                : "v"(y4), "v"(y8), "v"(y16), "v"(y32), "v"(y2), "v"(y3));
-#undef BVQ_LVL
 }
+
+// The same for the mid band (three cells per lane): factors yy^3, yy^6,
+// yy^12, yy^24, and the carry from the lanes below into cell 1 only,
+// d1 += yy^2 x_{k-1} (cell 2 is the lane's top, x itself).
+__device__ __forceinline__ void quad_scan3(float& x, float& tx, float& d1, float& td1, float y3, float y6,
+                                           float y12, float y24, float y2) {
+  asm volatile("s_nop 1\n\t"
+               BVQ_LVL("row_shr:1", "%4") "s_nop 0\n\t"
+               BVQ_LVL("row_shr:2", "%5") "s_nop 0\n\t"
+               BVQ_LVL("row_shr:4", "%6") "s_nop 0\n\t"
+               BVQ_LVL("row_shr:8", "%7") "s_nop 1\n\t"
+               "v_fmac_f32_dpp %2, %0, %8 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+               "v_fmac_f32_dpp %3, %1, %8 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+               : "+v"(x), "+v"(tx), "+v"(d1), "+v"(td1)
+               : "v"(y3), "v"(y6), "v"(y12), "v"(y24), "v"(y2));
+}
+#undef BVQ_LVL
 
 // Sum over each 16-lane row; lane 15 of the row holds the total.
 __device__ __forceinline__ float row_total(float v) {
@@ -685,22 +718,26 @@ __device__ __host__ constexpr int quad_sel_stride(int sel_cols) { return 4 * sel
 
 }  // namespace
 
+template <int kC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void band_quad_kernel(BArgs a) {
+  static_assert(kC == kQuadCells || kC == kMidCells, "the wide or the mid band");
+  constexpr int W = kQuadLanes * kC;  // the band's diagonals
+  const BandItem* const items = kC == kQuadCells ? a.items : a.items48;
   RowRec* tab = reinterpret_cast<RowRec*>(band_lds);
   const int C = a.sel_cols;
   const int lane = threadIdx.x;
   const int p = lane >> 4;  // pair slot: the DPP row
-  const int k = lane & 15;  // lane in the row: diagonals o + 4k .. o + 4k + 3
+  const int k = lane & 15;  // lane in the row: diagonals o + kC k .. o + kC k + kC - 1
   const bool left_lane = k == 0;
   const bool right_lane = k == kQuadLanes - 1;
-  const int n_units = *a.item_count / kQuadPairs;
+  const int n_units = *(kC == kQuadCells ? a.item_count : a.item48_count) / kQuadPairs;
   const int n_chunks = (n_units + kQuadChunk - 1) / kQuadChunk;
   int accepted = 0;
   unsigned long long cells = 0;
   // chunks claimed from a counter per XCD, as in band_forward_kernel
   const bool by_xcd = gridDim.x % kXcdCount == 0;
   const int xcd = by_xcd ? static_cast<int>(blockIdx.x % kXcdCount) : 0;
-  int* const claim = a.quad_claims + xcd * kCtrStride;
+  int* const claim = (kC == kQuadCells ? a.quad_claims : a.quad48_claims) + xcd * kCtrStride;
   _Float16* const selL = reinterpret_cast<_Float16*>(band_lds + kTabBytes) + p * quad_sel_stride(C);  // [4][C]
   for (;;) {
     int cl = 0;
@@ -710,8 +747,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
     if (chunk >= n_chunks) break;
     const int u_end = min(n_units, (chunk + 1) * kQuadChunk);
     for (int u = chunk * kQuadChunk; u < u_end; ++u) {
-      const BandItem w = a.items[kQuadPairs * u + p];
-      const BandItem w0 = a.items[kQuadPairs * u];  // slot 0 of a unit is never padding
+      const BandItem w = items[kQuadPairs * u + p];
+      const BandItem w0 = items[kQuadPairs * u];  // slot 0 of a unit is never padding
       const bool live = w.hap >= 0;
       const int read = __builtin_amdgcn_readfirstlane(w0.read);
       const int hap = live ? w.hap : w0.hap;  // padding mirrors slot 0
@@ -722,12 +759,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
       const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
       // start slide, as in band_forward_kernel<64>: off(r) is the offset the
       // schedule asks for on row r
-      const int Pt = min(kBandSlide, max(0, (H - 1) - (o_lo + kBandLanes - 1)));
+      const int Pt = min(kBandSlide, max(0, (H - 1) - (o_lo + W - 1)));
       auto off = [&](int r) { return o_lo + max(0, Pt + 1 - r); };
       const int wb = o_lo;  // window slot s is haplotype column wb + 1 + s
 
       __syncthreads();  // the previous unit's LDS reads are done
-      for (int s = k; s < R + kBandLanes + Pt; s += kQuadLanes) {
+      for (int s = k; s < R + W + Pt; s += kQuadLanes) {
         const int j = wb + 1 + s;
         const uint32_t m = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : 0u;
 #pragma unroll
@@ -738,25 +775,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
       const float yy = a.ph2pr[cq0];
       const float gx = 1.f - a.ph2pr[cq0];
       const double igx = 1.0 / static_cast<double>(gx);
-      const float yy2 = yy * yy, yy3 = yy2 * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4, yy16 = yy8 * yy8,
-                  yy32 = yy16 * yy16;
+      // decay factors of the D scan over lanes: yy^kC, yy^2kC, yy^4kC, yy^8kC
+      const float yy2 = yy * yy, yy3 = yy2 * yy, yy4 = yy2 * yy2;
+      const float yc = kC == kQuadCells ? yy4 : yy3;
+      const float yc2 = yc * yc, yc4 = yc2 * yc2, yc8 = yc4 * yc4;
 
       // Row 0: M = I = 0, D = K/H for columns 0..H (times g); T = F on the
       // edge zones active there.
       int o_cur = off(1);
-      const _Float16* sl = selL + (4 * k + o_cur - wb);  // this lane's cell 0 slot, less the row's part
+      const _Float16* sl = selL + (kC * k + o_cur - wb);  // this lane's cell 0 slot, less the row's part
       // D is carried divided by its row's my (D' = D / my, constant along a
       // row): the chain along a row is then D'_d = M_{d-1} + yy * D'_{d-1},
       // with no multiply per cell, and the next row takes my * D' in one FMA.
       // Row 0's D is kept as is (myp = 1).
-      float M[4], I[4], D[4], TM[4], TI[4], TD[4];
+      float M[kC], I[kC], D[kC], TM[kC], TI[kC], TD[kC];
       float myp = 1.f;
       {
         const float d0 = gx * (kK / static_cast<float>(H));
-        const bool ez0 = (left_lane && o_cur - 1 >= 0) || (right_lane && o_cur + kBandLanes <= H);
+        const bool ez0 = (left_lane && o_cur - 1 >= 0) || (right_lane && o_cur + W <= H);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int d = o_cur + 4 * k + c;
+        for (int c = 0; c < kC; ++c) {
+          const int d = o_cur + kC * k + c;
           M[c] = I[c] = TM[c] = TI[c] = 0.f;
           D[c] = (d >= 0 && d <= H) ? d0 : 0.f;
           TD[c] = ez0 ? D[c] : 0.f;
@@ -800,9 +839,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
           const int i = i0 + v;
           const RowRec rr = tab[v];
           const char* sb = reinterpret_cast<const char*>(sl) + rr.selo;
-          float pr[4];
+          float pr[kC];
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
+          for (int c = 0; c < kC; ++c)
             pr[c] = __builtin_fmaf(static_cast<float>(*reinterpret_cast<const _Float16*>(sb + 2 * c)), rr.dp, rr.pe);
           if (kX == 1 || (kX == 2 && i > R - kEdgeRunRows)) {
             const int c0 = i - 1 + o_cur;  // the bottom cell's column on row i - 1
@@ -811,77 +850,86 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
           }
           bool ez = false;
           if constexpr (full) {
-            const bool lon = i + o_cur >= 2, ron = i + o_cur + kBandLanes <= H;
+            const bool lon = i + o_cur >= 2, ron = i + o_cur + W <= H;
             ez = (left_lane && lon) || (right_lane && ron);
           }
-          float Mn[4], In[4], TMn[4], TIn[4], Y[4], TY[4];
+          float Mn[kC], In[kC], TMn[kC], TIn[kC], Y[kC], TY[kC];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
+          for (int c = 0; c < kC; ++c) {
             Mn[c] = __builtin_fmaf(M[c], rr.mm, __builtin_fmaf(D[c], myp, I[c])) * pr[c];
             TMn[c] = __builtin_fmaf(TM[c], rr.mm, __builtin_fmaf(TD[c], myp, TI[c])) * pr[c];
           }
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
+          for (int c = 0; c < kC - 1; ++c) {
             In[c] = __builtin_fmaf(M[c + 1], rr.mx, I[c + 1] * rr.xx);
             TIn[c] = __builtin_fmaf(TM[c + 1], rr.mx, TI[c + 1] * rr.xx);
           }
-          In[3] = dpp<kRowShl1>(I[0]) * rr.xx;
-          TIn[3] = dpp<kRowShl1>(TI[0]) * rr.xx;
+          In[kC - 1] = dpp<kRowShl1>(I[0]) * rr.xx;
+          TIn[kC - 1] = dpp<kRowShl1>(TI[0]) * rr.xx;
           if constexpr (full) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < kC; ++c)
               if (ez) TMn[c] = Mn[c];
           }
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
+          for (int c = 0; c < kC; ++c) {
             Y[c] = Mn[c];
             TY[c] = TMn[c];
           }
           myp = rr.my;
-          quad_cross(In[3], TIn[3], Y[0], TY[0], M[0], TM[0], Y[3], TY[3], rr.mx, yy);
-          // D chain along the lane: d1 = Y0 + yy*carry (done above), d2, d3
-          float d1 = Y[0], td1 = TY[0];
-          float d2 = __builtin_fmaf(d1, yy, Y[1]), td2 = __builtin_fmaf(td1, yy, TY[1]);
-          float x = __builtin_fmaf(d2, yy, Y[2]), tx = __builtin_fmaf(td2, yy, TY[2]);
-          quad_scan(x, tx, d1, td1, d2, td2, yy4, yy8, yy16, yy32, yy2, yy3);
-          // cell 0 of the lane above: Y3 + yy*D3 of this lane
-          const float z = __builtin_fmaf(x, yy, Y[3]), tz = __builtin_fmaf(tx, yy, TY[3]);
-          const float Dn0 = dpp<kRowShr1>(z);
-          float TDn0 = dpp<kRowShr1>(tz);
-          float TDn1 = td1, TDn2 = td2, TDn3 = tx;
+          quad_cross(In[kC - 1], TIn[kC - 1], Y[0], TY[0], M[0], TM[0], Y[kC - 1], TY[kC - 1], rr.mx, yy);
+          // D chain along the lane: cell 1 = Y0 + yy*carry (done above), then
+          // the cells above; the lane's top value x goes into the scan
+          float Dn[kC], TDn[kC];
+          float x, tx;
+          if constexpr (kC == 4) {
+            float d1 = Y[0], td1 = TY[0];
+            float d2 = __builtin_fmaf(d1, yy, Y[1]), td2 = __builtin_fmaf(td1, yy, TY[1]);
+            x = __builtin_fmaf(d2, yy, Y[2]);
+            tx = __builtin_fmaf(td2, yy, TY[2]);
+            quad_scan(x, tx, d1, td1, d2, td2, yc, yc2, yc4, yc8, yy2, yy3);
+            Dn[1] = d1;
+            Dn[2] = d2;
+            TDn[1] = td1;
+            TDn[2] = td2;
+          } else {
+            float d1 = Y[0], td1 = TY[0];
+            x = __builtin_fmaf(d1, yy, Y[1]);
+            tx = __builtin_fmaf(td1, yy, TY[1]);
+            quad_scan3(x, tx, d1, td1, yc, yc2, yc4, yc8, yy2);
+            Dn[1] = d1;
+            TDn[1] = td1;
+          }
+          Dn[kC - 1] = x;
+          TDn[kC - 1] = tx;
+          // cell 0 of the lane above: Y[kC - 1] + yy * (this lane's top D)
+          const float z = __builtin_fmaf(x, yy, Y[kC - 1]), tz = __builtin_fmaf(tx, yy, TY[kC - 1]);
+          Dn[0] = dpp<kRowShr1>(z);
+          TDn[0] = dpp<kRowShr1>(tz);
           if constexpr (full) {
             if (ez) {
 #pragma unroll
-              for (int c = 0; c < 4; ++c) TIn[c] = In[c];
-              TDn0 = Dn0;
-              TDn1 = d1;
-              TDn2 = d2;
-              TDn3 = x;
+              for (int c = 0; c < kC; ++c) {
+                TIn[c] = In[c];
+                TDn[c] = Dn[c];
+              }
             }
           } else {
-            if (i >= ez_from) TIn[3] = In[3];  // left lanes whose zone is on
-            if (i <= ez_to) {                  // right lanes whose zone is on
-              TDn0 = Dn0;
-              TDn1 = d1;
-              TDn2 = d2;
-              TDn3 = x;
+            if (i >= ez_from) TIn[kC - 1] = In[kC - 1];  // left lanes whose zone is on
+            if (i <= ez_to) {                            // right lanes whose zone is on
+#pragma unroll
+              for (int c = 0; c < kC; ++c) TDn[c] = Dn[c];
             }
           }
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
+          for (int c = 0; c < kC; ++c) {
             M[c] = Mn[c];
             I[c] = In[c];
             TM[c] = TMn[c];
             TI[c] = TIn[c];
+            D[c] = Dn[c];
+            TD[c] = TDn[c];
           }
-          D[0] = Dn0;
-          D[1] = d1;
-          D[2] = d2;
-          D[3] = x;
-          TD[0] = TDn0;
-          TD[1] = TDn1;
-          TD[2] = TDn2;
-          TD[3] = TDn3;
         };
         using T_ = std::true_type;
         using F_ = std::false_type;
@@ -903,16 +951,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
           const float ig = static_cast<float>(igx);
           float wl = 0.f;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) wl += __builtin_fmaf(__builtin_fmaf(D[c], myp, I[c]), ig, M[c]);
-          const float wtop = __builtin_fmaf(__builtin_fmaf(D[3], myp, I[3]), ig, M[3]);
+          for (int c = 0; c < kC; ++c) wl += __builtin_fmaf(__builtin_fmaf(D[c], myp, I[c]), ig, M[c]);
+          const float wtop = __builtin_fmaf(__builtin_fmaf(D[kC - 1], myp, I[kC - 1]), ig, M[kC - 1]);
           const float tot = __shfl(row_total(wl), lane | 15);
           const float top = __shfl(wtop, lane | 15);
           const bool go = want && top <= kBandHold * tot;
           auto shift = [&](float* s) {
-            const float below = dpp<kRowShr1>(s[3]);
-            s[3] = go ? s[2] : s[3];
-            s[2] = go ? s[1] : s[2];
-            s[1] = go ? s[0] : s[1];
+            const float below = dpp<kRowShr1>(s[kC - 1]);
+#pragma unroll
+            for (int c = kC - 1; c > 0; --c) s[c] = go ? s[c - 1] : s[c];
             s[0] = go ? below : s[0];
           };
           shift(M);
@@ -929,7 +976,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
         // Steady rows: the overrides of a zone that was on the row before;
         // full rows where a left zone turns on (row 2 - o of its pair).
         ez_from = left_lane ? 2 - o_cur : INT_MAX;
-        ez_to = right_lane ? H - o_cur - kBandLanes : INT_MIN;
+        ez_to = right_lane ? H - o_cur - W : INT_MIN;
         const int act0 = __builtin_amdgcn_readlane(2 - o_cur, 0);
         const int act1 = __builtin_amdgcn_readlane(2 - o_cur, 16);
         const int act2 = __builtin_amdgcn_readlane(2 - o_cur, 32);
@@ -967,8 +1014,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
       const float ig = static_cast<float>(igx);
       float sS = 0.f, sT = 0.f;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int j = R + o_cur + 4 * k + c;
+      for (int c = 0; c < kC; ++c) {
+        const int j = R + o_cur + kC * k + c;
         if (j >= 1 && j <= H) {
           sS += __builtin_fmaf(I[c], ig, M[c]);
           sT += __builtin_fmaf(TI[c], ig, TM[c]);
@@ -978,7 +1025,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
       const float S = row_total(sS);
       const float Tm = row_total(sT);
       if (right_lane && live) {
-        cells += static_cast<unsigned long long>(R) * kBandLanes;
+        cells += static_cast<unsigned long long>(R) * W;
         if (S >= kMinAccepted && Tm <= static_cast<float>(a.tol) * S) {
           a.out[w.out] = log10_pos(S) - kLog10K;
           a.form[w.out] = kFormBandFP32;
@@ -1316,14 +1363,29 @@ static size_t quad_lds_bytes(int sel_cols) {
   return static_cast<size_t>(kTabBytes) + static_cast<size_t>(kQuadPairs) * quad_sel_stride(sel_cols) * sizeof(_Float16);
 }
 
-// grid: the wide kernel of the plan's shape; grid1: the one-pair-per-wave
-// kernels that follow it (the quad shape's leftovers, the narrow band).
-hipError_t launch_band_forward(const BArgs& a, int grid, int grid1, hipStream_t st) {
+// The banded short-read kernels, in two launches: the quad kernels (quad
+// shape only) over their groups of four, and the one-pair-per-wavefront
+// kernels (every wide pair with the one-pair shape, else the quad shape's
+// leftovers; the narrow band).  grid: the wide kernel of the plan's shape;
+// grid1: the one-pair kernels that follow the quad shape.
+static hipError_t check_band_args(const BArgs& a) {
   if (a.sel_cols < band_sel_cols(0) || a.sel_cols > band_sel_cols(kBandReadMax)) return hipErrorInvalidValue;
   if (a.wide_pairs != 1 && a.wide_pairs != kQuadPairs) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+hipError_t launch_band_quads(const BArgs& a, int grid, hipStream_t st) {
+  if (hipError_t e = check_band_args(a); e != hipSuccess) return e;
+  if (a.wide_pairs != kQuadPairs) return hipSuccess;
+  hipLaunchKernelGGL(band_quad_kernel<kQuadCells>, dim3(grid), dim3(kBandLanes), quad_lds_bytes(a.sel_cols), st, a);
+  if (a.mid_on)
+    hipLaunchKernelGGL(band_quad_kernel<kMidCells>, dim3(grid), dim3(kBandLanes), quad_lds_bytes(a.sel_cols), st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_band_one(const BArgs& a, int grid, int grid1, hipStream_t st) {
+  if (hipError_t e = check_band_args(a); e != hipSuccess) return e;
   const bool quad = a.wide_pairs == kQuadPairs;
-  if (quad) hipLaunchKernelGGL(band_quad_kernel, dim3(grid), dim3(kBandLanes), quad_lds_bytes(a.sel_cols), st, a);
-  // one pair per wavefront: every wide pair, or the quad shape's leftovers
   hipLaunchKernelGGL(band_forward_kernel<kBandLanes>, dim3(quad ? grid1 : grid), dim3(kBandLanes),
                      band_lds_bytes(1, a.sel_cols), st, a);
   hipLaunchKernelGGL(band_forward_kernel<kBandNarrow>, dim3(grid1), dim3(kBandLanes),
@@ -1335,7 +1397,8 @@ int band_blocks_per_cu(int sel_cols, int wide_pairs) {
   int n = 0;
   const hipError_t e =
       wide_pairs == kQuadPairs
-          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_quad_kernel, kBandLanes, quad_lds_bytes(sel_cols))
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_quad_kernel<kQuadCells>, kBandLanes,
+                                                         quad_lds_bytes(sel_cols))
           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_forward_kernel<kBandLanes>, kBandLanes,
                                                          band_lds_bytes(1, sel_cols));
   if (e != hipSuccess) return 8;

@@ -22,7 +22,8 @@
 
 namespace bvp {
 hipError_t launch_band_dispatch(const BArgs& a, hipStream_t st);
-hipError_t launch_band_forward(const BArgs& a, int grid, int grid1, hipStream_t st);
+hipError_t launch_band_quads(const BArgs& a, int grid, hipStream_t st);
+hipError_t launch_band_one(const BArgs& a, int grid, int grid1, hipStream_t st);
 int band_blocks_per_cu(int sel_cols, int wide_pairs);
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
 int band_long_blocks_per_cu();
@@ -38,7 +39,7 @@ hipError_t launch_expand_bases(const int64_t* read_off, int64_t n_reads, const u
                                const int64_t* hexc, int64_t n_hexc, uint8_t* hmask, hipStream_t st);
 int band_prologue_grid(int cus, int n_tasks);
 hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, const float* params, int margin,
-                                int max_width, int allow_narrow, int grid, hipStream_t st);
+                                int max_width, int allow, int grid, hipStream_t st);
 hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
                               float* H2, hipStream_t st);
 hipError_t launch_mlp_backward(const float* params, const float* X, const float* Y, const float* T,
@@ -49,7 +50,7 @@ hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, f
                        hipStream_t st);
 hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
                                        const float* X, int64_t n, int margin, int max_width,
-                                       int allow_narrow, int2* band, hipStream_t st);
+                                       int allow, int2* band, hipStream_t st);
 hipError_t launch_hap_tables(const uint8_t* hmask, const int64_t* hap_off, const int64_t* tab_off,
                              int64_t n_haps, unsigned long long* tab, hipStream_t st);
 template <typename T>
@@ -95,12 +96,14 @@ int fail(int code, const std::string& msg) {
 // log10-sum, :105-109 / :145-149 ph2pr and K), computed once on the host and
 // uploaded.  Only qualities below 128 are reachable (the kernel masks & 127,
 // avx-pairhmm-template.h:110-125), so the match-to-match table stops there.
-constexpr int kCounters = 80;  // device counters of a plan, kCtrStride ints apart (see bv_plan)
+constexpr int kCounters = 88;  // device counters of a plan, kCtrStride ints apart (see bv_plan)
 constexpr int kFeatCounters = 24;  // [24..31]: the features kernel's per-XCD task claims
 constexpr int kPredCounters = 72;  // [72..79]: the split prologue's second kernel's task claims
 constexpr int kBandClaims = 32;    // [32..47]: the banded kernels' per-XCD chunk claims
 constexpr int kQuadClaims = 56;    // [56..63]: the quad banded kernel's per-XCD chunk claims
 constexpr int kItem1Count = 64;    // [64]: the quad shape's leftover wide pairs
+constexpr int kItem48Count = 13;   // [13]: the mid band's item count
+constexpr int kQuad48Claims = 80;  // [80..87]: the mid-band quad kernel's per-XCD chunk claims
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -282,7 +285,7 @@ void parallel_for(int64_t n, F&& f, int64_t serial_below = 1 << 16) {
 struct bv_plan {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[9] = {};
   bv_options opt{};
   int64_t n_pairs = 0, cells = 0, n_reads = 0, n_haps = 0, read_bytes = 0, hap_bytes = 0;
   int hmax = 0;
@@ -293,6 +296,7 @@ struct bv_plan {
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
   int sel_cols = 0;  // short-read banded kernel: selector columns per class
   int wide_pairs = kQuadPairs;  // wide-band shape: pairs per wavefront (1 or kQuadPairs)
+  bool mid_on = false;          // the quad shape's 48-diagonal band (learned bands may take it)
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -325,7 +329,8 @@ struct bv_plan {
   int64_t n_longhap_pairs = 0;      // short-read pairs with such haplotypes (that list's capacity)
   int2* d_band = nullptr;
   BandItem* d_items = nullptr;    // banded work lists, built on the device each run
-  BandItem* d_items1 = nullptr;   // the quad shape's leftover wide pairs
+  BandItem* d_items1 = nullptr;   // the quad shape's leftover wide and mid pairs
+  BandItem* d_items48 = nullptr;  // the mid band's quad groups
   BandItem* d_items32 = nullptr;
   BandItem* d_items_long = nullptr;  // long reads on the moving band
   Task* d_long_full = nullptr;       // long reads left to the full-matrix FP64 pass
@@ -352,15 +357,16 @@ struct bv_plan {
   // [0] FP32 queue, [1] FP64 queue, [2] rescue count, [3] full-matrix FP32
   // list count, [4] its queue, [5] accepted banded pairs, [6] rescued cells
   // (u64), [8] swept cells (u64), [10] wide banded item count, [11] stacked
-  // FP32 queue, [12] narrow banded item count, [14] verify count, [15] its
-  // queue, [16] FP64-first queue, [17] moving-band item count, [18] its
+  // FP32 queue, [12] narrow banded item count, [13] mid banded item count,
+  // [14] verify count, [15] its queue, [16] FP64-first queue, [17] moving-band item count, [18] its
   // queue, [19] long full-matrix count, [20] its queue, [21] accepted
   // moving-band pairs, [22] FP64 cells (u64), [24..31] features task claims
   // (one per XCD), [32..47] banded chunk claims (per XCD, wide then narrow
   // kernel), [48] full-matrix list with haplotypes longer than kStackHmax,
   // [49] its queue, [56..63] quad banded chunk claims (per XCD), [64] the
-  // quad shape's leftover wide pairs, [72..79] the split prologue's second
-  // kernel's task claims (per XCD); the rest unused
+  // quad shape's leftover wide and mid pairs, [72..79] the split prologue's
+  // second kernel's task claims (per XCD), [80..87] the mid-band quad
+  // kernel's chunk claims (per XCD); the rest unused
   int* d_counters = nullptr;
   unsigned long long* d_rescue_cells = nullptr;
   double* d_out = nullptr;
@@ -408,7 +414,7 @@ void free_plan(bv_plan* p) {
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
                   p->d_stasks, p->d_tasks_rest, p->d_fallback_long,
                   p->d_rescue, p->d_counters,
-                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items,
+                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items, p->d_items48,
                   p->d_items1, p->d_items32, p->d_items_long, p->d_long_full,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off, p->d_rec, p->d_trec,
                   p->d_region_tab, p->d_hap_allele, p->d_n_alleles, p->d_allele_off, p->d_lnorm,
@@ -597,13 +603,16 @@ int band_grid_cap(const bv_plan* p, int wide_pairs) {
 
 int ensure_band_lists(bv_plan* p) {
   const int64_t n = std::max<int64_t>(1, p->n_pairs);
-  // the quad shape leaves at most kQuadPairs - 1 wide pairs per task (per
-  // 64 haplotypes of a task in the fused prologue) to the one-pair kernel;
-  // narrow items are padded to an even count per task (or piece)
+  // the quad shape leaves at most kQuadPairs - 1 wide and as many mid pairs
+  // per task (per 64 haplotypes of a task in the fused prologue) to the
+  // one-pair kernel; narrow items are padded to an even count per task (or
+  // piece)
   if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
+  if (!p->d_items48 && hipMalloc(&p->d_items48, sizeof(BandItem) * n) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_items1 &&
-      hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, (kQuadPairs - 1) * p->n_subtasks)) !=
+      hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, 2 * (kQuadPairs - 1) * p->n_subtasks)) !=
           hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_items32 && hipMalloc(&p->d_items32, sizeof(BandItem) * (n + p->n_subtasks)) != hipSuccess)
@@ -692,6 +701,8 @@ BArgs band_args(const bv_plan* p) {
   b.item1_count = ctr(p, kItem1Count);
   b.items32 = p->d_items32;
   b.item32_count = ctr(p, 12);
+  b.items48 = p->d_items48;
+  b.item48_count = ctr(p, kItem48Count);
   b.full = p->d_fallback;
   b.full_count = ctr(p, 3);
   b.full_long = p->d_fallback_long;
@@ -704,6 +715,7 @@ BArgs band_args(const bv_plan* p) {
   b.accepted = ctr(p, 5);
   b.claims = ctr(p, kBandClaims);
   b.quad_claims = ctr(p, kQuadClaims);
+  b.quad48_claims = ctr(p, kQuad48Claims);
   b.cells_done = reinterpret_cast<unsigned long long*>(ctr(p, 8));
   b.tasks64 = p->d_tasks64;
   b.n_tasks64 = p->n_tasks64;
@@ -720,7 +732,13 @@ BArgs band_args(const bv_plan* p) {
   b.fp64_cells = reinterpret_cast<unsigned long long*>(ctr(p, 22));
   b.sel_cols = p->sel_cols;
   b.wide_pairs = p->wide_pairs;
+  b.mid_on = p->mid_on;
   return b;
+}
+
+// Band classes the learned rules may pick besides the wide band.
+int band_allow(const bv_plan* p) {
+  return (p->opt.band_narrow ? kAllowNarrow : 0) | (p->mid_on && p->wide_pairs == kQuadPairs ? kAllowMid : 0);
 }
 
 // Long reads in FP64 first; pairs within reach of FP32's range are queued
@@ -783,17 +801,18 @@ int enqueue_run(bv_plan* p, bool timed) {
       f.next = ctr(p, kPredCounters);
     }
     HIP_TRY(launch_band_prologue(f, band_args(p), p->d_band, p->d_model, p->band_margin, p->band_max_width,
-                                 p->opt.band_narrow, p->grid_prologue, p->stream));
+                                 band_allow(p), p->grid_prologue, p->stream));
   } else if (fp32 && p->has_model) {
     // Learned bands: seed features, MLP, band table -- all on the device.
     HIP_TRY(launch_features(p));
     HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
     HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_ends, p->d_X, p->n_pairs, p->band_margin,
-                                        p->band_max_width, p->opt.band_narrow, p->d_band, p->stream));
+                                        p->band_max_width, band_allow(p), p->d_band, p->stream));
   }
   if (fp32 && p->has_model) {
     // Long reads (outside the network's training range): start bands from
     // the seeds of their first bases; the moving band follows from there.
+    // (After the prologue: its separate-kernel form writes every pair's band.)
     const int grid_anchor = std::max(1, std::min(p->n_tasks64, g_ctx.cus * 32));
     HIP_TRY(launch_long_anchors(p->d_rclass, p->d_read_off, p->d_htab_off, p->d_htab, p->d_tasks64,
                                 p->n_tasks64, p->d_band, grid_anchor, p->stream));
@@ -816,7 +835,8 @@ int enqueue_run(bv_plan* p, bool timed) {
     const BArgs b = band_args(p);
     if (!fused) HIP_TRY(launch_band_dispatch(b, p->stream));  // (the fused prologue wrote the lists)
     if (timed) HIP_TRY(hipEventRecord(p->ev[5], p->stream));
-    HIP_TRY(launch_band_forward(b, p->grid_band, p->grid_band1, p->stream));
+    HIP_TRY(launch_band_quads(b, p->grid_band, p->stream));
+    HIP_TRY(launch_band_one(b, p->grid_band, p->grid_band1, p->stream));
     if (timed) HIP_TRY(hipEventRecord(p->ev[6], p->stream));
   } else if (fp32) {
     // Pass 1: every pair in FP32 on the full matrix: stacked reads, and the
@@ -1435,6 +1455,15 @@ int bv_plan_set_band_shape(bv_plan* p, int32_t wide_pairs) {
   return BV_OK;
 }
 
+int bv_plan_set_band_mid(bv_plan* p, int32_t on) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  if ((on != 0) != p->mid_on) {
+    p->mid_on = on != 0;
+    drop_graph(p);
+  }
+  return BV_OK;
+}
+
 int bv_plan_set_band_prologue(bv_plan* p, int32_t mode) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
   if (mode < 0 || mode > 2) return fail(BV_EINVAL, "prologue mode is 0, 1 or 2");
@@ -1446,6 +1475,12 @@ int bv_plan_set_band_prologue(bv_plan* p, int32_t mode) {
   // the buffers the mode needs, once a band model is set (not under capture)
   if (p->has_model)
     if (int rc = ensure_feature_buffers(p)) return rc;
+  return BV_OK;
+}
+
+int bv_plan_band_mid(bv_plan* p, int32_t* on) {
+  if (!p || !on) return fail(BV_EINVAL, "NULL argument");
+  *on = p->mid_on ? 1 : 0;
   return BV_OK;
 }
 

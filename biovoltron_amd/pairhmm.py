@@ -86,7 +86,7 @@ EXPORTS = (
     "bv_pairhmm_tables", "bv_compute_full_prob", "bv_options_default", "bv_plan_create",
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
     "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_set_band_shape", "bv_plan_band_shape",
-    "bv_plan_set_band_prologue",
+    "bv_plan_set_band_prologue", "bv_plan_set_band_mid", "bv_plan_band_mid",
     "bv_plan_postprocess", "bv_sw_align", "bv_sw_release",
     "bv_pairhmm_batch",
 )
@@ -121,6 +121,8 @@ def library() -> C.CDLL:
         lib.bv_plan_set_band_shape.argtypes = [vp, i32]
         lib.bv_plan_band_shape.argtypes = [vp, vp]
         lib.bv_plan_set_band_prologue.argtypes = [vp, i32]
+        lib.bv_plan_set_band_mid.argtypes = [vp, i32]
+        lib.bv_plan_band_mid.argtypes = [vp, vp]
         lib.bv_plan_postprocess.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
@@ -284,6 +286,17 @@ class Plan:
         v = C.c_int32()
         _check(self._lib.bv_plan_band_shape(self._h, C.byref(v)))
         return int(v.value)
+
+    def set_band_mid(self, on: bool) -> None:
+        """The quad shape's 48-diagonal mid band on or off (default): faster,
+        but not within the 1e-3 bound on every pair (bv_plan_set_band_mid);
+        with it off, bands of 33-48 diagonals run as 64-diagonal bands."""
+        _check(self._lib.bv_plan_set_band_mid(self._h, 1 if on else 0))
+
+    def band_mid(self) -> bool:
+        v = C.c_int32()
+        _check(self._lib.bv_plan_band_mid(self._h, C.byref(v)))
+        return bool(v.value)
 
     def set_grids(self, grids: dict) -> None:
         """Sets persistent grids (keys band, fp32, fp64, long; missing or 0

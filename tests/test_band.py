@@ -22,6 +22,26 @@ def fixture_batch(name="workload1.npz", step=1):
     return sel, pairhmm.PackedBatch.from_regions([([r], [h]) for r, h in sel])
 
 
+def read_groups(step, n_haps=4, seed=5):
+    """Fixture pairs as one read against n_haps haplotypes each: the pair's
+    own and n_haps - 1 copies with 2 % of their bases substituted, one
+    region per read.  Returns the pairs in output order and the batch."""
+    pairs, _ = load_pairs("workload1.npz")
+    rng = np.random.default_rng(seed)
+    regions, flat = [], []
+    for rd, hap in pairs[::step]:
+        h = np.frombuffer(hap, np.uint8)
+        haps = [hap]
+        for _ in range(n_haps - 1):
+            m = h.copy()
+            k = rng.random(m.size) < 0.02
+            m[k] = rng.choice(list(b"ACGT"), int(k.sum()))
+            haps.append(m.tobytes())
+        regions.append(([rd], haps))
+        flat += [(rd, x) for x in haps]
+    return flat, pairhmm.PackedBatch.from_regions(regions)
+
+
 def oracle_span(oracle, rd, hap, frac=band.LABEL_MASS):
     R = len(rd["seq"])
     _, mass, _ = oracle.band_posterior(rd, hap)
@@ -64,9 +84,11 @@ def test_targets_roundtrip():
 
 # ---- GPU ----------------------------------------------------------------------
 
-def run_plan(pb, bands=None, model=None, shape=0, **kw):
+def run_plan(pb, bands=None, model=None, shape=0, mid=None, **kw):
     with pairhmm.Plan(pb) as plan:
         plan.set_band_shape(shape)
+        if mid is not None:
+            plan.set_band_mid(mid)
         if bands is not None:
             band.set_plan_bands(plan, bands)
         if model is not None:
@@ -372,10 +394,10 @@ HOLD = 1e-9  # a move that drops more of its row's mass is held
 
 
 def slide_schedule(R, H, o_lo, width):
-    """Per-row offsets the wide band's start slide asks for: up to SLIDE
-    diagonals higher on rows 0-1, one lower per row down to o_lo; capped
-    where the matrix ends.  The narrow band does not slide."""
-    P = SLIDE if width == 64 else 0
+    """Per-row offsets the wide and mid bands' start slide asks for: up to
+    SLIDE diagonals higher on rows 0-1, one lower per row down to o_lo;
+    capped where the matrix ends.  The narrow band does not slide."""
+    P = SLIDE if width >= 48 else 0
     pt = min(P, max(0, (H - 1) - (o_lo + width - 1)))
     off = [o_lo + max(0, pt + 1 - r) for r in range(1, R + 1)]
     return np.array([off[0]] + off, np.int32)
@@ -383,9 +405,11 @@ def slide_schedule(R, H, o_lo, width):
 
 def oracle_band(oracle, rd, hap, o_lo, width):
     """The oracle's restatement of the banded kernels: the slide (held by the
-    hold rule) and the insertion runs that leave through the bottom edge."""
+    hold rule) and the insertion runs that leave through the bottom edge; the
+    mid band's edge zones are three diagonals wide, the others' four."""
     R, H = len(rd["seq"]), len(hap)
-    v, ratio, _, _ = oracle.band_follow(rd, hap, o_lo, width, 4, max_move=1, hold_eps=HOLD, irun=2,
+    edge = 3 if width == 48 else 4
+    v, ratio, _, _ = oracle.band_follow(rd, hap, o_lo, width, edge, max_move=1, hold_eps=HOLD, irun=2,
                                         sched=slide_schedule(R, H, o_lo, width))
     return v, ratio
 
@@ -397,21 +421,25 @@ def test_slide_schedule_shape():
     # a band already past the matrix's top does not slide
     assert (slide_schedule(20, 30, -22, 64) == -22).all()
     assert (slide_schedule(100, 400, 50, 32) == 50).all()
+    s = slide_schedule(100, 400, 50, 48)
+    assert s[0] == 58 and s[100] == 50
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("width,shape", [(64, 4), (64, 1), (32, 4)])
+@pytest.mark.parametrize("width,shape", [(64, 4), (64, 1), (48, 4), (32, 4)])
 def test_band_kernel_matches_oracle_band_forward(oracle, width, shape):
     """The banded kernels against the oracle's FP64 restatement of the same
     band (likelihood and the touch-mass decision), on bands placed around and
     across the alignment; the wide band with its start slide, in both shapes
-    (four pairs of one read per wavefront, or one)."""
-    sel, pb = fixture_batch(step=2)
+    (four pairs of one read per wavefront, or one), and the mid band (48
+    diagonals, three per lane, edge zones of three).  Each read runs against
+    four haplotypes, so the quad kernels take whole groups."""
+    sel, pb = read_groups(step=2)
     lab = band.labels(pb)
     rng = np.random.default_rng(width)
     centre = (lab[:, 0] + lab[:, 1]) // 2 + rng.integers(-width // 2, width // 2 + 1, len(sel))
     bands = np.stack([centre - width // 2, centre - width // 2 + width - 1], 1).astype(np.int32)
-    out, form, st, _ = run_plan(pb, bands=bands, shape=shape)
+    out, form, st, _ = run_plan(pb, bands=bands, shape=shape, mid=width == 48)
     tol = pairhmm.default_options().band_tol
     n_acc = n_rej = 0
     worst = 0.0
@@ -428,6 +456,32 @@ def test_band_kernel_matches_oracle_band_forward(oracle, width, shape):
         f"max |dlog10| vs oracle band {worst:.3g}")
     assert n_acc > 50 and n_rej > 10
     assert worst <= 2e-5
+
+
+@pytest.mark.gpu
+def test_mid_band_on_off():
+    """Learned bands with the 48-diagonal mid band on and off (default), and
+    with the one-pair shape (no mid band): every result within 1e-3 of the
+    full matrix on these regions, 48-wide bands only with the quad shape and
+    the mid band on, and fewer cells swept with it."""
+    ev = synth.generate(1, seed=synth.SEED_BASE + 1234, count=40)
+    pb = pairhmm.PackedBatch.from_synth(ev)
+    full, _, _, _ = run_plan(pb)
+    model = band.BandModel.load(band.DEFAULT_MODEL)
+    res = {}
+    for name, shape, mid in (("mid", 4, True), ("no_mid", 4, False), ("one_pair", 1, True)):
+        out, form, st, used = run_plan(pb, model=model, shape=shape, mid=mid)
+        w = used[:, 1] - used[:, 0] + 1
+        n48 = int((w == 48).sum())
+        err = float(np.abs(out - full).max())
+        res[name] = (n48, int(st.cells_swept), int(st.band_pairs), int(st.fallback_pairs), err)
+        say(name, "48-wide", n48, "cells", st.cells_swept, "band", st.band_pairs, "fallback",
+            st.fallback_pairs, "max |dlog10|", err)
+        assert err <= 1e-3, name
+    assert res["mid"][0] > 0.5 * res["mid"][2]
+    assert res["no_mid"][0] == 0 and res["one_pair"][0] == 0
+    assert res["mid"][1] < 0.9 * res["no_mid"][1]
+    assert res["mid"][3] <= res["no_mid"][3] + max(5, res["mid"][2] // 200)
 
 
 @pytest.mark.gpu
@@ -496,11 +550,15 @@ def test_graph_with_long_reads_on_second_stream():
 def test_autotune_changes_speed_not_results():
     """Grids change only the speed (bit-identical results); the wide band's
     two shapes give the same forms and agree to FP32 rounding (the D scan
-    sums in another order), and so does whatever shape the autotuner picks."""
+    sums in another order), and so does whatever shape the autotuner picks.
+    (With the mid band off: the one-pair shape has none, so the two shapes
+    then run the same bands.)"""
     ev = synth.generate(1, seed=synth.SEED_BASE + 78, count=40)
     pb = pairhmm.PackedBatch.from_synth(ev)
     with pairhmm.Plan(pb) as plan:
         band.set_plan_model(plan, band.BandModel.load(band.DEFAULT_MODEL))
+        plan.set_band_mid(False)
+        assert not plan.band_mid()
         res = {}
         for shape in (4, 1):
             plan.set_band_shape(shape)
