@@ -534,10 +534,6 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
 //    that matches nowhere is emitted as a run of insertions whose diagonals
 //    sweep up to its length away from the alignment, out of sight of the
 //    kernel's edge-zone test);
-//  - otherwise (with the mid band allowed: the quad shape) 48 diagonals
-//    centred on the predicted centre when the span fits between the mid
-//    band's edge zones (three diagonals per side), every seed hit lies inside
-//    them, and the anchoring seeds pass the wide band's end tests below;
 //  - otherwise 64 diagonals centred on the predicted centre when the span is
 //    at most `max_width` wide, every seed hit lies inside the wide band's edge
 //    zones, and the anchoring seeds cover at least half the read and reach
@@ -558,7 +554,7 @@ __device__ __forceinline__ bool hits_inside(const float* x, int lo, int hi) {
 
 __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const int* __restrict__ anchor,
                                             const int2* __restrict__ ends, const float* __restrict__ X,
-                                            int64_t n, int margin, int max_width, int allow,
+                                            int64_t n, int margin, int max_width, int allow_narrow,
                                             int2* __restrict__ band) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -577,20 +573,14 @@ __global__ void band_from_prediction_kernel(const float* __restrict__ Y, const i
   const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(Y[i * 2], -8.f), 8.f)));
   const int2 e = ends[i];
   constexpr int kIn = kBandNarrow / 2 - kBandEdge;  // narrow interior: [-kIn, kIn - 1]
-  const bool narrow = (allow & kAllowNarrow) && width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
+  const bool narrow = allow_narrow && width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
                       hits_inside(x, -kIn, kIn - 1) && e.x >= 0 && e.x <= kEndSlack &&
                       e.y >= R - kEndSlack;
-  const bool ends_wide = e.x >= 0 && e.x <= kEndSlackWide && e.y >= R - kEndSlackWide && 2 * (e.y - e.x) >= R;
-  constexpr int kInM = kBandMid / 2 - kMidEdge;  // mid interior: [-kInM, kInM - 1] around the centre
-  const bool mid = (allow & kAllowMid) && width <= kBandMid - 2 * kMidEdge &&
-                   hits_inside(x, dc - kInM, dc + kInM - 1) && ends_wide;
   constexpr int kInW = kBandLanes / 2 - kBandEdge;  // wide interior: [-kInW, kInW - 1]
-  const bool wide = hits_inside(x, dc - kInW, dc + kInW - 1) && ends_wide;
+  const bool wide = hits_inside(x, dc - kInW, dc + kInW - 1) && e.x >= 0 && e.x <= kEndSlackWide &&
+                    e.y >= R - kEndSlackWide && 2 * (e.y - e.x) >= R;
   if (narrow) {
     band[i] = int2{anchor[i] - kBandNarrow / 2, anchor[i] + kBandNarrow / 2 - 1};
-  } else if (mid) {
-    const int c = anchor[i] + dc;
-    band[i] = int2{c - kBandMid / 2, c + kBandMid / 2 - 1};
   } else if (wide) {
     const int c = anchor[i] + dc;
     band[i] = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
@@ -621,7 +611,7 @@ struct ProlArgs {
   BArgs b;           // lists and counters written; b.band unused
   int2* band;        // [pairs]: the band table (short-read pairs)
   const float* params;
-  int margin, max_width, allow;  // allow: kAllowNarrow | kAllowMid
+  int margin, max_width, allow_narrow;
 };
 
 // Per-wavefront buffer of up to 64 pairs (LDS).  A bin count fits a byte: a
@@ -717,19 +707,13 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
     if (!(H > kBandHapMax || !hits || width > A.max_width)) {
       const int dc = static_cast<int>(lrintf(16.f * fminf(fmaxf(y0, -8.f), 8.f)));
       constexpr int kIn = kBandNarrow / 2 - kBandEdge;
-      const bool narrow = (A.allow & kAllowNarrow) && width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
+      const bool narrow = A.allow_narrow && width <= kBandNarrow - 2 * kBandEdge && abs(dc) <= 4 &&
                           inside(-kIn, kIn - 1) && pf >= 0 && pf <= kEndSlack && pl >= R - kEndSlack;
-      const bool ends_wide = pf >= 0 && pf <= kEndSlackWide && pl >= R - kEndSlackWide && 2 * (pl - pf) >= R;
-      constexpr int kInM = kBandMid / 2 - kMidEdge;
-      const bool mid =
-          (A.allow & kAllowMid) && width <= kBandMid - 2 * kMidEdge && inside(dc - kInM, dc + kInM - 1) && ends_wide;
       constexpr int kInW = kBandLanes / 2 - kBandEdge;
-      const bool wide = inside(dc - kInW, dc + kInW - 1) && ends_wide;
+      const bool wide = inside(dc - kInW, dc + kInW - 1) && pf >= 0 && pf <= kEndSlackWide &&
+                        pl >= R - kEndSlackWide && 2 * (pl - pf) >= R;
       if (narrow) {
         band = int2{anchor - kBandNarrow / 2, anchor + kBandNarrow / 2 - 1};
-      } else if (mid) {
-        const int c = anchor + dc;
-        band = int2{c - kBandMid / 2, c + kBandMid / 2 - 1};
       } else if (wide) {
         const int c = anchor + dc;
         band = int2{c - kBandLanes / 2, c + kBandLanes / 2 - 1};
@@ -744,9 +728,7 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
   if (valid && B.gc[s] && band.x <= band.y && H <= kBandHapMax && R <= kBandReadMax && band.x >= -(1 << 26) &&
       band.y <= (1 << 26)) {
     const int wd = band.y - band.x + 1;
-    cls = wd <= kBandNarrow ? kBandNarrow
-          : (wd <= kBandMid && a.mid_on && a.wide_pairs == kQuadPairs) ? kBandMid
-          : (wd <= kBandLanes ? kBandLanes : 0);
+    cls = wd <= kBandNarrow ? kBandNarrow : (wd <= kBandLanes ? kBandLanes : 0);
   }
   // this pair's task piece: lanes [start, end)
   const unsigned long long lt = below(lane);
@@ -759,28 +741,23 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
   const unsigned long long w64 = __ballot(cls == kBandLanes);
   const int n64 = __popcll(w64 & seg), r64 = __popcll(w64 & seg & lt);
   const bool to_q = cls == kBandLanes && (!quad || r64 < (n64 & ~(kQuadPairs - 1)));
-  const unsigned long long w48 = __ballot(cls == kBandMid);
-  const int n48 = __popcll(w48 & seg), r48 = __popcll(w48 & seg & lt);
-  const bool to_q48 = cls == kBandMid && r48 < (n48 & ~(kQuadPairs - 1));
-  const bool to_1 = (cls == kBandLanes && !to_q) || (cls == kBandMid && !to_q48);
+  const bool to_1 = cls == kBandLanes && !to_q;
   const unsigned long long w32 = __ballot(cls == kBandNarrow);
   const bool first = valid && ((seg_first >> lane) & 1ull);
   const bool odd = first && (__popcll(w32 & seg) & 1);
   const bool to_f = valid && cls == 0 && H <= kStackHmax;
   const bool to_fl = valid && cls == 0 && H > kStackHmax;
-  const unsigned long long mq = __ballot(to_q), mq48 = __ballot(to_q48), m1 = __ballot(to_1), mo = __ballot(odd),
-                           mf = __ballot(to_f), mfl = __ballot(to_fl);
-  int bq = 0, bq48 = 0, b1 = 0, b32 = 0, bf = 0, bfl = 0;
+  const unsigned long long mq = __ballot(to_q), m1 = __ballot(to_1), mo = __ballot(odd), mf = __ballot(to_f),
+                           mfl = __ballot(to_fl);
+  int bq = 0, b1 = 0, b32 = 0, bf = 0, bfl = 0;
   if (lane == 0) {
     if (mq) bq = atomicAdd(a.item_count, __popcll(mq));
-    if (mq48) bq48 = atomicAdd(a.item48_count, __popcll(mq48));
     if (m1) b1 = atomicAdd(a.item1_count, __popcll(m1));
     if (w32) b32 = atomicAdd(a.item32_count, __popcll(w32) + __popcll(mo));
     if (mf) bf = atomicAdd(a.full_count, __popcll(mf));
     if (mfl) bfl = atomicAdd(a.full_long_count, __popcll(mfl));
   }
   bq = __shfl(bq, 0);
-  bq48 = __shfl(bq48, 0);
   b1 = __shfl(b1, 0);
   b32 = __shfl(b32, 0);
   bf = __shfl(bf, 0);
@@ -788,11 +765,8 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
   const int pads_before = __popcll(mo & below(start));
   if (cls) {
     const int wd = band.y - band.x + 1;
-    // the one-pair list takes mid leftovers as 64-wide bands
-    const int kw = to_1 ? kBandLanes : cls;
-    const BandItem it{read, hap, band.x - ((kw - wd) >> 1), 0, out};
+    const BandItem it{read, hap, band.x - ((cls - wd) >> 1), 0, out};
     if (to_q) a.items[bq + __popcll(mq & lt)] = it;
-    else if (to_q48) a.items48[bq48 + __popcll(mq48 & lt)] = it;
     else if (to_1) a.items1[b1 + __popcll(m1 & lt)] = it;
     else a.items32[b32 + __popcll(w32 & lt) + pads_before] = it;
   } else if (to_f) {
@@ -1121,9 +1095,9 @@ int band_prologue_grid(int cus, int n_tasks) {
 }
 
 hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, const float* params, int margin,
-                                int max_width, int allow, int grid, hipStream_t st) {
+                                int max_width, int allow_narrow, int grid, hipStream_t st) {
   if (f.n_tasks <= 0) return hipSuccess;
-  ProlArgs A{f, b, band, params, margin, max_width, allow};
+  ProlArgs A{f, b, band, params, margin, max_width, allow_narrow};
   if (f.rec != nullptr)
     hipLaunchKernelGGL(band_prologue_kernel<true>, dim3(grid), dim3(64 * kProWaves), 0, st, A);
   else
@@ -1163,11 +1137,11 @@ hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, f
 
 hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
                                        const float* X, int64_t n, int margin, int max_width,
-                                       int allow, int2* band, hipStream_t st) {
+                                       int allow_narrow, int2* band, hipStream_t st) {
   const int64_t blocks = (n + 255) / 256;
   if (blocks > 0)
     hipLaunchKernelGGL(band_from_prediction_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
-                       Y, anchor, ends, X, n, margin, max_width, allow, band);
+                       Y, anchor, ends, X, n, margin, max_width, allow_narrow, band);
   return hipGetLastError();
 }
 

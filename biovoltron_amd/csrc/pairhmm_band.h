@@ -27,13 +27,13 @@ constexpr int kQuadCells = 4;     // diagonals per lane
 static_assert(kQuadLanes * kQuadCells == kBandLanes, "a quad pair covers the wide band");
 static_assert(kQuadCells == kBandEdge, "an edge zone is one lane of the quad shape");
 // The mid band: 48 diagonals, the quad shape with three diagonals per lane;
-// its edge zones are one lane (three diagonals) per side.
+// its edge zones are one lane (three diagonals) per side.  Bands of 33-48
+// diagonals a caller sets take it when it is on (the learned rules do not:
+// a 48-diagonal band misses alignments 20-30 diagonals from the predicted
+// centre on a few pairs in 10^7 without its edge zones seeing them).
 constexpr int kMidCells = 3;
 constexpr int kBandMid = kQuadLanes * kMidCells;
 constexpr int kMidEdge = kMidCells;
-// Band classes a learned band may take besides the wide one (rule flags).
-constexpr int kAllowNarrow = 1;
-constexpr int kAllowMid = 2;
 
 // One banded pair: diagonals j - r in [o_lo, o_lo + width).  In the narrow
 // list items 2u and 2u+1 share a read (hap < 0 marks padding), in the wide
@@ -117,7 +117,7 @@ struct BArgs {
   // lane) for each read's whole groups of four
   int wide_pairs;
   // the quad shape also runs 48-wide bands (three diagonals per lane); else
-  // a band of 33-48 diagonals takes the wide lists
+  // a band of 33-48 diagonals takes the wide lists (as a 64-wide band)
   int mid_on;
 };
 

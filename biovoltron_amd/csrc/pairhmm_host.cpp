@@ -39,7 +39,7 @@ hipError_t launch_expand_bases(const int64_t* read_off, int64_t n_reads, const u
                                const int64_t* hexc, int64_t n_hexc, uint8_t* hmask, hipStream_t st);
 int band_prologue_grid(int cus, int n_tasks);
 hipError_t launch_band_prologue(const FeatArgs& f, const BArgs& b, int2* band, const float* params, int margin,
-                                int max_width, int allow, int grid, hipStream_t st);
+                                int max_width, int allow_narrow, int grid, hipStream_t st);
 hipError_t launch_mlp_forward(const float* params, const float* X, int64_t n, float* Y, float* H1,
                               float* H2, hipStream_t st);
 hipError_t launch_mlp_backward(const float* params, const float* X, const float* Y, const float* T,
@@ -50,7 +50,7 @@ hipError_t launch_adam(float* params, const float* grad, float* m1, float* m2, f
                        hipStream_t st);
 hipError_t launch_band_from_prediction(const float* Y, const int* anchor, const int2* ends,
                                        const float* X, int64_t n, int margin, int max_width,
-                                       int allow, int2* band, hipStream_t st);
+                                       int allow_narrow, int2* band, hipStream_t st);
 hipError_t launch_hap_tables(const uint8_t* hmask, const int64_t* hap_off, const int64_t* tab_off,
                              int64_t n_haps, unsigned long long* tab, hipStream_t st);
 template <typename T>
@@ -296,7 +296,7 @@ struct bv_plan {
   int grid32_max = 0, grid64_max = 0, grid_band_max = 0, grid_long_max = 1;  // scratch is sized for these
   int sel_cols = 0;  // short-read banded kernel: selector columns per class
   int wide_pairs = kQuadPairs;  // wide-band shape: pairs per wavefront (1 or kQuadPairs)
-  bool mid_on = false;          // the quad shape's 48-diagonal band (learned bands may take it)
+  bool mid_on = false;          // bands of 33-48 diagonals from bv_plan_set_bands on the 48-diagonal quad kernel
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -609,7 +609,7 @@ int ensure_band_lists(bv_plan* p) {
   // piece)
   if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
-  if (!p->d_items48 && hipMalloc(&p->d_items48, sizeof(BandItem) * n) != hipSuccess)
+  if (p->mid_on && !p->d_items48 && hipMalloc(&p->d_items48, sizeof(BandItem) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_items1 &&
       hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, 2 * (kQuadPairs - 1) * p->n_subtasks)) !=
@@ -732,14 +732,10 @@ BArgs band_args(const bv_plan* p) {
   b.fp64_cells = reinterpret_cast<unsigned long long*>(ctr(p, 22));
   b.sel_cols = p->sel_cols;
   b.wide_pairs = p->wide_pairs;
-  b.mid_on = p->mid_on;
+  b.mid_on = p->mid_on && p->d_items48 != nullptr;
   return b;
 }
 
-// Band classes the learned rules may pick besides the wide band.
-int band_allow(const bv_plan* p) {
-  return (p->opt.band_narrow ? kAllowNarrow : 0) | (p->mid_on && p->wide_pairs == kQuadPairs ? kAllowMid : 0);
-}
 
 // Long reads in FP64 first; pairs within reach of FP32's range are queued
 // for FP32 and keep the FP32 result when it does not underflow (the
@@ -801,13 +797,13 @@ int enqueue_run(bv_plan* p, bool timed) {
       f.next = ctr(p, kPredCounters);
     }
     HIP_TRY(launch_band_prologue(f, band_args(p), p->d_band, p->d_model, p->band_margin, p->band_max_width,
-                                 band_allow(p), p->grid_prologue, p->stream));
+                                 p->opt.band_narrow, p->grid_prologue, p->stream));
   } else if (fp32 && p->has_model) {
     // Learned bands: seed features, MLP, band table -- all on the device.
     HIP_TRY(launch_features(p));
     HIP_TRY(launch_mlp_forward(p->d_model, p->d_X, p->n_pairs, p->d_Y, nullptr, nullptr, p->stream));
     HIP_TRY(launch_band_from_prediction(p->d_Y, p->d_anchor, p->d_ends, p->d_X, p->n_pairs, p->band_margin,
-                                        p->band_max_width, band_allow(p), p->d_band, p->stream));
+                                        p->band_max_width, p->opt.band_narrow, p->d_band, p->stream));
   }
   if (fp32 && p->has_model) {
     // Long reads (outside the network's training range): start bands from
@@ -1457,10 +1453,14 @@ int bv_plan_set_band_shape(bv_plan* p, int32_t wide_pairs) {
 
 int bv_plan_set_band_mid(bv_plan* p, int32_t on) {
   if (!p) return fail(BV_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(g_ctx.device));
   if ((on != 0) != p->mid_on) {
     p->mid_on = on != 0;
     drop_graph(p);
   }
+  // the mid band's list, once the plan has bands (else when they are set)
+  if (p->mid_on && p->d_items)
+    if (int rc = ensure_band_lists(p)) return rc;
   return BV_OK;
 }
 

@@ -288,9 +288,10 @@ class Plan:
         return int(v.value)
 
     def set_band_mid(self, on: bool) -> None:
-        """The quad shape's 48-diagonal mid band on or off (default): faster,
-        but not within the 1e-3 bound on every pair (bv_plan_set_band_mid);
-        with it off, bands of 33-48 diagonals run as 64-diagonal bands."""
+        """The quad shape's 48-diagonal mid band on or off (default): with it
+        on, bands of 33-48 diagonals from set_plan_bands run as 48-diagonal
+        bands, else as 64-diagonal ones (bv_plan_set_band_mid).  Learned
+        bands never take 48 diagonals."""
         _check(self._lib.bv_plan_set_band_mid(self._h, 1 if on else 0))
 
     def band_mid(self) -> bool:

@@ -346,18 +346,16 @@ int bv_plan_set_grids(bv_plan* plan, const int32_t* grids);
  * maximum; bv_plan_autotune also searches it.  No reference counterpart. */
 int bv_plan_set_band_shape(bv_plan* plan, int32_t wide_pairs);
 int bv_plan_band_shape(bv_plan* plan, int32_t* wide_pairs);
-/* The quad shape's mid band (default off): learned bands whose predicted
- * span fits 48 diagonals (between edge zones of three diagonals per side)
- * take 48 instead of 64, run four pairs of one read per wavefront with three
- * diagonals per lane (about 25 % less banded kernel time on the combined
- * workload).  Not within the 1e-3 bound: on 10,000 fresh short-read regions
- * (45.4 M pairs) 12 pairs missed the full-matrix likelihood by more than
- * 1e-4 and 4 by more than 1e-3 (worst 1.2e-2), alignments 20-30 diagonals
- * from the predicted centre that the narrower band's edge zones do not see;
- * the 64-diagonal band keeps all of them below 4e-5.  A band of 33-48
- * diagonals from bv_plan_set_bands runs there too; with it off, or with the
- * one-pair shape, such bands run as 64-diagonal bands.  No reference
- * counterpart. */
+/* The quad shape's mid band (default off): with it on, bands of 33-48
+ * diagonals from bv_plan_set_bands run as 48-diagonal bands, four pairs of
+ * one read per wavefront with three diagonals per lane and edge zones of
+ * three diagonals (about 25 % less banded kernel time than 64); with it off,
+ * or with the one-pair shape, such bands run as 64-diagonal bands with the
+ * same centre.  Learned bands never take 48 diagonals: on the combined
+ * workload 48-diagonal learned bands missed 20 pairs of 4.04e8 by more than
+ * 1e-3 (worst 2.98 log10 units), alignments 20-30 diagonals from the
+ * predicted centre that a 48-diagonal band's edge zones do not see.  No
+ * reference counterpart. */
 int bv_plan_set_band_mid(bv_plan* plan, int32_t on);
 int bv_plan_band_mid(bv_plan* plan, int32_t* on);
 /* How a run with a band model builds its bands and work lists.  2 (the

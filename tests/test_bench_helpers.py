@@ -14,7 +14,7 @@ def test_profiled_traffic_reads_the_committed_summary():
     import bench
 
     band = bench.profiled_traffic("pairhmm_workload4.md", "learned",
-                                  ["band_quad_kernel", "band_forward_kernel<64>", "band_forward_kernel<32>"])
+                                  ["band_quad_kernel<4>", "band_forward_kernel<64>", "band_forward_kernel<32>"])
     full = bench.profiled_traffic("pairhmm_workload4.md", "full", ["full-matrix FP32 pass (short reads, stacked)"])
     assert band is not None and full is not None
     # the three banded kernels together move over a hundred GB per run; the
