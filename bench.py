@@ -206,7 +206,8 @@ def sw_cpu_baseline(pairs, device_results, budget_s: float):
                 sample=f"{done} of {len(pairs)} pairs, {cells:.3e} cells, {t:.1f} s"), done, mismatches
 
 
-FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_rejected_full_fp32", 4: "band_fp64_long"}
+FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_rejected_full_fp32", 4: "band_fp64_long",
+              5: "band_fp32_long"}
 
 
 def oracle_form_errors(batch, legs, rr, hh, short_per_form: int = 600, long_per_form: int = 6):
@@ -405,6 +406,7 @@ def main():
             band_kernel_ms=round(sb.last_ms_band, 3), band_cells=band_cells,
             long_pairs=int(ran_long.sum()), long_band_pairs=int(sb.long_band_pairs),
             long_band_kernel_ms=round(sb.last_ms_long, 3), long_band_cells=long_cells,
+            long_band_bits=plan.long_precision(),
             grids=tuned or dict(band=None, fp32=sb.grid_f32, fp64=sb.grid_f64))
         # End to end for one evaluation from host buffers: plan set-up (host
         # work lists, input upload), the band model's set-up (haplotype k-mer
@@ -528,12 +530,15 @@ def main():
         line["roofline"] = roof
         line["config"]["full_matrix"]["roofline"] = full_roof
         if band_info["long_band_kernel_ms"] > 0:
-            ach64 = (BAND_FLOPS_PER_CELL * band_info["long_band_cells"]
-                     / (band_info["long_band_kernel_ms"] / 1e3) / 1e12)
+            achl = (BAND_FLOPS_PER_CELL * band_info["long_band_cells"]
+                    / (band_info["long_band_kernel_ms"] / 1e3) / 1e12)
+            fp32_long = band_info["long_band_bits"] == 32
+            peak = PEAK_FP32_TFLOPS if fp32_long else PEAK_FP64_TFLOPS
             band_info["long_roofline"] = {
-                "bound": "valu", "kernel": "band_long_kernel (FP64)", "achieved": round(ach64, 3),
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach64 / PEAK_FP64_TFLOPS, 4),
-                "flops_per_cell": BAND_FLOPS_PER_CELL, "kernel_ms": band_info["long_band_kernel_ms"]}
+                "bound": "valu", "kernel": f"band_long_kernel<{'float' if fp32_long else 'double'}>",
+                "achieved": round(achl, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achl / peak, 4),
+                "flops_per_cell": BAND_FLOPS_PER_CELL, "kernel_ms": band_info["long_band_kernel_ms"],
+                "note": "shares the GPU with the short-read passes (second stream)"}
     else:
         line["roofline"] = full_roof
     if not args.no_cpu:

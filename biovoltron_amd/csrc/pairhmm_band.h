@@ -99,6 +99,7 @@ struct BArgs {
   const Task* tasks64;
   int n_tasks64;
   int long_on;  // 0: every long pair takes the full matrix
+  int long_fp32;  // the moving band in FP32 (rescaled every 64 rows), else FP64
   BandItem* long_items;
   int* long_count;
   int* long_queue;

@@ -1065,8 +1065,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
 // result), it goes to the full-matrix FP64 pass.
 //
 // Precision: the DP starts from D = 1/H (the reference's K is a power of two
-// and cancels), and the state is rescaled by a power of two every 64 rows,
-// which is exact; log10 L = log10 S + (removed exponent) * log10 2.
+// and cancels), and the state is rescaled by a power of two every 64 rows
+// (FP64) or 16 rows (FP32), which is exact; log10 L = log10 S + (removed
+// exponent) * log10 2.  The FP32 form computes the same recurrence with the
+// FP32 tables and the short-read kernels' DPP-folded I and D scans.
 //
 // Per 64-row block the row records (doubles) and a 192-column window of the
 // haplotype's class masks are staged in LDS: with at most three moves of 4
@@ -1080,20 +1082,22 @@ static_assert(2 * kBandLanes + 2 * kLongSlack <= kLongWin, "long-band window too
 static_assert(kLongMove <= kBandEdge, "a move may only drop edge-zone lanes");
 constexpr double kLog10Two = 0.30102999566398119521;
 
-struct alignas(16) RowRec64 {
-  double dp;  // pm - pe
-  double pe;  // e / 3 (pm for a wildcard read base)
-  double mm;
-  double g;   // 1 - ph2pr[gcp]: insert/delete -> match
-  double mx;
-  double xx;
-  double my;
+template <typename T>
+struct alignas(16) RowRecL {
+  T dp;  // pm - pe
+  T pe;  // e / 3 (pm for a wildcard read base)
+  T mm;
+  T g;   // 1 - ph2pr[gcp]: insert/delete -> match
+  T mx;
+  T xx;
+  T my;
   uint32_t cls;
   uint32_t pad;
 };
 
+template <typename T>
 struct LongLds {
-  RowRec64 tab[kBandLanes];
+  RowRecL<T> tab[kBandLanes];
   uint8_t hm[kLongWin];
 };
 
@@ -1120,17 +1124,26 @@ __device__ __forceinline__ double decay_scan64(double x, double y1, double y2, d
   return x;
 }
 
-__device__ __forceinline__ double wmax(double v) {
+template <typename T>
+__device__ __forceinline__ T wmax(T v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
 }
 
+template <typename T>
+__device__ __forceinline__ T wsumT(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Lane k takes lane k + s's value; lanes past the band edge take 0.
-__device__ __forceinline__ double take(double v, int k, int s) {
+template <typename T>
+__device__ __forceinline__ T take(T v, int k, int s) {
   const int src = k + s;
-  const double x = __shfl(v, src & 63);
-  return (src >= 0 && src < kBandLanes) ? x : 0.0;
+  const T x = __shfl(v, src & 63);
+  return (src >= 0 && src < kBandLanes) ? x : T(0);
 }
 
 }  // namespace
@@ -1154,8 +1167,12 @@ __global__ __launch_bounds__(256) void band_dispatch_long_kernel(BArgs a) {
   }
 }
 
+template <typename T>
 __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
-  __shared__ LongLds s;
+  constexpr bool kF32 = sizeof(T) == 4;
+  __shared__ LongLds<T> s;
+  const T* const ph2pr = kF32 ? reinterpret_cast<const T*>(a.ph2pr) : reinterpret_cast<const T*>(a.ph2pr_d);
+  const T* const mmt = kF32 ? reinterpret_cast<const T*>(a.mm) : reinterpret_cast<const T*>(a.mm_d);
   const int k = threadIdx.x;
   const bool left_lane = k < kBandEdge;
   const bool right_lane = k >= kBandLanes - kBandEdge;
@@ -1177,23 +1194,23 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
     const int64_t ho = a.hap_off[hap];
     const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
 
-    const double yy = a.ph2pr_d[a.gcp[ro] & 127];
-    const double yy2 = yy * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4;
-    double f15 = (row16 & 1) ? 1.0 : 0.0, f31 = (row16 >= 2) ? 1.0 : 0.0;
+    const T yy = ph2pr[a.gcp[ro] & 127];
+    const T yy2 = yy * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4;
+    T f15 = (row16 & 1) ? T(1) : T(0), f31 = (row16 >= 2) ? T(1) : T(0);
     for (int m = 0; m <= (k & 15); ++m) f15 *= yy;
     for (int m = 31; m < k; ++m) f31 *= yy;
     int ez_from = 2 - o, ez_to = H - o - kBandLanes;
 
     // Row 0: D = 1/H on columns 0..H.
-    double Mp = 0.0, Ip = 0.0;
-    double Dp = (o + k >= 0 && o + k <= H) ? 1.0 / static_cast<double>(H) : 0.0;
-    double TMp = 0.0, TIp = 0.0, TDp = 0.0;
+    T Mp = 0, Ip = 0;
+    T Dp = (o + k >= 0 && o + k <= H) ? T(1) / static_cast<T>(H) : T(0);
+    T TMp = 0, TIp = 0, TDp = 0;
     if ((left_lane && o - 1 >= 0) || (right_lane && o + kBandLanes <= H)) TDp = Dp;
     int lg = 0;  // exponent removed by rescaling
     int jb = 0;  // haplotype column of window slot 0
     // a read end explained as insertions, leaving through the bottom lane
     // (as in band_forward_kernel)
-    double xg = 0.0;
+    T xg = 0;
 
     for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
       const int n = min(kBandLanes, R - i0 + 1);
@@ -1202,8 +1219,9 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
         if (i > 1) {
           // Lanes past the haplotype end hold values that never feed
           // in-matrix cells; clear them before the row statistics.
-          if ((i - 1) + o + k > H) Mp = Ip = Dp = TMp = TIp = TDp = 0.0;
-          const double wgt = Mp + Ip + Dp;
+          if ((i - 1) + o + k > H) Mp = Ip = Dp = TMp = TIp = TDp = T(0);
+          // the move statistics in FP64 in either precision
+          const double wgt = static_cast<double>(Mp) + static_cast<double>(Ip) + static_cast<double>(Dp);
           const double W = wsum(wgt);
           const double C = wsum(static_cast<double>(k) * wgt);
           int sft = 0;
@@ -1229,34 +1247,37 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
             ez_to = H - o - kBandLanes;
           }
         }
-        if (v0 == 0) {
-          if (i0 > 1) {
-            const double m = wmax(fmax(Mp, fmax(Ip, Dp)));
-            if (m > 0.0) {
-              const int e = ilogb(m);
-              const double sc = ldexp(1.0, -e);
-              Mp *= sc; Ip *= sc; Dp *= sc;
-              TMp *= sc; TIp *= sc; TDp *= sc;
-              xg *= sc;
-              lg += e;
-            }
+        // Rescale by a power of two: every 64 rows in FP64, every 16 in
+        // FP32 (a run of indels costs ~3e-5 a row, so 64 rows can take the
+        // mass below FP32's range).
+        if (kF32 ? i > 1 : (v0 == 0 && i0 > 1)) {
+          const T m = wmax(fmax(Mp, fmax(Ip, Dp)));
+          if (m > T(0)) {
+            const int e = ilogb(m);
+            const T sc = ldexp(T(1), -e);
+            Mp *= sc; Ip *= sc; Dp *= sc;
+            TMp *= sc; TIp *= sc; TDp *= sc;
+            xg *= sc;
+            lg += e;
           }
+        }
+        if (v0 == 0) {
           __syncthreads();  // the previous block's LDS reads are done
           if (k < n) {
             const int64_t p = ro + i0 - 1 + k;
             const uint8_t rc = a.rclass[p];
             const int iq = a.ins[p] & 127, dq = a.del[p] & 127, cq = a.gcp[p] & 127;
             const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
-            const double e = a.ph2pr_d[a.q[p] & 127];
-            RowRec64 rr;
-            const double pm = 1.0 - e;
-            rr.pe = (rc & kWildcard) ? pm : e / 3.0;
+            const T e = ph2pr[a.q[p] & 127];
+            RowRecL<T> rr;
+            const T pm = T(1) - e;
+            rr.pe = (rc & kWildcard) ? pm : e / T(3);
             rr.dp = pm - rr.pe;
-            rr.mm = a.mm_d[((hi * (hi + 1)) >> 1) + lo];
-            rr.g = 1.0 - a.ph2pr_d[cq];
-            rr.mx = a.ph2pr_d[iq];
-            rr.xx = a.ph2pr_d[cq];
-            rr.my = a.ph2pr_d[dq];
+            rr.mm = mmt[((hi * (hi + 1)) >> 1) + lo];
+            rr.g = T(1) - ph2pr[cq];
+            rr.mx = ph2pr[iq];
+            rr.xx = ph2pr[cq];
+            rr.my = ph2pr[dq];
             rr.cls = rc & 3;
             rr.pad = 0;
             s.tab[k] = rr;
@@ -1271,32 +1292,45 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
         const int v1 = min(n, v0 + kLongEvery);
         for (int v = v0; v < v1; ++v) {
           const int ii = i0 + v;
-          const RowRec64 rr = s.tab[v];
+          const RowRecL<T> rr = s.tab[v];
           const uint32_t hmv = s.hm[ii + o + k - jb];  // in [0, 151)
-          const double sel = static_cast<double>((hmv >> rr.cls) & 1u);
-          const double prior = fma(sel, rr.dp, rr.pe);
+          const T sel = static_cast<T>((hmv >> rr.cls) & 1u);
+          const T prior = fma(sel, rr.dp, rr.pe);
           const bool ez = (left_lane && ii >= ez_from) || (right_lane && ii <= ez_to);
           const int col = ii + o + k;
           if (ii > R - kEdgeRunRows)
-            xg = fma(xg, yy, (k == 0 && col - 1 >= 1 && col - 1 <= H) ? fma(Mp, rr.mx, Ip * rr.xx) : 0.0);
+            xg = fma(xg, yy, (k == 0 && col - 1 >= 1 && col - 1 <= H) ? fma(Mp, rr.mx, Ip * rr.xx) : T(0));
 
-          double t = Ip * rr.g;
+          T t = Ip * rr.g;
           t = fma(Dp, rr.g, t);
           t = fma(Mp, rr.mm, t);
-          const double Mn = t * prior;
-          const double In = fma(dpp64<kWaveShl1>(Mp), rr.mx, dpp64<kWaveShl1>(Ip) * rr.xx);
-          const double Dn = decay_scan64(dpp64<kWaveShr1>(Mn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
-
-          double tt = TIp * rr.g;
+          const T Mn = t * prior;
+          T tt = TIp * rr.g;
           tt = fma(TDp, rr.g, tt);
           tt = fma(TMp, rr.mm, tt);
-          double TMn = tt * prior;
-          double TIn = fma(dpp64<kWaveShl1>(TMp), rr.mx, dpp64<kWaveShl1>(TIp) * rr.xx);
-          if (ez) {
-            TMn = Mn;
-            TIn = In;
+          T TMn = tt * prior;
+          T In, TIn, Dn, TDn;
+          if constexpr (kF32) {
+            // the FP32 kernel's DPP-folded I and the two D scans in one block
+            In = insert_next(Mp, Ip, rr.mx, rr.xx);
+            TIn = insert_next(TMp, TIp, rr.mx, rr.xx);
+            if (ez) {
+              TMn = Mn;
+              TIn = In;
+            }
+            Dn = dpp<kWaveShr1>(Mn) * rr.my;
+            TDn = dpp<kWaveShr1>(TMn) * rr.my;
+            decay_scan2<kBandLanes>(Dn, TDn, yy, yy2, yy4, yy8, f15, f31);
+          } else {
+            In = fma(dpp64<kWaveShl1>(Mp), rr.mx, dpp64<kWaveShl1>(Ip) * rr.xx);
+            Dn = decay_scan64(dpp64<kWaveShr1>(Mn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
+            TIn = fma(dpp64<kWaveShl1>(TMp), rr.mx, dpp64<kWaveShl1>(TIp) * rr.xx);
+            if (ez) {
+              TMn = Mn;
+              TIn = In;
+            }
+            TDn = decay_scan64(dpp64<kWaveShr1>(TMn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
           }
-          double TDn = decay_scan64(dpp64<kWaveShr1>(TMn) * rr.my, yy, yy2, yy4, yy8, f15, f31);
           if (ez) TDn = Dn;
 
           Mp = Mn;
@@ -1311,14 +1345,14 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
     // S = sum over the last row of M + I, columns 1..H.
     const int j = R + o + k;
     const bool in = j >= 1 && j <= H;
-    const double S = wsum(in ? Mp + Ip : 0.0);
-    const double Tm = wsum((in ? TMp + TIp : 0.0) + xg);
+    const T S = wsumT(in ? Mp + Ip : T(0));
+    const T Tm = wsumT((in ? TMp + TIp : T(0)) + xg);
     cells += static_cast<unsigned long long>(R) * kBandLanes;
     if (k == 0) {
-      const double L = log10(S) + static_cast<double>(lg) * kLog10Two;
-      if (S > 0.0 && Tm <= static_cast<double>(a.tol) * S && L < a.verify_above) {
+      const double L = log10(static_cast<double>(S)) + static_cast<double>(lg) * kLog10Two;
+      if (S > T(0) && Tm <= static_cast<T>(a.tol) * S && L < a.verify_above) {
         a.out[w.out] = L;
-        a.form[w.out] = kFormBandFP64;
+        a.form[w.out] = kF32 ? kFormBandLong32 : kFormBandFP64;
         ++accepted;
       } else {
         const int q = atomicAdd(a.long_full_count, 1);
@@ -1330,7 +1364,7 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
     if (accepted) atomicAdd(a.long_accepted, accepted);
     if (cells) {
       if (a.cells_done) atomicAdd(a.cells_done, cells);
-      if (a.fp64_cells) atomicAdd(a.fp64_cells, cells);
+      if (!kF32 && a.fp64_cells) atomicAdd(a.fp64_cells, cells);
     }
   }
 }
@@ -1338,14 +1372,18 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st) {
   if (a.n_tasks64 <= 0) return hipSuccess;
   hipLaunchKernelGGL(band_dispatch_long_kernel, dim3((a.n_tasks64 + 255) / 256), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(band_long_kernel, dim3(grid), dim3(kBandLanes), 0, st, a);
+  if (a.long_fp32)
+    hipLaunchKernelGGL(band_long_kernel<float>, dim3(grid), dim3(kBandLanes), 0, st, a);
+  else
+    hipLaunchKernelGGL(band_long_kernel<double>, dim3(grid), dim3(kBandLanes), 0, st, a);
   return hipGetLastError();
 }
 
-int band_long_blocks_per_cu() {
+int band_long_blocks_per_cu(int fp32) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_long_kernel, kBandLanes, 0) != hipSuccess)
-    return 8;
+  const hipError_t e = fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_long_kernel<float>, kBandLanes, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, band_long_kernel<double>, kBandLanes, 0);
+  if (e != hipSuccess) return 8;
   return n > 0 ? n : 1;
 }
 

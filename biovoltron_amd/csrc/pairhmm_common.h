@@ -23,6 +23,7 @@ enum Form : uint8_t {
   kFormBandFP32 = 2,    // banded FP32 forward accepted
   kFormBandFallback = 3,// band rejected (boundary mass), full FP32 used
   kFormBandFP64 = 4,    // long read: moving-band FP64 forward accepted
+  kFormBandLong32 = 5,  // long read: moving-band FP32 forward (rescaled per 64 rows) accepted
   kFormPending = 255,   // internal: queued for the FP64 pass
 };
 

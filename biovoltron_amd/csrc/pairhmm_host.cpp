@@ -26,7 +26,7 @@ hipError_t launch_band_quads(const BArgs& a, int grid, hipStream_t st);
 hipError_t launch_band_one(const BArgs& a, int grid, int grid1, hipStream_t st);
 int band_blocks_per_cu(int sel_cols, int wide_pairs);
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
-int band_long_blocks_per_cu();
+int band_long_blocks_per_cu(int fp32);
 hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, const int64_t* htab_off,
                                const unsigned long long* htab, const Task* tasks, int n_tasks, int2* band,
                                int grid, hipStream_t st);
@@ -212,7 +212,7 @@ int ensure_init_locked(int device) {
   if (!g_ctx.d_noalloc_scratch) return fail(BV_EHIP, "no device address for the allocation-free scratch");
   g_ctx.bpc_f32 = max_blocks_per_cu<float>();
   g_ctx.bpc_f64 = max_blocks_per_cu<double>();
-  g_ctx.bpc_long = band_long_blocks_per_cu();
+  g_ctx.bpc_long = std::max(band_long_blocks_per_cu(1), band_long_blocks_per_cu(0));
   g_ctx.ready = true;
   return BV_OK;
 }
@@ -297,6 +297,7 @@ struct bv_plan {
   int sel_cols = 0;  // short-read banded kernel: selector columns per class
   int wide_pairs = kQuadPairs;  // wide-band shape: pairs per wavefront (1 or kQuadPairs)
   bool mid_on = false;          // bands of 33-48 diagonals from bv_plan_set_bands on the 48-diagonal quad kernel
+  bool long_fp32 = true;        // the long reads' moving band in FP32 (rescaled every 64 rows), else FP64
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
   uint8_t* d_q = nullptr;
@@ -720,6 +721,7 @@ BArgs band_args(const bv_plan* p) {
   b.tasks64 = p->d_tasks64;
   b.n_tasks64 = p->n_tasks64;
   b.long_on = p->opt.fp64_renorm != 0;
+  b.long_fp32 = p->long_fp32 ? 1 : 0;
   b.long_items = p->d_items_long;
   b.long_count = ctr(p, 17);
   b.long_queue = ctr(p, 18);
@@ -1478,6 +1480,22 @@ int bv_plan_set_band_prologue(bv_plan* p, int32_t mode) {
   return BV_OK;
 }
 
+int bv_plan_set_long_precision(bv_plan* p, int32_t bits) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  if (bits != 32 && bits != 64) return fail(BV_EINVAL, "the moving band runs in 32 or 64 bits");
+  if ((bits == 32) != p->long_fp32) {
+    p->long_fp32 = bits == 32;
+    drop_graph(p);
+  }
+  return BV_OK;
+}
+
+int bv_plan_long_precision(bv_plan* p, int32_t* bits) {
+  if (!p || !bits) return fail(BV_EINVAL, "NULL argument");
+  *bits = p->long_fp32 ? 32 : 64;
+  return BV_OK;
+}
+
 int bv_plan_band_mid(bv_plan* p, int32_t* on) {
   if (!p || !on) return fail(BV_EINVAL, "NULL argument");
   *on = p->mid_on ? 1 : 0;
@@ -1582,8 +1600,9 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   s->n_pairs = p->n_pairs;
   s->cells = p->cells;
   const bool long_banded = fp32 && p->has_band && p->n_tasks64 > 0;
-  // FP64 pairs: rescued ones and every long read (full matrix or moving band)
-  s->fp64_pairs = fp32 ? cnt[2] + p->n_tasks64 : p->n_pairs;
+  // FP64 pairs: rescued ones and every long read (full matrix, or the moving
+  // band when it runs in FP64)
+  s->fp64_pairs = fp32 ? cnt[2] + p->n_tasks64 - (long_banded && p->long_fp32 ? cnt[21] : 0) : p->n_pairs;
   s->fp64_cells = static_cast<int64_t>(c64);
   s->tasks = p->n_tasks;
   s->grid_f32 = p->grid32;

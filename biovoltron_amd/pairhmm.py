@@ -32,7 +32,7 @@ from ._build import HIP_LIB
 BV_OK = 0
 MODE_AVX = 0
 MODE_SCALAR = 1
-FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_fallback", 4: "band_fp64"}
+FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_fallback", 4: "band_fp64", 5: "band_long_fp32"}
 
 
 class PairHMMError(RuntimeError):
@@ -87,6 +87,7 @@ EXPORTS = (
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
     "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_set_band_shape", "bv_plan_band_shape",
     "bv_plan_set_band_prologue", "bv_plan_set_band_mid", "bv_plan_band_mid",
+    "bv_plan_set_long_precision", "bv_plan_long_precision",
     "bv_plan_postprocess", "bv_sw_align", "bv_sw_release",
     "bv_pairhmm_batch",
 )
@@ -123,6 +124,8 @@ def library() -> C.CDLL:
         lib.bv_plan_set_band_prologue.argtypes = [vp, i32]
         lib.bv_plan_set_band_mid.argtypes = [vp, i32]
         lib.bv_plan_band_mid.argtypes = [vp, vp]
+        lib.bv_plan_set_long_precision.argtypes = [vp, i32]
+        lib.bv_plan_long_precision.argtypes = [vp, vp]
         lib.bv_plan_postprocess.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         lib.bv_plan_destroy.argtypes = [vp]
         lib.bv_plan_destroy.restype = None
@@ -293,6 +296,17 @@ class Plan:
         bands, else as 64-diagonal ones (bv_plan_set_band_mid).  Learned
         bands never take 48 diagonals."""
         _check(self._lib.bv_plan_set_band_mid(self._h, 1 if on else 0))
+
+    def set_long_precision(self, bits: int) -> None:
+        """Arithmetic of the long reads' moving band: 32 (default, FP32
+        rescaled every 64 rows, form 5) or 64 (FP64, form 4)
+        (bv_plan_set_long_precision)."""
+        _check(self._lib.bv_plan_set_long_precision(self._h, int(bits)))
+
+    def long_precision(self) -> int:
+        v = C.c_int32()
+        _check(self._lib.bv_plan_long_precision(self._h, C.byref(v)))
+        return int(v.value)
 
     def band_mid(self) -> bool:
         v = C.c_int32()

@@ -52,6 +52,8 @@ extern "C" {
 #define BV_FORM_BAND_FP32 2
 #define BV_FORM_BAND_FALLBACK 3
 #define BV_FORM_BAND_FP64 4 /* long read (>= 1024 bases): moving-band FP64 forward */
+#define BV_FORM_BAND_LONG32 5 /* long read: moving-band FP32 forward, rescaled by a power of
+                                 two every 64 rows (bv_plan_set_long_precision, default) */
 
 /* Initialise the device context and upload the tables (idempotent).  device
  * is a HIP ordinal; -1 keeps the current device. */
@@ -357,6 +359,16 @@ int bv_plan_band_shape(bv_plan* plan, int32_t* wide_pairs);
  * predicted centre that a 48-diagonal band's edge zones do not see.  No
  * reference counterpart. */
 int bv_plan_set_band_mid(bv_plan* plan, int32_t on);
+/* Arithmetic of the long reads' moving band: 32 (default) FP32 with the
+ * state rescaled by a power of two every 64 rows (exact, so it never
+ * underflows; log10 L adds the removed exponent), form BV_FORM_BAND_LONG32;
+ * or 64, FP64 as before, form BV_FORM_BAND_FP64.  Either way a pair whose
+ * touch mass exceeds the tolerance, or whose likelihood FP32 would reach
+ * unscaled (the reference's FP32 rule), takes the full-matrix FP64 pass.
+ * No reference counterpart: the reference runs long reads in FP32 and,
+ * after underflow, in FP64 on the full matrix. */
+int bv_plan_set_long_precision(bv_plan* plan, int32_t bits);
+int bv_plan_long_precision(bv_plan* plan, int32_t* bits);
 int bv_plan_band_mid(bv_plan* plan, int32_t* on);
 /* How a run with a band model builds its bands and work lists.  2 (the
  * default): the features kernel writes a compact record per pair (bin

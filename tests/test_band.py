@@ -539,15 +539,16 @@ def test_graph_with_long_reads_on_second_stream():
         band.set_plan_model(graph, model)
         eager.run()
         ref, ref_form = eager.results()
-        say("long-read pairs on the moving band", int((ref_form == 4).sum()), "of", len(ref))
-        assert (ref_form == 4).sum() > 0
+        on_band = np.isin(ref_form, (4, 5))  # the moving band in FP64 or FP32
+        say("long-read pairs on the moving band", int(on_band.sum()), "of", len(ref))
+        assert on_band.sum() > 0
         for k in range(3):
             graph.run()
             out, form = graph.results()
             assert np.array_equal(out, ref) and np.array_equal(form, ref_form), k
             eager.run()
             eager.results()
-        assert graph.stats().long_band_pairs == int((ref_form == 4).sum())
+        assert graph.stats().long_band_pairs == int(on_band.sum())
 
 
 @pytest.mark.gpu

@@ -1,4 +1,5 @@
-"""Long reads (1024+ bases, run in FP64) on the band that follows the alignment.
+"""Long reads (1024+ bases) on the band that follows the alignment, in FP32
+(the default, rescaled every 64 rows) and in FP64.
 
 The moving-band kernel against the oracle's restatement of the same band
 (bvo_band_follow_forward: likelihood, touch mass, dropped share and the
@@ -14,7 +15,11 @@ from biovoltron_amd import band, pairhmm, synth
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-3
-FORM_FP64, FORM_BAND_FP64 = 1, 4
+FORM_FP64 = 1
+FORM_BAND = {64: 4, 32: 5}  # the moving band's form per arithmetic
+# the kernel against the oracle's FP64 moving band: FP64 rounding, or FP32
+# rounding over reads of up to 10 kb
+BAND_EPS = {64: 1e-8, 32: 2e-4}
 VERIFY_ABOVE = -28.0 - 120.0 * np.log10(2.0) - 0.05
 # the moving band's rules: moves that would drop more than 1e-9 of a row's
 # mass are held; the touch mass counts the insertion runs leaving the bottom
@@ -36,8 +41,10 @@ def long_sample(regions=1, seed=None):
     return b, sel
 
 
-def run_plan(pb, bands=None, model=None):
+def run_plan(pb, bands=None, model=None, bits=32):
     with pairhmm.Plan(pb) as plan:
+        plan.set_long_precision(bits)
+        assert plan.long_precision() == bits
         if bands is not None:
             band.set_plan_bands(plan, bands)
         if model is not None:
@@ -49,7 +56,7 @@ def run_plan(pb, bands=None, model=None):
     return out, form, st, used
 
 
-def check_against_oracle(oracle, sel, out, form, used, tol):
+def check_against_oracle(oracle, sel, out, form, used, tol, bits):
     n_acc = n_rej = 0
     worst = 0.0
     for k, (rd, hap) in enumerate(sel):
@@ -59,7 +66,7 @@ def check_against_oracle(oracle, sel, out, form, used, tol):
             continue
         o = b0 - (64 - (b1 - b0 + 1)) // 2
         v, ratio, lost, _ = oracle.band_follow(rd, hap, o, **RULES)
-        if form[k] == FORM_BAND_FP64:
+        if form[k] == FORM_BAND[bits]:
             n_acc += 1
             worst = max(worst, abs(out[k] - v))
             assert ratio < 10 * tol and v < VERIFY_ABOVE + 1e-6, k
@@ -69,25 +76,27 @@ def check_against_oracle(oracle, sel, out, form, used, tol):
     return n_acc, n_rej, worst
 
 
-def test_long_band_matches_oracle_and_full_matrix(oracle):
+@pytest.mark.parametrize("bits", [32, 64])
+def test_long_band_matches_oracle_and_full_matrix(oracle, bits):
     """Learned mode on long-read regions: start bands from the seeds, the
     moving band for every long pair that has one.  Accepted pairs equal the
-    oracle's moving band to FP64 rounding; rejected ones took the full
-    matrix; every result is within 1e-3 of the full-matrix FP64 oracle."""
+    oracle's FP64 moving band to the kernel's rounding; rejected ones took
+    the full matrix; every result is within 1e-3 of the full-matrix FP64
+    oracle."""
     b, sel = long_sample(regions=2)
     pb = pairhmm.PackedBatch.from_synth(b)
     say("long pairs", len(sel), "cells", b.cells)
     full, ff, _, _ = run_plan(pb)
     model = band.BandModel.load(band.DEFAULT_MODEL)
-    out, form, st, used = run_plan(pb, model=model)
+    out, form, st, used = run_plan(pb, model=model, bits=bits)
     tol = pairhmm.default_options().band_tol
-    n_acc, n_rej, worst = check_against_oracle(oracle, sel, out, form, used, tol)
+    n_acc, n_rej, worst = check_against_oracle(oracle, sel, out, form, used, tol, bits)
     err = float(np.abs(out - full).max())
-    say(f"accepted {n_acc}, rejected {n_rej}, max |dlog10| vs oracle band {worst:.3g}, "
+    say(f"{bits}-bit band: accepted {n_acc}, rejected {n_rej}, max |dlog10| vs oracle band {worst:.3g}, "
         f"vs full {err:.3g}, long band pairs {st.long_band_pairs}, fp64 cells {st.fp64_cells}")
-    assert n_acc == st.long_band_pairs == int((form == FORM_BAND_FP64).sum())
+    assert n_acc == st.long_band_pairs == int((form == FORM_BAND[bits]).sum())
     assert n_acc >= 0.9 * len(sel)
-    assert worst <= 1e-8
+    assert worst <= BAND_EPS[bits]
     assert err <= TOL
     # the banded pairs computed far fewer FP64 cells than the full matrix
     assert st.fp64_cells < 0.1 * b.cells
@@ -99,7 +108,8 @@ def test_long_band_matches_oracle_and_full_matrix(oracle):
         assert abs(out[k] - ex) <= TOL
 
 
-def test_long_band_misplaced_start_falls_back():
+@pytest.mark.parametrize("bits", [32, 64])
+def test_long_band_misplaced_start_falls_back(bits):
     """Start bands 300 diagonals away from the alignment: the moving band
     cannot reach it, the touch mass says so, and the full matrix decides."""
     b, sel = long_sample(regions=1)
@@ -110,10 +120,10 @@ def test_long_band_misplaced_start_falls_back():
     good = used[:, 0] <= used[:, 1]
     bands = used.copy()
     bands[good] += 300
-    out, form, st, _ = run_plan(pb, bands=bands)
-    say("misplaced", int(good.sum()), "rejected", int((form[good] != FORM_BAND_FP64).sum()))
-    assert (form[good] != FORM_BAND_FP64).mean() >= 0.9
-    rej = form != FORM_BAND_FP64
+    out, form, st, _ = run_plan(pb, bands=bands, bits=bits)
+    say(bits, "misplaced", int(good.sum()), "rejected", int((form[good] != FORM_BAND[bits]).sum()))
+    assert (form[good] != FORM_BAND[bits]).mean() >= 0.9
+    rej = form != FORM_BAND[bits]
     assert np.array_equal(out[rej], full[rej])
     assert np.abs(out - full).max() <= TOL
 
@@ -125,10 +135,11 @@ def _drift_pair(rng, hap, pieces, q=30):
     return rd
 
 
-def test_long_band_follows_planted_drift(oracle):
+@pytest.mark.parametrize("bits", [32, 64])
+def test_long_band_follows_planted_drift(oracle, bits):
     """Reads whose alignment drifts by 60-100 diagonals (20-base deletions
     and insertions), started 12 diagonals off centre: accepted, equal to the
-    oracle's moving band and to the full matrix."""
+    oracle's moving band (to the kernel's rounding) and to the full matrix."""
     rng = np.random.default_rng(11)
     hap = bytes(rng.choice(list(b"ACGT"), 3000).astype(np.uint8))
     ins = bytes(rng.choice(list(b"ACGT"), 20).astype(np.uint8))
@@ -144,12 +155,12 @@ def test_long_band_follows_planted_drift(oracle):
     sel = [(rd, hap) for rd in reads]
     pb = pairhmm.PackedBatch.from_regions([([rd], [hap]) for rd in reads])
     bands = np.array([[s - 32 + 12, s + 31 + 12] for s in starts], np.int32)
-    out, form, st, used = run_plan(pb, bands=bands)
+    out, form, st, used = run_plan(pb, bands=bands, bits=bits)
     for k, (rd, h) in enumerate(sel):
         v, ratio, lost, offs = oracle.band_follow(rd, h, int(bands[k, 0]), **RULES)
         ex = oracle.log10_exact(rd, h, 0)
         say(f"drift read {k}: R={len(rd['seq'])} form {form[k]} gpu {out[k]:.9f} band {v:.9f} "
             f"exact {ex:.9f} ratio {ratio:.2e} lost {lost:.2e} final offset {offs[-1]}")
-        assert form[k] == FORM_BAND_FP64
-        assert abs(out[k] - v) <= 1e-8
-        assert abs(out[k] - ex) <= 1e-6
+        assert form[k] == FORM_BAND[bits]
+        assert abs(out[k] - v) <= BAND_EPS[bits]
+        assert abs(out[k] - ex) <= 1e-6 + BAND_EPS[bits]

@@ -142,7 +142,7 @@ def test_combined_workload_against_full_and_oracle():
     assert d.max() <= TOL
     # every path of the run is exercised: banded, rejected, FP64 rescue,
     # long reads on the moving band and in the full-matrix FP64 pass
-    for code in (1, 2, 3, 4):
+    for code in (1, 2, 3, 5):  # 5: the long reads' moving band (FP32, the default)
         assert (fb == code).sum() > 0, pairhmm.FORM_NAMES[code]
     assert (ff == 1).sum() > 0 and ((ff == 1) & long_pair).sum() > 0
     assert ((fb == 1) & long_pair).sum() > 0  # long pairs the moving band left
