@@ -206,7 +206,7 @@ def sw_cpu_baseline(pairs, device_results, budget_s: float):
                 sample=f"{done} of {len(pairs)} pairs, {cells:.3e} cells, {t:.1f} s"), done, mismatches
 
 
-FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_rejected_full_fp32", 4: "band_fp64_long",
+FORM_NAMES = {0: "fp32", 1: "fp64", 2: "band_fp32", 3: "band_rejected_full_fp32", 4: "band_fp64",
               5: "band_fp32_long"}
 
 

@@ -106,6 +106,14 @@ struct BArgs {
   Task* long_full;
   int* long_full_count;
   int* long_accepted;
+  int long_cap;  // entries of long_items: the long kernel runs min(*long_count, long_cap) of them
+  // Short-read pairs whose FP32 band sum fell below 1e-28 (the reference
+  // would re-run them in FP64): the FP64 moving band over a 64-diagonal band
+  // with the same centre, at most band_rescue_cap of them (the rest take the
+  // full-matrix lists as before); null when off
+  BandItem* band_rescue;
+  int* band_rescue_count;
+  int band_rescue_cap;
   const double* ph2pr_d;  // [128]
   const double* mm_d;     // [8256]
   double verify_above;    // log10 L at or above which FP32 might not underflow

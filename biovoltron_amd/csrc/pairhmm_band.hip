@@ -30,6 +30,7 @@
 // per-column class masks are staged in LDS once per pair.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <climits>
 #include <type_traits>
 
@@ -309,6 +310,25 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   }
 }
 
+// A pair the FP32 band did not accept.  One whose band sum fell below 1e-28
+// while still a normal float with the touch test passed (the band holds its
+// alignment, its likelihood is beyond FP32's reach): to the FP64 band rescue
+// list (a 64-diagonal band with the same centre) while it has room.  Any
+// other (the touch test failed, or the sum is not a normal float): the
+// full-matrix FP32 list, whose underflow sends it on to the FP64 pass.
+__device__ __forceinline__ void band_reject(const BArgs& a, int read, int hap, int o_lo, int width, float S,
+                                            float Tm, long long out) {
+  if (a.band_rescue != nullptr && S < kMinAccepted && S >= FLT_MIN && Tm <= static_cast<float>(a.tol) * S) {
+    const int q = atomicAdd(a.band_rescue_count, 1);
+    if (q < a.band_rescue_cap) {
+      a.band_rescue[q] = BandItem{read, hap, o_lo - ((kBandLanes - width) >> 1), 0, out};
+      return;
+    }
+  }
+  const int q = atomicAdd(a.full_count, 1);
+  a.full[q] = Task{read, hap, 1, -2, out};
+}
+
 // Banded forward over one list.  G = 64: one pair per wavefront.  G = 32: the
 // two pairs 2u, 2u+1 (one read, two haplotypes) in lanes 0-31 and 32-63.
 template <int G>
@@ -586,8 +606,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
         a.form[w.out] = kFormBandFP32;
         ++accepted;
       } else {
-        const int q = atomicAdd(a.full_count, 1);
-        a.full[q] = Task{read, hap, 1, -2, w.out};
+        band_reject(a, read, hap, o_lo, G, S, Tm, w.out);
       }
     }
   }
@@ -1031,8 +1050,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ba
           a.form[w.out] = kFormBandFP32;
           ++accepted;
         } else {
-          const int q = atomicAdd(a.full_count, 1);
-          a.full[q] = Task{read, hap, 1, -2, w.out};
+          band_reject(a, read, hap, o_lo, W, S, Tm, w.out);
         }
       }
     }
@@ -1177,7 +1195,7 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
   const bool left_lane = k < kBandEdge;
   const bool right_lane = k >= kBandLanes - kBandEdge;
   const int row16 = k >> 4;
-  const int n_items = *a.long_count;
+  const int n_items = min(*a.long_count, a.long_cap);
   int accepted = 0;
   unsigned long long cells = 0;
   for (;;) {
@@ -1367,6 +1385,23 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
       if (!kF32 && a.fp64_cells) atomicAdd(a.fp64_cells, cells);
     }
   }
+}
+
+// The FP64 band rescue (band_reject): the FP64 moving-band kernel over the
+// rescue list, its rejects to the FP64 full-matrix rescue list.
+hipError_t launch_band_rescue(const BArgs& b, Task* rescue, int* rescue_count, int* queue, int* accepted, int grid,
+                              hipStream_t st) {
+  if (b.band_rescue == nullptr) return hipSuccess;
+  BArgs a = b;
+  a.long_items = b.band_rescue;
+  a.long_count = b.band_rescue_count;
+  a.long_cap = b.band_rescue_cap;
+  a.long_queue = queue;
+  a.long_full = rescue;
+  a.long_full_count = rescue_count;
+  a.long_accepted = accepted;
+  hipLaunchKernelGGL(band_long_kernel<double>, dim3(grid), dim3(kBandLanes), 0, st, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st) {

@@ -22,7 +22,7 @@ enum Form : uint8_t {
   kFormFP64 = 1,        // FP32 underflowed, FP64 re-run
   kFormBandFP32 = 2,    // banded FP32 forward accepted
   kFormBandFallback = 3,// band rejected (boundary mass), full FP32 used
-  kFormBandFP64 = 4,    // long read: moving-band FP64 forward accepted
+  kFormBandFP64 = 4,    // moving-band FP64 forward accepted (long read; short pair whose FP32 band underflowed)
   kFormBandLong32 = 5,  // long read: moving-band FP32 forward (rescaled per 64 rows) accepted
   kFormPending = 255,   // internal: queued for the FP64 pass
 };

@@ -26,6 +26,8 @@ hipError_t launch_band_quads(const BArgs& a, int grid, hipStream_t st);
 hipError_t launch_band_one(const BArgs& a, int grid, int grid1, hipStream_t st);
 int band_blocks_per_cu(int sel_cols, int wide_pairs);
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st);
+hipError_t launch_band_rescue(const BArgs& b, Task* rescue, int* rescue_count, int* queue, int* accepted, int grid,
+                              hipStream_t st);
 int band_long_blocks_per_cu(int fp32);
 hipError_t launch_long_anchors(const uint8_t* rclass, const int64_t* read_off, const int64_t* htab_off,
                                const unsigned long long* htab, const Task* tasks, int n_tasks, int2* band,
@@ -104,6 +106,9 @@ constexpr int kQuadClaims = 56;    // [56..63]: the quad banded kernel's per-XCD
 constexpr int kItem1Count = 64;    // [64]: the quad shape's leftover wide pairs
 constexpr int kItem48Count = 13;   // [13]: the mid band's item count
 constexpr int kQuad48Claims = 80;  // [80..87]: the mid-band quad kernel's per-XCD chunk claims
+constexpr int kBandRescueCount = 50;  // [50]: short pairs whose FP32 band underflowed, on the FP64 band
+constexpr int kBandRescueQueue = 51;  // [51]: its queue
+constexpr int kBandRescueAccepted = 52;  // [52]: the pairs it accepted
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -332,6 +337,9 @@ struct bv_plan {
   BandItem* d_items = nullptr;    // banded work lists, built on the device each run
   BandItem* d_items1 = nullptr;   // the quad shape's leftover wide and mid pairs
   BandItem* d_items48 = nullptr;  // the mid band's quad groups
+  BandItem* d_band_rescue = nullptr;  // short pairs whose FP32 band sum underflowed
+  int band_rescue_cap = 0;
+  int grid_rescue = 1;
   BandItem* d_items32 = nullptr;
   BandItem* d_items_long = nullptr;  // long reads on the moving band
   Task* d_long_full = nullptr;       // long reads left to the full-matrix FP64 pass
@@ -364,7 +372,9 @@ struct bv_plan {
   // moving-band pairs, [22] FP64 cells (u64), [24..31] features task claims
   // (one per XCD), [32..47] banded chunk claims (per XCD, wide then narrow
   // kernel), [48] full-matrix list with haplotypes longer than kStackHmax,
-  // [49] its queue, [56..63] quad banded chunk claims (per XCD), [64] the
+  // [49] its queue, [50] FP64 band rescue count, [51] its queue, [52] its
+  // accepted pairs, [56..63]
+  // quad banded chunk claims (per XCD), [64] the
   // quad shape's leftover wide and mid pairs, [72..79] the split prologue's
   // second kernel's task claims (per XCD), [80..87] the mid-band quad
   // kernel's chunk claims (per XCD); the rest unused
@@ -415,7 +425,7 @@ void free_plan(bv_plan* p) {
                   p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
                   p->d_stasks, p->d_tasks_rest, p->d_fallback_long,
                   p->d_rescue, p->d_counters,
-                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items, p->d_items48,
+                  p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items, p->d_items48, p->d_band_rescue,
                   p->d_items1, p->d_items32, p->d_items_long, p->d_long_full,
                   p->d_X, p->d_anchor, p->d_Y, p->d_model, p->d_ends, p->d_htab, p->d_htab_off, p->d_rec, p->d_trec,
                   p->d_region_tab, p->d_hap_allele, p->d_n_alleles, p->d_allele_off, p->d_lnorm,
@@ -612,6 +622,13 @@ int ensure_band_lists(bv_plan* p) {
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (p->mid_on && !p->d_items48 && hipMalloc(&p->d_items48, sizeof(BandItem) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
+  // FP64 band rescue: room for a sixteenth of the pairs (more go the
+  // full-matrix way)
+  if (!p->d_band_rescue) {
+    p->band_rescue_cap = static_cast<int>(std::min<int64_t>(INT_MAX / 2, std::max<int64_t>(4096, n / 16)));
+    if (hipMalloc(&p->d_band_rescue, sizeof(BandItem) * p->band_rescue_cap) != hipSuccess)
+      return fail(BV_ENOMEM, "band work list allocation failed");
+  }
   if (!p->d_items1 &&
       hipMalloc(&p->d_items1, sizeof(BandItem) * std::max<int64_t>(1, 2 * (kQuadPairs - 1) * p->n_subtasks)) !=
           hipSuccess)
@@ -735,6 +752,10 @@ BArgs band_args(const bv_plan* p) {
   b.sel_cols = p->sel_cols;
   b.wide_pairs = p->wide_pairs;
   b.mid_on = p->mid_on && p->d_items48 != nullptr;
+  b.long_cap = std::max(1, p->n_tasks64);
+  b.band_rescue = p->d_band_rescue;
+  b.band_rescue_count = ctr(p, kBandRescueCount);
+  b.band_rescue_cap = p->band_rescue_cap;
   return b;
 }
 
@@ -882,6 +903,13 @@ int enqueue_run(bv_plan* p, bool timed) {
     }
   }
   if (timed) HIP_TRY(hipEventRecord(p->ev[4], p->stream));
+  if (fp32 && p->has_band) {
+    // Short pairs whose FP32 band sum fell below 1e-28: the FP64 moving band
+    // (64 diagonals, same centre); the pairs it does not accept join the
+    // FP64 full-matrix list.
+    HIP_TRY(launch_band_rescue(band_args(p), p->d_rescue, ctr(p, 2), ctr(p, kBandRescueQueue),
+                               ctr(p, kBandRescueAccepted), p->grid_rescue, p->stream));
+  }
   {
     // Pass 3: FP64 over the pairs whose FP32 sum fell below 1e-28 (or all
     // pairs in the FP64-only modes).
@@ -1272,6 +1300,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   p->grid_long = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * g_ctx.bpc_long, p->n_tasks64)));
   p->grid_long_max = p->grid_long;
+  p->grid_rescue = std::max(1, g_ctx.cus * g_ctx.bpc_long);
   p->grid_stack = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>(static_cast<int64_t>(g_ctx.cus) * stack_blocks_per_cu(p->stack_cols),
                            std::max(1, p->n_stasks))));
@@ -1602,14 +1631,15 @@ int bv_plan_stats_get(bv_plan* p, bv_plan_stats* s) {
   const bool long_banded = fp32 && p->has_band && p->n_tasks64 > 0;
   // FP64 pairs: rescued ones and every long read (full matrix, or the moving
   // band when it runs in FP64)
-  s->fp64_pairs = fp32 ? cnt[2] + p->n_tasks64 - (long_banded && p->long_fp32 ? cnt[21] : 0) : p->n_pairs;
+  s->fp64_pairs =
+      fp32 ? cnt[2] + cnt[kBandRescueAccepted] + p->n_tasks64 - (long_banded && p->long_fp32 ? cnt[21] : 0) : p->n_pairs;
   s->fp64_cells = static_cast<int64_t>(c64);
   s->tasks = p->n_tasks;
   s->grid_f32 = p->grid32;
   s->grid_f64 = p->grid64;
   s->last_ms_f32 = p->last_ms32;
   s->last_ms_f64 = p->last_ms64;
-  s->band_pairs = p->has_band ? cnt[5] + cnt[21] : 0;
+  s->band_pairs = p->has_band ? cnt[5] + cnt[21] + cnt[kBandRescueAccepted] : 0;
   s->fallback_pairs = p->has_band ? cnt[3] + cnt[48] : 0;
   s->cells_swept = static_cast<int64_t>(swept);
   s->last_ms_fallback = p->last_msfb;

@@ -51,7 +51,9 @@ extern "C" {
 #define BV_FORM_FP64 1
 #define BV_FORM_BAND_FP32 2
 #define BV_FORM_BAND_FALLBACK 3
-#define BV_FORM_BAND_FP64 4 /* long read (>= 1024 bases): moving-band FP64 forward */
+#define BV_FORM_BAND_FP64 4 /* moving-band FP64 forward: a long read (>= 1024 bases) with
+                               bv_plan_set_long_precision(plan, 64), or a short-read pair whose
+                               FP32 band sum fell below 1e-28 (the reference's FP64 re-run) */
 #define BV_FORM_BAND_LONG32 5 /* long read: moving-band FP32 forward, rescaled by a power of
                                  two every 64 rows (bv_plan_set_long_precision, default) */
 
