@@ -489,6 +489,36 @@ def test_mid_band_on_off():
 
 
 @pytest.mark.gpu
+def test_fp64_band_rescue_of_underflowing_short_pairs(oracle):
+    """Short-read pairs whose FP32 band sum falls below 1e-28 (beyond FP32's
+    reach: the reference re-runs them in FP64) take the FP64 moving band
+    (form 4) when their band passed the touch test: equal to the FP64
+    full-matrix oracle, and counted as FP64 and banded pairs."""
+    b = synth.generate(1, seed=synth.SEED_BASE + 555, count=300)
+    pb = pairhmm.PackedBatch.from_synth(b)
+    full, ff, _, _ = run_plan(pb)
+    out, form, st, _ = run_plan(pb, model=band.BandModel.load(band.DEFAULT_MODEL))
+    rr, hh = b.pair_table()
+    resc = np.flatnonzero(form == 4)
+    say("pairs", pb.n_pairs, "FP64 in the full leg", int((ff == 1).sum()), "on the FP64 band", resc.size)
+    assert resc.size > 0
+    assert (ff[resc] == 1).all() and (full[resc] < -64.1).all()
+    assert st.fp64_pairs >= resc.size and st.band_pairs >= resc.size
+    assert np.abs(out - full).max() <= 1e-3
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for k in rng.choice(resc, size=min(40, resc.size), replace=False):
+        r, h = int(rr[k]), int(hh[k])
+        a, e = int(b.read_off[r]), int(b.read_off[r + 1])
+        rd = dict(seq=b.read_bases[a:e].tobytes(), q=b.read_qual[a:e].tobytes(), i=b.read_ins[a:e].tobytes(),
+                  d=b.read_del[a:e].tobytes(), c=b.read_gcp[a:e].tobytes())
+        hap = b.hap_bases[int(b.hap_off[h]):int(b.hap_off[h + 1])].tobytes()
+        worst = max(worst, abs(out[k] - oracle.log10_exact(rd, hap, 0)))
+    say("max |dlog10| of the FP64 band rescue vs the FP64 oracle", worst)
+    assert worst <= 1e-6
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("learned", [False, True])
 def test_graph_replays_match_eager_runs(learned):
     """bv_plan_run replays one captured graph; every replay, with eager runs
