@@ -99,7 +99,7 @@ struct BArgs {
   const Task* tasks64;
   int n_tasks64;
   int long_on;  // 0: every long pair takes the full matrix
-  int long_fp32;  // the moving band in FP32 (rescaled every 64 rows), else FP64
+  int long_fp32;  // the moving band in FP32 (rescaled every 16 rows), else FP64 (every 64)
   BandItem* long_items;
   int* long_count;
   int* long_queue;
@@ -107,6 +107,13 @@ struct BArgs {
   int* long_full_count;
   int* long_accepted;
   int long_cap;  // entries of long_items: the long kernel runs min(*long_count, long_cap) of them
+  // FP32 moving band, quad shape: the long reads one task each (all their
+  // haplotypes), and the units of four pairs of one read they give
+  const Task* tasks64r;
+  int n_tasks64r;
+  BandItem* long_items4;
+  int* long4_count;
+  int* long4_queue;
   // Short-read pairs whose FP32 band sum fell below 1e-28 (the reference
   // would re-run them in FP64): the FP64 moving band over a 64-diagonal band
   // with the same centre, at most band_rescue_cap of them (the rest take the

@@ -1387,6 +1387,332 @@ __global__ __launch_bounds__(64) void band_long_kernel(BArgs a) {
   }
 }
 
+// ---- long reads, quad shape (FP32): four pairs of one long read per
+// wavefront ------------------------------------------------------------------
+//
+// The FP32 moving band over the quad kernel's layout: pair slot p owns the
+// 16-lane DPP row p, lane k holds the diagonals o_p + 4k .. o_p + 4k + 3 of
+// its pair's 64-diagonal band, and a row step is the quad kernel's (D
+// carried divided by its row's my, I and D times g, the three-FMA chain and
+// the decayed scan, T = F on the active edge zones of every row).  The four
+// pairs share the read and so the row records; each has its own band that
+// moves as in band_long_kernel: before every 16th row its centroid moves it
+// by up to 4 diagonals unless the cells it would drop hold more than
+// kBandHold of the pair's row mass (cells travel between lanes by
+// ds_bpermute), and its state is rescaled by a power of two.  Per 64-row
+// block a 192-column window of the haplotype's match selectors (f16, per
+// read class) is staged per pair: with at most three moves of 4 in a block,
+// row i lane k cell c reads window slot (i - i0) + (o - o0) + 12 + 4k + c.
+// A read's pairs come in units of four (band_dispatch_long4_kernel); the
+// rest run on band_long_kernel<float>.
+
+// Splits the long reads' pairs: units of four pairs of one read with a band
+// to the quad list, the rest with a band to the one-pair list, pairs
+// without one to the full-matrix FP64 list.  One thread per long read.
+__global__ __launch_bounds__(256) void band_dispatch_long4_kernel(BArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_tasks64r) return;
+  const Task tk = a.tasks64r[t];
+  const bool gc = a.long_on && a.gcp_const[tk.read] != 0;
+  auto band_ok = [&](int h, int2& b) {
+    b = a.band[tk.out0 + h];
+    const int64_t w = static_cast<int64_t>(b.y) - b.x + 1;
+    return gc && w >= 1 && w <= kBandLanes && b.x >= -(1 << 26) && b.y <= (1 << 26);
+  };
+  int n_ok = 0;
+  for (int h = 0; h < tk.nhap; ++h) {
+    int2 b;
+    n_ok += band_ok(h, b);
+  }
+  const int nq = n_ok & ~(kQuadPairs - 1);
+  int bq = nq > 0 ? atomicAdd(a.long4_count, nq) : 0;
+  int kq = 0;
+  for (int h = 0; h < tk.nhap; ++h) {
+    int2 b;
+    const int64_t o = tk.out0 + h;
+    if (band_ok(h, b)) {
+      const int w = b.y - b.x + 1;
+      const BandItem it{tk.read, tk.hap0 + h, b.x - ((kBandLanes - w) >> 1), 0, o};
+      if (kq < nq) {
+        a.long_items4[bq + kq++] = it;
+      } else {
+        const int q = atomicAdd(a.long_count, 1);
+        a.long_items[q] = it;
+      }
+    } else {
+      const int q = atomicAdd(a.long_full_count, 1);
+      a.long_full[q] = Task{tk.read, tk.hap0 + h, 1, 0, o};
+    }
+  }
+}
+
+// Maximum over each 16-lane row, returned to every lane of the row.
+__device__ __forceinline__ float row_max(float v, int lane) {
+  v = fmaxf(v, dpp<kRowShr + 1>(v));
+  v = fmaxf(v, dpp<kRowShr + 2>(v));
+  v = fmaxf(v, dpp<kRowShr + 4>(v));
+  v = fmaxf(v, dpp<kRowShr + 8>(v));
+  return __shfl(v, lane | 15);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void band_long_quad_kernel(BArgs a) {
+  __shared__ RowRec tab[kBandLanes];
+  __shared__ _Float16 selw[kQuadPairs][4][kLongWin];
+  const int lane = threadIdx.x;
+  const int p = lane >> 4;  // pair slot: the DPP row
+  const int k = lane & 15;  // lane in the row: diagonals o + 4k .. o + 4k + 3
+  const bool left_lane = k == 0;
+  const bool right_lane = k == kQuadLanes - 1;
+  const int n_units = min(*a.long4_count, a.long_cap) / kQuadPairs;
+  int accepted = 0;
+  unsigned long long cells = 0;
+  for (;;) {
+    int u = 0;
+    if (lane == 0) u = atomicAdd(a.long4_queue, 1);
+    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0));
+    if (u >= n_units) break;
+    const BandItem w = a.long_items4[kQuadPairs * u + p];
+    const int read = __builtin_amdgcn_readfirstlane(a.long_items4[kQuadPairs * u].read);
+    const int hap = w.hap;
+    int o = w.o_lo;
+    const int64_t ro = a.read_off[read];
+    const int R = static_cast<int>(a.read_off[read + 1] - ro);
+    const int64_t ho = a.hap_off[hap];
+    const int H = static_cast<int>(a.hap_off[hap + 1] - ho);
+    const int cq0 = a.gcp[ro] & 127;
+    const float yy = a.ph2pr[cq0];
+    const float gx = 1.f - a.ph2pr[cq0];
+    const float ig = static_cast<float>(1.0 / static_cast<double>(gx));
+    const float yy2 = yy * yy, yy3 = yy2 * yy, yy4 = yy2 * yy2, yy8 = yy4 * yy4, yy16 = yy8 * yy8,
+                yy32 = yy16 * yy16;
+
+    // Row 0: D = g / H on columns 0..H (the constant K of the reference
+    // cancels); T = F on the edge zones active there.
+    float M[4], I[4], D[4], TM[4], TI[4], TD[4];
+    float myp = 1.f;
+    {
+      const float d0 = gx / static_cast<float>(H);
+      const bool ez0 = (left_lane && o - 1 >= 0) || (right_lane && o + kBandLanes <= H);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int d = o + 4 * k + c;
+        M[c] = I[c] = TM[c] = TI[c] = 0.f;
+        D[c] = (d >= 0 && d <= H) ? d0 : 0.f;
+        TD[c] = ez0 ? D[c] : 0.f;
+      }
+    }
+    int lg = 0;       // exponent removed by rescaling (this pair's)
+    float xg = 0.f;   // read-end insertion runs leaving through the bottom lane
+    int o0 = o;       // the band's offset when the block's window was staged
+    for (int i0 = 1; i0 <= R; i0 += kBandLanes) {
+      const int n = min(kBandLanes, R - i0 + 1);
+      for (int v0 = 0; v0 < n; v0 += kLongEvery) {
+        const int i = i0 + v0;  // next row; the state holds row i - 1
+        if (i > 1) {
+          // cells past the haplotype end never feed in-matrix cells
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if ((i - 1) + o + 4 * k + c > H) M[c] = I[c] = D[c] = TM[c] = TI[c] = TD[c] = 0.f;
+          float wv[4], wl = 0.f, cl = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            wv[c] = __builtin_fmaf(__builtin_fmaf(D[c], myp, I[c]), ig, M[c]);
+            wl += wv[c];
+            cl = __builtin_fmaf(static_cast<float>(4 * k + c), wv[c], cl);
+          }
+          const float W = __shfl(row_total(wl), lane | 15);
+          const float Cn = __shfl(row_total(cl), lane | 15);
+          int sft = 0;
+          if (W > 0.f) {
+            sft = static_cast<int>(floorf(Cn / W - 0.5f * (kBandLanes - 1) + 0.5f));
+            sft = min(kLongMove, max(-kLongMove, sft));
+          }
+          // hold: no move that drops more than kBandHold of the pair's row mass
+          float gone = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int t = 4 * k + c;
+            if (t - sft < 0 || t - sft >= kBandLanes) gone += wv[c];
+          }
+          const float G = __shfl(row_total(gone), lane | 15);
+          if (sft != 0 && G > kBandHold * W) sft = 0;
+          if (__ballot(sft != 0) != 0) {
+            // cell t of the pair takes the old cell t + sft (0 outside): the
+            // old cells of lanes kA and kA + 1 of the same row
+            const int base = 4 * k + sft;              // in [-4, 67]
+            const int kA = ((base + 64) >> 2) - 16;    // floor(base / 4)
+            const int r = base - 4 * kA;               // 0..3
+            const int la = (lane & ~15) | (kA & 15), lb = (lane & ~15) | ((kA + 1) & 15);
+            const bool okA = kA >= 0 && kA < kQuadLanes, okB = kA + 1 >= 0 && kA + 1 < kQuadLanes;
+            auto shift = [&](float* sv) {
+              float A[4], B[4];
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                A[c] = __shfl(sv[c], la);
+                B[c] = __shfl(sv[c], lb);
+              }
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                const int q = r + c;  // 0..6
+                const float va = q == 0 ? A[0] : q == 1 ? A[1] : q == 2 ? A[2] : A[3];
+                const float vb = q == 4 ? B[0] : q == 5 ? B[1] : B[2];
+                sv[c] = q < 4 ? (okA ? va : 0.f) : (okB ? vb : 0.f);
+              }
+            };
+            shift(M);
+            shift(I);
+            shift(D);
+            shift(TM);
+            shift(TI);
+            shift(TD);
+            o += sft;
+          }
+          // rescale this pair by a power of two
+          float m = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) m = fmaxf(m, fmaxf(M[c], fmaxf(I[c], D[c])));
+          m = row_max(m, lane);
+          if (m > 0.f) {
+            const int e = ilogbf(m);
+            const float sc = ldexpf(1.f, -e);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              M[c] *= sc; I[c] *= sc; D[c] *= sc;
+              TM[c] *= sc; TI[c] *= sc; TD[c] *= sc;
+            }
+            xg *= sc;
+            lg += e;
+          }
+        }
+        if (v0 == 0) {
+          __syncthreads();  // the previous block's LDS reads are done
+          if (lane < n) {
+            const int64_t q = ro + i0 - 1 + lane;
+            const uint8_t rc = a.rclass[q];
+            const int iq = a.ins[q] & 127, dq = a.del[q] & 127;
+            const int lo = iq < dq ? iq : dq, hi = iq < dq ? dq : iq;
+            const float e = a.ph2pr[a.q[q] & 127];
+            RowRec rr;
+            const float pm = 1.f - e;
+            rr.pe = (rc & kWildcard) ? pm : e / 3.f;
+            rr.dp = pm - rr.pe;
+            rr.mm = a.mm[((hi * (hi + 1)) >> 1) + lo];
+            rr.mx = gx * a.ph2pr[iq];
+            rr.xx = yy;
+            rr.my = gx * a.ph2pr[dq];
+            rr.selo = static_cast<uint32_t>(2 * ((rc & 3) * kLongWin + lane));
+            rr.pad = 0.f;
+            tab[lane] = rr;
+          }
+          o0 = o;
+          const int jb = i0 + o0 - kLongSlack;  // column of window slot 0
+          for (int s = k; s < kLongWin; s += kQuadLanes) {
+            const int j = jb + s;
+            const uint32_t mk = (j >= 1 && j <= H) ? a.hmask[ho + j - 1] : 0u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) selw[p][q][s] = ((mk >> q) & 1u) ? _Float16(1) : _Float16(0);
+          }
+          __syncthreads();
+        }
+        const int v1 = min(n, v0 + kLongEvery);
+        // this lane's cell 0 slot in its pair's window, less the row's part
+        const _Float16* sl = &selw[p][0][0] + ((o - o0) + kLongSlack + 4 * k);
+        for (int v = v0; v < v1; ++v) {
+          const int ii = i0 + v;
+          const RowRec rr = tab[v];
+          const char* sb = reinterpret_cast<const char*>(sl) + rr.selo;
+          float pr[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            pr[c] = __builtin_fmaf(static_cast<float>(*reinterpret_cast<const _Float16*>(sb + 2 * c)), rr.dp, rr.pe);
+          if (ii > R - kEdgeRunRows) {
+            const int c0 = ii - 1 + o;  // the bottom cell's column on row ii - 1
+            const float xe = __builtin_fmaf(M[0], rr.mx, I[0] * rr.xx);
+            xg = __builtin_fmaf(xg, yy, (c0 >= 1 && c0 <= H) ? xe : 0.f);
+          }
+          const bool ez = (left_lane && ii + o >= 2) || (right_lane && ii + o + kBandLanes <= H);
+          float Mn[4], In[4], TMn[4], TIn[4], Y[4], TY[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            Mn[c] = __builtin_fmaf(M[c], rr.mm, __builtin_fmaf(D[c], myp, I[c])) * pr[c];
+            TMn[c] = __builtin_fmaf(TM[c], rr.mm, __builtin_fmaf(TD[c], myp, TI[c])) * pr[c];
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            In[c] = __builtin_fmaf(M[c + 1], rr.mx, I[c + 1] * rr.xx);
+            TIn[c] = __builtin_fmaf(TM[c + 1], rr.mx, TI[c + 1] * rr.xx);
+          }
+          In[3] = dpp<kRowShl1>(I[0]) * rr.xx;
+          TIn[3] = dpp<kRowShl1>(TI[0]) * rr.xx;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if (ez) TMn[c] = Mn[c];
+            Y[c] = Mn[c];
+            TY[c] = TMn[c];
+          }
+          myp = rr.my;
+          quad_cross(In[3], TIn[3], Y[0], TY[0], M[0], TM[0], Y[3], TY[3], rr.mx, yy);
+          float d1 = Y[0], td1 = TY[0];
+          float d2 = __builtin_fmaf(d1, yy, Y[1]), td2 = __builtin_fmaf(td1, yy, TY[1]);
+          float x = __builtin_fmaf(d2, yy, Y[2]), tx = __builtin_fmaf(td2, yy, TY[2]);
+          quad_scan(x, tx, d1, td1, d2, td2, yy4, yy8, yy16, yy32, yy2, yy3);
+          const float z = __builtin_fmaf(x, yy, Y[3]), tz = __builtin_fmaf(tx, yy, TY[3]);
+          const float Dn0 = dpp<kRowShr1>(z);
+          float TDn0 = dpp<kRowShr1>(tz);
+          float TDn1 = td1, TDn2 = td2, TDn3 = tx;
+          if (ez) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) TIn[c] = In[c];
+            TDn0 = Dn0;
+            TDn1 = d1;
+            TDn2 = d2;
+            TDn3 = x;
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            M[c] = Mn[c];
+            I[c] = In[c];
+            TM[c] = TMn[c];
+            TI[c] = TIn[c];
+          }
+          D[0] = Dn0; D[1] = d1; D[2] = d2; D[3] = x;
+          TD[0] = TDn0; TD[1] = TDn1; TD[2] = TDn2; TD[3] = TDn3;
+        }
+      }
+    }
+    // S = sum over the last row of M + I, columns 1..H (I carried times g);
+    // lane 15 of the row gets its pair's totals.
+    float sS = 0.f, sT = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = R + o + 4 * k + c;
+      if (j >= 1 && j <= H) {
+        sS += __builtin_fmaf(I[c], ig, M[c]);
+        sT += __builtin_fmaf(TI[c], ig, TM[c]);
+      }
+    }
+    if (left_lane) sT += xg * ig;
+    const float S = row_total(sS);
+    const float Tm = row_total(sT);
+    if (right_lane) {
+      cells += static_cast<unsigned long long>(R) * kBandLanes;
+      const double L = log10(static_cast<double>(S)) + static_cast<double>(lg) * kLog10Two;
+      if (S > 0.f && Tm <= static_cast<float>(a.tol) * S && L < a.verify_above) {
+        a.out[w.out] = L;
+        a.form[w.out] = kFormBandLong32;
+        ++accepted;
+      } else {
+        const int q = atomicAdd(a.long_full_count, 1);
+        a.long_full[q] = Task{read, hap, 1, -2, w.out};
+      }
+    }
+  }
+  if (right_lane) {
+    if (accepted) atomicAdd(a.long_accepted, accepted);
+    if (cells && a.cells_done) atomicAdd(a.cells_done, cells);
+  }
+}
+
 // The FP64 band rescue (band_reject): the FP64 moving-band kernel over the
 // rescue list, its rejects to the FP64 full-matrix rescue list.
 hipError_t launch_band_rescue(const BArgs& b, Task* rescue, int* rescue_count, int* queue, int* accepted, int grid,
@@ -1406,6 +1732,14 @@ hipError_t launch_band_rescue(const BArgs& b, Task* rescue, int* rescue_count, i
 
 hipError_t launch_band_long(const BArgs& a, int grid, hipStream_t st) {
   if (a.n_tasks64 <= 0) return hipSuccess;
+  if (a.long_fp32 && a.n_tasks64r > 0) {
+    // FP32: a read's pairs in units of four on the quad kernel, the rest on
+    // the one-pair kernel
+    hipLaunchKernelGGL(band_dispatch_long4_kernel, dim3((a.n_tasks64r + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(band_long_quad_kernel, dim3(grid), dim3(kBandLanes), 0, st, a);
+    hipLaunchKernelGGL(band_long_kernel<float>, dim3(grid), dim3(kBandLanes), 0, st, a);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(band_dispatch_long_kernel, dim3((a.n_tasks64 + 255) / 256), dim3(256), 0, st, a);
   if (a.long_fp32)
     hipLaunchKernelGGL(band_long_kernel<float>, dim3(grid), dim3(kBandLanes), 0, st, a);

@@ -109,6 +109,8 @@ constexpr int kQuad48Claims = 80;  // [80..87]: the mid-band quad kernel's per-X
 constexpr int kBandRescueCount = 50;  // [50]: short pairs whose FP32 band underflowed, on the FP64 band
 constexpr int kBandRescueQueue = 51;  // [51]: its queue
 constexpr int kBandRescueAccepted = 52;  // [52]: the pairs it accepted
+constexpr int kLong4Count = 53;  // [53]: the FP32 moving band's quad-unit items
+constexpr int kLong4Queue = 54;  // [54]: their queue
 constexpr int kJac = 80001;
 constexpr int kMmEntries = (128 * 129) / 2;
 
@@ -327,6 +329,9 @@ struct bv_plan {
   int grid_fbstack = 1;  // stacked kernel over the learned-band run's full-matrix list
   int stack_cols = 0;  // LDS carry row length of the stacked kernel
   Task* d_tasks64 = nullptr;       // long reads, FP64 first (FP32 mode), one pair each
+  Task* d_tasks64r = nullptr;      // the same long reads, one task per read (all its haplotypes)
+  int n_tasks64r = 0;
+  BandItem* d_items_long4 = nullptr;  // the FP32 moving band's quad units (four pairs of a read)
   Task* d_verify = nullptr;        // their pairs that may not underflow FP32
   int n_tasks64 = 0;
   Task* d_rescue = nullptr;
@@ -373,7 +378,8 @@ struct bv_plan {
   // (one per XCD), [32..47] banded chunk claims (per XCD, wide then narrow
   // kernel), [48] full-matrix list with haplotypes longer than kStackHmax,
   // [49] its queue, [50] FP64 band rescue count, [51] its queue, [52] its
-  // accepted pairs, [56..63]
+  // accepted pairs, [53] the FP32 moving band's quad-unit items, [54] their
+  // queue, [56..63]
   // quad banded chunk claims (per XCD), [64] the
   // quad shape's leftover wide and mid pairs, [72..79] the split prologue's
   // second kernel's task claims (per XCD), [80..87] the mid-band quad
@@ -422,7 +428,8 @@ void free_plan(bv_plan* p) {
   if (!p) return;
   // d_rescue_cells points into d_counters and is not freed on its own.
   void* bufs[] = {p->d_rclass, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_gcp_const, p->d_read_off,
-                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_verify,
+                  p->d_hmask, p->d_hap_off, p->d_tasks, p->d_tasks_region, p->d_tasks64, p->d_tasks64r,
+                  p->d_items_long4, p->d_verify,
                   p->d_stasks, p->d_tasks_rest, p->d_fallback_long,
                   p->d_rescue, p->d_counters,
                   p->d_out, p->d_form, p->d_carry32, p->d_carry64, p->d_fallback, p->d_band, p->d_items, p->d_items48, p->d_band_rescue,
@@ -641,6 +648,8 @@ int ensure_band_lists(bv_plan* p) {
   const int64_t n64 = std::max(1, p->n_tasks64);
   if (!p->d_items_long && hipMalloc(&p->d_items_long, sizeof(BandItem) * n64) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
+  if (!p->d_items_long4 && hipMalloc(&p->d_items_long4, sizeof(BandItem) * n64) != hipSuccess)
+    return fail(BV_ENOMEM, "band work list allocation failed");
   if (!p->d_long_full && hipMalloc(&p->d_long_full, sizeof(Task) * n64) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (p->n_tasks64 > 0) {
@@ -753,6 +762,11 @@ BArgs band_args(const bv_plan* p) {
   b.wide_pairs = p->wide_pairs;
   b.mid_on = p->mid_on && p->d_items48 != nullptr;
   b.long_cap = std::max(1, p->n_tasks64);
+  b.tasks64r = p->d_tasks64r;
+  b.n_tasks64r = p->n_tasks64r;
+  b.long_items4 = p->d_items_long4;
+  b.long4_count = ctr(p, kLong4Count);
+  b.long4_queue = ctr(p, kLong4Queue);
   b.band_rescue = p->d_band_rescue;
   b.band_rescue_count = ctr(p, kBandRescueCount);
   b.band_rescue_cap = p->band_rescue_cap;
@@ -1140,7 +1154,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   // Built per chunk of consecutive regions on its own thread, then joined
   // in region order (the same lists a single pass would build).
   struct Built {
-    std::vector<Task> tasks, tasks64, rest;
+    std::vector<Task> tasks, tasks64, tasks64r, rest;
     std::vector<uint64_t> cost, cost64, cost_rest, cost_stack;
     std::vector<StackTask> stasks;
     int64_t cells = 0, longhap = 0;
@@ -1204,6 +1218,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       const int64_t out0 = pair_base + (r - r0) * nh;
       o.cells += R * hsum;
       if (fp32_mode && R >= kFp64FirstRows) {
+        o.tasks64r.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h0), static_cast<int32_t>(nh), 0, out0});
         for (int64_t h = h0; h < h1; ++h) {
           o.tasks64.push_back(Task{static_cast<int32_t>(r), static_cast<int32_t>(h), 1, 0, out0 + (h - h0)});
           o.cost64.push_back(static_cast<uint64_t>(R * (hoff[h + 1] - hoff[h])));
@@ -1253,12 +1268,13 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
         std::copy((parts[c].*member).begin(), (parts[c].*member).end(), dst.begin() + at[c]);
     }, 1);
   };
-  std::vector<Task> tasks, tasks64, tasks_rest;
+  std::vector<Task> tasks, tasks64, tasks64r, tasks_rest;
   std::vector<uint64_t> cost, cost64, cost_rest, cost_stack;
   std::vector<StackTask> stasks;
   join(&Built::tasks, tasks);
   join(&Built::cost, cost);
   join(&Built::tasks64, tasks64);
+  join(&Built::tasks64r, tasks64r);
   join(&Built::cost64, cost64);
   join(&Built::rest, tasks_rest);
   join(&Built::cost_rest, cost_rest);
@@ -1284,6 +1300,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
   for (const Task& t : tasks) p->n_subtasks += (t.nhap + 63) / 64;
   p->grid_prologue = band_prologue_grid(g_ctx.cus, p->n_tasks);
   p->n_tasks64 = static_cast<int>(tasks64.size());
+  p->n_tasks64r = static_cast<int>(tasks64r.size());
 
   const int bpc32 = g_ctx.bpc_f32, bpc64 = g_ctx.bpc_f64;
   p->grid32 = opt.grid_f32 > 0 ? opt.grid_f32 : g_ctx.cus * bpc32;
@@ -1334,6 +1351,7 @@ int bv_plan_create(const bv_region_batch* b, const bv_options* optp, bv_plan** o
       (rc = upload_longest_first(&p->d_tasks_rest, static_cast<const Task*>(nullptr), tasks_rest, cost_rest, st,
                                  up)) ||
       (rc = upload_longest_first(&p->d_tasks64, static_cast<const Task*>(nullptr), tasks64, cost64, st, up)) ||
+      (rc = dev_upload(&p->d_tasks64r, tasks64r.data(), static_cast<int64_t>(tasks64r.size()), st, 0, up)) ||
       (rc = dev_upload(&p->d_verify, static_cast<const Task*>(nullptr), 0, st,
                        std::max<int64_t>(1, static_cast<int64_t>(tasks64.size())))))
     return cleanup(rc);
