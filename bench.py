@@ -535,7 +535,10 @@ def main():
             fp32_long = band_info["long_band_bits"] == 32
             peak = PEAK_FP32_TFLOPS if fp32_long else PEAK_FP64_TFLOPS
             band_info["long_roofline"] = {
-                "bound": "valu", "kernel": f"band_long_kernel<{'float' if fp32_long else 'double'}>",
+                "bound": "valu",
+                "kernel": ("band_long_quad_kernel + band_long_kernel<float> (four pairs of a read per "
+                           "wavefront, then the rest one per wavefront)" if fp32_long
+                           else "band_long_kernel<double>"),
                 "achieved": round(achl, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achl / peak, 4),
                 "flops_per_cell": BAND_FLOPS_PER_CELL, "kernel_ms": band_info["long_band_kernel_ms"],
                 "note": "shares the GPU with the short-read passes (second stream)"}
