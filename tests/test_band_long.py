@@ -1,5 +1,7 @@
 """Long reads (1024+ bases) on the band that follows the alignment, in FP32
-(the default, rescaled every 64 rows) and in FP64.
+(the default, rescaled every 16 rows; four pairs of a read per wavefront, a
+read's last one to three pairs one per wavefront) and in FP64 (rescaled every
+64 rows).
 
 The moving-band kernel against the oracle's restatement of the same band
 (bvo_band_follow_forward: likelihood, touch mass, dropped share and the
@@ -164,3 +166,43 @@ def test_long_band_follows_planted_drift(oracle, bits):
         assert form[k] == FORM_BAND[bits]
         assert abs(out[k] - v) <= BAND_EPS[bits]
         assert abs(out[k] - ex) <= 1e-6 + BAND_EPS[bits]
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_long_band_haplotypes_drifting_apart(oracle, bits):
+    """One long read against six haplotypes whose alignments drift in
+    different directions (20-30 base insertions or deletions in the
+    haplotype at different places, a longer prefix): in FP32 four of the pairs share a
+    wavefront, each band moving on its own, and the other two run one per
+    wavefront.  Every pair is accepted and equals the oracle's moving band
+    (to the kernel's rounding) and the full-matrix oracle."""
+    rng = np.random.default_rng(23)
+    base = bytes(rng.choice(list(b"ACGT"), 3200).astype(np.uint8))
+    extra = bytes(rng.choice(list(b"ACGT"), 20).astype(np.uint8))
+    # base quality 10: log10 L of -135 to -182, out of FP32's reach (a pair
+    # FP32 computes takes the FP32 full-matrix result, as in the reference).
+    # The read skips 15 bases at row 1200 (+15 diagonals on every haplotype
+    # that keeps them); the haplotypes add or drop 20-30 bases elsewhere.
+    rd = _drift_pair(rng, base, [(200, 1400), (1415, 2700)], q=10)
+    haps = [
+        base,                                                # +15 at row 1200
+        base[:900] + extra + base[900:],                     # +20 at row 700, then +15
+        base[:1800] + base[1830:],                           # +15, then -30 at row 1600
+        extra + base,                                        # starts at 220
+        base[:500] + extra + base[500:1000] + base[1020:],   # +20 at row 300, -20 at row 800
+        base[:2000] + base[2020:],                           # +15, then -20 at row 1800
+    ]
+    starts = [200, 200, 200, 220, 200, 200]
+    sel = [(rd, h) for h in haps]
+    pb = pairhmm.PackedBatch.from_regions([([rd], haps)])
+    bands = np.array([[s - 32 + 6, s + 31 + 6] for s in starts], np.int32)
+    out, form, st, used = run_plan(pb, bands=bands, bits=bits)
+    for k, (r_, h) in enumerate(sel):
+        v, ratio, lost, offs = oracle.band_follow(r_, h, int(bands[k, 0]), **RULES)
+        ex = oracle.log10_exact(r_, h, 0)
+        say(f"hap {k}: R={len(r_['seq'])} H={len(h)} form {form[k]} gpu {out[k]:.9f} band {v:.9f} "
+            f"exact {ex:.9f} ratio {ratio:.2e} offsets {offs[0]} -> {offs[-1]}")
+        assert form[k] == FORM_BAND[bits]
+        assert abs(out[k] - v) <= BAND_EPS[bits]
+        assert abs(out[k] - ex) <= 1e-6 + BAND_EPS[bits]
+    assert st.long_band_pairs == len(sel)
