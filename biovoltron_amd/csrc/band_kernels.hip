@@ -740,10 +740,18 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
   const bool quad = a.wide_pairs == kQuadPairs;
   const unsigned long long w64 = __ballot(cls == kBandLanes);
   const int n64 = __popcll(w64 & seg), r64 = __popcll(w64 & seg & lt);
-  const bool to_q = cls == kBandLanes && (!quad || r64 < (n64 & ~(kQuadPairs - 1)));
+  // quad slots of the piece (band_dispatch_kernel's rule): whole units, and
+  // one more, padded, when at least quad_fill wide pairs are left over
+  const int rem64 = n64 & (kQuadPairs - 1);
+  const int n64q = !quad ? n64 : (rem64 >= a.quad_fill ? n64 + kQuadPairs - rem64 : n64 - rem64);
+  const bool to_q = cls == kBandLanes && r64 < n64q;
   const bool to_1 = cls == kBandLanes && !to_q;
   const unsigned long long w32 = __ballot(cls == kBandNarrow);
   const bool first = valid && ((seg_first >> lane) & 1ull);
+  // the piece's padding slots (0-3), written by its first lane after its
+  // quad items
+  const int qpad = first ? max(0, n64q - n64) : 0;
+  const unsigned long long mp1 = __ballot(qpad & 1), mp2 = __ballot(qpad & 2);
   const bool odd = first && (__popcll(w32 & seg) & 1);
   const bool to_f = valid && cls == 0 && H <= kStackHmax;
   const bool to_fl = valid && cls == 0 && H > kStackHmax;
@@ -751,7 +759,7 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
                            mfl = __ballot(to_fl);
   int bq = 0, b1 = 0, b32 = 0, bf = 0, bfl = 0;
   if (lane == 0) {
-    if (mq) bq = atomicAdd(a.item_count, __popcll(mq));
+    if (mq) bq = atomicAdd(a.item_count, __popcll(mq) + __popcll(mp1) + 2 * __popcll(mp2));
     if (m1) b1 = atomicAdd(a.item1_count, __popcll(m1));
     if (w32) b32 = atomicAdd(a.item32_count, __popcll(w32) + __popcll(mo));
     if (mf) bf = atomicAdd(a.full_count, __popcll(mf));
@@ -763,10 +771,11 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
   bf = __shfl(bf, 0);
   bfl = __shfl(bfl, 0);
   const int pads_before = __popcll(mo & below(start));
+  const int qpads_before = __popcll(mp1 & below(start)) + 2 * __popcll(mp2 & below(start));
   if (cls) {
     const int wd = band.y - band.x + 1;
     const BandItem it{read, hap, band.x - ((cls - wd) >> 1), 0, out};
-    if (to_q) a.items[bq + __popcll(mq & lt)] = it;
+    if (to_q) a.items[bq + __popcll(mq & lt) + qpads_before] = it;
     else if (to_1) a.items1[b1 + __popcll(m1 & lt)] = it;
     else a.items32[b32 + __popcll(w32 & lt) + pads_before] = it;
   } else if (to_f) {
@@ -775,6 +784,8 @@ __device__ __forceinline__ void prologue_flush(const ProlArgs& A, const float* _
     a.full_long[bfl + __popcll(mfl & lt)] = Task{read, hap, 1, -1, out};
   }
   if (odd) a.items32[b32 + __popcll(w32 & below(end)) + pads_before] = BandItem{read, -1, 0, 0, -1};
+  for (int z = 0; z < qpad; ++z)
+    a.items[bq + __popcll(mq & below(end)) + qpads_before + z] = BandItem{read, -1, 0, 0, -1};
 }
 
 // The two-kernel prologue's buffer filling: the features come as compact

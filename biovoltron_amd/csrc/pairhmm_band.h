@@ -63,9 +63,10 @@ struct BArgs {
   int n_tasks;
   const int2* band;
   // banded work lists (dispatch output, band kernel input): 64-wide bands
-  // (with the quad shape: groups of four pairs of one read, and in items1
-  // the rest of each read's wide pairs, fewer than four, for the one-pair
-  // kernel), and 32-wide bands in read-sharing pairs
+  // (with the quad shape: groups of four pairs of one read, a group's last
+  // slots possibly padding, and in items1 the rest of each read's wide
+  // pairs, fewer than four, for the one-pair kernel), and 32-wide bands in
+  // read-sharing pairs
   BandItem* items;
   int* item_count;
   BandItem* items1;
@@ -135,6 +136,10 @@ struct BArgs {
   // the quad shape also runs 48-wide bands (three diagonals per lane); else
   // a band of 33-48 diagonals takes the wide lists (as a 64-wide band)
   int mid_on;
+  // quad shape: a task's (or task piece's) last wide pairs, fewer than
+  // four, fill one more unit of four (the empty slots padding) when there
+  // are at least quad_fill of them, else they go to items1; 4: never
+  int quad_fill;
 };
 
 // Dynamic LDS bytes of the short-read banded kernel for `pairs` pairs per

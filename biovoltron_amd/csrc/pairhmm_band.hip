@@ -256,11 +256,14 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
   }
   const int n32p = (n32 + 1) & ~1;
   // the quad shape takes a task's wide pairs four at a time; the rest (fewer
-  // than four) go to the one-pair kernel's list
+  // than four) fill one more unit, padded, when there are at least
+  // quad_fill of them, else go to the one-pair kernel's list.  n64q: the
+  // quad slots, padding included.
   const bool quad = a.wide_pairs == kQuadPairs;
-  const int n64q = quad ? n64 & ~(kQuadPairs - 1) : n64;
+  const int rem64 = n64 & (kQuadPairs - 1);
+  const int n64q = !quad ? n64 : (rem64 >= a.quad_fill ? n64 + kQuadPairs - rem64 : n64 - rem64);
   const int n48q = n48 & ~(kQuadPairs - 1);  // mid pairs exist only with the quad shape
-  const int n64r = n64 - n64q + n48 - n48q;
+  const int n64r = max(0, n64 - n64q) + n48 - n48q;
   int t64 = 0, t48 = 0, t64r = 0, t32 = 0, tf = 0, tfl = 0;
   const int o64 = wave_excl_scan(n64q, lane, t64);
   const int o48 = wave_excl_scan(n48q, lane, t48);
@@ -306,6 +309,7 @@ __global__ __launch_bounds__(256) void band_dispatch_kernel(BArgs a) {
         a.full_long[bfl++] = Task{tk.read, tk.hap0 + h, 1, -1, o};
       }
     }
+    for (int z = n64; z < n64q; ++z) a.items[b64++] = BandItem{tk.read, -1, 0, 0, -1};
     if (n32 & 1) a.items32[b32] = BandItem{tk.read, -1, 0, 0, -1};
   }
 }

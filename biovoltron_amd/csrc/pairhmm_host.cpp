@@ -304,6 +304,7 @@ struct bv_plan {
   int sel_cols = 0;  // short-read banded kernel: selector columns per class
   int wide_pairs = kQuadPairs;  // wide-band shape: pairs per wavefront (1 or kQuadPairs)
   bool mid_on = false;          // bands of 33-48 diagonals from bv_plan_set_bands on the 48-diagonal quad kernel
+  int quad_fill = 3;            // quad shape: a read's last wide pairs fill a padded unit when at least this many
   bool long_fp32 = true;        // the long reads' moving band in FP32 (rescaled every 64 rows), else FP64
   int64_t stride32 = 0, stride64 = 0;
   uint8_t* d_rclass = nullptr;
@@ -625,7 +626,11 @@ int ensure_band_lists(bv_plan* p) {
   // per task (per 64 haplotypes of a task in the fused prologue) to the
   // one-pair kernel; narrow items are padded to an even count per task (or
   // piece)
-  if (!p->d_items && hipMalloc(&p->d_items, sizeof(BandItem) * n) != hipSuccess)
+  // the quad list: the pairs, and at most kQuadPairs - 1 padding slots per
+  // task piece (a task or a 64-haplotype part of it; the prologue never
+  // splits a piece over two flushes)
+  if (!p->d_items &&
+      hipMalloc(&p->d_items, sizeof(BandItem) * (n + (kQuadPairs - 1) * p->n_subtasks)) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
   if (p->mid_on && !p->d_items48 && hipMalloc(&p->d_items48, sizeof(BandItem) * n) != hipSuccess)
     return fail(BV_ENOMEM, "band work list allocation failed");
@@ -761,6 +766,7 @@ BArgs band_args(const bv_plan* p) {
   b.sel_cols = p->sel_cols;
   b.wide_pairs = p->wide_pairs;
   b.mid_on = p->mid_on && p->d_items48 != nullptr;
+  b.quad_fill = p->quad_fill;
   b.long_cap = std::max(1, p->n_tasks64);
   b.tasks64r = p->d_tasks64r;
   b.n_tasks64r = p->n_tasks64r;
@@ -1540,6 +1546,23 @@ int bv_plan_set_long_precision(bv_plan* p, int32_t bits) {
 int bv_plan_long_precision(bv_plan* p, int32_t* bits) {
   if (!p || !bits) return fail(BV_EINVAL, "NULL argument");
   *bits = p->long_fp32 ? 32 : 64;
+  return BV_OK;
+}
+
+int bv_plan_set_quad_fill(bv_plan* p, int32_t min_pairs) {
+  if (!p) return fail(BV_EINVAL, "plan is NULL");
+  if (min_pairs < 1 || min_pairs > kQuadPairs) return fail(BV_EINVAL, "quad fill is 1 to 4 pairs");
+  HIP_TRY(hipSetDevice(g_ctx.device));
+  if (min_pairs != p->quad_fill) {
+    p->quad_fill = min_pairs;
+    drop_graph(p);
+  }
+  return BV_OK;
+}
+
+int bv_plan_quad_fill(bv_plan* p, int32_t* min_pairs) {
+  if (!p || !min_pairs) return fail(BV_EINVAL, "NULL argument");
+  *min_pairs = p->quad_fill;
   return BV_OK;
 }
 

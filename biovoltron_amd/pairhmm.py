@@ -87,6 +87,7 @@ EXPORTS = (
     "bv_plan_run", "bv_plan_time", "bv_plan_results", "bv_plan_stats_get", "bv_plan_destroy",
     "bv_plan_autotune", "bv_plan_set_grids", "bv_plan_set_band_shape", "bv_plan_band_shape",
     "bv_plan_set_band_prologue", "bv_plan_set_band_mid", "bv_plan_band_mid",
+    "bv_plan_set_quad_fill", "bv_plan_quad_fill",
     "bv_plan_set_long_precision", "bv_plan_long_precision",
     "bv_plan_postprocess", "bv_sw_align", "bv_sw_release",
     "bv_pairhmm_batch",
@@ -124,6 +125,8 @@ def library() -> C.CDLL:
         lib.bv_plan_set_band_prologue.argtypes = [vp, i32]
         lib.bv_plan_set_band_mid.argtypes = [vp, i32]
         lib.bv_plan_band_mid.argtypes = [vp, vp]
+        lib.bv_plan_set_quad_fill.argtypes = [vp, i32]
+        lib.bv_plan_quad_fill.argtypes = [vp, vp]
         lib.bv_plan_set_long_precision.argtypes = [vp, i32]
         lib.bv_plan_long_precision.argtypes = [vp, vp]
         lib.bv_plan_postprocess.argtypes = [vp, i32, vp, vp, vp, vp, vp]
@@ -299,13 +302,25 @@ class Plan:
 
     def set_long_precision(self, bits: int) -> None:
         """Arithmetic of the long reads' moving band: 32 (default, FP32
-        rescaled every 64 rows, form 5) or 64 (FP64, form 4)
+        rescaled every 16 rows, form 5) or 64 (FP64, form 4)
         (bv_plan_set_long_precision)."""
         _check(self._lib.bv_plan_set_long_precision(self._h, int(bits)))
 
     def long_precision(self) -> int:
         v = C.c_int32()
         _check(self._lib.bv_plan_long_precision(self._h, C.byref(v)))
+        return int(v.value)
+
+    def set_quad_fill(self, min_pairs: int) -> None:
+        """The quad shape's leftovers: a read's last 1-3 wide pairs fill one
+        more four-pair wavefront (empty slots idle) when at least min_pairs
+        of them are left, else run one per wavefront; 4 never fills
+        (bv_plan_set_quad_fill, default 3)."""
+        _check(self._lib.bv_plan_set_quad_fill(self._h, int(min_pairs)))
+
+    def quad_fill(self) -> int:
+        v = C.c_int32()
+        _check(self._lib.bv_plan_quad_fill(self._h, C.byref(v)))
         return int(v.value)
 
     def band_mid(self) -> bool:

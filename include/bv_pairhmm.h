@@ -362,7 +362,7 @@ int bv_plan_band_shape(bv_plan* plan, int32_t* wide_pairs);
  * reference counterpart. */
 int bv_plan_set_band_mid(bv_plan* plan, int32_t on);
 /* Arithmetic of the long reads' moving band: 32 (default) FP32 with the
- * state rescaled by a power of two every 64 rows (exact, so it never
+ * state rescaled by a power of two every 16 rows (exact, so it never
  * underflows; log10 L adds the removed exponent), form BV_FORM_BAND_LONG32;
  * or 64, FP64 as before, form BV_FORM_BAND_FP64.  Either way a pair whose
  * touch mass exceeds the tolerance, or whose likelihood FP32 would reach
@@ -372,6 +372,15 @@ int bv_plan_set_band_mid(bv_plan* plan, int32_t on);
 int bv_plan_set_long_precision(bv_plan* plan, int32_t bits);
 int bv_plan_long_precision(bv_plan* plan, int32_t* bits);
 int bv_plan_band_mid(bv_plan* plan, int32_t* on);
+/* The quad shape's leftovers: a read's wide pairs run four to a wavefront,
+ * and the last one to three of them either fill one more wavefront, its
+ * empty slots idle, when there are at least min_pairs of them, or run one
+ * pair per wavefront.  min_pairs 1-4; 4: never fill; default 3 (three
+ * pairs in a quad wavefront cost about 0.7 of three one-pair wavefronts).
+ * Results are the same banded likelihoods either way, to FP32 rounding.
+ * No reference counterpart. */
+int bv_plan_set_quad_fill(bv_plan* plan, int32_t min_pairs);
+int bv_plan_quad_fill(bv_plan* plan, int32_t* min_pairs);
 /* How a run with a band model builds its bands and work lists.  2 (the
  * default): the features kernel writes a compact record per pair (bin
  * counts as bytes, anchor, seed-end span), then one kernel runs the MLP,

@@ -489,6 +489,47 @@ def test_mid_band_on_off():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n_haps", [5, 6, 7])
+def test_quad_fill_of_leftover_pairs(n_haps):
+    """A read's last one to three wide pairs either fill one more four-pair
+    wavefront (padding slots idle) or run one pair per wavefront: with
+    64-diagonal caller bands, every fill setting (1-4) and the one-pair shape
+    give the same forms on all but a few pairs at the accept threshold and
+    the same likelihoods to FP32 rounding, all within 1e-3 of the full
+    matrix."""
+    sel, pb = read_groups(step=8, n_haps=n_haps)
+    lab = band.labels(pb)
+    centre = (lab[:, 0] + lab[:, 1]) // 2
+    bands = np.stack([centre - 32, centre + 31], 1).astype(np.int32)
+    full, _, _, _ = run_plan(pb)
+    base = None
+    for shape, fill in ((4, 4), (4, 3), (4, 2), (4, 1), (1, 4)):
+        with pairhmm.Plan(pb) as plan:
+            plan.set_band_shape(shape)
+            plan.set_quad_fill(fill)
+            assert plan.quad_fill() == fill
+            band.set_plan_bands(plan, bands)
+            plan.run()
+            out, form = plan.results()
+        err = float(np.abs(out - full).max())
+        say(f"{n_haps} haplotypes, shape {shape}, fill {fill}: banded {int((form == 2).sum())} of {len(sel)}, "
+            f"max |dlog10| vs full {err:.3g}")
+        assert err <= 1e-3
+        assert (form == 2).mean() > 0.9
+        if base is None:
+            base = (out, form)
+            continue
+        same = form == base[1]
+        assert same.mean() >= 0.999
+        both = same & (form == 2)
+        assert float(np.abs(out[both] - base[0][both]).max()) <= 1e-4
+    with pairhmm.Plan(pb) as plan:
+        for bad in (0, 5):
+            with pytest.raises(pairhmm.PairHMMError):
+                plan.set_quad_fill(bad)
+
+
+@pytest.mark.gpu
 def test_fp64_band_rescue_of_underflowing_short_pairs(oracle):
     """Short-read pairs whose FP32 band sum falls below 1e-28 (beyond FP32's
     reach: the reference re-runs them in FP64) take the FP64 moving band
